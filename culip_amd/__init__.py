@@ -1,0 +1,293 @@
+"""culip_amd: MI355X-native BILDLt preconditioner + SQMR for KKT systems.
+
+Python mirror of the reference's C++ interfaces (dthuerck/culip):
+  BlockiLDLt   libs/la/preconditioner/block_ildlt.cuh:188-311
+  CSRMatrixSPMV, AugmentedMatrixSPMV   libs/la/spmv.cuh:36-149
+  SQMR         libs/la/iterative_solvers/sqmr.cuh:21-72
+  read_matrix_csr / read_dense_vector  libs/test/test.h
+over the C-ABI of libculip_amd.so. Device vectors are torch CUDA (HIP)
+float64 tensors; torch only provides device memory and streams.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT,
+                   ORDER_NATURAL, ORDER_BLOCK_COLOR, check, lib)
+
+__all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
+           "SQMR", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
+           "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR"]
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def csr_struct(row_ptr, col, val, n=None):
+    """Builds a Csr view; returns (struct, keepalive)."""
+    rp, ci, cv = _i32(row_ptr), _i32(col), _f64(val)
+    m = len(rp) - 1
+    s = _lib.Csr(m, m if n is None else n, len(ci), _ptr(rp, C.c_int32), _ptr(ci, C.c_int32),
+                 _ptr(cv, C.c_double))
+    return s, (rp, ci, cv)
+
+
+def _copy_csr(c):
+    m, nnz = c.m, c.nnz
+    rp = np.ctypeslib.as_array(c.row_ptr, shape=(m + 1,)).copy()
+    ci = np.ctypeslib.as_array(c.col, shape=(nnz,)).copy() if nnz else np.zeros(0, np.int32)
+    cv = np.ctypeslib.as_array(c.val, shape=(nnz,)).copy() if nnz else np.zeros(0)
+    return rp, ci, cv
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (float64, contiguous)."""
+    if t is None:
+        return None
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise CulipError(1, "device vectors must be CUDA (HIP) torch tensors")
+    if t.dtype != torch.float64 or not t.is_contiguous():
+        raise CulipError(1, "device vectors must be contiguous float64")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+class KKT:
+    """Synthetic KKT system [[H, A'], [A, -delta I]] (host)."""
+
+    def __init__(self, n, m, h_offdiag_per_row=4, a_nnz_per_col=3, h_diag=(1.0, 10.0),
+                 h_diag_log=False, h_offdiag_std=0.1, delta=1e-8, seed=0):
+        p = _lib.KktParams(n, m, h_offdiag_per_row, a_nnz_per_col, h_diag[0], h_diag[1],
+                           1 if h_diag_log else 0, h_offdiag_std, delta, seed)
+        h = C.c_void_p()
+        check(lib().culip_kkt_generate(C.byref(p), C.byref(h)))
+        self._h = h
+        self.n, self.m = n, m
+        c = _lib.Csr()
+        rhs = C.POINTER(C.c_double)()
+        check(lib().culip_kkt_view(h, C.byref(c), C.byref(rhs)))
+        self.row_ptr, self.col, self.val = _copy_csr(c)
+        self.N = c.m
+        self.rhs = np.ctypeslib.as_array(rhs, shape=(c.m,)).copy()
+
+    def barrier_values(self, w_lo=1e-4, w_hi=1e4, seed=0):
+        out = np.empty(len(self.val))
+        check(lib().culip_kkt_barrier_values(self._h, w_lo, w_hi, seed, _ptr(out, C.c_double)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_kkt_destroy(self._h)
+            self._h = None
+
+
+class Prep:
+    """Ordering, contiguous blocking and static 1x1 pivot starts."""
+
+    def __init__(self, row_ptr, col, val, n_primal, max_block=32, ordering=ORDER_NATURAL):
+        s, keep = csr_struct(row_ptr, col, val)
+        h = C.c_void_p()
+        check(lib().culip_prep_create(C.byref(s), n_primal, max_block, ordering, C.byref(h)))
+        self._h = h
+        c = _lib.Csr()
+        perm = C.POINTER(C.c_int32)()
+        nbk = C.c_int32()
+        bs = C.POINTER(C.c_int32)()
+        npv = C.c_int32()
+        ps = C.POINTER(C.c_int32)()
+        check(lib().culip_prep_view(h, C.byref(c), C.byref(perm), C.byref(nbk), C.byref(bs),
+                                    C.byref(npv), C.byref(ps)))
+        self.row_ptr, self.col, self.val = _copy_csr(c)
+        self.perm = np.ctypeslib.as_array(perm, shape=(c.m,)).copy()
+        self.block_starts = np.ctypeslib.as_array(bs, shape=(nbk.value,)).copy()
+        self.piv_starts = np.ctypeslib.as_array(ps, shape=(npv.value,)).copy()
+
+    def permute_values(self, vals):
+        v = _f64(vals)
+        out = np.empty_like(v)
+        check(lib().culip_prep_permute_values(self._h, _ptr(v, C.c_double),
+                                              _ptr(out, C.c_double)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_prep_destroy(self._h)
+            self._h = None
+
+
+class BlockiLDLt:
+    """BlockiLDLt<T, false, false> (static pivots); prec 'f64' or 'f32' = T."""
+
+    def __init__(self, row_ptr, col, val, piv_starts, block_starts, prec="f64"):
+        s, keep = csr_struct(row_ptr, col, val)
+        ps, bs = _i32(piv_starts), _i32(block_starts)
+        h = C.c_void_p()
+        pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
+        check(lib().culip_bildlt_create(C.byref(s), len(ps), _ptr(ps, C.c_int32), len(bs),
+                                        _ptr(bs, C.c_int32), pr, C.byref(h)))
+        self._h = h
+        self.N = len(row_ptr) - 1
+        self.prec = "f32" if pr == PREC_F32 else "f64"
+
+    def n(self):
+        return lib().culip_bildlt_n(self._h)
+
+    def symbolic(self, fill_level=0, fill_factor=1.0):
+        check(lib().culip_bildlt_symbolic(self._h, fill_level, fill_factor))
+
+    def compute(self, fill_level=0, fill_factor=1.0, threshold=1e-4, stream=None):
+        check(lib().culip_bildlt_compute(self._h, fill_level, fill_factor, threshold,
+                                         _stream(stream)))
+        return 0.0   # the reference's compute() returns 0 (block_ildlt.impl.cuh:145)
+
+    def refactor(self, vals, threshold=1e-4, stream=None):
+        import torch
+        if isinstance(vals, torch.Tensor):
+            check(lib().culip_bildlt_refactor(self._h, _dptr(vals), 1, threshold,
+                                              _stream(stream)))
+        else:
+            v = _f64(vals)
+            check(lib().culip_bildlt_refactor(self._h, v.ctypes.data_as(C.c_void_p), 0,
+                                              threshold, _stream(stream)))
+
+    def _solve(self, side, b, x, transpose, stream):
+        check(lib().culip_bildlt_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),
+                                       _stream(stream)))
+
+    def solve_left(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_LEFT, b, x, transpose, stream)
+
+    def solve_middle(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_MIDDLE, b, x, transpose, stream)
+
+    def solve_right(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_RIGHT, b, x, transpose, stream)
+
+    def get_int(self, name):
+        p = C.POINTER(C.c_int32)()
+        n = C.c_int64()
+        check(lib().culip_bildlt_get_int(self._h, name.encode(), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.int32)
+
+    def get_real(self, name):
+        p = C.POINTER(C.c_double)()
+        n = C.c_int64()
+        check(lib().culip_bildlt_get_real(self._h, name.encode(), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0)
+
+    def device_bytes(self):
+        return lib().culip_bildlt_device_bytes(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_bildlt_destroy(self._h)
+            self._h = None
+
+
+class _Spmv:
+    def multiply(self, x, y, alpha=1.0, beta=0.0, transpose=False, stream=None):
+        check(lib().culip_spmv_multiply(self._h, _dptr(x), _dptr(y), alpha, beta,
+                                        1 if transpose else 0, _stream(stream)))
+
+    def m(self):
+        return lib().culip_spmv_m(self._h)
+
+    def n(self):
+        return lib().culip_spmv_n(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_spmv_destroy(self._h)
+            self._h = None
+
+
+class CSRMatrixSPMV(_Spmv):
+    def __init__(self, row_ptr, col, val, n=None):
+        s, self._keep = csr_struct(row_ptr, col, val, n)
+        h = C.c_void_p()
+        check(lib().culip_spmv_create_csr(C.byref(s), C.byref(h)))
+        self._h = h
+
+    def update_values(self, vals, stream=None):
+        v = _f64(vals)
+        check(lib().culip_spmv_update_values(self._h, v.ctypes.data_as(C.c_void_p), 0,
+                                             _stream(stream)))
+
+
+class AugmentedMatrixSPMV(_Spmv):
+    """K = [diag(d) A'; A delta*I(ix)] (spmv.cuh:116-149), dimension n + m."""
+
+    def __init__(self, a_row_ptr, a_col, a_val, a_ncols, d, delta=0.0, ix=None):
+        s, self._keep = csr_struct(a_row_ptr, a_col, a_val, a_ncols)
+        dd = _f64(d)
+        h = C.c_void_p()
+        ixa = _i32(ix if ix is not None else [])
+        check(lib().culip_spmv_create_kkt(C.byref(s), _ptr(dd, C.c_double), delta, len(ixa),
+                                          _ptr(ixa, C.c_int32) if len(ixa) else None,
+                                          C.byref(h)))
+        self._h = h
+
+
+class SQMR:
+    """SQMR<double>(spmv, tol, max_it, prec); solve(b, x) -> (residual, iterations)."""
+
+    def __init__(self, spmv, tol=1e-8, max_it=1000, prec=None, true_residual_period=0,
+                 max_restarts=0):
+        self._spmv, self._prec = spmv, prec
+        o = _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts)
+        h = C.c_void_p()
+        check(lib().culip_sqmr_create(spmv._h, prec._h if prec is not None else None,
+                                      C.byref(o), C.byref(h)))
+        self._h = h
+        self.result = None
+
+    def solve(self, b, x, stream=None):
+        r = _lib.SqmrResult()
+        check(lib().culip_sqmr_solve(self._h, _dptr(b), _dptr(x), C.byref(r), _stream(stream)))
+        self.result = {"residual": r.residual, "iterations": r.iterations,
+                       "converged": bool(r.converged), "restarts": r.restarts,
+                       "true_rel_residual": r.true_rel_residual}
+        return r.residual, r.iterations
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_sqmr_destroy(self._h)
+            self._h = None
+
+
+def read_matrix_csr(path):
+    """Test<T>::read_matrix_csr (libs/test/test.impl.h:218-228)."""
+    h = C.c_void_p()
+    check(lib().culip_mm_read_csr(str(path).encode(), C.byref(h)))
+    c = _lib.Csr()
+    check(lib().culip_host_matrix_view(h, C.byref(c)))
+    out = _copy_csr(c) + (c.n,)
+    lib().culip_host_matrix_destroy(h)
+    return out
+
+
+def read_dense_vector(path):
+    """Test<T>::read_dense_vector (libs/test/test.impl.h:236-300)."""
+    n = C.c_int64()
+    p = C.POINTER(C.c_double)()
+    check(lib().culip_mm_read_vector(str(path).encode(), C.byref(n), C.byref(p)))
+    v = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+    lib().culip_host_free(p)
+    return v

@@ -1,0 +1,62 @@
+// Numeric BILDLt factor on the device: shared declarations.
+#pragma once
+
+#include <cstdint>
+
+#include "../host/symbolic.hpp"
+
+namespace culip {
+
+// Device pointers used by the factor kernels of one system.
+template<typename VT>
+struct FactorArgs {
+    int32_t n = 0, nb = 0;
+    const int32_t* bs = nullptr;          // block starts (nb + 1)
+    const int32_t* is_piv = nullptr;      // n
+    const BlockMeta* bin = nullptr;       // symbolic metadata (input stores)
+    BlockMeta* bout = nullptr;            // factored metadata (output stores)
+    const int32_t* csr_off = nullptr;     // coarse CSR offsets (block ids)
+    const int32_t* csc_off = nullptr;
+    const int32_t* csc_idx = nullptr;
+    const int64_t* upd_off = nullptr;     // Schur pairs per block
+    const int32_t* upd_ki = nullptr;
+    const int32_t* upd_kj = nullptr;
+    const int32_t* lvlb_idx = nullptr;    // level block list
+    const int32_t* block_in_lvl = nullptr;
+    const int32_t* lvl_rows = nullptr;
+    const int32_t* ix_in = nullptr;       // input stores
+    const VT* val_in = nullptr;
+    int32_t* ix_out = nullptr;            // output stores
+    VT* val_out = nullptr;
+    double* tri = nullptr;                // 3n, tridiagonal D
+    VT* rn = nullptr;                     // n, row sums of squares
+    // per level position scratch
+    VT* if_val = nullptr;                 // 1024 per position
+    uint16_t* if_idx = nullptr;           // 1024 per position
+    int32_t* if_cnt = nullptr;
+    VT* rn_part = nullptr;                // 32 per position
+    int32_t* drop_cnt = nullptr;
+    uint8_t* overflow = nullptr;
+    int32_t* piv_loc = nullptr;           // nb
+    int32_t* carry = nullptr;             // pool, out_ix, out_val
+    VT threshold = 0;
+};
+
+struct LevelRange {
+    int32_t lo;        // offset of the level in the level block list
+    int32_t npiv;      // pivot blocks (first in the level)
+    int32_t size;      // blocks in the level
+    int32_t row_lo;    // offset of the level in lvl_rows
+};
+
+// Host entry points (factor.hip). All launches go to `stream`.
+template<typename VT>
+void factor_begin(const FactorArgs<VT>& a, void* stream);
+template<typename VT>
+void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream);
+// val_in[slot[q]] = vals[q] for slot[q] >= 0 (val_in zeroed first)
+template<typename VT>
+void gather_values(const double* vals, const int64_t* slot, int64_t nnz, VT* val_in,
+    int64_t val_space, void* stream);
+
+}  // namespace culip

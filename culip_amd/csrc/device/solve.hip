@@ -1,0 +1,318 @@
+// Solve layout construction and level-scheduled block triangular solves.
+//
+// Reference: solve_U_block / solve_Ut_block (block_ildlt.impl.cuh:2378-2426,
+// 2557-2606) with k_solve_blocks_offdiagonal_atomic / _diagonal_atomic
+// (block_ildlt_kernels.impl.cuh:2354-2606): per level, x[rows] -= U_blk x
+// for the level's off-diagonal blocks, then a 32-step substitution with the
+// pivot block. Here the factor is first re-laid out per fine row (entries of
+// a block row contiguous), the pivot blocks are inverted once, and each level
+// is one launch: one wave per block row gathers its rows with coalesced loads
+// and a segmented wave reduction (fixed order, no atomics), then applies
+// U_ii^-1. k_solve_D follows helper_kernels.impl.cuh:720-772.
+#include "solve.hpp"
+#include "wave.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace culip {
+
+namespace {
+
+// columns of U_ii^-1 by back substitution, in double; X[r][c] (32 x 32 LDS)
+template<typename VT>
+__device__ void diag_inverse(const VT* U, int h, double* X)
+{
+    const int c = lane_id();
+    for(int r = 31; r >= 0; --r) {
+        if(c < 32) {
+            double v = 0.0;
+            if(c < h && r < h && r <= c) {
+                if(r == c) {
+                    v = 1.0 / (double) U[r * h + r];
+                } else {
+                    double s = 0.0;
+                    for(int k = r + 1; k <= c; ++k) s = fma((double) U[r * h + k], X[32 * k + c], s);
+                    v = -s / (double) U[r * h + r];
+                }
+            }
+            X[32 * r + c] = v;
+        }
+    }
+    wave_sync();
+}
+
+template<typename VT, bool FILL>
+__global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+{
+    __shared__ double X[1024];
+    const int lane = lane_id();
+    const int i = blockIdx.x;
+    const int fr = t.bs[i];
+    const int h = t.bs[i + 1] - fr;
+    // pivot block inverse rows (ui) and columns (ti)
+    const BlockMeta dm = t.bout[t.csr_off[i]];
+    diag_inverse(val_out + dm.val_ptr, h, X);
+    if(lane < h) {
+        int cr = 0, cc = 0;
+        int32_t wr = 0, wc = 0;
+        if(FILL) { wr = p.ui_ptr[fr + lane]; wc = p.ti_ptr[fr + lane]; }
+        for(int c = 0; c < h; ++c) {
+            const double v = X[32 * lane + c];  // row lane of U^-1
+            if(v != 0.0) {
+                if(FILL) { p.ui_col[wr + cr] = fr + c; p.ui_val[wr + cr] = (VT) v; p.ui_lr[wr + cr] = (uint8_t) lane; }
+                ++cr;
+            }
+            const double u = X[32 * c + lane];  // column lane of U^-1
+            if(u != 0.0) {
+                if(FILL) { p.ti_col[wc + cc] = fr + c; p.ti_val[wc + cc] = (VT) u; p.ti_lr[wc + cc] = (uint8_t) lane; }
+                ++cc;
+            }
+        }
+        if(!FILL) { p.ui_ptr[fr + lane + 1] = cr; p.ti_ptr[fr + lane + 1] = cc; }
+    }
+    // off-diagonal blocks of block row i (uo), by block id
+    if(lane < h) {
+        int cnt = 0;
+        const int32_t base = FILL ? p.uo_ptr[fr + lane] : 0;
+        for(int b = t.csr_off[i] + 1; b < t.csr_off[i + 1]; ++b) {
+            const BlockMeta m = t.bout[b];
+            const int fc = t.bs[m.block_col];
+            const int w = t.bs[m.block_col + 1] - fc;
+            const VT* v = val_out + m.val_ptr;
+            if(m.format == FMT_DENSE) {
+                for(int c = 0; c < w; ++c) {
+                    const VT x = v[lane * w + c];
+                    if(x != (VT) 0) {
+                        if(FILL) { p.uo_col[base + cnt] = fc + c; p.uo_val[base + cnt] = x; p.uo_lr[base + cnt] = (uint8_t) lane; }
+                        ++cnt;
+                    }
+                }
+            } else {
+                const int32_t* ix = t.ix_out + m.col_ptr;
+                for(int e = 0; e < m.nnz; ++e) {
+                    const int q = ix[e];
+                    const int r = q / w;
+                    if(r == lane) {
+                        if(FILL) { p.uo_col[base + cnt] = fc + (q - r * w); p.uo_val[base + cnt] = v[e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+        if(!FILL) p.uo_ptr[fr + lane + 1] = cnt;
+    }
+}
+
+template<typename VT, bool FILL>
+__global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+{
+    const int lane = lane_id();
+    const int j = blockIdx.x;
+    const int fc = t.bs[j];
+    const int w = t.bs[j + 1] - fc;
+    if(lane >= w) return;
+    int cnt = 0;
+    const int32_t base = FILL ? p.to_ptr[fc + lane] : 0;
+    // blocks of column j above the diagonal, ascending block row
+    for(int q = t.csc_off[j]; q < t.csc_off[j + 1] - 1; ++q) {
+        const BlockMeta m = t.bout[t.csc_idx[q]];
+        const int fr = t.bs[m.block_row];
+        const int h = t.bs[m.block_row + 1] - fr;
+        const VT* v = val_out + m.val_ptr;
+        if(m.format == FMT_DENSE) {
+            for(int r = 0; r < h; ++r) {
+                const VT x = v[r * w + lane];
+                if(x != (VT) 0) {
+                    if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = x; p.to_lr[base + cnt] = (uint8_t) lane; }
+                    ++cnt;
+                }
+            }
+        } else {
+            const int32_t* ix = t.ix_out + m.col_ptr;
+            for(int e = 0; e < m.nnz; ++e) {
+                const int qq = ix[e];
+                const int r = qq / w;
+                if(qq - r * w == lane) {
+                    if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = v[e]; p.to_lr[base + cnt] = (uint8_t) lane; }
+                    ++cnt;
+                }
+            }
+        }
+    }
+    if(!FILL) p.to_ptr[fc + lane + 1] = cnt;
+}
+
+// inclusive scan of a[1..n] in place (a[0] = 0), one workgroup, in chunks
+__global__ __launch_bounds__(1024) void k_scan_inplace(int32_t* a, int32_t n, int32_t* total)
+{
+    __shared__ int32_t s[1024];
+    const int tid = threadIdx.x;
+    int32_t carry = 0;
+    if(tid == 0) a[0] = 0;
+    for(int32_t c0 = 1; c0 <= n; c0 += 1024) {
+        const int32_t i = c0 + tid;
+        int32_t v = (i <= n) ? a[i] : 0;
+        s[tid] = v;
+        __syncthreads();
+        for(int o = 1; o < 1024; o <<= 1) {
+            const int32_t u = (tid >= o) ? s[tid - o] : 0;
+            __syncthreads();
+            s[tid] += u;
+            __syncthreads();
+        }
+        if(i <= n) a[i] = carry + s[tid];
+        const int32_t tot = s[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if(tid == 0 && total) *total = carry;
+}
+
+// segmented inclusive sum over lanes; segments are runs of equal sg
+template<typename T>
+__device__ __forceinline__ T seg_scan(T v, int sg)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for(int o = 1; o < 64; o <<= 1) {
+        const T u = __shfl_up(v, o, 64);
+        const int su = __shfl_up(sg, o, 64);
+        if(lane >= o && su == sg) v += u;
+    }
+    return v;
+}
+
+// one level: one wave per block row; y = x_i - sum(off-diag) ; x_i = Uinv y
+template<typename VT>
+__global__ __launch_bounds__(64) void k_trsv_level(const int32_t* rows, const int32_t* bs,
+    const int32_t* o_ptr, const int32_t* o_col, const VT* o_val, const uint8_t* o_lr,
+    const int32_t* i_ptr, const int32_t* i_col, const VT* i_val, const uint8_t* i_lr,
+    double* x)
+{
+    __shared__ VT y[33];
+    __shared__ VT res[33];
+    const int lane = lane_id();
+    const int i = rows[blockIdx.x];
+    const int fr = bs[i];
+    const int h = bs[i + 1] - fr;
+    if(lane < h) { y[lane] = (VT) x[fr + lane]; res[lane] = 0; }
+    wave_sync();
+    const int s = o_ptr[fr], e_end = o_ptr[fr + h];
+    for(int e0 = s; e0 < e_end; e0 += 64) {
+        const int e = e0 + lane;
+        VT v = 0;
+        int sg = 64;
+        if(e < e_end) {
+            v = o_val[e] * (VT) x[o_col[e]];
+            sg = o_lr[e];
+        }
+        v = seg_scan(v, sg);
+        const int nsg = __shfl_down(sg, 1, 64);
+        if(sg < 64 && (lane == 63 || nsg != sg)) y[sg] -= v;
+        wave_sync();
+    }
+    const int s2 = i_ptr[fr], e2 = i_ptr[fr + h];
+    for(int e0 = s2; e0 < e2; e0 += 64) {
+        const int e = e0 + lane;
+        VT v = 0;
+        int sg = 64;
+        if(e < e2) {
+            v = i_val[e] * y[i_col[e] - fr];
+            sg = i_lr[e];
+        }
+        v = seg_scan(v, sg);
+        const int nsg = __shfl_down(sg, 1, 64);
+        if(sg < 64 && (lane == 63 || nsg != sg)) res[sg] += v;
+        wave_sync();
+    }
+    if(lane < h) x[fr + lane] = (double) res[lane];
+}
+
+__global__ void k_solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b,
+    double* x)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g >= n) return;
+    const int this_piv = is_piv[g];
+    const int next_piv = (g < n - 1) ? is_piv[g + 1] : 1;
+    const double p11 = tri[3 * (int64_t) g + 1];
+    const double p12 = tri[3 * (int64_t) g + 2];
+    const double p22 = !next_piv ? tri[3 * (int64_t) (g + 1) + 1] : 1.0;
+    const double r1 = b[g];
+    const double r2 = (g < n - 1) ? b[g + 1] : 0.0;
+    const double gr = sqrt(fma(p11, p11, p12 * p12));
+    const double c = p11 / gr, s = p12 / gr;
+    double g1 = fma(c, r1, s * r2);
+    double g2 = fma(-s, r1, c * r2);
+    const double t12 = fma(c, p12, s * p22);
+    const double t22 = fma(-s, p12, c * p22);
+    g2 = g2 / t22;
+    g1 = fma(-t12, g2, g1) / gr;
+    if(this_piv) x[g] = g1;
+    if(!next_piv) x[g + 1] = g2;
+}
+
+}  // namespace
+
+template<typename VT>
+void solve_plan_count(const SolveTopo& t, const VT* val_out, VT*, SolvePlan<VT>& p, void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    k_plan_rows<VT, false><<<t.nb, 64, 0, st>>>(t, val_out, p);
+    k_plan_cols<VT, false><<<t.nb, 64, 0, st>>>(t, val_out, p);
+}
+
+template<typename VT>
+void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT*, SolvePlan<VT>& p,
+    void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    k_plan_rows<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
+    k_plan_cols<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
+}
+
+void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, void* stream)
+{
+    k_scan_inplace<<<1, 1024, 0, (hipStream_t) stream>>>(ptr, n, total_dev);
+}
+
+template<typename VT>
+void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    for(int l = t.num_levels - 1; l >= 0; --l) {
+        const int lo = t.lvl_off_h[l], cnt = t.lvl_off_h[l + 1] - lo;
+        if(cnt > 0)
+            k_trsv_level<VT><<<cnt, 64, 0, st>>>(t.lvl_rows + lo, t.bs, p.uo_ptr, p.uo_col,
+                p.uo_val, p.uo_lr, p.ui_ptr, p.ui_col, p.ui_val, p.ui_lr, x);
+    }
+}
+
+template<typename VT>
+void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    for(int l = 0; l < t.num_levels; ++l) {
+        const int lo = t.lvl_off_h[l], cnt = t.lvl_off_h[l + 1] - lo;
+        if(cnt > 0)
+            k_trsv_level<VT><<<cnt, 64, 0, st>>>(t.lvl_rows + lo, t.bs, p.to_ptr, p.to_col,
+                p.to_val, p.to_lr, p.ti_ptr, p.ti_col, p.ti_val, p.ti_lr, x);
+    }
+}
+
+void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
+    void* stream)
+{
+    k_solve_D<<<(n + 255) / 256, 256, 0, (hipStream_t) stream>>>(n, tri, is_piv, b, x);
+}
+
+template void solve_plan_count<double>(const SolveTopo&, const double*, double*, SolvePlan<double>&, void*);
+template void solve_plan_count<float>(const SolveTopo&, const float*, float*, SolvePlan<float>&, void*);
+template void solve_plan_fill<double>(const SolveTopo&, const double*, const double*, SolvePlan<double>&, void*);
+template void solve_plan_fill<float>(const SolveTopo&, const float*, const float*, SolvePlan<float>&, void*);
+template void trsv_upper<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
+template void trsv_upper<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
+template void trsv_upper_t<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
+template void trsv_upper_t<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
+
+}  // namespace culip

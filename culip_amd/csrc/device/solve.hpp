@@ -1,0 +1,59 @@
+// Block triangular solves with the BILDLt factor on the device.
+#pragma once
+
+#include <cstdint>
+
+#include "factor.hpp"
+
+namespace culip {
+
+// Solve layout built once per factorization from the block storage.
+// For the U solve every fine row r holds the entries of the off-diagonal
+// blocks of its block row (uo) and row r of U_ii^-1 (ui); for the U' solve
+// every fine row c holds column c of the off-diagonal blocks above its
+// block (to) and column c of U_jj^-1 (ti). Each row's entries come in block
+// order, so all sums run in a fixed order. lr* = local row of each entry.
+template<typename VT>
+struct SolvePlan {
+    int32_t* uo_ptr; int32_t* uo_col; VT* uo_val; uint8_t* uo_lr;
+    int32_t* ui_ptr; int32_t* ui_col; VT* ui_val; uint8_t* ui_lr;
+    int32_t* to_ptr; int32_t* to_col; VT* to_val; uint8_t* to_lr;
+    int32_t* ti_ptr; int32_t* ti_col; VT* ti_val; uint8_t* ti_lr;
+};
+
+struct SolveTopo {
+    int32_t n, nb, num_levels;
+    const int32_t* bs;          // block starts
+    const int32_t* lvl_rows;    // block rows by level (device)
+    const int32_t* lvl_off_h;   // level offsets (host)
+    const int32_t* csr_off;     // coarse CSR offsets (block ids of a row)
+    const int32_t* csc_off;     // coarse CSC offsets
+    const int32_t* csc_idx;     // block ids of a column by row
+    const BlockMeta* bout;      // factored metadata
+    const int32_t* ix_out;
+    const double* tri;
+    const int32_t* is_piv;
+};
+
+// Counting pass of the layout: fills the four count arrays (n entries each,
+// stored in the *_ptr arrays shifted by one) and the pivot-block inverses.
+template<typename VT>
+void solve_plan_count(const SolveTopo& t, const VT* val_out, VT* dinv, SolvePlan<VT>& p,
+    void* stream);
+// Fill pass after the *_ptr arrays hold exclusive prefix sums.
+template<typename VT>
+void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT* dinv, SolvePlan<VT>& p,
+    void* stream);
+
+// exclusive prefix sum of cnt[1..n] into ptr[0..n] (ptr[0] = 0), in place
+void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, void* stream);
+
+// x <- U \ x (in place); x <- U' \ x; x <- D \ x.
+template<typename VT>
+void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
+template<typename VT>
+void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
+void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
+    void* stream);
+
+}  // namespace culip

@@ -1,0 +1,102 @@
+// CSR SpMV (cuSPARSE csrmv semantics, sparse_la_instances.cu:130-232) and
+// the matrix-free KKT operator (AugmentedMatrixSPMV, spmv.impl.cuh:362-427).
+// y = alpha * A x + beta * y. Eight lanes per row, coalesced entry loads,
+// fixed-order reduction (xor tree within the eight lanes).
+#include "spmv.hpp"
+#include "wave.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace culip {
+
+namespace {
+
+constexpr int LPR = 8;   // lanes per row
+
+__global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __restrict__ rp,
+    const int32_t* __restrict__ ci, const double* __restrict__ cv, const double* __restrict__ x,
+    double* __restrict__ y, double alpha, double beta)
+{
+    const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row0 = tid / LPR;
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x / LPR;
+    for(int64_t row = row0; row < m; row += stride) {
+        double acc = 0.0;
+        const int32_t s = rp[row], e = rp[row + 1];
+        for(int32_t q = s + sub; q < e; q += LPR) acc = fma(cv[q], x[ci[q]], acc);
+#pragma unroll
+        for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        if(sub == 0) {
+            const double r = alpha * acc;
+            y[row] = (beta == 0.0) ? r : fma(beta, y[row], r);
+        }
+    }
+}
+
+// KKT: b1 = d .* x1 + A' x2 ; b2 = delta * x2(ix) + A x1 ; y = alpha b + beta y
+__global__ __launch_bounds__(256) void k_kkt_primal(int32_t n, const double* d,
+    const int32_t* atp, const int32_t* ati, const double* atv, const double* x, double* y,
+    double alpha, double beta)
+{
+    const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row = tid / LPR;
+    double acc = 0.0;
+    if(row < n) {
+        const double* x2 = x + n;
+        for(int32_t q = atp[row] + sub; q < atp[row + 1]; q += LPR) acc = fma(atv[q], x2[ati[q]], acc);
+    }
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if(row < n && sub == 0) {
+        const double b = fma(d[row], x[row], acc);
+        y[row] = (beta == 0.0) ? alpha * b : fma(beta, y[row], alpha * b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_kkt_dual(int32_t n, int32_t m, const double* dix,
+    const int32_t* ap, const int32_t* ai, const double* av, const double* x, double* y,
+    double alpha, double beta)
+{
+    const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row = tid / LPR;
+    double acc = 0.0;
+    if(row < m) {
+        for(int32_t q = ap[row] + sub; q < ap[row + 1]; q += LPR) acc = fma(av[q], x[ai[q]], acc);
+    }
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if(row < m && sub == 0) {
+        const double b = fma(dix[row], x[n + row], acc);
+        y[n + row] = (beta == 0.0) ? alpha * b : fma(beta, y[n + row], alpha * b);
+    }
+}
+
+}  // namespace
+
+void csr_spmv(const DevCsr& A, const double* x, double* y, double alpha, double beta,
+    void* stream)
+{
+    if(A.m == 0) return;
+    const int64_t threads = (int64_t) A.m * LPR;
+    int64_t grid = (threads + 255) / 256;
+    if(grid > 65536) grid = 65536;
+    k_csr_spmv<<<(unsigned) grid, 256, 0, (hipStream_t) stream>>>(A.m, A.row_ptr, A.col, A.val,
+        x, y, alpha, beta);
+}
+
+void kkt_spmv(const DevKkt& K, const double* x, double* y, double alpha, double beta,
+    void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    if(K.n > 0)
+        k_kkt_primal<<<(unsigned) (((int64_t) K.n * LPR + 255) / 256), 256, 0, st>>>(K.n, K.d,
+            K.At.row_ptr, K.At.col, K.At.val, x, y, alpha, beta);
+    if(K.m > 0)
+        k_kkt_dual<<<(unsigned) (((int64_t) K.m * LPR + 255) / 256), 256, 0, st>>>(K.n, K.m,
+            K.dix, K.A.row_ptr, K.A.col, K.A.val, x, y, alpha, beta);
+}
+
+}  // namespace culip

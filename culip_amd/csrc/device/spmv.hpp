@@ -1,0 +1,29 @@
+// Device sparse operators.
+#pragma once
+
+#include <cstdint>
+
+namespace culip {
+
+struct DevCsr {
+    int32_t m = 0, n = 0;
+    int64_t nnz = 0;
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    double* val = nullptr;
+};
+
+// K = [diag(d) A'; A diag(dix)] with dix = delta on the rows in ix, else 0.
+struct DevKkt {
+    int32_t n = 0, m = 0;
+    DevCsr A, At;
+    double* d = nullptr;    // n
+    double* dix = nullptr;  // m
+};
+
+void csr_spmv(const DevCsr& A, const double* x, double* y, double alpha, double beta,
+    void* stream);
+void kkt_spmv(const DevKkt& K, const double* x, double* y, double alpha, double beta,
+    void* stream);
+
+}  // namespace culip

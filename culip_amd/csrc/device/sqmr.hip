@@ -1,0 +1,559 @@
+// SQMR on the device, FP64 recurrences.
+//
+// Reference: libs/la/iterative_solvers/sqmr.impl.cuh:109-355. The reference
+// issues per iteration 4 dots, 2 nrm2, 5 axpy, 4 scal, 3 D-solves, 6 copies
+// and ~22 single-thread scalar kernels. Here one iteration is six vector
+// kernels plus the preconditioned operator:
+//   k_alpha   alpha's two dots; D \ Av is formed on the fly, never stored
+//   k_vnext   v_{k+1} = Av - alpha v_k - beta_k v_{k-1}, ||v_{k+1}||^2
+//   k_vscale  v_{k+1} /= delta, z = D \ v_{k+1}, w_{k+1} = eta delta z, ev = z
+//   [right(ev); ev2 = A ev; left(ev2)]                   (Av for k + 1)
+//   k_beta    beta's two dots
+//   k_pupdate w_{k+1} /= beta, p_k, x += tau p_k, ||x||^2
+//   k_scalars the scalar recurrence, quasi-residual and the stopping test
+// D \ v_k is reused from the previous iteration (the reference recomputes the
+// same vector). Every workgroup reduces the per-workgroup partial sums in the
+// same fixed order, so all scalars are identical everywhere and run to run.
+// The host checks the stopping flags with the reference's 4-iteration
+// throttle (:309-331).
+#include "sqmr.hpp"
+#include "wave.hpp"
+#include "../host/common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+namespace culip {
+
+namespace {
+
+constexpr int TPB = 256;
+
+// scalar slots (sqmr.impl.cuh:138-156)
+enum {
+    S_ALPHA = 0, S_BETA = 1, S_DELTA = 4, S_NORMA = 7, S_EPS = 8, S_BROT = 9, S_BRR = 10,
+    S_AROT = 11, S_C = 12, S_S = 15, S_TAU = 18, S_RHO = 19, S_GAMMA = 20, S_NORMPB = 21,
+    S_NORMX = 22, S_ETA = 26, S_ITER = 27, NSCAL = 32
+};
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if(e_ != hipSuccess) fail(CULIP_ERR_HIP, std::string("HIP: ") + hipGetErrorString(e_)); \
+    } while(0)
+
+__device__ double block_sum(double v, double* sh)
+{
+    const int tid = threadIdx.x;
+    v = wave_sum(v);
+    __syncthreads();
+    if((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if(tid == 0) {
+        for(int q = 0; q < (int) (blockDim.x >> 6); ++q) t += sh[q];
+        sh[8] = t;
+    }
+    __syncthreads();
+    t = sh[8];
+    __syncthreads();
+    return t;
+}
+
+// every workgroup sums the partials part[q * stride + off], q < G, the same way
+__device__ double reduce_part(const double* part, int G, int stride, int off, double* sh)
+{
+    double v = 0.0;
+    for(int q = threadIdx.x; q < G; q += blockDim.x) v += part[q * stride + off];
+    return block_sum(v, sh);
+}
+
+struct DP { double g1, g2; bool w1, w2; };
+
+// k_solve_D at row g (helper_kernels.impl.cuh:720-772)
+__device__ __forceinline__ DP dsolve(int32_t g, int32_t n, const double* tri, const int32_t* ip,
+    double r1, double r2)
+{
+    DP o;
+    if(!tri) { o.g1 = r1; o.g2 = 0; o.w1 = true; o.w2 = false; return o; }
+    const int this_piv = ip[g];
+    const int next_piv = (g < n - 1) ? ip[g + 1] : 1;
+    const double p11 = tri[3 * (int64_t) g + 1];
+    const double p12 = tri[3 * (int64_t) g + 2];
+    const double p22 = !next_piv ? tri[3 * (int64_t) (g + 1) + 1] : 1.0;
+    const double gr = sqrt(fma(p11, p11, p12 * p12));
+    const double c = p11 / gr, s = p12 / gr;
+    double g1 = fma(c, r1, s * r2);
+    double g2 = fma(-s, r1, c * r2);
+    const double t12 = fma(c, p12, s * p22);
+    const double t22 = fma(-s, p12, c * p22);
+    g2 = g2 / t22;
+    g1 = fma(-t12, g2, g1) / gr;
+    o.g1 = g1; o.g2 = g2; o.w1 = this_piv != 0; o.w2 = !next_piv;
+    return o;
+}
+
+__device__ __forceinline__ double sgn(double v) { return v < 0 ? -1.0 : 1.0; }
+
+// k_sym_ortho (helper_kernels.impl.cuh:662-699)
+__device__ void sym_ortho(double a, double b, double& c, double& s, double& r)
+{
+    if(b == 0) { s = 0; r = fabs(a); c = (a == 0) ? 1 : sgn(a); }
+    else if(a == 0) { c = 0; s = sgn(b); r = fabs(b); }
+    else if(fabs(b) >= fabs(a)) {
+        const double tao = a / b;
+        s = sgn(b) / sqrt(1 + tao * tao);
+        c = s * tao;
+        r = b / s;
+    } else {
+        const double tao = b / a;
+        c = sgn(a) / sqrt(1 + tao * tao);
+        s = c * tao;
+        r = a / c;
+    }
+}
+
+// scalar part of one iteration (sqmr.impl.cuh:192-283), on a private copy
+__device__ void scalar_step(double* S, int k, double alpha, double delta1, double beta1)
+{
+    const int k0 = k % 3, k1 = (k + 1) % 3, km1 = (k + 2) % 3;
+    S[S_ALPHA] = alpha;
+    S[S_DELTA + k1] = delta1;
+    S[S_BETA + k1] = beta1;
+    S[S_ETA] = (delta1 / beta1) * S[S_ETA];
+    const double bk = S[S_BETA + k0];
+    const double nrm3 = sqrt(bk * bk + alpha * alpha + delta1 * delta1);
+    S[S_NORMA] = fmax(S[S_NORMA], nrm3);
+    S[S_EPS] = S[S_S + km1] * bk;
+    S[S_BROT] = (-S[S_C + km1]) * bk;
+    S[S_BRR] = S[S_C + k0] * S[S_BROT] + S[S_S + k0] * alpha;
+    S[S_AROT] = S[S_S + k0] * S[S_BROT] + (-S[S_C + k0]) * alpha;
+    double c, s, r;
+    sym_ortho(S[S_AROT], delta1, c, s, r);
+    S[S_C + k1] = c;
+    S[S_S + k1] = s;
+    S[S_RHO] = r;
+    S[S_TAU] = c * S[S_GAMMA];
+}
+
+__global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, const double* v,
+    const double* z, const double* tri, const int32_t* ip, double* part)
+{
+    __shared__ double sh[16];
+    double num = 0.0, den = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
+        const DP d = dsolve(g, n, tri, ip, Av[g], (g < n - 1) ? Av[g + 1] : 0.0);
+        if(d.w1) num = fma(v[g], d.g1, num);
+        if(d.w2) num = fma(v[g + 1], d.g2, num);
+        den = fma(v[g], z[g], den);
+    }
+    num = block_sum(num, sh);
+    den = block_sum(den, sh);
+    if(threadIdx.x == 0) { part[8 * blockIdx.x + 0] = num; part[8 * blockIdx.x + 1] = den; }
+}
+
+__global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* Av,
+    const double* vk, const double* vkm1, double* tv, const double* scal, double* part)
+{
+    __shared__ double sh[16];
+    const int G = gridDim.x;
+    const double alpha = reduce_part(part, G, 8, 0, sh) / reduce_part(part, G, 8, 1, sh);
+    const double beta = scal[S_BETA + k % 3];
+    double ss = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+        const double t = fma(-beta, vkm1[g], fma(-alpha, vk[g], Av[g]));
+        tv[g] = t;
+        ss = fma(t, t, ss);
+    }
+    ss = block_sum(ss, sh);
+    if(threadIdx.x == 0) part[8 * blockIdx.x + 2] = ss;
+}
+
+__global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, double* v1,
+    double* z, double* w1, double* ev, const double* tri, const int32_t* ip, const double* scal,
+    const double* part)
+{
+    __shared__ double sh[16];
+    const int G = gridDim.x;
+    const double delta = sqrt(reduce_part(part, G, 8, 2, sh));
+    const double t = 1.0 / delta;
+    const double we = scal[S_ETA] * delta;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+        const double r1 = tv[g] * t;
+        const double r2 = (g < n - 1) ? tv[g + 1] * t : 0.0;
+        v1[g] = r1;
+        const DP d = dsolve(g, n, tri, ip, r1, r2);
+        if(d.w1) { z[g] = d.g1; w1[g] = d.g1 * we; ev[g] = d.g1; }
+        if(d.w2) { z[g + 1] = d.g2; w1[g + 1] = d.g2 * we; ev[g + 1] = d.g2; }
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const double* Av,
+    const double* vk, double* part)
+{
+    __shared__ double sh[16];
+    double num = 0.0, den = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
+        num = fma(wk[g], Av[g], num);
+        den = fma(wk[g], vk[g], den);
+    }
+    num = block_sum(num, sh);
+    den = block_sum(den, sh);
+    if(threadIdx.x == 0) { part[8 * blockIdx.x + 3] = num; part[8 * blockIdx.x + 4] = den; }
+}
+
+__device__ void iteration_scalars(double* S, const double* scal, const double* part, int G,
+    int k, double* sh)
+{
+    for(int q = 0; q < NSCAL; ++q) S[q] = scal[q];
+    const double alpha = reduce_part(part, G, 8, 0, sh) / reduce_part(part, G, 8, 1, sh);
+    const double delta = sqrt(reduce_part(part, G, 8, 2, sh));
+    const double beta = reduce_part(part, G, 8, 3, sh) / reduce_part(part, G, 8, 4, sh);
+    scalar_step(S, k, alpha, delta, beta);
+}
+
+__global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
+    double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
+    double* part)
+{
+    __shared__ double sh[16];
+    double S[NSCAL];
+    const int G = gridDim.x;
+    iteration_scalars(S, scal, part, G, k, sh);
+    const double ib = 1.0 / S[S_BETA + (k + 1) % 3];
+    const double nbrr = -S[S_BRR], neps = -S[S_EPS], ir = 1.0 / S[S_RHO], tau = S[S_TAU];
+    double ss = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+        w1[g] = w1[g] * ib;
+        double p = vk[g];
+        p = fma(nbrr, pkm1[g], p);
+        p = fma(neps, pkm2[g], p);
+        p = p * ir;
+        pk[g] = p;
+        const double xv = fma(tau, p, x[g]);
+        x[g] = xv;
+        ss = fma(xv, xv, ss);
+    }
+    ss = block_sum(ss, sh);
+    if(threadIdx.x == 0) part[8 * blockIdx.x + 5] = ss;
+}
+
+// k_sqmr_test (:89-107) and the state update; one workgroup
+__global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, const double* part,
+    int32_t* flags, double tol)
+{
+    __shared__ double sh[16];
+    double S[NSCAL];
+    iteration_scalars(S, scal, part, G, k, sh);
+    const double nx = sqrt(reduce_part(part, G, 8, 5, sh));
+    if(threadIdx.x == 0) {
+        const int k1 = (k + 1) % 3;
+        S[S_NORMX] = nx;
+        S[S_GAMMA] = S[S_S + k1] * S[S_GAMMA];
+        flags[0] |= (fabs(S[S_DELTA + k1]) <= tol);
+        flags[1] |= (S[S_GAMMA] / (S[S_NORMA] * S[S_NORMX] + S[S_NORMPB]) <= tol);
+        S[S_ITER] = (double) k;
+        for(int q = 0; q < NSCAL; ++q) scal[q] = S[q];
+    }
+}
+
+// start: tv = Pl \ b (done by the caller); v0 = tv / ||tv||, z = D \ v0
+__global__ __launch_bounds__(TPB) void k_sumsq(int32_t n, const double* a, double* part, int slot)
+{
+    __shared__ double sh[16];
+    double ss = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) ss = fma(a[g], a[g], ss);
+    ss = block_sum(ss, sh);
+    if(threadIdx.x == 0) part[8 * blockIdx.x + slot] = ss;
+}
+
+__global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, double* v0, double* z,
+    const double* tri, const int32_t* ip, double* part)
+{
+    __shared__ double sh[16];
+    const int G = gridDim.x;
+    const double t = 1.0 / sqrt(reduce_part(part, G, 8, 6, sh));
+    double ss = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+        const double r1 = tv[g] * t;
+        const double r2 = (g < n - 1) ? tv[g + 1] * t : 0.0;
+        v0[g] = r1;
+        const DP d = dsolve(g, n, tri, ip, r1, r2);
+        if(d.w1) { z[g] = d.g1; ss = fma(d.g1, d.g1, ss); }
+        if(d.w2) { z[g + 1] = d.g2; ss = fma(d.g2, d.g2, ss); }
+    }
+    ss = block_sum(ss, sh);
+    if(threadIdx.x == 0) part[8 * blockIdx.x + 7] = ss;
+}
+
+__global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, double* w0, double* ev,
+    double* scal, const double* part, int32_t* flags)
+{
+    __shared__ double sh[16];
+    const int G = gridDim.x;
+    const double norm_pb = sqrt(reduce_part(part, G, 8, 6, sh));
+    const double eta = 1.0 / sqrt(reduce_part(part, G, 8, 7, sh));
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+        w0[g] = z[g] * eta;
+        ev[g] = z[g];
+    }
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        for(int q = 0; q < NSCAL; ++q) scal[q] = 0.0;
+        scal[S_C + 0] = -1.0;
+        scal[23] = 1.0;
+        scal[S_NORMPB] = norm_pb;
+        scal[S_GAMMA] = norm_pb;
+        scal[S_ETA] = eta;
+        flags[0] = flags[1] = 0;
+    }
+}
+
+__global__ void k_fill(int32_t n, double* a, double v)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) a[g] = v;
+}
+
+__global__ void k_copy(int32_t n, const double* a, double* b)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) b[g] = a[g];
+}
+
+// b <- D \ a (out of place)
+__global__ void k_dsolve(int32_t n, const double* a, double* b, const double* tri, const int32_t* ip)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g >= n) return;
+    const DP d = dsolve(g, n, tri, ip, a[g], (g < n - 1) ? a[g + 1] : 0.0);
+    if(d.w1) b[g] = d.g1;
+    if(d.w2) b[g + 1] = d.g2;
+}
+
+// r = b - r (r holds A x); x_acc += x
+__global__ void k_residual(int32_t n, const double* b, double* r)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) r[g] = b[g] - r[g];
+}
+
+__global__ void k_axpy1(int32_t n, const double* a, double* b)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) b[g] += a[g];
+}
+
+__global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, const double* b,
+    double* part)
+{
+    __shared__ double sh[16];
+    double sa = 0.0, sb = 0.0;
+    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
+        sa = fma(a[g], a[g], sa);
+        sb = fma(b[g], b[g], sb);
+    }
+    sa = block_sum(sa, sh);
+    sb = block_sum(sb, sh);
+    if(threadIdx.x == 0) { part[8 * blockIdx.x + 0] = sa; part[8 * blockIdx.x + 1] = sb; }
+}
+
+__global__ __launch_bounds__(TPB) void k_norm_final(int G, const double* part, double* out)
+{
+    __shared__ double sh[16];
+    const double a = reduce_part(part, G, 8, 0, sh);
+    const double b = reduce_part(part, G, 8, 1, sh);
+    if(threadIdx.x == 0) { out[0] = sqrt(a); out[1] = sqrt(b); }
+}
+
+inline unsigned blocks_for(int32_t n) { return (unsigned) ((n + 255) / 256); }
+
+}  // namespace
+
+int32_t sqmr_grid(int32_t n)
+{
+    int32_t g = (n + TPB - 1) / TPB;
+    if(g > 512) g = 512;
+    if(g < 1) g = 1;
+    return g;
+}
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t) 255; }
+
+size_t sqmr_work_bytes(int32_t n)
+{
+    const size_t vec = align_up(sizeof(double) * (size_t) (n > 0 ? n : 1));
+    return 19 * vec + align_up(sizeof(double) * 8 * sqmr_grid(n)) + align_up(sizeof(double) * 64) +
+        align_up(sizeof(int32_t) * 8);
+}
+
+void sqmr_work_bind(SqmrWork& w, int32_t n, char* base, int32_t* host_flags, double* host_scal)
+{
+    const size_t vec = align_up(sizeof(double) * (size_t) (n > 0 ? n : 1));
+    w.n = n;
+    w.grid = sqmr_grid(n);
+    double** vecs[18] = {&w.v[0], &w.v[1], &w.v[2], &w.w[0], &w.w[1], &w.w[2], &w.p[0], &w.p[1],
+        &w.p[2], &w.z, &w.tv, &w.ev, &w.av[0], &w.av[1], &w.xt, &w.rt, &w.xacc, &w.rhs};
+    size_t off = 0;
+    for(int q = 0; q < 18; ++q) { *vecs[q] = (double*) (base + off); off += vec; }
+    off += vec;  // spare
+    w.part = (double*) (base + off);
+    off += align_up(sizeof(double) * 8 * w.grid);
+    w.scal = (double*) (base + off);
+    off += align_up(sizeof(double) * 64);
+    w.flags = (int32_t*) (base + off);
+    w.host_flags = host_flags;
+    w.host_scal = host_scal;
+}
+
+// ||b - A x|| and ||b|| in FP64; r = b - A x is left in w.rt
+static void residual_norms(const SqmrOps& ops, SqmrWork& w, const double* b, const double* x,
+    hipStream_t st, double& nr, double& nb)
+{
+    const int32_t n = ops.n;
+    ops.spmv(x, w.rt, st);
+    k_residual<<<blocks_for(n), 256, 0, st>>>(n, b, w.rt);
+    k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, w.rt, b, w.part);
+    k_norm_final<<<1, TPB, 0, st>>>(w.grid, w.part, w.scal + 40);
+    HIPCHK(hipMemcpyAsync(w.host_scal + 40, w.scal + 40, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    nr = w.host_scal[40];
+    nb = w.host_scal[41];
+}
+
+double true_rel_residual(const SqmrOps& ops, SqmrWork& w, const double* b, const double* x,
+    void* stream)
+{
+    double nr, nb;
+    residual_norms(ops, w, b, x, (hipStream_t) stream, nr, nb);
+    return nb > 0 ? nr / nb : nr;
+}
+
+// One SQMR run from x = 0 (sqmr.impl.cuh:109-355). Returns the solution
+// x = Pr \ (D \ y) in xout. check_every > 0 adds the true-residual gate.
+static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* xout,
+    const culip_sqmr_opts& opt, culip_sqmr_result& res, hipStream_t st, int32_t& total_it,
+    bool& gate_met, double gate_abs)
+{
+    const int32_t n = ops.n;
+    const int G = w.grid;
+    const unsigned nb = blocks_for(n);
+    gate_met = false;
+    // x (y-iterate) lives in w.xt; vectors start at zero (:126-135)
+    for(double* a : {w.v[0], w.v[1], w.v[2], w.w[0], w.w[1], w.w[2], w.p[0], w.p[1], w.p[2], w.xt})
+        k_fill<<<nb, 256, 0, st>>>(n, a, 0.0);
+    // v0 = Pl \ b normalised; w0 = eta D \ v0; Av = Pl \ A Pr \ D \ v0
+    k_copy<<<nb, 256, 0, st>>>(n, b, w.tv);
+    if(ops.left) ops.left(w.tv, st);
+    k_sumsq<<<G, TPB, 0, st>>>(n, w.tv, w.part, 6);
+    k_init_v<<<G, TPB, 0, st>>>(n, w.tv, w.v[0], w.z, ops.tri, ops.is_piv, w.part);
+    k_init_w<<<G, TPB, 0, st>>>(n, w.z, w.w[0], w.ev, w.scal, w.part, w.flags);
+    int cur = 0;
+    if(ops.right) ops.right(w.ev, st);
+    ops.spmv(w.ev, w.av[cur], st);
+    if(ops.left) ops.left(w.av[cur], st);
+
+    hipEvent_t ev_copy;
+    HIPCHK(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
+    int32_t it = 0;
+    const int period = opt.true_residual_period;
+    while(it < opt.max_it) {
+        const int k0 = it % 3, k1 = (it + 1) % 3, km1 = (it + 2) % 3, km2 = (it + 1) % 3;
+        k_alpha<<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
+        k_vnext<<<G, TPB, 0, st>>>(n, it, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal, w.part);
+        k_vscale<<<G, TPB, 0, st>>>(n, w.tv, w.v[k1], w.z, w.w[k1], w.ev, ops.tri, ops.is_piv,
+            w.scal, w.part);
+        const int nxt = cur ^ 1;
+        if(ops.right) ops.right(w.ev, st);
+        ops.spmv(w.ev, w.av[nxt], st);
+        if(ops.left) ops.left(w.av[nxt], st);
+        k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
+        k_pupdate<<<G, TPB, 0, st>>>(n, it, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
+            w.scal, w.part);
+        k_scalars<<<1, TPB, 0, st>>>(it, G, w.scal, w.part, w.flags, opt.tol);
+        cur = nxt;
+        HIPCHK(hipGetLastError());
+
+        if(period > 0 && (it + 1) % period == 0) {
+            // true solution and residual of the current iterate
+            k_dsolve<<<nb, 256, 0, st>>>(n, w.xt, xout, ops.tri, ops.is_piv);
+            if(!ops.tri) k_copy<<<nb, 256, 0, st>>>(n, w.xt, xout);
+            if(ops.right) ops.right(xout, st);
+            double nr, nbn;
+            residual_norms(ops, w, b, xout, st, nr, nbn);
+            if(nr <= gate_abs) { gate_met = true; ++it; break; }
+        }
+        // throttled convergence check (:309-331)
+        if(it > 0 && (it % 4) == 0) {
+            HIPCHK(hipEventSynchronize(ev_copy));
+            if(w.host_flags[0] || w.host_flags[1]) break;
+        } else if(((it + 1) % 4) == 0) {
+            HIPCHK(hipMemcpyAsync(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipEventRecord(ev_copy, st));
+        }
+        ++it;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipEventDestroy(ev_copy));
+    total_it += it;
+    HIPCHK(hipMemcpyAsync(w.host_scal, w.scal, NSCAL * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    res.residual = w.host_scal[S_GAMMA] / w.host_scal[S_NORMPB];
+    res.converged = gate_met || w.host_flags[0] || w.host_flags[1];
+    if(!gate_met) {
+        // x <- Pr \ (D \ x) (:346-351)
+        if(ops.tri) k_dsolve<<<nb, 256, 0, st>>>(n, w.xt, xout, ops.tri, ops.is_piv);
+        else k_copy<<<nb, 256, 0, st>>>(n, w.xt, xout);
+        if(ops.right) ops.right(xout, st);
+    }
+}
+
+void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
+    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    const int32_t n = ops.n;
+    const unsigned nb = blocks_for(n);
+    std::memset(&res, 0, sizeof(res));
+    int32_t total_it = 0;
+    bool gate = false;
+    double gate_abs = -1.0;
+    if(opt.true_residual_period > 0) {
+        double nbn;
+        k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, b, b, w.part);
+        k_norm_final<<<1, TPB, 0, st>>>(w.grid, w.part, w.scal + 40);
+        HIPCHK(hipMemcpyAsync(w.host_scal + 40, w.scal + 40, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        nbn = w.host_scal[40];
+        gate_abs = opt.tol * nbn;
+    }
+    sqmr_run(ops, w, b, x, opt, res, st, total_it, gate, gate_abs);
+    double nr, nbn;
+    residual_norms(ops, w, b, x, st, nr, nbn);
+    double rr = nbn > 0 ? nr / nbn : nr;
+    // iterative refinement: solve A d = r with a fresh recurrence, x += d,
+    // while the FP64 true residual misses the tolerance
+    int restarts = 0;
+    while(rr > opt.tol && restarts < opt.max_restarts && total_it < opt.max_it) {
+        k_copy<<<nb, 256, 0, st>>>(n, x, w.xacc);
+        k_copy<<<nb, 256, 0, st>>>(n, w.rt, w.rhs);
+        culip_sqmr_opts o2 = opt;
+        o2.max_it = opt.max_it - total_it;
+        culip_sqmr_result r2{};
+        bool g2 = false;
+        sqmr_run(ops, w, w.rhs, x, o2, r2, st, total_it, g2, gate_abs);
+        k_axpy1<<<nb, 256, 0, st>>>(n, w.xacc, x);
+        residual_norms(ops, w, b, x, st, nr, nbn);
+        rr = nbn > 0 ? nr / nbn : nr;
+        res.residual = r2.residual;
+        ++restarts;
+    }
+    res.iterations = total_it;
+    res.restarts = restarts;
+    res.true_rel_residual = rr;
+    if(opt.true_residual_period > 0 || opt.max_restarts > 0) res.converged = rr <= opt.tol;
+}
+
+}  // namespace culip

@@ -1,0 +1,54 @@
+// SQMR (libs/la/iterative_solvers/sqmr.impl.cuh:109-355) on the device.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+
+#include "culip_amd.h"
+
+namespace culip {
+
+struct SqmrOps {
+    int32_t n = 0;
+    std::function<void(const double* x, double* y, void* stream)> spmv;   // y = A x
+    std::function<void(double* x, void* stream)> left;                    // x = Pl \ x (NULL: I)
+    std::function<void(double* x, void* stream)> right;                   // x = Pr \ x (NULL: I)
+    const double* tri = nullptr;      // middle D (NULL: identity)
+    const int32_t* is_piv = nullptr;
+};
+
+// Device work space of one solve (allocated by the caller of sqmr_solve).
+struct SqmrWork {
+    int32_t n = 0, grid = 0;
+    double* v[3] = {nullptr, nullptr, nullptr};
+    double* w[3] = {nullptr, nullptr, nullptr};
+    double* p[3] = {nullptr, nullptr, nullptr};
+    double* z = nullptr;        // D \ v_k
+    double* tv = nullptr;       // unscaled v_{k+1}
+    double* ev = nullptr;       // preconditioner work
+    double* av[2] = {nullptr, nullptr};
+    double* xt = nullptr;       // true solution / residual work
+    double* rt = nullptr;
+    double* xacc = nullptr;     // refinement: accumulated solution
+    double* rhs = nullptr;      // refinement: current residual
+    double* part = nullptr;     // 8 * grid partial sums
+    double* scal = nullptr;     // 32 scalars
+    int32_t* flags = nullptr;   // 4 ints: test[0], test[1]
+    int32_t* host_flags = nullptr;  // pinned
+    double* host_scal = nullptr;    // pinned, 32 doubles
+};
+
+int32_t sqmr_grid(int32_t n);
+size_t sqmr_work_bytes(int32_t n);
+void sqmr_work_bind(SqmrWork& w, int32_t n, char* dev_base, int32_t* host_flags,
+    double* host_scal);
+
+// Solves A x = b. b, x device vectors. Throws Error on HIP failures.
+void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
+    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream);
+
+// ||b - A x|| / ||b|| in FP64 (synchronises the stream).
+double true_rel_residual(const SqmrOps& ops, SqmrWork& w, const double* b, const double* x,
+    void* stream);
+
+}  // namespace culip

@@ -1,0 +1,291 @@
+// BlockiLDLt object: symbolic phase on the host, pattern upload, numeric
+// factor and solves on the device (block_ildlt.impl.cuh:44-246, 1284-1390).
+#include "objects.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace culip {
+
+void hip_check(hipError_t e, const char* what)
+{
+    if(e != hipSuccess) fail(CULIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void Bildlt::symbolic(int32_t fill_level, double fill_factor)
+{
+    symbolic_analyze(sym, A, piv_starts, block_starts, fill_level, fill_factor, f32());
+    have_sym = true;
+    uploaded = false;
+    factored = false;
+    host_int.clear();
+    host_real.clear();
+}
+
+void Bildlt::upload_pattern()
+{
+    const Symbolic& s = sym;
+    upload(d_bs, s.block_starts);
+    upload(d_is_piv, s.is_piv);
+    upload(d_bin, s.blocks);
+    upload(d_csr_off, s.csr_off);
+    upload(d_csc_off, s.csc_off);
+    upload(d_csc_idx, s.csc_idx);
+    upload(d_upd_off, s.upd_off);
+    upload(d_upd_ki, s.upd_ki);
+    upload(d_upd_kj, s.upd_kj);
+    upload(d_lvlb_idx, s.lvlb_idx);
+    upload(d_block_in_lvl, s.block_in_lvl);
+    upload(d_lvl_rows, s.lvl_rows);
+    upload(d_ix_in, s.ix_store);
+    upload(d_value_slot, s.value_slot);
+
+    const int32_t n = s.n, K = s.coarse_nnz;
+    const int64_t mlb = std::max<int64_t>(1, s.max_level_blocks);
+    d_vals.ensure(sizeof(double) * std::max<int64_t>(1, A.nnz()));
+    d_val_in.ensure(vt() * std::max<int64_t>(1, s.val_space));
+    d_bout.ensure(sizeof(BlockMeta) * std::max(1, K));
+    d_ix_out.ensure(sizeof(int32_t) * std::max<int64_t>(1, s.ix_space));
+    d_val_out.ensure(vt() * std::max<int64_t>(1, s.val_space));
+    d_tri.ensure(sizeof(double) * 3 * (size_t) n);
+    d_rn.ensure(vt() * n);
+    d_if_val.ensure(vt() * 1024 * mlb);
+    d_if_idx.ensure(sizeof(uint16_t) * 1024 * mlb);
+    d_if_cnt.ensure(sizeof(int32_t) * mlb);
+    d_rn_part.ensure(vt() * 32 * mlb);
+    d_drop_cnt.ensure(sizeof(int32_t) * mlb);
+    d_overflow.ensure(mlb);
+    d_piv_loc.ensure(sizeof(int32_t) * s.nb);
+    d_carry.ensure(sizeof(int32_t) * 4);
+    for(int q = 0; q < 4; ++q) d_ptr[q].ensure(sizeof(int32_t) * (size_t) (n + 1));
+    d_totals.ensure(sizeof(int32_t) * 4);
+    uploaded = true;
+}
+
+template<typename VT>
+FactorArgs<VT> Bildlt::factor_args() const
+{
+    FactorArgs<VT> a;
+    a.n = sym.n;
+    a.nb = sym.nb;
+    a.bs = d_bs.as<int32_t>();
+    a.is_piv = d_is_piv.as<int32_t>();
+    a.bin = d_bin.as<BlockMeta>();
+    a.bout = d_bout.as<BlockMeta>();
+    a.csr_off = d_csr_off.as<int32_t>();
+    a.csc_off = d_csc_off.as<int32_t>();
+    a.csc_idx = d_csc_idx.as<int32_t>();
+    a.upd_off = d_upd_off.as<int64_t>();
+    a.upd_ki = d_upd_ki.as<int32_t>();
+    a.upd_kj = d_upd_kj.as<int32_t>();
+    a.lvlb_idx = d_lvlb_idx.as<int32_t>();
+    a.block_in_lvl = d_block_in_lvl.as<int32_t>();
+    a.lvl_rows = d_lvl_rows.as<int32_t>();
+    a.ix_in = d_ix_in.as<int32_t>();
+    a.val_in = d_val_in.as<VT>();
+    a.ix_out = d_ix_out.as<int32_t>();
+    a.val_out = d_val_out.as<VT>();
+    a.tri = d_tri.as<double>();
+    a.rn = d_rn.as<VT>();
+    a.if_val = d_if_val.as<VT>();
+    a.if_idx = d_if_idx.as<uint16_t>();
+    a.if_cnt = d_if_cnt.as<int32_t>();
+    a.rn_part = d_rn_part.as<VT>();
+    a.drop_cnt = d_drop_cnt.as<int32_t>();
+    a.overflow = d_overflow.as<uint8_t>();
+    a.piv_loc = d_piv_loc.as<int32_t>();
+    a.carry = d_carry.as<int32_t>();
+    a.threshold = (VT) threshold;
+    return a;
+}
+
+template<typename VT>
+SolvePlan<VT> Bildlt::plan() const
+{
+    SolvePlan<VT> p;
+    int32_t** ptrs[4] = {&p.uo_ptr, &p.ui_ptr, &p.to_ptr, &p.ti_ptr};
+    int32_t** cols[4] = {&p.uo_col, &p.ui_col, &p.to_col, &p.ti_col};
+    VT** vals[4] = {&p.uo_val, &p.ui_val, &p.to_val, &p.ti_val};
+    uint8_t** lrs[4] = {&p.uo_lr, &p.ui_lr, &p.to_lr, &p.ti_lr};
+    for(int q = 0; q < 4; ++q) {
+        *ptrs[q] = d_ptr[q].as<int32_t>();
+        *cols[q] = d_col[q].as<int32_t>();
+        *vals[q] = d_val[q].as<VT>();
+        *lrs[q] = d_lr[q].as<uint8_t>();
+    }
+    return p;
+}
+
+SolveTopo Bildlt::topo() const
+{
+    SolveTopo t;
+    t.n = sym.n;
+    t.nb = sym.nb;
+    t.num_levels = sym.num_levels;
+    t.bs = d_bs.as<int32_t>();
+    t.lvl_rows = d_lvl_rows.as<int32_t>();
+    t.lvl_off_h = sym.lvl_off.data();
+    t.csr_off = d_csr_off.as<int32_t>();
+    t.csc_off = d_csc_off.as<int32_t>();
+    t.csc_idx = d_csc_idx.as<int32_t>();
+    t.bout = d_bout.as<BlockMeta>();
+    t.ix_out = d_ix_out.as<int32_t>();
+    t.tri = d_tri.as<double>();
+    t.is_piv = d_is_piv.as<int32_t>();
+    return t;
+}
+
+template<typename VT>
+void Bildlt::factor_impl(hipStream_t st)
+{
+    const Symbolic& s = sym;
+    FactorArgs<VT> a = factor_args<VT>();
+    gather_values<VT>(d_vals.as<double>(), d_value_slot.as<int64_t>(), A.nnz(), d_val_in.as<VT>(),
+        s.val_space, st);
+    factor_begin<VT>(a, st);
+    for(int32_t l = 0; l < s.num_levels; ++l) {
+        LevelRange r;
+        r.lo = s.lvlb_off[l];
+        r.size = s.lvlb_off[l + 1] - r.lo;
+        r.npiv = s.lvl_off[l + 1] - s.lvl_off[l];
+        r.row_lo = s.lvl_off[l];
+        factor_level<VT>(a, r, st);
+    }
+    HIP_CHECK(hipGetLastError());
+
+    // solve layout: count, scan, size, fill
+    SolveTopo t = topo();
+    SolvePlan<VT> p = plan<VT>();
+    solve_plan_count<VT>(t, a.val_out, nullptr, p, st);
+    int32_t* tot = d_totals.as<int32_t>();
+    for(int q = 0; q < 4; ++q) scan_counts(d_ptr[q].as<int32_t>(), s.n, tot + q, st);
+    int32_t htot[4];
+    HIP_CHECK(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    for(int q = 0; q < 4; ++q) {
+        const size_t cnt = (size_t) std::max(1, htot[q]);
+        d_col[q].ensure(sizeof(int32_t) * cnt);
+        d_val[q].ensure(sizeof(VT) * cnt);
+        d_lr[q].ensure(cnt);
+    }
+    p = plan<VT>();
+    solve_plan_fill<VT>(t, a.val_out, nullptr, p, st);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t st)
+{
+    if(!have_sym) fail(CULIP_ERR_STATE, "factor: symbolic phase has not run");
+    if(!uploaded) upload_pattern();
+    threshold = thr;
+    const size_t bytes = sizeof(double) * (size_t) A.nnz();
+    if(bytes) {
+        if(vals == nullptr) vals = A.val.data(), on_device = false;
+        HIP_CHECK(hipMemcpyAsync(d_vals.p, vals, bytes,
+            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    }
+    if(f32()) factor_impl<float>(st);
+    else factor_impl<double>(st);
+    factored = true;
+    host_int.erase("fblocks");
+    host_int.erase("fix_store");
+    host_real.erase("fval_store");
+    host_real.erase("tri");
+    host_real.erase("row_norms");
+}
+
+void Bildlt::trsv_u(double* x, hipStream_t st) const
+{
+    if(f32()) trsv_upper<float>(topo(), plan<float>(), x, st);
+    else trsv_upper<double>(topo(), plan<double>(), x, st);
+}
+
+void Bildlt::trsv_ut(double* x, hipStream_t st) const
+{
+    if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
+    else trsv_upper_t<double>(topo(), plan<double>(), x, st);
+}
+
+void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
+    hipStream_t st) const
+{
+    if(!factored) fail(CULIP_ERR_STATE, "solve: preconditioner has not been computed");
+    const size_t bytes = sizeof(double) * (size_t) n();
+    if(side == CULIP_SOLVE_MIDDLE) {
+        solve_D(n(), d_tri.as<double>(), d_is_piv.as<int32_t>(), b, x, st);
+        return;
+    }
+    // transpose swaps left and right (block_ildlt.impl.cuh:192-201, 236-245)
+    const bool right = (side == CULIP_SOLVE_RIGHT) != transpose;
+    if(b != x) HIP_CHECK(hipMemcpyAsync(x, b, bytes, hipMemcpyDeviceToDevice, st));
+    // P is the identity for static pivoting (k_permute_P / k_permute_Pt)
+    if(right) trsv_u(x, st);
+    else trsv_ut(x, st);
+}
+
+int64_t Bildlt::device_bytes() const
+{
+    int64_t b = 0;
+    for(const DevBuf* d : {&d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off,
+            &d_upd_ki, &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in,
+            &d_value_slot, &d_vals, &d_val_in, &d_bout, &d_ix_out, &d_val_out, &d_tri, &d_rn,
+            &d_if_val, &d_if_idx, &d_if_cnt, &d_rn_part, &d_drop_cnt, &d_overflow, &d_piv_loc,
+            &d_carry, &d_totals})
+        b += (int64_t) d->bytes;
+    for(int q = 0; q < 4; ++q)
+        b += (int64_t) (d_ptr[q].bytes + d_col[q].bytes + d_val[q].bytes + d_lr[q].bytes);
+    return b;
+}
+
+void Bildlt::download_factor()
+{
+    if(!factored) fail(CULIP_ERR_STATE, "factor has not been computed");
+    if(host_int.count("fblocks")) return;
+    HIP_CHECK(hipDeviceSynchronize());
+    int32_t carry[4];
+    HIP_CHECK(hipMemcpy(carry, d_carry.p, sizeof(carry), hipMemcpyDeviceToHost));
+    const int32_t K = sym.coarse_nnz;
+    std::vector<int32_t> fb((size_t) K * 9);
+    HIP_CHECK(hipMemcpy(fb.data(), d_bout.p, sizeof(BlockMeta) * K, hipMemcpyDeviceToHost));
+    host_int["fblocks"] = std::move(fb);
+    std::vector<int32_t> fix(carry[1]);
+    if(carry[1]) HIP_CHECK(hipMemcpy(fix.data(), d_ix_out.p, sizeof(int32_t) * carry[1], hipMemcpyDeviceToHost));
+    host_int["fix_store"] = std::move(fix);
+    host_int["carry"] = std::vector<int32_t>(carry, carry + 4);
+    std::vector<double> fv(carry[2]);
+    std::vector<double> rn(n());
+    if(f32()) {
+        std::vector<float> t(std::max(carry[2], n()));
+        if(carry[2]) HIP_CHECK(hipMemcpy(t.data(), d_val_out.p, sizeof(float) * carry[2], hipMemcpyDeviceToHost));
+        for(int32_t q = 0; q < carry[2]; ++q) fv[q] = t[q];
+        HIP_CHECK(hipMemcpy(t.data(), d_rn.p, sizeof(float) * n(), hipMemcpyDeviceToHost));
+        for(int32_t q = 0; q < n(); ++q) rn[q] = t[q];
+    } else {
+        if(carry[2]) HIP_CHECK(hipMemcpy(fv.data(), d_val_out.p, sizeof(double) * carry[2], hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(rn.data(), d_rn.p, sizeof(double) * n(), hipMemcpyDeviceToHost));
+    }
+    host_real["fval_store"] = std::move(fv);
+    host_real["row_norms"] = std::move(rn);
+    std::vector<double> tri(3 * (size_t) n());
+    HIP_CHECK(hipMemcpy(tri.data(), d_tri.p, sizeof(double) * tri.size(), hipMemcpyDeviceToHost));
+    host_real["tri"] = std::move(tri);
+}
+
+// ---------------------------------------------------------------------------
+
+void Spmv::multiply(const double* x, double* y, double alpha, double beta, bool transpose,
+    hipStream_t st) const
+{
+    if(kkt) { kkt_spmv(K, x, y, alpha, beta, st); return; }
+    if(transpose && have_at) csr_spmv(At, x, y, alpha, beta, st);
+    else csr_spmv(A, x, y, alpha, beta, st);
+}
+
+Sqmr::~Sqmr()
+{
+    if(host_flags) (void) hipHostFree(host_flags);
+    if(host_scal) (void) hipHostFree(host_scal);
+}
+
+}  // namespace culip

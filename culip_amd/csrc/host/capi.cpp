@@ -1,0 +1,588 @@
+// C-ABI of culip_amd (include/culip_amd.h). Every entry point converts
+// library errors into status codes; the message is kept per thread.
+#include "objects.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "kkt.hpp"
+
+struct culip_kkt { culip::Kkt* k; };
+struct culip_prep { culip::Prep* p; };
+struct culip_host_matrix { culip::HostCsr a; };
+struct culip_bildlt { culip::Bildlt b; };
+struct culip_spmv { culip::Spmv s; };
+struct culip_sqmr { culip::Sqmr q; };
+
+using namespace culip;
+
+static thread_local std::string g_last_error;
+
+template<typename F>
+static culip_status guarded(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return CULIP_OK;
+    } catch(const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch(const std::bad_alloc&) {
+        g_last_error = "out of host memory";
+        return CULIP_ERR_NOMEM;
+    } catch(const std::exception& e) {
+        g_last_error = e.what();
+        return CULIP_ERR_INVALID;
+    }
+}
+
+static HostCsr to_host(const culip_csr* c)
+{
+    CULIP_REQUIRE(c && c->row_ptr && c->m >= 0 && c->n >= 0, "csr: null or malformed matrix");
+    HostCsr h;
+    h.m = c->m;
+    h.n = c->n;
+    h.row_ptr.assign(c->row_ptr, c->row_ptr + c->m + 1);
+    CULIP_REQUIRE(h.row_ptr[0] == 0 && (int64_t) h.row_ptr[c->m] == c->nnz, "csr: row_ptr inconsistent with nnz");
+    for(int32_t i = 0; i < c->m; ++i) CULIP_REQUIRE(h.row_ptr[i] <= h.row_ptr[i + 1], "csr: row_ptr not monotone");
+    if(c->nnz) {
+        CULIP_REQUIRE(c->col && c->val, "csr: null col/val");
+        h.col.assign(c->col, c->col + c->nnz);
+        h.val.assign(c->val, c->val + c->nnz);
+    }
+    for(int32_t q : h.col) CULIP_REQUIRE(q >= 0 && q < h.n, "csr: column index out of range");
+    return h;
+}
+
+static void view(const HostCsr& h, culip_csr* out)
+{
+    out->m = h.m;
+    out->n = h.n;
+    out->nnz = h.nnz();
+    out->row_ptr = h.row_ptr.data();
+    out->col = h.col.data();
+    out->val = h.val.data();
+}
+
+static void device_present()
+{
+    int n = 0;
+    if(hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail(CULIP_ERR_NO_DEVICE, "no HIP device available");
+}
+
+extern "C" {
+
+const char* culip_last_error(void) { return g_last_error.c_str(); }
+const char* culip_version(void) { return "culip_amd 0.1 (gfx950)"; }
+
+// ---- KKT generator / prep -------------------------------------------------
+
+culip_status culip_kkt_generate(const culip_kkt_params* p, culip_kkt** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(p && out, "kkt: null argument");
+        auto* h = new culip_kkt;
+        h->k = kkt_generate(*p);
+        *out = h;
+    });
+}
+
+culip_status culip_kkt_view(const culip_kkt* k, culip_csr* K, const double** rhs)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(k && K, "kkt: null argument");
+        view(k->k->K, K);
+        if(rhs) *rhs = k->k->rhs.data();
+    });
+}
+
+culip_status culip_kkt_barrier_values(const culip_kkt* k, double w_lo, double w_hi,
+    uint64_t seed, double* vals_out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(k && vals_out, "kkt: null argument");
+        kkt_barrier_values(*k->k, w_lo, w_hi, seed, vals_out);
+    });
+}
+
+void culip_kkt_destroy(culip_kkt* k)
+{
+    if(k) { delete k->k; delete k; }
+}
+
+culip_status culip_prep_create(const culip_csr* K, int32_t n_primal, int32_t max_block,
+    culip_ordering ordering, culip_prep** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out, "prep: null argument");
+        HostCsr h = to_host(K);
+        auto* p = new culip_prep;
+        p->p = prep_create(h, n_primal, max_block, ordering);
+        *out = p;
+    });
+}
+
+culip_status culip_prep_view(const culip_prep* p, culip_csr* K_perm, const int32_t** perm,
+    int32_t* num_blocks, const int32_t** block_starts, int32_t* num_pivs,
+    const int32_t** piv_starts)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(p, "prep: null argument");
+        if(K_perm) view(p->p->K, K_perm);
+        if(perm) *perm = p->p->perm.data();
+        if(num_blocks) *num_blocks = (int32_t) p->p->block_starts.size();
+        if(block_starts) *block_starts = p->p->block_starts.data();
+        if(num_pivs) *num_pivs = (int32_t) p->p->piv_starts.size();
+        if(piv_starts) *piv_starts = p->p->piv_starts.data();
+    });
+}
+
+culip_status culip_prep_permute_values(const culip_prep* p, const double* vals_K,
+    double* vals_perm)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(p && vals_K && vals_perm, "prep: null argument");
+        const auto& src = p->p->entry_src;
+        for(size_t q = 0; q < src.size(); ++q) vals_perm[q] = vals_K[src[q]];
+    });
+}
+
+void culip_prep_destroy(culip_prep* p)
+{
+    if(p) { delete p->p; delete p; }
+}
+
+// ---- MatrixMarket (libs/test/test.impl.h:25-300) ---------------------------
+
+static std::vector<std::string> mm_lines(const char* path, std::string& banner)
+{
+    std::ifstream f(path);
+    if(!f.good()) fail(CULIP_ERR_INVALID, std::string("cannot open ") + path);
+    std::vector<std::string> lines;
+    std::string line;
+    bool first = true;
+    while(std::getline(f, line)) {
+        if(first) { banner = line; first = false; continue; }
+        if(!line.empty() && line[0] == '%') continue;
+        if(line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        lines.push_back(line);
+    }
+    std::transform(banner.begin(), banner.end(), banner.begin(), ::tolower);
+    return lines;
+}
+
+culip_status culip_mm_read_csr(const char* path, culip_host_matrix** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(path && out, "mm: null argument");
+        std::string banner;
+        auto lines = mm_lines(path, banner);
+        CULIP_REQUIRE(banner.find("coordinate") != std::string::npos, "mm: only coordinate matrices");
+        const bool sym = banner.find("symmetric") != std::string::npos;
+        CULIP_REQUIRE(!lines.empty(), "mm: missing size line");
+        int64_t m, n, nnz;
+        { std::istringstream ss(lines[0]); ss >> m >> n >> nnz; }
+        std::vector<int32_t> ii, jj;
+        std::vector<double> cc;
+        for(int64_t q = 0; q < nnz; ++q) {
+            CULIP_REQUIRE((size_t) q + 1 < lines.size(), "mm: truncated file");
+            std::istringstream ss(lines[q + 1]);
+            int64_t i, j;
+            double v = 1.0;
+            ss >> i >> j >> v;
+            ii.push_back((int32_t) (i - 1));
+            jj.push_back((int32_t) (j - 1));
+            cc.push_back(v);
+        }
+        // symmetric expansion: mirror off-diagonal entries (:76-91)
+        if(sym) {
+            const size_t base = ii.size();
+            for(size_t q = 0; q < base; ++q)
+                if(ii[q] != jj[q]) { ii.push_back(jj[q]); jj.push_back(ii[q]); cc.push_back(cc[q]); }
+        }
+        // COO -> CSR, per-row sort by column keeping duplicates (:111-172)
+        auto* h = new culip_host_matrix;
+        HostCsr& a = h->a;
+        a.m = (int32_t) m;
+        a.n = (int32_t) n;
+        a.row_ptr.assign(m + 1, 0);
+        for(int32_t r : ii) a.row_ptr[r + 1]++;
+        for(int64_t r = 0; r < m; ++r) a.row_ptr[r + 1] += a.row_ptr[r];
+        a.col.resize(ii.size());
+        a.val.resize(ii.size());
+        std::vector<int32_t> fillp(a.row_ptr.begin(), a.row_ptr.end() - 1);
+        for(size_t q = 0; q < ii.size(); ++q) {
+            const int32_t pos = fillp[ii[q]]++;
+            a.col[pos] = jj[q];
+            a.val[pos] = cc[q];
+        }
+        for(int64_t r = 0; r < m; ++r) {
+            std::vector<std::pair<int32_t, double>> row;
+            for(int32_t q = a.row_ptr[r]; q < a.row_ptr[r + 1]; ++q) row.emplace_back(a.col[q], a.val[q]);
+            std::stable_sort(row.begin(), row.end(),
+                [](const auto& x, const auto& y) { return x.first < y.first; });
+            for(size_t q = 0; q < row.size(); ++q) {
+                a.col[a.row_ptr[r] + q] = row[q].first;
+                a.val[a.row_ptr[r] + q] = row[q].second;
+            }
+        }
+        *out = h;
+    });
+}
+
+culip_status culip_mm_read_vector(const char* path, int64_t* len, double** vals)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(path && len && vals, "mm: null argument");
+        std::string banner;
+        auto lines = mm_lines(path, banner);
+        CULIP_REQUIRE(banner.find("array") != std::string::npos, "mm: only array vectors");
+        int64_t m, n;
+        { std::istringstream ss(lines.at(0)); ss >> m >> n; }
+        const int64_t k = std::max(m, n);
+        double* v = (double*) std::malloc(sizeof(double) * (size_t) std::max<int64_t>(1, k));
+        for(int64_t q = 0; q < k; ++q) {
+            CULIP_REQUIRE((size_t) q + 1 < lines.size(), "mm: truncated vector");
+            v[q] = std::strtod(lines[q + 1].c_str(), nullptr);
+        }
+        *len = k;
+        *vals = v;
+    });
+}
+
+culip_status culip_host_matrix_view(const culip_host_matrix* a, culip_csr* out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(a && out, "mm: null argument");
+        view(a->a, out);
+    });
+}
+
+void culip_host_matrix_destroy(culip_host_matrix* a) { delete a; }
+void culip_host_free(void* p) { std::free(p); }
+
+// ---- BlockiLDLt -------------------------------------------------------------
+
+culip_status culip_bildlt_create(const culip_csr* hA, int32_t num_pivs, const int32_t* piv_starts,
+    int32_t num_blocks, const int32_t* block_starts, culip_prec prec, culip_bildlt** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out, "bildlt: null argument");
+        CULIP_REQUIRE(prec == CULIP_PREC_F64 || prec == CULIP_PREC_F32, "bildlt: unknown precision");
+        CULIP_REQUIRE(num_pivs > 0 && piv_starts, "bildlt: pivot starts required");
+        CULIP_REQUIRE(num_blocks > 0 && block_starts, "bildlt: block starts required");
+        auto* h = new culip_bildlt;
+        try {
+            h->b.A = to_host(hA);
+            CULIP_REQUIRE(h->b.A.m == h->b.A.n, "bildlt: matrix must be square");
+            h->b.prec = prec;
+            h->b.piv_starts.assign(piv_starts, piv_starts + num_pivs);
+            h->b.block_starts.assign(block_starts, block_starts + num_blocks);
+        } catch(...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+culip_status culip_bildlt_symbolic(culip_bildlt* h, int32_t fill_level, double fill_factor)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null handle");
+        h->b.symbolic(fill_level, fill_factor);
+    });
+}
+
+culip_status culip_bildlt_compute(culip_bildlt* h, int32_t fill_level, double fill_factor,
+    double threshold, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null handle");
+        device_present();
+        if(!h->b.have_sym || h->b.sym.fill_level != fill_level || h->b.sym.fill_factor != fill_factor)
+            h->b.symbolic(fill_level, fill_factor);
+        h->b.factor(nullptr, false, threshold, (hipStream_t) stream);
+    });
+}
+
+culip_status culip_bildlt_refactor(culip_bildlt* h, const double* vals, int32_t vals_on_device,
+    double threshold, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && vals, "bildlt: null argument");
+        device_present();
+        h->b.factor(vals, vals_on_device != 0, threshold, (hipStream_t) stream);
+    });
+}
+
+culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t transpose,
+    const double* b, double* x, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && b && x, "bildlt: null argument");
+        CULIP_REQUIRE(side == CULIP_SOLVE_LEFT || side == CULIP_SOLVE_MIDDLE || side == CULIP_SOLVE_RIGHT,
+            "bildlt: unknown side");
+        CULIP_REQUIRE(side == CULIP_SOLVE_MIDDLE || b != x || true, "");
+        h->b.solve(side, transpose != 0, b, x, (hipStream_t) stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int32_t culip_bildlt_n(const culip_bildlt* h) { return h ? h->b.n() : -1; }
+
+int64_t culip_bildlt_device_bytes(const culip_bildlt* h) { return h ? h->b.device_bytes() : 0; }
+
+culip_status culip_bildlt_get_int(culip_bildlt* h, const char* name, const int32_t** ptr,
+    int64_t* len)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && name && ptr && len, "bildlt: null argument");
+        Bildlt& b = h->b;
+        const Symbolic& s = b.sym;
+        const std::string nm(name);
+        auto put = [&](const std::vector<int32_t>& v) { *ptr = v.data(); *len = (int64_t) v.size(); };
+        if(nm == "fblocks" || nm == "fix_store" || nm == "carry") {
+            b.download_factor();
+            put(b.host_int.at(nm));
+            return;
+        }
+        CULIP_REQUIRE(b.have_sym, "bildlt: symbolic phase has not run");
+        auto& cache = b.host_int;
+        if(nm == "block_starts") put(s.block_starts);
+        else if(nm == "coarse_csr_off") put(s.csr_off);
+        else if(nm == "coarse_csr_col") put(s.csr_col);
+        else if(nm == "coarse_csc_off") put(s.csc_off);
+        else if(nm == "coarse_csc_idx") put(s.csc_idx);
+        else if(nm == "a_block_nnz") put(s.a_block_nnz);
+        else if(nm == "lvl_off") put(s.lvl_off);
+        else if(nm == "lvl_rows") put(s.lvl_rows);
+        else if(nm == "row_level") put(s.row_level);
+        else if(nm == "lvlb_off") put(s.lvlb_off);
+        else if(nm == "lvlb_idx") put(s.lvlb_idx);
+        else if(nm == "block_in_lvl") put(s.block_in_lvl);
+        else if(nm == "row_first_block_offset") put(s.row_first_block_offset);
+        else if(nm == "solve_off") put(s.solve_off);
+        else if(nm == "solve_idx") put(s.solve_idx);
+        else if(nm == "is_piv") put(s.is_piv);
+        else if(nm == "ix_store") put(s.ix_store);
+        else if(nm == "blocks") {
+            std::vector<int32_t> v((size_t) s.coarse_nnz * 9);
+            std::memcpy(v.data(), s.blocks.data(), sizeof(BlockMeta) * s.blocks.size());
+            cache[nm] = std::move(v);
+            put(cache[nm]);
+        } else if(nm == "upd_off") {
+            std::vector<int32_t> v(s.upd_off.begin(), s.upd_off.end());
+            cache[nm] = std::move(v);
+            put(cache[nm]);
+        } else if(nm == "upd_ki") put(s.upd_ki);
+        else if(nm == "upd_kj") put(s.upd_kj);
+        else if(nm == "scalars") {
+            cache[nm] = {s.n, s.nb, s.fine_u_nnz, s.init_coarse_nnz, s.fill_in_nnz, s.coarse_nnz,
+                s.num_levels, s.max_level_size, s.max_level_blocks, s.blks_sparse, s.blks_dense,
+                (int32_t) s.ix_space, (int32_t) s.val_space};
+            put(cache[nm]);
+        } else fail(CULIP_ERR_INVALID, "bildlt: unknown structure '" + nm + "'");
+    });
+}
+
+culip_status culip_bildlt_get_real(culip_bildlt* h, const char* name, const double** ptr,
+    int64_t* len)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && name && ptr && len, "bildlt: null argument");
+        Bildlt& b = h->b;
+        const std::string nm(name);
+        if(nm == "val_store") {
+            CULIP_REQUIRE(b.have_sym, "bildlt: symbolic phase has not run");
+            auto& v = b.host_real[nm];
+            v.assign(b.sym.val_space, 0.0);
+            symbolic_gather_values(b.sym, b.A.val.data(), v.data());
+            if(b.f32()) for(double& q : v) q = (double) (float) q;
+            *ptr = v.data();
+            *len = (int64_t) v.size();
+            return;
+        }
+        if(nm == "fval_store" || nm == "tri" || nm == "row_norms") {
+            b.download_factor();
+            const auto& v = b.host_real.at(nm);
+            *ptr = v.data();
+            *len = (int64_t) v.size();
+            return;
+        }
+        fail(CULIP_ERR_INVALID, "bildlt: unknown structure '" + nm + "'");
+    });
+}
+
+void culip_bildlt_destroy(culip_bildlt* h) { delete h; }
+
+// ---- SPMV -----------------------------------------------------------------
+
+static void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h)
+{
+    upload(br, h.row_ptr);
+    upload(bc, h.col);
+    upload(bv, h.val);
+    d.m = h.m;
+    d.n = h.n;
+    d.nnz = h.nnz();
+    d.row_ptr = br.as<int32_t>();
+    d.col = bc.as<int32_t>();
+    d.val = bv.as<double>();
+}
+
+static HostCsr transpose_csr(const HostCsr& a)
+{
+    HostCsr t;
+    t.m = a.n;
+    t.n = a.m;
+    t.row_ptr.assign(a.n + 1, 0);
+    for(int32_t c : a.col) t.row_ptr[c + 1]++;
+    for(int32_t i = 0; i < a.n; ++i) t.row_ptr[i + 1] += t.row_ptr[i];
+    t.col.resize(a.col.size());
+    t.val.resize(a.col.size());
+    std::vector<int32_t> fillp(t.row_ptr.begin(), t.row_ptr.end() - 1);
+    for(int32_t r = 0; r < a.m; ++r)
+        for(int32_t q = a.row_ptr[r]; q < a.row_ptr[r + 1]; ++q) {
+            const int32_t p = fillp[a.col[q]]++;
+            t.col[p] = r;
+            t.val[p] = a.val[q];
+        }
+    return t;
+}
+
+culip_status culip_spmv_create_csr(const culip_csr* hA, culip_spmv** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out, "spmv: null argument");
+        device_present();
+        HostCsr h = to_host(hA);
+        auto* s = new culip_spmv;
+        try {
+            upload_csr(s->s.A, s->s.b_row, s->s.b_col, s->s.b_val, h);
+            // explicit transpose for A' x (spmv.impl.cuh:57-66), unless A is
+            // structurally and numerically symmetric
+            HostCsr t = transpose_csr(h);
+            const bool symm = t.row_ptr == h.row_ptr && t.col == h.col && t.val == h.val;
+            if(!symm) {
+                upload_csr(s->s.At, s->s.bt_row, s->s.bt_col, s->s.bt_val, t);
+                s->s.have_at = true;
+            }
+            s->s.m = h.m;
+            s->s.n = h.n;
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
+
+culip_status culip_spmv_create_kkt(const culip_csr* hA, const double* d, double delta,
+    int32_t n_ix, const int32_t* ix, culip_spmv** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out && d, "spmv: null argument");
+        device_present();
+        HostCsr a = to_host(hA);
+        auto* s = new culip_spmv;
+        try {
+            Spmv& o = s->s;
+            o.kkt = true;
+            o.K.n = a.n;
+            o.K.m = a.m;
+            upload_csr(o.K.A, o.b_arow, o.b_acol, o.b_aval, a);
+            upload_csr(o.K.At, o.b_atrow, o.b_atcol, o.b_atval, transpose_csr(a));
+            upload(o.b_d, std::vector<double>(d, d + a.n));
+            std::vector<double> dix(a.m, 0.0);
+            for(int32_t q = 0; q < n_ix; ++q) {
+                CULIP_REQUIRE(ix && ix[q] >= 0 && ix[q] < a.m, "spmv: ix out of range");
+                dix[ix[q]] += delta;
+            }
+            upload(o.b_dix, dix);
+            o.K.d = o.b_d.as<double>();
+            o.K.dix = o.b_dix.as<double>();
+            o.m = o.n = a.n + a.m;
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
+
+culip_status culip_spmv_update_values(culip_spmv* op, const double* vals, int32_t on_device,
+    void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(op && vals && !op->s.kkt, "spmv: values can be replaced on CSR operators only");
+        HIP_CHECK(hipMemcpyAsync(op->s.A.val, vals, sizeof(double) * (size_t) op->s.A.nnz,
+            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, (hipStream_t) stream));
+        CULIP_REQUIRE(!op->s.have_at, "spmv: value update of a non-symmetric operator");
+    });
+}
+
+culip_status culip_spmv_multiply(const culip_spmv* op, const double* x, double* y, double alpha,
+    double beta, int32_t transpose, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(op && x && y, "spmv: null argument");
+        op->s.multiply(x, y, alpha, beta, transpose != 0, (hipStream_t) stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int32_t culip_spmv_m(const culip_spmv* op) { return op ? op->s.m : -1; }
+int32_t culip_spmv_n(const culip_spmv* op) { return op ? op->s.n : -1; }
+void culip_spmv_destroy(culip_spmv* op) { delete op; }
+
+// ---- SQMR -----------------------------------------------------------------
+
+culip_status culip_sqmr_create(const culip_spmv* A, const culip_bildlt* prec,
+    const culip_sqmr_opts* opts, culip_sqmr** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(A && opts && out, "sqmr: null argument");
+        CULIP_REQUIRE(A->s.m == A->s.n, "sqmr: operator must be square");
+        CULIP_REQUIRE(!prec || prec->b.n() == A->s.m, "sqmr: preconditioner size mismatch");
+        CULIP_REQUIRE(opts->max_it >= 0 && opts->tol >= 0, "sqmr: bad options");
+        device_present();
+        auto* s = new culip_sqmr;
+        try {
+            Sqmr& q = s->q;
+            q.A = &A->s;
+            q.prec = prec ? &prec->b : nullptr;
+            q.opts = *opts;
+            const int32_t n = A->s.m;
+            q.work.ensure(sqmr_work_bytes(n));
+            HIP_CHECK(hipHostMalloc((void**) &q.host_flags, 8 * sizeof(int32_t), hipHostMallocDefault));
+            HIP_CHECK(hipHostMalloc((void**) &q.host_scal, 64 * sizeof(double), hipHostMallocDefault));
+            sqmr_work_bind(q.w, n, q.work.as<char>(), q.host_flags, q.host_scal);
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
+
+culip_status culip_sqmr_solve(culip_sqmr* s, const double* b, double* x, culip_sqmr_result* res,
+    void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s && b && x && res, "sqmr: null argument");
+        Sqmr& q = s->q;
+        if(q.prec) CULIP_REQUIRE(q.prec->factored, "sqmr: preconditioner has not been computed");
+        SqmrOps ops;
+        ops.n = q.A->m;
+        const Spmv* A = q.A;
+        ops.spmv = [A](const double* xx, double* yy, void* st) {
+            A->multiply(xx, yy, 1.0, 0.0, false, (hipStream_t) st);
+        };
+        if(q.prec) {
+            const Bildlt* P = q.prec;
+            ops.left = [P](double* xx, void* st) { P->trsv_ut(xx, (hipStream_t) st); };
+            ops.right = [P](double* xx, void* st) { P->trsv_u(xx, (hipStream_t) st); };
+            ops.tri = P->d_tri.as<double>();
+            ops.is_piv = P->d_is_piv.as<int32_t>();
+        }
+        sqmr_solve(ops, q.w, b, x, q.opts, *res, stream);
+    });
+}
+
+void culip_sqmr_destroy(culip_sqmr* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// Library objects behind the C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <map>
+#include <memory>
+#include <string>
+
+#include "common.hpp"
+#include "symbolic.hpp"
+#include "../device/factor.hpp"
+#include "../device/solve.hpp"
+#include "../device/spmv.hpp"
+#include "../device/sqmr.hpp"
+
+namespace culip {
+
+void hip_check(hipError_t e, const char* what);
+#define HIP_CHECK(x) ::culip::hip_check((x), #x)
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if(p) (void) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b)
+    {
+        if(b <= bytes && p) return;
+        release();
+        const size_t want = b ? b : 16;
+        HIP_CHECK(hipMalloc(&p, want));
+        bytes = want;
+    }
+    template<typename T> T* as() const { return (T*) p; }
+};
+
+template<typename T>
+void upload(DevBuf& d, const std::vector<T>& h)
+{
+    d.ensure(sizeof(T) * h.size());
+    if(!h.empty()) HIP_CHECK(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+struct Bildlt {
+    culip_prec prec = CULIP_PREC_F64;
+    HostCsr A;
+    std::vector<int32_t> piv_starts, block_starts;
+    Symbolic sym;
+    bool have_sym = false;
+    bool uploaded = false;
+    bool factored = false;
+    double threshold = 0.0;
+
+    // static (per pattern)
+    DevBuf d_bs, d_is_piv, d_bin, d_csr_off, d_csc_off, d_csc_idx, d_upd_off, d_upd_ki, d_upd_kj,
+        d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot;
+    // numeric
+    DevBuf d_vals, d_val_in, d_bout, d_ix_out, d_val_out, d_tri, d_rn;
+    DevBuf d_if_val, d_if_idx, d_if_cnt, d_rn_part, d_drop_cnt, d_overflow, d_piv_loc, d_carry;
+    // solve layout
+    DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4];
+    DevBuf d_totals;
+
+    // host copies for inspection
+    std::map<std::string, std::vector<int32_t>> host_int;
+    std::map<std::string, std::vector<double>> host_real;
+
+    int32_t n() const { return A.m; }
+    bool f32() const { return prec == CULIP_PREC_F32; }
+    size_t vt() const { return f32() ? sizeof(float) : sizeof(double); }
+
+    void symbolic(int32_t fill_level, double fill_factor);
+    void upload_pattern();
+    void factor(const double* vals, bool vals_on_device, double thr, hipStream_t st);
+    void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
+    void trsv_u(double* x, hipStream_t st) const;
+    void trsv_ut(double* x, hipStream_t st) const;
+    SolveTopo topo() const;
+    int64_t device_bytes() const;
+    void download_factor();
+
+    template<typename VT> FactorArgs<VT> factor_args() const;
+    template<typename VT> SolvePlan<VT> plan() const;
+    template<typename VT> void factor_impl(hipStream_t st);
+};
+
+struct Spmv {
+    bool kkt = false;
+    DevCsr A, At;           // At only for a non-symmetric CSR transpose product
+    bool have_at = false;
+    DevKkt K;
+    DevBuf b_row, b_col, b_val, bt_row, bt_col, bt_val, b_d, b_dix, b_arow, b_acol, b_aval,
+        b_atrow, b_atcol, b_atval;
+    int32_t m = 0, n = 0;
+    void multiply(const double* x, double* y, double alpha, double beta, bool transpose,
+        hipStream_t st) const;
+};
+
+struct Sqmr {
+    const Spmv* A = nullptr;
+    const Bildlt* prec = nullptr;
+    culip_sqmr_opts opts{};
+    DevBuf work;
+    int32_t* host_flags = nullptr;
+    double* host_scal = nullptr;
+    SqmrWork w;
+    ~Sqmr();
+};
+
+}  // namespace culip

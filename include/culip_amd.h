@@ -1,0 +1,221 @@
+/*
+ * culip_amd -- MI355X (gfx950) block incomplete LDL' (BILDLt) preconditioner
+ * and SQMR solver for KKT systems [[H, A'], [A, -delta I]].
+ *
+ * Thin C-ABI: plain pointers and sizes, int32 indices (the reference's
+ * mat_int_t, libs/utils/defines.h:56-62), explicit streams, status codes
+ * instead of the reference's raise(SIGSEGV) (libs/utils/types.cuh:32).
+ * Streams are hipStream_t passed as void* (NULL = default stream).
+ * Device pointers are HIP device memory owned by the caller; every object
+ * created here owns its own device storage.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository dthuerck/culip).
+ */
+#ifndef CULIP_AMD_H
+#define CULIP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CULIP_OK = 0,
+    CULIP_ERR_INVALID = 1,      /* bad argument / inconsistent structure */
+    CULIP_ERR_HIP = 2,          /* HIP runtime error */
+    CULIP_ERR_NOMEM = 3,        /* host or device allocation failed */
+    CULIP_ERR_STATE = 4,        /* call out of order (e.g. solve before factor) */
+    CULIP_ERR_NUMERIC = 5,      /* non-finite values / breakdown */
+    CULIP_ERR_BUDGET = 6,       /* request exceeds the device memory budget */
+    CULIP_ERR_UNSUPPORTED = 7,  /* option not supported by this build */
+    CULIP_ERR_NO_DEVICE = 8     /* no HIP device available */
+} culip_status;
+
+typedef enum { CULIP_PREC_F64 = 0, CULIP_PREC_F32 = 1 } culip_prec;
+typedef enum { CULIP_SOLVE_LEFT = 0, CULIP_SOLVE_MIDDLE = 1, CULIP_SOLVE_RIGHT = 2 } culip_side;
+typedef enum { CULIP_ORDER_NATURAL = 0, CULIP_ORDER_BLOCK_COLOR = 1 } culip_ordering;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char * culip_last_error(void);
+const char * culip_version(void);
+
+/* Host CSR view (borrowed pointers). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    const int32_t * row_ptr;
+    const int32_t * col;
+    const double * val;
+} culip_csr;
+
+/* ------------------------------------------------------------------------ */
+/* Input preparation (host only). The reference reads the matrix, ordering,
+ * blocking and pivot structure from MatrixMarket files
+ * (apps/block_ildlt/block_ildlt_invariants.h:88-111); these entry points
+ * produce them for the synthetic KKT workloads instead.                    */
+
+typedef struct {
+    int32_t n;                  /* primal size (H is n x n) */
+    int32_t m;                  /* dual size (A is m x n) */
+    int32_t h_offdiag_per_row;  /* average off-diagonals per row of H */
+    int32_t a_nnz_per_col;      /* nonzeros per column of A */
+    double h_diag_lo, h_diag_hi;/* H diagonal range */
+    int32_t h_diag_log;         /* 1: log-uniform in [lo, hi]; 0: uniform */
+    double h_offdiag_std;       /* std of H off-diagonal values */
+    double delta;               /* K(2,2) = -delta I */
+    uint64_t seed;
+} culip_kkt_params;
+
+typedef struct culip_kkt culip_kkt;
+culip_status culip_kkt_generate(const culip_kkt_params * p, culip_kkt ** out);
+/* Full symmetric CSR of K (sorted columns) and the N(0,1) right-hand side. */
+culip_status culip_kkt_view(const culip_kkt * k, culip_csr * K, const double ** rhs);
+/* Serving refactor input: H_ii <- H_ii * w_i, w_i log-uniform in
+ * [w_lo, w_hi] drawn from `seed`; writes the new values of K (same pattern)
+ * into vals_out (nnz doubles). */
+culip_status culip_kkt_barrier_values(const culip_kkt * k, double w_lo,
+    double w_hi, uint64_t seed, double * vals_out);
+void culip_kkt_destroy(culip_kkt * k);
+
+/* Ordering, contiguous blocking (blocks <= max_block, split at n_primal) and
+ * static 1x1 pivot starts; K_perm(i, j) = K(perm[i], perm[j]) as the
+ * reference's host_permute with perm_is_old_to_new = false
+ * (block_ildlt_invariants.h:29-69, sparse_la.impl.cuh:602-661). */
+typedef struct culip_prep culip_prep;
+culip_status culip_prep_create(const culip_csr * K, int32_t n_primal,
+    int32_t max_block, culip_ordering ordering, culip_prep ** out);
+culip_status culip_prep_view(const culip_prep * p, culip_csr * K_perm,
+    const int32_t ** perm, int32_t * num_blocks, const int32_t ** block_starts,
+    int32_t * num_pivs, const int32_t ** piv_starts);
+/* Apply the prep permutation to a value array of K's pattern
+ * (vals_K[nnz] -> vals_perm[nnz] in K_perm's entry order). */
+culip_status culip_prep_permute_values(const culip_prep * p,
+    const double * vals_K, double * vals_perm);
+void culip_prep_destroy(culip_prep * p);
+
+/* MatrixMarket (libs/test/test.impl.h:25-300 semantics: 1-based, symmetric
+ * expansion, duplicates kept, per-row sort). */
+typedef struct culip_host_matrix culip_host_matrix;
+culip_status culip_mm_read_csr(const char * path, culip_host_matrix ** out);
+culip_status culip_mm_read_vector(const char * path, int64_t * len, double ** vals);
+culip_status culip_host_matrix_view(const culip_host_matrix * a, culip_csr * out);
+void culip_host_matrix_destroy(culip_host_matrix * a);
+void culip_host_free(void * p);
+
+/* ------------------------------------------------------------------------ */
+/* BlockiLDLt (libs/la/preconditioner/block_ildlt.cuh:188-311).            */
+
+typedef struct culip_bildlt culip_bildlt;
+
+/* ctor (block_ildlt.impl.cuh:44-79): full symmetric host CSR with sorted
+ * columns, num_pivs pivot starts, num_blocks block starts (no sentinels
+ * needed). The matrix values are copied. prec selects BlockiLDLt<double>
+ * or BlockiLDLt<float> semantics (storage and arithmetic of the factor). */
+culip_status culip_bildlt_create(const culip_csr * hA, int32_t num_pivs,
+    const int32_t * piv_starts, int32_t num_blocks,
+    const int32_t * block_starts, culip_prec prec, culip_bildlt ** out);
+
+/* compute(fill_level, fill_factor, threshold) (block_ildlt.impl.cuh:103-146):
+ * symbolic analysis on the host (once per pattern), upload, numeric factor
+ * on the device. Also performs solve_analysis() (:2050-2150). Returns after
+ * the factor has finished on `stream`. */
+culip_status culip_bildlt_compute(culip_bildlt * h, int32_t fill_level,
+    double fill_factor, double threshold, void * stream);
+
+/* Symbolic phase only (host): build_initial_coarse .. build_blocks and the
+ * solve analysis. compute() calls it when it has not run yet. */
+culip_status culip_bildlt_symbolic(culip_bildlt * h, int32_t fill_level,
+    double fill_factor);
+
+/* Numeric refactor for new values of the same pattern (interior-point
+ * serving: barrier diagonals change). vals: nnz doubles in hA's entry order,
+ * host (vals_on_device = 0) or device memory. No reference counterpart: the
+ * reference re-runs compute() from scratch. */
+culip_status culip_bildlt_refactor(culip_bildlt * h, const double * vals,
+    int32_t vals_on_device, double threshold, void * stream);
+
+/* solve_left / solve_middle / solve_right (block_ildlt.impl.cuh:183-246):
+ * left x = U' \ (P' b), middle x = D \ b, right x = P (U \ b); transpose
+ * swaps left and right. b, x: device doubles of length n; may alias only
+ * for CULIP_SOLVE_MIDDLE. */
+culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
+    int32_t transpose, const double * b, double * x, void * stream);
+
+int32_t culip_bildlt_n(const culip_bildlt * h);
+
+/* Host views of the symbolic structures and (after compute) of the factor
+ * downloaded from the device. Names: see DESIGN of the symbolic dump. */
+culip_status culip_bildlt_get_int(culip_bildlt * h, const char * name,
+    const int32_t ** ptr, int64_t * len);
+culip_status culip_bildlt_get_real(culip_bildlt * h, const char * name,
+    const double ** ptr, int64_t * len);
+/* Device bytes held by this object. */
+int64_t culip_bildlt_device_bytes(const culip_bildlt * h);
+void culip_bildlt_destroy(culip_bildlt * h);
+
+/* ------------------------------------------------------------------------ */
+/* SPMV (libs/la/spmv.cuh:18-149).                                          */
+
+typedef struct culip_spmv culip_spmv;
+/* CSRMatrixSPMV (spmv.cuh:36-66): uploads a copy of the host CSR. */
+culip_status culip_spmv_create_csr(const culip_csr * hA, culip_spmv ** out);
+/* AugmentedMatrixSPMV (spmv.cuh:116-149): K = [diag(d) A'; A delta I(ix)]
+ * matrix-free from A (m x n), d (n), delta and ix (n_ix rows of the dual
+ * block; ix == NULL: none). Dimensions are n + m (the reference's m()/n()
+ * return A->m, spmv.impl.cuh:319-337; fixed here). */
+culip_status culip_spmv_create_kkt(const culip_csr * hA, const double * d,
+    double delta, int32_t n_ix, const int32_t * ix, culip_spmv ** out);
+/* Replace the values of a CSR operator (same pattern), host array. */
+culip_status culip_spmv_update_values(culip_spmv * op, const double * vals,
+    int32_t vals_on_device, void * stream);
+/* multiply (spmv.cuh:24-31): y = alpha * op(A) x + beta * y. */
+culip_status culip_spmv_multiply(const culip_spmv * op, const double * x,
+    double * y, double alpha, double beta, int32_t transpose, void * stream);
+int32_t culip_spmv_m(const culip_spmv * op);
+int32_t culip_spmv_n(const culip_spmv * op);
+void culip_spmv_destroy(culip_spmv * op);
+
+/* ------------------------------------------------------------------------ */
+/* SQMR (libs/la/iterative_solvers/sqmr.cuh:21-72).                         */
+
+typedef struct {
+    double tol;             /* reference: 1e-8 in the driver */
+    int32_t max_it;         /* reference: 1000 in the driver */
+    /* 0: reference semantics only (quasi-residual test, 4-iteration
+     * throttle). k > 0: every k iterations recompute x and the true
+     * residual ||b - A x|| / ||b|| in FP64; stop when it is <= tol and
+     * restart from x (iterative refinement) when the recurrence has
+     * drifted. */
+    int32_t true_residual_period;
+    int32_t max_restarts;
+} culip_sqmr_opts;
+
+typedef struct {
+    double residual;            /* gamma / ||Pl \ b|| (sqmr.impl.cuh:343) */
+    int32_t iterations;         /* total SQMR iterations */
+    int32_t converged;          /* 1 if the stopping test was met */
+    int32_t restarts;           /* refinement restarts */
+    double true_rel_residual;   /* ||b - A x|| / ||b|| at exit, FP64 */
+} culip_sqmr_result;
+
+typedef struct culip_sqmr culip_sqmr;
+
+/* SQMR ctor (sqmr.impl.cuh:19-56): operator, optional preconditioner
+ * (NULL: none), options; allocates the device work vectors once. The
+ * operator and preconditioner are borrowed and must outlive the solver. */
+culip_status culip_sqmr_create(const culip_spmv * A, const culip_bildlt * prec,
+    const culip_sqmr_opts * opts, culip_sqmr ** out);
+
+/* SQMR::solve (sqmr.impl.cuh:109-355). b, x: device doubles. Synchronises
+ * `stream` before returning. */
+culip_status culip_sqmr_solve(culip_sqmr * s, const double * b, double * x,
+    culip_sqmr_result * res, void * stream);
+void culip_sqmr_destroy(culip_sqmr * s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,65 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    return GOLDEN
+
+
+def sym_problem(n_blocks=6, bmax=8, density=0.15, two_by_two=True, seed=1, diag_shift=4.0):
+    """Random symmetric indefinite test matrix with a contiguous blocking and
+    a static 1x1/2x2 pivot structure (pivots never straddle blocks)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(2, bmax + 1, size=n_blocks)
+    bs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    n = int(sizes.sum())
+    A = np.zeros((n, n))
+    mask = rng.random((n, n)) < density
+    vals = rng.normal(size=(n, n))
+    A[mask] = vals[mask]
+    A = np.triu(A, 1)
+    A = A + A.T
+    d = rng.normal(size=n) * diag_shift
+    d[np.abs(d) < 0.5] += 2.0
+    A[np.arange(n), np.arange(n)] = d
+    is_piv = np.ones(n, dtype=np.int32)
+    if two_by_two:
+        for b, s in zip(bs, sizes):
+            r = b
+            while r < b + s:
+                if r + 1 < b + s and rng.random() < 0.35:
+                    is_piv[r + 1] = 0
+                    A[r, r + 1] = A[r + 1, r] = rng.normal() * 3.0
+                    r += 2
+                else:
+                    r += 1
+    piv = np.nonzero(is_piv)[0].astype(np.int32)
+    rp = [0]
+    ci, cv = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        ci.extend(nz)
+        cv.extend(A[i, nz])
+        rp.append(len(ci))
+    return (np.array(rp, np.int32), np.array(ci, np.int32), np.array(cv)), piv, bs, A
+
+
+@pytest.fixture(scope="session")
+def hip_device():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("gpu test run without a HIP device")
+    return torch.device("cuda:0")

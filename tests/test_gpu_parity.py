@@ -1,0 +1,198 @@
+"""GPU parity: the HIP path against the CPU oracle on identical inputs.
+
+FP64: the numeric factor (block metadata after dropping, compacted storage,
+tridiagonal D, row norms) is compared bit for bit; the kernels perform the
+oracle's floating-point operations in the same order. Solves and SQMR are
+compared to the tolerances written in each test.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import culip_amd as ca
+import oracle_lib as ol
+from conftest import sym_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _kkt(n, m, B, order, seed=0, **kw):
+    k = ca.KKT(n, m, kw.pop("o", 4), kw.pop("c", 3), seed=seed, **kw)
+    p = ca.Prep(k.row_ptr, k.col, k.val, n, B, order)
+    return k, p
+
+
+def _pair(csr, piv, bs, t, fill_level=0, ff=1.0, thr=1e-4):
+    rp, ci, cv = csr
+    g = ca.BlockiLDLt(rp, ci, cv, piv, bs, "f32" if t == "f" else "f64")
+    g.compute(fill_level, ff, thr)
+    o = ol.OracleBildlt(rp, ci, cv, piv, bs, t)
+    o.symbolic(fill_level, ff)
+    o.factor(thr)
+    return g, o
+
+
+def _factor_equal(g, o, exact=True, rtol=0.0):
+    fb_g, fb_o = g.get_int("fblocks"), o.get("fblocks")
+    assert np.array_equal(fb_g, fb_o), "factored block metadata differs"
+    assert np.array_equal(g.get_int("fix_store"), o.get("fix_store"))
+    for nm in ("fval_store", "tri", "row_norms"):
+        a, b = g.get_real(nm), o.get(nm)
+        assert a.shape == b.shape, nm
+        if exact:
+            assert np.array_equal(a, b), f"{nm}: max diff {np.max(np.abs(a - b))}"
+        else:
+            np.testing.assert_allclose(a, b, rtol=rtol, atol=rtol, err_msg=nm)
+
+
+@pytest.mark.parametrize("order", [ca.ORDER_NATURAL, ca.ORDER_BLOCK_COLOR])
+def test_factor_bitwise_kkt_fp64(hip_device, order):
+    k, p = _kkt(2000, 800, 16, order)
+    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, "d")
+    _factor_equal(g, o)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_factor_bitwise_2x2_fp64(hip_device, seed):
+    csr, piv, bs, _ = sym_problem(n_blocks=9, bmax=32, density=0.12, seed=seed)
+    g, o = _pair(csr, piv, bs, "d", fill_level=1, ff=2.0, thr=1e-3)
+    _factor_equal(g, o)
+
+
+def test_factor_overflow_and_leftover_fp64(hip_device):
+    # fill factor 1 with level-1 fill and a large threshold: blocks overflow
+    # their capacity, borrow leftover space and keep the top entries
+    for seed, thr, dens in ((7, 0.05, 0.05), (8, 0.2, 0.1), (8, 0.01, 0.1)):
+        csr, piv, bs, _ = sym_problem(n_blocks=12, bmax=24, density=dens, two_by_two=False,
+                                      seed=seed)
+        g, o = _pair(csr, piv, bs, "d", fill_level=1, ff=1.0, thr=thr)
+        _factor_equal(g, o)
+        fb = o.get("fblocks").reshape(-1, 9)
+        assert (fb[:, 4] > 0).any(), "case must exercise borrowing from the leftover pool"
+        assert ((fb[:, 1] == 0) & (fb[:, 2] == fb[:, 3])).any(), "case must hit capacity"
+
+
+def test_factor_fp32_close_to_oracle(hip_device):
+    k, p = _kkt(2000, 800, 16, ca.ORDER_NATURAL)
+    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, "f")
+    fb_g, fb_o = g.get_int("fblocks").reshape(-1, 9), o.get("fblocks").reshape(-1, 9)
+    # structure: same formats and capacities; values to FP32 rounding
+    assert np.array_equal(fb_g[:, 1], fb_o[:, 1])
+    same = (fb_g == fb_o).all(axis=1).mean()
+    assert same > 0.999
+    np.testing.assert_allclose(g.get_real("tri"), o.get("tri"), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("t", ["d", "f"])
+def test_solves_match_oracle(hip_device, t):
+    import torch
+    k, p = _kkt(2000, 800, 16, ca.ORDER_BLOCK_COLOR)
+    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, t)
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=p.row_ptr.size - 1)
+    db = torch.tensor(b, device=hip_device)
+    tol = 1e-11 if t == "d" else 2e-4
+    for side in ("left", "middle", "right"):
+        x = torch.zeros_like(db)
+        getattr(g, f"solve_{side}")(db, x)
+        torch.cuda.synchronize()
+        ref = o.solve(side, b)
+        err = np.linalg.norm(x.cpu().numpy() - ref) / np.linalg.norm(ref)
+        assert err < tol, (side, err)
+
+
+def test_solve_identities(hip_device):
+    """U (U \\ b) = b and D (D \\ b) = b on the device factor."""
+    import torch
+    csr, piv, bs, _ = sym_problem(n_blocks=10, bmax=32, density=0.1, seed=4)
+    g, o = _pair(csr, piv, bs, "d", fill_level=1, ff=3.0, thr=0.0)
+    n = csr[0].size - 1
+    b = np.random.default_rng(3).normal(size=n)
+    x = torch.zeros(n, dtype=torch.float64, device=hip_device)
+    g.solve_right(torch.tensor(b, device=hip_device), x)
+    torch.cuda.synchronize()
+    # rebuild U densely from the oracle's factor (bitwise equal to the GPU's)
+    U = _dense_U(o, n)
+    np.testing.assert_allclose(U @ x.cpu().numpy(), b, rtol=1e-10, atol=1e-10)
+
+
+def _dense_U(o, n):
+    bs = list(o.get("block_starts"))
+    fb = o.get("fblocks").reshape(-1, 9)
+    fix, fv = o.get("fix_store"), o.get("fval_store")
+    U = np.zeros((n, n))
+    for blk in fb:
+        _, fmt, nnz, _, _, br, bc, cp, vp = blk
+        r0, r1, c0, c1 = bs[br], bs[br + 1], bs[bc], bs[bc + 1]
+        h, w = r1 - r0, c1 - c0
+        if fmt == 1:
+            T = fv[vp:vp + h * w].reshape(h, w)
+            if br == bc:
+                T = np.triu(T)
+            U[r0:r1, c0:c1] = T
+        else:
+            for q in range(nnz):
+                ix = fix[cp + q]
+                U[r0 + ix // w, c0 + ix % w] = fv[vp + q]
+    return U
+
+
+def test_spmv_matches_fixture(hip_device, golden):
+    """AugmentedMatrixSPMV vs tests/la/data/spmv_K.mtx (test_spmv.cc:191-210)."""
+    import torch
+    ar, ac, av, an = ca.read_matrix_csr(f"{golden}/spmv_A.mtx")
+    d = ca.read_dense_vector(f"{golden}/spmv_d.mtx")
+    kr, kc, kv, kn = ca.read_matrix_csr(f"{golden}/spmv_K.mtx")
+    N = kr.size - 1
+    x = torch.full((N,), 0.1, dtype=torch.float64, device=hip_device)
+    yk = torch.zeros_like(x)
+    ya = torch.zeros_like(x)
+    K = ca.CSRMatrixSPMV(kr, kc, kv)
+    Aug = ca.AugmentedMatrixSPMV(ar, ac, av, an, d)
+    K.multiply(x, yk)
+    Aug.multiply(x, ya)
+    Aug2 = torch.zeros_like(x)
+    Aug.multiply(x, Aug2, transpose=True)
+    torch.cuda.synchronize()
+    ref = ol.csr_spmv(kr, kc, kv, np.full(N, 0.1))
+    np.testing.assert_allclose(yk.cpu().numpy(), ref, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(ya.cpu().numpy(), ref, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(Aug2.cpu().numpy(), ref, rtol=1e-10, atol=1e-10)
+
+
+def test_sqmr_reference_semantics_vs_oracle(hip_device):
+    import torch
+    k, p = _kkt(2000, 800, 16, ca.ORDER_NATURAL)
+    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, "d")
+    b = k.rhs[p.perm]
+    xo, ro = ol.sqmr_bildlt(p.row_ptr, p.col, p.val, o, b, 1e-8, 1000)
+    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    s = ca.SQMR(A, 1e-8, 1000, g)
+    x = torch.zeros(b.size, dtype=torch.float64, device=hip_device)
+    res, its = s.solve(torch.tensor(b, device=hip_device), x)
+    assert abs(its - ro.iterations) <= 4
+    assert abs(res - ro.residual) <= 0.5 * ro.residual
+    K = sp.csr_matrix((p.val, p.col, p.row_ptr))
+    xg = x.cpu().numpy()
+    rel = np.linalg.norm(b - K @ xg) / np.linalg.norm(b)
+    assert rel < 1e-6
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_sqmr_true_residual_gate(hip_device, prec):
+    """Configurations 1 and 3: true relative residual <= 1e-8 in FP64."""
+    import torch
+    for kw in ({}, {"h_diag": (1e-6, 1e6), "h_diag_log": True, "delta": 1e-10}):
+        k, p = _kkt(2000, 800, 16, ca.ORDER_BLOCK_COLOR, **dict(kw))
+        g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, prec)
+        g.compute(0, 1.0, 1e-4)
+        A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+        s = ca.SQMR(A, 1e-8, 2000, g, true_residual_period=20, max_restarts=20)
+        b = k.rhs[p.perm]
+        x = torch.zeros(b.size, dtype=torch.float64, device=hip_device)
+        s.solve(torch.tensor(b, device=hip_device), x)
+        K = sp.csr_matrix((p.val, p.col, p.row_ptr))
+        rel = np.linalg.norm(b - K @ x.cpu().numpy()) / np.linalg.norm(b)
+        assert rel <= 1e-8, (prec, kw, rel, s.result)
+        assert s.result["true_rel_residual"] <= 1e-8
