@@ -163,6 +163,7 @@ void Bildlt::factor_impl(hipStream_t st)
     HIP_CHECK(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     for(int q = 0; q < 4; ++q) {
+        plan_tot[q] = htot[q];
         const size_t cnt = (size_t) std::max(1, htot[q]);
         d_col[q].ensure(sizeof(int32_t) * cnt);
         d_val[q].ensure(sizeof(VT) * cnt);

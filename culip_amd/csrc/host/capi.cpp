@@ -344,6 +344,12 @@ culip_status culip_bildlt_get_int(culip_bildlt* h, const char* name, const int32
         const Symbolic& s = b.sym;
         const std::string nm(name);
         auto put = [&](const std::vector<int32_t>& v) { *ptr = v.data(); *len = (int64_t) v.size(); };
+        if(nm == "plan_sizes") {
+            CULIP_REQUIRE(b.factored, "bildlt: factor has not been computed");
+            b.host_int[nm] = std::vector<int32_t>(b.plan_tot, b.plan_tot + 4);
+            put(b.host_int[nm]);
+            return;
+        }
         if(nm == "fblocks" || nm == "fix_store" || nm == "carry") {
             b.download_factor();
             put(b.host_int.at(nm));

@@ -70,6 +70,7 @@ struct Bildlt {
     // solve layout
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4];
     DevBuf d_totals;
+    int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
 
     // host copies for inspection
     std::map<std::string, std::vector<int32_t>> host_int;

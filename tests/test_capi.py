@@ -1,0 +1,109 @@
+"""The C-ABI library loads on any host, exports every entry point declared
+in include/culip_amd.h, reports errors as status codes, and its host-side
+input preparation (generator, ordering, blocking) behaves as documented."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import culip_amd as ca
+from culip_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "culip_amd.h")
+
+
+def test_library_exports_every_declared_symbol():
+    text = open(HEADER).read()
+    declared = set(re.findall(r"\b(culip_[a-z0-9_]+)\s*\(", text))
+    assert len(declared) > 30
+    h = _lib.lib()
+    missing = [s for s in sorted(declared) if not hasattr(h, s)]
+    assert not missing, missing
+    assert set(_lib.EXPORTED) <= declared
+
+
+def test_errors_are_status_codes():
+    rp = np.array([0, 1, 2], np.int32)
+    ci = np.array([0, 1], np.int32)
+    cv = np.array([1.0, 2.0])
+    # a block start that is not a pivot start: 2x2 pivot straddling blocks
+    g = ca.BlockiLDLt(rp, ci, cv, [0], [0, 1])
+    with pytest.raises(ca.CulipError) as e:
+        g.symbolic(0, 1.0)
+    assert e.value.status == 1 and "pivot" in str(e.value)
+    with pytest.raises(ca.CulipError) as e:
+        ca.BlockiLDLt(rp, ci, cv, [0, 1], [0, 1]).symbolic(0, 0.5)
+    assert "fill factor" in str(e.value)
+    with pytest.raises(ca.CulipError):
+        ca.BlockiLDLt(rp, np.array([0, 5], np.int32), cv, [0, 1], [0, 1])
+    with pytest.raises(ca.CulipError):
+        ca.Prep(rp, ci, cv, 1, 64)
+
+
+def test_generator_is_deterministic_and_symmetric():
+    a = ca.KKT(500, 200, 4, 3, seed=7)
+    b = ca.KKT(500, 200, 4, 3, seed=7)
+    c = ca.KKT(500, 200, 4, 3, seed=8)
+    assert np.array_equal(a.val, b.val) and np.array_equal(a.rhs, b.rhs)
+    assert not np.array_equal(a.val, c.val)
+    K = sp.csr_matrix((a.val, a.col, a.row_ptr))
+    assert (abs(K - K.T) > 0).nnz == 0
+    n, m = 500, 200
+    H = K[:n, :n]
+    A = K[n:, :n]
+    assert np.allclose(K[n:, n:].toarray(), -1e-8 * np.eye(m))
+    assert H.nnz == n + n * 4 // 2 * 2
+    assert (np.diff(A.tocsc().indptr) == 3).all()
+    d = H.diagonal()
+    assert d.min() >= 1.0 and d.max() <= 10.0
+    off = (H - sp.diags(d)).data
+    assert abs(off.std() - 0.1) < 0.02
+
+
+def test_barrier_values_rescale_h_diagonal_only():
+    k = ca.KKT(300, 100, 4, 3, seed=1)
+    v = k.barrier_values(1e-4, 1e4, seed=3)
+    K0 = sp.csr_matrix((k.val, k.col, k.row_ptr))
+    K1 = sp.csr_matrix((v, k.col, k.row_ptr))
+    D = (K1 - K0)
+    assert (D - sp.diags(D.diagonal())).nnz == 0 or abs(D - sp.diags(D.diagonal())).max() == 0
+    w = K1.diagonal()[:300] / K0.diagonal()[:300]
+    assert w.min() >= 1e-4 and w.max() <= 1e4
+    assert np.array_equal(K1.diagonal()[300:], K0.diagonal()[300:])
+
+
+@pytest.mark.parametrize("order", [ca.ORDER_NATURAL, ca.ORDER_BLOCK_COLOR])
+def test_prep_permutes_and_blocks(order):
+    k = ca.KKT(1000, 400, 4, 3, seed=2)
+    p = ca.Prep(k.row_ptr, k.col, k.val, 1000, 16, order)
+    K = sp.csr_matrix((k.val, k.col, k.row_ptr))
+    P = sp.csr_matrix((p.val, p.col, p.row_ptr))
+    assert sorted(p.perm) == list(range(1400))
+    assert (abs(K[p.perm][:, p.perm] - P) > 0).nnz == 0
+    assert np.array_equal(p.permute_values(k.val), p.val)
+    bs = list(p.block_starts) + [1400]
+    assert all(1 <= bs[i + 1] - bs[i] <= 16 for i in range(len(bs) - 1))
+    assert np.array_equal(p.piv_starts, np.arange(1400))
+    # primal rows first, dual rows after, blocks never mix them
+    assert (p.perm[:1000] < 1000).all()
+    assert 1000 in bs
+    if order == ca.ORDER_BLOCK_COLOR:
+        g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+        g.symbolic(0, 1.0)
+        gn = ca.BlockiLDLt(k.row_ptr, k.col, k.val, p.piv_starts,
+                           ca.Prep(k.row_ptr, k.col, k.val, 1000, 16).block_starts)
+        gn.symbolic(0, 1.0)
+        assert g.get_int("scalars")[6] < gn.get_int("scalars")[6]
+
+
+def test_matrix_market_reader(golden):
+    rp, ci, cv, n = ca.read_matrix_csr(f"{golden}/sqmr_small_A.mtx")
+    assert rp.size - 1 == 62 and n == 62 and rp[-1] == 342
+    A = sp.csr_matrix((cv, ci, rp))
+    assert (abs(A - A.T) > 0).nnz == 0
+    assert all((np.diff(ci[rp[i]:rp[i + 1]]) >= 0).all() for i in range(62))
+    p = ca.read_dense_vector(f"{golden}/sqmr_small_prec_p.mtx")
+    assert sorted(p.astype(int)) == list(range(62))

@@ -113,7 +113,27 @@ def test_solve_identities(hip_device):
     torch.cuda.synchronize()
     # rebuild U densely from the oracle's factor (bitwise equal to the GPU's)
     U = _dense_U(o, n)
-    np.testing.assert_allclose(U @ x.cpu().numpy(), b, rtol=1e-10, atol=1e-10)
+    xs = x.cpu().numpy()
+    # normwise backward error of the computed solution (stable: ~n eps)
+    bwd = np.linalg.norm(b - U @ xs) / (np.linalg.norm(U, 2) * np.linalg.norm(xs) + np.linalg.norm(b))
+    assert bwd < 1e-13, bwd
+    z = torch.zeros_like(x)
+    g.solve_middle(torch.tensor(b, device=hip_device), z)
+    torch.cuda.synchronize()
+    D = _dense_D(o, n)
+    zs = z.cpu().numpy()
+    assert np.linalg.norm(b - D @ zs) / (np.linalg.norm(D, 2) * np.linalg.norm(zs)) < 1e-14
+
+
+def _dense_D(o, n):
+    tri = o.get("tri")
+    D = np.zeros((n, n))
+    for i in range(n):
+        D[i, i] = tri[3 * i + 1]
+        if i + 1 < n:
+            D[i + 1, i] = tri[3 * i + 2]
+            D[i, i + 1] = tri[3 * (i + 1)]
+    return D
 
 
 def _dense_U(o, n):
