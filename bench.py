@@ -26,9 +26,9 @@ WORKLOADS = {
                      delta=1e-8),
     # CPU-runnable case: n=2,000, m=800, 4 off-diagonals/row, 3 nnz/col, block 16.
     "kkt-2k": dict(n=2000, m=800, o=4, c=3, B=16, h_diag=(1.0, 10.0), log=False, delta=1e-8),
-    # Barrier-conditioned case: H diagonal 10^U(-6,6), delta 1e-10.
-    "kkt-2k-barrier": dict(n=2000, m=800, o=4, c=3, B=16, h_diag=(1e-6, 1e6), log=True,
-                           delta=1e-10),
+    # Barrier-conditioned case: H = H0 + diag(w), w = 10^U(-6,6), delta 1e-10.
+    "kkt-2k-barrier": dict(n=2000, m=800, o=4, c=3, B=16, h_diag=(1.0, 10.0), log=False,
+                           delta=1e-10, barrier=(1e-6, 1e6)),
 }
 
 
@@ -72,7 +72,7 @@ def cpu_baseline(wl, order, seed, fill_factor, threshold):
     scale = 10 if wl["n"] >= 100000 else 1
     n, m = wl["n"] // scale, wl["m"] // scale
     kkt = ca.KKT(n, m, wl["o"], wl["c"], h_diag=wl["h_diag"], h_diag_log=wl["log"],
-                 delta=wl["delta"], seed=seed)
+                 delta=wl["delta"], seed=seed, barrier=wl.get("barrier"))
     prep = ca.Prep(kkt.row_ptr, kkt.col, kkt.val, n, wl["B"], order)
     b = kkt.rhs[prep.perm]
     o = ol.OracleBildlt(prep.row_ptr, prep.col, prep.val, prep.piv_starts, prep.block_starts,
@@ -105,7 +105,7 @@ def main():
     wl = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     kkt = ca.KKT(wl["n"], wl["m"], wl["o"], wl["c"], h_diag=wl["h_diag"], h_diag_log=wl["log"],
-                 delta=wl["delta"], seed=a.seed)
+                 delta=wl["delta"], seed=a.seed, barrier=wl.get("barrier"))
     order = ca.ORDER_BLOCK_COLOR if a.order == "color" else ca.ORDER_NATURAL
     prep = ca.Prep(kkt.row_ptr, kkt.col, kkt.val, wl["n"], wl["B"], order)
     N = prep.row_ptr.size - 1

@@ -73,9 +73,10 @@ class KKT:
     """Synthetic KKT system [[H, A'], [A, -delta I]] (host)."""
 
     def __init__(self, n, m, h_offdiag_per_row=4, a_nnz_per_col=3, h_diag=(1.0, 10.0),
-                 h_diag_log=False, h_offdiag_std=0.1, delta=1e-8, seed=0):
+                 h_diag_log=False, h_offdiag_std=0.1, delta=1e-8, seed=0, barrier=None):
+        bl, bh = barrier if barrier else (0.0, 0.0)
         p = _lib.KktParams(n, m, h_offdiag_per_row, a_nnz_per_col, h_diag[0], h_diag[1],
-                           1 if h_diag_log else 0, h_offdiag_std, delta, seed)
+                           1 if h_diag_log else 0, h_offdiag_std, delta, seed, bl, bh)
         h = C.c_void_p()
         check(lib().culip_kkt_generate(C.byref(p), C.byref(h)))
         self._h = h

@@ -34,7 +34,8 @@ class KktParams(C.Structure):
     _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("h_offdiag_per_row", C.c_int32),
                 ("a_nnz_per_col", C.c_int32), ("h_diag_lo", C.c_double),
                 ("h_diag_hi", C.c_double), ("h_diag_log", C.c_int32),
-                ("h_offdiag_std", C.c_double), ("delta", C.c_double), ("seed", C.c_uint64)]
+                ("h_offdiag_std", C.c_double), ("delta", C.c_double), ("seed", C.c_uint64),
+                ("barrier_lo", C.c_double), ("barrier_hi", C.c_double)]
 
 
 class SqmrOpts(C.Structure):

@@ -2,23 +2,27 @@
 //
 // Reference: libs/la/preconditioner/block_ildlt.impl.cuh:1519-1671 (schedule)
 // and block_ildlt_kernels.impl.cuh:1146-2074 (kernels). Per level:
-//   k_pivots   one wave per pivot block: A_ii -= sum_k U_ki' D_k U_ki over
-//              the block column, static 1x1/2x2 LDL' of the tile, row sums
-//   k_offdiag  one wave per off-diagonal block: A_ij -= sum_k U_ki' D_k U_kj
-//              over the symbolic Schur pairs, U_ij = D_i \ (U_ii' \ A_ij),
-//              per-row partial sums of squares; the result leaves as a dense
-//              tile or as a compact (index, value) list of |v| > 1e-10
-//   k_rowdrop  one wave per block row: row norms in a fixed order, count of
-//              entries surviving |v| >= tau * sqrtf(norm) per sparse block
-//   k_scan     one workgroup: leftover-space pool, granted extra space and
-//              output offsets for the level's blocks in level-list order
-//   k_store    one wave per block: compacted final storage
+//   k_pivots    one wave per pivot block: A_ii -= sum_k U_ki' D_k U_ki over
+//               the block column, static 1x1/2x2 LDL' of the tile, row sums
+//   k_offdiag   one wave per off-diagonal block: A_ij -= sum_k U_ki' D_k U_kj
+//               over the block's symbolic Schur pairs, U_ij = D_i \ (U_ii' \
+//               A_ij), per-row partial sums of squares; the result leaves as
+//               a dense tile or as a compact (index, value) list of the
+//               entries with |v| > 1e-10
+//   k_rownorm   one wave per block row: row sums of squares in a fixed order
+//   k_dropcount one lane per sparse block: entries surviving the threshold
+//   k_scan_sizes one workgroup: leftover-space pool, granted extra space and
+//               output offsets for the level's blocks in level-list order
+//   k_store     one wave per block: compacted final storage
 //
-// Every floating-point operation of the reference's kernels is performed in
-// the same order with the same fused multiply-adds as the CPU oracle's
-// restatement, so FP64 factors agree bit for bit. Work on zero entries that
-// the reference performs (padded 32x32 products) is skipped only where it
-// cannot change a result.
+// The dependency blocks of a tile are walked with their metadata fetched
+// lane-parallel up front and each block's entries fetched one step ahead
+// into registers, so a dependency costs LDS work rather than a chain of
+// dependent global loads. The update U' D U is applied over the nonzero rows
+// of the right factor only, with D applied on the fly: every term the
+// reference adds is added, in the same order with the same fused
+// multiply-adds as the CPU oracle's restatement; the skipped terms are exact
+// zeros. FP64 factors therefore agree with the oracle bit for bit.
 #include "factor.hpp"
 #include "wave.hpp"
 
@@ -38,8 +42,8 @@ __device__ __forceinline__ void tile_zero(VT* T)
     for(int t = 0; t < 16; ++t) T[64 * t + lane] = (VT) 0;
 }
 
-// d_load_block (kernels :22-81). TRANS: T[32 c + r] = B[r][c].
-template<typename VT, bool TRANS>
+// d_load_block (kernels :22-81) for an input block: zero, then fill
+template<typename VT>
 __device__ void tile_load(VT* T, const BlockMeta& m, const int32_t* bs, const int32_t* ixs,
     const VT* vals)
 {
@@ -52,76 +56,182 @@ __device__ void tile_load(VT* T, const BlockMeta& m, const int32_t* bs, const in
 #pragma unroll
         for(int t = 0; t < 16; ++t) {
             const int idx = 64 * t + lane, r = idx >> 5, c = idx & 31;
-            if(r < h && c < w) {
-                const VT v = vals[(int64_t) m.val_ptr + r * w + c];
-                T[TRANS ? (32 * c + r) : idx] = v;
-            }
+            if(r < h && c < w) T[idx] = vals[(int64_t) m.val_ptr + r * w + c];
         }
     } else {
         for(int e = lane; e < m.nnz; e += 64) {
             const int ix = ixs[(int64_t) m.col_ptr + e];
             const int r = ix / w, c = ix - r * w;
-            T[TRANS ? (32 * c + r) : (32 * r + c)] = vals[(int64_t) m.val_ptr + e];
+            T[32 * r + c] = vals[(int64_t) m.val_ptr + e];
         }
     }
     wave_sync();
 }
 
-// Lt[c][l] = (U_ki' D_k)[c][l] from src(c, l) = U_ki[l][c]
-// (d_multiply_block_by_block_diagonal, kernels :372-411)
-template<typename VT, bool SRC_T>
-__device__ void build_lt(VT* Lt, const VT* S, const double* tri_k, int rk)
+// metadata of a factored dependency block, one per lane
+struct Dep {
+    int fmt, nnz, cp, vp, rb, rk;
+};
+
+__device__ __forceinline__ Dep load_dep(const BlockMeta* bout, const int32_t* bs, int blk)
+{
+    const BlockMeta m = bout[blk];
+    Dep d;
+    d.fmt = m.format;
+    d.nnz = m.nnz;
+    d.cp = m.col_ptr;
+    d.vp = m.val_ptr;
+    d.rb = bs[m.block_row];
+    d.rk = bs[m.block_row + 1] - d.rb;
+    return d;
+}
+
+__device__ __forceinline__ Dep bcast(const Dep& d, int src)
+{
+    Dep o;
+    o.fmt = wave_bcast(d.fmt, src);
+    o.nnz = wave_bcast(d.nnz, src);
+    o.cp = wave_bcast(d.cp, src);
+    o.vp = wave_bcast(d.vp, src);
+    o.rb = wave_bcast(d.rb, src);
+    o.rk = wave_bcast(d.rk, src);
+    return o;
+}
+
+// registers holding one dependency block: its (up to 64) sparse entries and
+// the tridiagonal D of its rows
+template<typename VT>
+struct Pre {
+    int ix;
+    VT v;
+    VT sup, dia, sub;
+};
+
+template<typename VT>
+__device__ __forceinline__ bool small_sparse(const Dep& d)
+{
+    return d.fmt == FMT_SPARSE && d.nnz <= 64;
+}
+
+template<typename VT>
+__device__ __forceinline__ Pre<VT> prefetch(const Dep& d, const int32_t* ixs, const VT* vals,
+    const double* tri, bool want_d)
 {
     const int lane = lane_id();
+    Pre<VT> p;
+    p.ix = 0;
+    p.v = 0;
+    p.sup = p.dia = p.sub = 0;
+    if(small_sparse<VT>(d) && lane < d.nnz) {
+        p.ix = ixs[(int64_t) d.cp + lane];
+        p.v = vals[(int64_t) d.vp + lane];
+    }
     const int l = lane & 31;
-    VT sup = 0, dia = 0, sub = 0;
-    if(l < rk) {
-        sup = (VT) tri_k[3 * l];
-        dia = (VT) tri_k[3 * l + 1];
-        sub = (VT) tri_k[3 * l + 2];
+    if(want_d && l < d.rk) {
+        const double* t = tri + 3 * (int64_t) (d.rb + l);
+        p.sup = (VT) t[0];
+        p.dia = (VT) t[1];
+        p.sub = (VT) t[2];
     }
-#pragma unroll
-    for(int t = 0; t < 16; ++t) {
-        const int c = (lane >> 5) + 2 * t;
-        auto src = [&](int ll) { return SRC_T ? S[32 * c + ll] : S[32 * ll + c]; };
-        const VT mid = src(l);
-        VT res = mid;
-        if(l < rk) {
-            const VT left = (l == 0) ? (VT) 0 : src(l - 1);
-            const VT right = (l == 31) ? (VT) 0 : src(l + 1);
-            res = fma(sub, right, fma(dia, mid, sup * left));
-        }
-        Lt[32 * c + l] = res;
-    }
-    wave_sync();
+    return p;
 }
 
-// T[c][j] -= sum_{l < bw} R[l][j] * Lt[c][l] for c < lh, j < rh
-// (d_apply_left_right_update, kernels :415-456)
+// Writes a dependency block (width w) into tile X and, when mask is set,
+// the nonzero rows of each column into mask[c] (LDS, integer atomics).
 template<typename VT>
-__device__ void apply_product(VT* T, const VT* Lt, const VT* R, int lh, int bw, int rh)
+__device__ void stage(VT* X, uint32_t* mask, const Dep& d, const Pre<VT>& p, int w,
+    const int32_t* ixs, const VT* vals)
+{
+    const int lane = lane_id();
+    if(small_sparse<VT>(d)) {
+        if(lane < d.nnz) {
+            const int r = p.ix / w, c = p.ix - r * w;
+            X[32 * r + c] = p.v;
+            if(mask && p.v != (VT) 0) atomicOr(&mask[c], 1u << r);
+        }
+    } else if(d.fmt == FMT_DENSE) {
+#pragma unroll
+        for(int t = 0; t < 16; ++t) {
+            const int idx = 64 * t + lane, r = idx >> 5, c = idx & 31;
+            if(r < d.rk && c < w) {
+                const VT v = vals[(int64_t) d.vp + r * w + c];
+                X[idx] = v;
+                if(mask && v != (VT) 0) atomicOr(&mask[c], 1u << r);
+            }
+        }
+    } else {
+        for(int e = lane; e < d.nnz; e += 64) {
+            const int ix = ixs[(int64_t) d.cp + e];
+            const VT v = vals[(int64_t) d.vp + e];
+            const int r = ix / w, c = ix - r * w;
+            X[32 * r + c] = v;
+            if(mask && v != (VT) 0) atomicOr(&mask[c], 1u << r);
+        }
+    }
+}
+
+template<typename VT>
+__device__ void unstage(VT* X, uint32_t* mask, const Dep& d, const Pre<VT>& p, int w)
+{
+    const int lane = lane_id();
+    if(small_sparse<VT>(d)) {
+        if(lane < d.nnz) {
+            const int r = p.ix / w, c = p.ix - r * w;
+            X[32 * r + c] = (VT) 0;
+        }
+    } else {
+        tile_zero(X);
+    }
+    if(mask && lane < 32) mask[lane] = 0;
+}
+
+template<typename VT>
+__device__ __forceinline__ void stage_d(VT* cs, VT* cd, VT* cb, const Pre<VT>& p)
+{
+    const int lane = lane_id();
+    if(lane < 32) {
+        cs[lane] = p.sup;
+        cd[lane] = p.dia;
+        cb[lane] = p.sub;
+    }
+}
+
+// T[c][j] -= sum_l R[l][j] * Lt[c][l] over the nonzero rows l of R's column
+// j, ascending, with Lt[c][l] = (A' D)[c][l] formed from A[l-1..l+1][c] as
+// d_multiply_block_by_block_diagonal does (kernels :372-456)
+template<typename VT>
+__device__ void product(VT* T, const VT* A, const VT* R, const uint32_t* mask, const VT* cs,
+    const VT* cd, const VT* cb, int lh, int rh)
 {
     const int lane = lane_id();
     const int j = lane & 31;
     const int c0 = lane >> 5;
-    VT acc[16];
+    uint32_t mk = (j < rh) ? mask[j] : 0u;
+    if(mk) {
+        VT acc[16];
 #pragma unroll
-    for(int t = 0; t < 16; ++t) acc[t] = 0;
-    for(int l = 0; l < bw; ++l) {
-        const VT r = R[32 * l + j];
-        if(r != (VT) 0) {
+        for(int t = 0; t < 16; ++t) acc[t] = 0;
+        while(mk) {
+            const int l = __ffs(mk) - 1;
+            mk &= mk - 1;
+            const VT r = R[32 * l + j];
+            const VT sup = cs[l], dia = cd[l], sub = cb[l];
 #pragma unroll
-            for(int t = 0; t < 16; ++t) acc[t] = fma(r, Lt[32 * (c0 + 2 * t) + l], acc[t]);
+            for(int t = 0; t < 16; ++t) {
+                const int c = c0 + 2 * t;
+                const VT left = (l == 0) ? (VT) 0 : A[32 * (l - 1) + c];
+                const VT mid = A[32 * l + c];
+                const VT right = (l == 31) ? (VT) 0 : A[32 * (l + 1) + c];
+                const VT lt = fma(sub, right, fma(dia, mid, sup * left));
+                acc[t] = fma(r, lt, acc[t]);
+            }
         }
-    }
-    if(j < rh) {
 #pragma unroll
         for(int t = 0; t < 16; ++t) {
             const int c = c0 + 2 * t;
             if(c < lh) T[32 * c + j] -= acc[t];
         }
     }
-    wave_sync();
 }
 
 // d_ldlt_1x1_step (kernels :538-585)
@@ -210,10 +320,21 @@ __device__ VT row_ssq(const VT* T, int r, int w)
 }
 
 template<typename VT>
-struct alignas(16) WaveTiles {
+struct alignas(16) PivLds {
     VT T[1024];
-    VT L[1024];
     VT R[1024];
+    VT cs[32], cd[32], cb[32];
+    uint32_t mask[32];
+};
+
+template<typename VT>
+struct alignas(16) OffLds {
+    VT T[1024];
+    VT A[1024];
+    VT R[1024];
+    VT cs[32], cd[32], cb[32];
+    uint32_t mask[32];
+    uint32_t umask[32];
 };
 
 // ---------------------------------------------------------------------------
@@ -222,7 +343,7 @@ struct alignas(16) WaveTiles {
 template<typename VT>
 __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
 {
-    __shared__ WaveTiles<VT> s;
+    __shared__ PivLds<VT> s;
     const int lane = lane_id();
     const int q = blockIdx.x;
     const int b = a.lvlb_idx[lr.lo + q];
@@ -230,14 +351,35 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
     const int i = m.block_row;
     const int fr = a.bs[i];
     const int len = a.bs[i + 1] - fr;
-    tile_load<VT, false>(s.T, m, a.bs, a.ix_in, a.val_in);
-    for(int ci = a.csc_off[i]; ci < a.csc_off[i + 1] - 1; ++ci) {
-        const BlockMeta dm = a.bout[a.csc_idx[ci]];
-        const int kr = dm.block_row;
-        const int rk = a.bs[kr + 1] - a.bs[kr];
-        tile_load<VT, false>(s.R, dm, a.bs, a.ix_out, a.val_out);
-        build_lt<VT, false>(s.L, s.R, a.tri + 3 * (int64_t) a.bs[kr], rk);
-        apply_product(s.T, s.L, s.R, len, rk, len);
+    const int cb0 = a.csc_off[i];
+    const int nd = a.csc_off[i + 1] - 1 - cb0;
+    tile_zero(s.R);
+    if(lane < 32) s.mask[lane] = 0;
+    tile_load(s.T, m, a.bs, a.ix_in, a.val_in);
+    // updates from the blocks (k, i) above the diagonal, in CSC order
+    for(int c0 = 0; c0 < nd; c0 += 64) {
+        Dep mine{};
+        if(c0 + lane < nd) mine = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + lane]);
+        const int cnt = min(64, nd - c0);
+        Dep cur = bcast(mine, 0);
+        Pre<VT> pc = prefetch(cur, a.ix_out, a.val_out, a.tri, true);
+        for(int d = 0; d < cnt; ++d) {
+            Dep nxt = cur;
+            Pre<VT> pn = pc;
+            if(d + 1 < cnt) {
+                nxt = bcast(mine, d + 1);
+                pn = prefetch(nxt, a.ix_out, a.val_out, a.tri, true);
+            }
+            stage(s.R, s.mask, cur, pc, len, a.ix_out, a.val_out);
+            stage_d(s.cs, s.cd, s.cb, pc);
+            wave_sync();
+            product(s.T, s.R, s.R, s.mask, s.cs, s.cd, s.cb, len, len);
+            wave_sync();
+            unstage(s.R, s.mask, cur, pc, len);
+            wave_sync();
+            cur = nxt;
+            pc = pn;
+        }
     }
     // d_factor_pivot_factorize with static pivots (kernels :956-1083)
     double* tri = a.tri + 3 * (int64_t) fr;
@@ -265,7 +407,7 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
 template<typename VT>
 __global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
 {
-    __shared__ WaveTiles<VT> s;
+    __shared__ OffLds<VT> s;
     const int lane = lane_id();
     const int q = lr.npiv + blockIdx.x;
     const int b = a.lvlb_idx[lr.lo + q];
@@ -274,67 +416,108 @@ __global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
     const int fr = a.bs[i];
     const int hgt = a.bs[i + 1] - fr;
     const int wid = a.bs[jb + 1] - a.bs[jb];
-    tile_load<VT, false>(s.T, m, a.bs, a.ix_in, a.val_in);
-    for(int64_t p = a.upd_off[b]; p < a.upd_off[b + 1]; ++p) {
-        const BlockMeta mk = a.bout[a.upd_ki[p]];
-        const int kr = mk.block_row;
-        const int rk = a.bs[kr + 1] - a.bs[kr];
-        tile_load<VT, true>(s.R, mk, a.bs, a.ix_out, a.val_out);
-        build_lt<VT, true>(s.L, s.R, a.tri + 3 * (int64_t) a.bs[kr], rk);
-        tile_load<VT, false>(s.R, a.bout[a.upd_kj[p]], a.bs, a.ix_out, a.val_out);
-        apply_product(s.T, s.L, s.R, hgt, rk, wid);
+    const int64_t p0 = a.upd_off[b];
+    const int np = (int) (a.upd_off[b + 1] - p0);
+    const int pq = a.piv_loc[i];
+    const bool all1x1 = (wave_ballot(lane < hgt && !a.is_piv[fr + (lane & 31)]) == 0);
+    tile_zero(s.A);
+    if(lane < 32) { s.mask[lane] = 0; s.umask[lane] = 0; }
+    if(np > 0) tile_zero(s.R);
+    tile_load(s.T, m, a.bs, a.ix_in, a.val_in);
+    for(int c0 = 0; c0 < np; c0 += 64) {
+        Dep mi{}, mj{};
+        if(c0 + lane < np) {
+            mi = load_dep(a.bout, a.bs, a.upd_ki[p0 + c0 + lane]);
+            mj = load_dep(a.bout, a.bs, a.upd_kj[p0 + c0 + lane]);
+        }
+        const int cnt = min(64, np - c0);
+        Dep ci = bcast(mi, 0), cj = bcast(mj, 0);
+        Pre<VT> pi = prefetch(ci, a.ix_out, a.val_out, a.tri, true);
+        Pre<VT> pj = prefetch(cj, a.ix_out, a.val_out, a.tri, false);
+        for(int d = 0; d < cnt; ++d) {
+            Dep ni = ci, nj = cj;
+            Pre<VT> qi = pi, qj = pj;
+            if(d + 1 < cnt) {
+                ni = bcast(mi, d + 1);
+                nj = bcast(mj, d + 1);
+                qi = prefetch(ni, a.ix_out, a.val_out, a.tri, true);
+                qj = prefetch(nj, a.ix_out, a.val_out, a.tri, false);
+            }
+            stage(s.A, (uint32_t*) nullptr, ci, pi, hgt, a.ix_out, a.val_out);
+            stage(s.R, s.mask, cj, pj, wid, a.ix_out, a.val_out);
+            stage_d(s.cs, s.cd, s.cb, pi);
+            wave_sync();
+            product(s.T, s.A, s.R, s.mask, s.cs, s.cd, s.cb, hgt, wid);
+            wave_sync();
+            unstage(s.A, (uint32_t*) nullptr, ci, pi, hgt);
+            unstage(s.R, s.mask, cj, pj, wid);
+            wave_sync();
+            ci = ni; cj = nj; pi = qi; pj = qj;
+        }
     }
 
-    // pivot tile U_ii (row-major) into L
-    const VT* piv = a.if_val + (int64_t) a.piv_loc[i] * 1024;
-    tile_zero(s.L);
-    wave_sync();
+    // pivot tile U_ii (row-major) into A; umask[r] = nonzero columns > r
+    const VT* piv = a.if_val + (int64_t) pq * 1024;
     for(int e = lane; e < hgt * hgt; e += 64) {
         const int r = e / hgt, c = e - r * hgt;
-        s.L[32 * r + c] = piv[e];
+        const VT v = piv[e];
+        s.A[32 * r + c] = v;
+        if(c > r && v != (VT) 0) atomicOr(&s.umask[r], 1u << c);
     }
     wave_sync();
 
-    const int j = lane;          // one column per lane, lanes >= 32 idle
-    const VT* U = s.L;
+    // column j per lane (lanes >= 32 idle): forward substitution with U_ii'
+    // (terms with a zero multiplier are exact no-ops and are skipped)
+    const int j = lane;
+    const VT* U = s.A;
     VT rhs[32];
 #pragma unroll
     for(int l = 0; l < 32; ++l) rhs[l] = (l < hgt && j < wid) ? s.T[32 * l + j] : (VT) 0;
 #pragma unroll
     for(int ii = 0; ii < 32; ++ii) {
         if(ii < hgt) {
-            const VT pv = rhs[ii] / U[32 * ii + ii];
+            const uint32_t um = s.umask[ii];
+            if(rhs[ii] != (VT) 0 && um) {
+                const VT pv = rhs[ii] / U[32 * ii + ii];
 #pragma unroll
-            for(int l = ii + 1; l < 32; ++l)
-                if(l < hgt) rhs[l] = fma(-U[32 * ii + l], pv, rhs[l]);
+                for(int l = ii + 1; l < 32; ++l)
+                    if((um >> l) & 1u) rhs[l] = fma(-U[32 * ii + l], pv, rhs[l]);
+            }
         }
     }
     const double* D = a.tri + 3 * (int64_t) fr;
     const int32_t* ip = a.is_piv + fr;
     wave_sync();
+    if(all1x1) {
+        // 1x1 pivots: the Givens-form solve reduces exactly to rhs / d
 #pragma unroll
-    for(int l = 0; l < 32; ++l) {
-        if(l < hgt) {
-            const bool p1 = ip[l] != 0;
-            const bool p2 = (l + 1 < hgt) && !ip[l + 1];
-            const double rl = (double) rhs[l];
-            const double right = (l < 31) ? (double) rhs[l + 1] : (double) rhs[31];
-            const double d11 = D[3 * l + 1];
-            const double d12 = D[3 * l + 2];
-            double d22 = (l + 1 < hgt) ? D[3 * (l + 1) + 1] : 0.0;
-            if(l == 31) d22 = d11;
-            const double r = sqrt(fma(d11, d11, d12 * d12));
-            const double c = d11 / r, sn = d12 / r;
-            double x1 = fma(c, rl, sn * right);
-            double x2 = fma(-sn, rl, c * right);
-            const double t11 = fma(c, d11, sn * d12);
-            const double t12 = fma(c, d12, sn * d22);
-            const double t22 = fma(-sn, d12, c * d22);
-            x2 = p2 ? (x2 / t22) : 0.0;
-            x1 = fma(-t12, x2, x1) / t11;
-            if(j < wid) {
-                if(p1) s.T[32 * l + j] = (VT) x1;
-                if(p2) s.T[32 * (l + 1) + j] = (VT) x2;
+        for(int l = 0; l < 32; ++l)
+            if(l < hgt && j < wid) s.T[32 * l + j] = (VT) ((double) rhs[l] / D[3 * l + 1]);
+    } else {
+#pragma unroll
+        for(int l = 0; l < 32; ++l) {
+            if(l < hgt) {
+                const bool p1 = ip[l] != 0;
+                const bool p2 = (l + 1 < hgt) && !ip[l + 1];
+                const double rl = (double) rhs[l];
+                const double right = (l < 31) ? (double) rhs[l + 1] : (double) rhs[31];
+                const double d11 = D[3 * l + 1];
+                const double d12 = D[3 * l + 2];
+                double d22 = (l + 1 < hgt) ? D[3 * (l + 1) + 1] : 0.0;
+                if(l == 31) d22 = d11;
+                const double r = sqrt(fma(d11, d11, d12 * d12));
+                const double c = d11 / r, sn = d12 / r;
+                double x1 = fma(c, rl, sn * right);
+                double x2 = fma(-sn, rl, c * right);
+                const double t11 = fma(c, d11, sn * d12);
+                const double t12 = fma(c, d12, sn * d22);
+                const double t22 = fma(-sn, d12, c * d22);
+                x2 = p2 ? (x2 / t22) : 0.0;
+                x1 = fma(-t12, x2, x1) / t11;
+                if(j < wid) {
+                    if(p1) s.T[32 * l + j] = (VT) x1;
+                    if(p2) s.T[32 * (l + 1) + j] = (VT) x2;
+                }
             }
         }
     }
@@ -356,13 +539,13 @@ __global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
             const int idx = 64 * t + lane, r = idx >> 5, c = idx & 31;
             const VT v = s.T[idx];
             const bool pred = r < hgt && c < wid && fabs((double) v) > NZ_EPS;
-            const uint64_t mask = wave_ballot(pred);
+            const uint64_t msk = wave_ballot(pred);
             if(pred) {
-                const int pos = base + lanes_below(mask);
+                const int pos = base + lanes_below(msk);
                 oix[pos] = (uint16_t) idx;
                 oval[pos] = v;
             }
-            base += __popcll(mask);
+            base += __popcll(msk);
         }
         if(lane == 0) a.if_cnt[q] = base;
     }
@@ -371,123 +554,183 @@ __global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
 template<typename VT>
 __device__ __forceinline__ VT drop_norm(VT rn) { return (VT) sqrtf((float) rn); }
 
-// row norms (fixed order: pivot, then off-diagonal blocks by id) and the
-// count of entries surviving the threshold drop (kernels :1919-1946)
+// row sums of squares: pivot block first, then the off-diagonal blocks of
+// the row by block id (contiguous in the level list at row_first_block_offset)
 template<typename VT>
-__global__ __launch_bounds__(64) void k_rowdrop(FactorArgs<VT> a, LevelRange lr)
+__global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
 {
-    __shared__ VT nrm[32];
     const int lane = lane_id();
     const int b = a.lvlb_idx[lr.lo + blockIdx.x];
     const int i = a.bin[b].block_row;
     const int fr = a.bs[i];
     const int h = a.bs[i + 1] - fr;
-    if(lane < h) {
-        VT acc = a.rn[fr + lane];
-        for(int bb = a.csr_off[i] + 1; bb < a.csr_off[i + 1]; ++bb)
-            acc += a.rn_part[(int64_t) (a.block_in_lvl[bb] - lr.lo) * 32 + lane];
-        a.rn[fr + lane] = acc;
-        nrm[lane] = drop_norm(acc);
-    }
-    wave_sync();
-    for(int bb = a.csr_off[i]; bb < a.csr_off[i + 1]; ++bb) {
-        if(a.bin[bb].format != FMT_SPARSE) continue;
-        const int qb = a.block_in_lvl[bb] - lr.lo;
-        const int cnt = a.if_cnt[qb];
-        const VT* vals = a.if_val + (int64_t) qb * 1024;
-        const uint16_t* ixs = a.if_idx + (int64_t) qb * 1024;
-        int keep = 0;
-        for(int e0 = 0; e0 < cnt; e0 += 64) {
-            const int e = e0 + lane;
-            bool k = false;
-            if(e < cnt) {
-                const VT v = vals[e];
-                k = !(fabs(v) < a.threshold * nrm[ixs[e] >> 5]);
-            }
-            keep += __popcll(wave_ballot(k));
+    const int nblk = a.csr_off[i + 1] - a.csr_off[i] - 1;
+    if(lane >= h) return;
+    VT acc = a.rn[fr + lane];
+    if(nblk > 0) {
+        const VT* part = a.rn_part + (int64_t) (lr.npiv + a.rfo[i]) * 32 + lane;
+        int t = 0;
+        for(; t + 4 <= nblk; t += 4) {
+            const VT v0 = part[32 * (t + 0)], v1 = part[32 * (t + 1)];
+            const VT v2 = part[32 * (t + 2)], v3 = part[32 * (t + 3)];
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
         }
-        if(lane == 0) a.drop_cnt[qb] = keep;
+        for(; t < nblk; ++t) acc += part[32 * t];
     }
+    a.rn[fr + lane] = acc;
+}
+
+// entries of each sparse block that survive |v| >= tau * sqrtf(norm)
+// (kernels :1919-1946)
+template<typename VT>
+__global__ __launch_bounds__(256) void k_dropcount(FactorArgs<VT> a, LevelRange lr)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= lr.size) return;
+    const BlockMeta m = a.bin[a.lvlb_idx[lr.lo + q]];
+    if(m.format != FMT_SPARSE) return;
+    const int fr = a.bs[m.block_row];
+    const int cnt = a.if_cnt[q];
+    const VT* vals = a.if_val + (int64_t) q * 1024;
+    const uint16_t* ixs = a.if_idx + (int64_t) q * 1024;
+    int keep = 0;
+    for(int e = 0; e < cnt; ++e) {
+        const VT v = vals[e];
+        const VT nrm = drop_norm(a.rn[fr + (ixs[e] >> 5)]);
+        keep += !(fabs(v) < a.threshold * nrm);
+    }
+    a.drop_cnt[q] = keep;
 }
 
 // leftover pool, granted extra space and output offsets, in level-list
 // order (kernels :1948-2008 serialised). Pool after a sparse block:
 // max(pool + max_nnz - nnz, 0); composition of such maps is a scan.
-__global__ __launch_bounds__(1024) void k_scan_sizes(const BlockMeta* bin, BlockMeta* bout,
+constexpr int SCAN_TPB = 1024;
+constexpr int SCAN_IPT = 8;
+
+__device__ __forceinline__ void map_compose(int64_t& A, int64_t& B, int64_t a, int64_t b)
+{
+    // first (A, B), then (a, b):  p -> max(max(p + A, B) + a, b)
+    A = A + a;
+    const int64_t nb = B + a;
+    B = nb > b ? nb : b;
+}
+
+__global__ __launch_bounds__(SCAN_TPB) void k_scan_sizes(const BlockMeta* bin, BlockMeta* bout,
     const int32_t* lvlb_idx, const int32_t* bs, const int32_t* drop_cnt, uint8_t* overflow,
     int32_t* carry, LevelRange lr)
 {
-    __shared__ int64_t sa[1024], sb[1024], sx[1024], sv[1024];
+    __shared__ int64_t sa[SCAN_TPB], sb[SCAN_TPB];
     const int tid = threadIdx.x;
     int64_t pool = carry[0];
     int64_t oix = carry[1], oval = carry[2];
-    for(int c0 = 0; c0 < lr.size; c0 += 1024) {
-        const int q = c0 + tid;
-        BlockMeta m{};
-        int32_t cnt = 0, h = 0, w = 0;
-        bool sparse = false;
-        int64_t fa = 0, fb = 0;
-        if(q < lr.size) {
-            m = bin[lvlb_idx[lr.lo + q]];
-            h = bs[m.block_row + 1] - bs[m.block_row];
-            w = bs[m.block_col + 1] - bs[m.block_col];
-            sparse = (m.format == FMT_SPARSE);
-            if(sparse) { cnt = drop_cnt[q]; fa = (int64_t) m.max_nnz - cnt; }
+    for(int c0 = 0; c0 < lr.size; c0 += SCAN_TPB * SCAN_IPT) {
+        BlockMeta mm[SCAN_IPT];
+        int32_t cnt[SCAN_IPT];
+        int64_t A = 0, B = 0;
+#pragma unroll
+        for(int k = 0; k < SCAN_IPT; ++k) {
+            const int q = c0 + tid * SCAN_IPT + k;
+            cnt[k] = 0;
+            if(q < lr.size) {
+                mm[k] = bin[lvlb_idx[lr.lo + q]];
+                if(mm[k].format == FMT_SPARSE) {
+                    cnt[k] = drop_cnt[q];
+                    map_compose(A, B, (int64_t) mm[k].max_nnz - cnt[k], 0);
+                }
+            } else {
+                mm[k].format = -1;
+            }
         }
-        // inclusive scan of maps p -> max(p + a, b)
-        sa[tid] = fa; sb[tid] = fb;
+        // exclusive scan of the per-thread maps
+        sa[tid] = A; sb[tid] = B;
         __syncthreads();
-        for(int o = 1; o < 1024; o <<= 1) {
+        for(int o = 1; o < SCAN_TPB; o <<= 1) {
             int64_t a1 = 0, b1 = 0;
             const bool has = tid >= o;
             if(has) { a1 = sa[tid - o]; b1 = sb[tid - o]; }
             __syncthreads();
             if(has) {
-                const int64_t a2 = sa[tid], b2 = sb[tid];
-                sa[tid] = a1 + a2;
-                sb[tid] = (b1 + a2 > b2) ? (b1 + a2) : b2;
+                int64_t a2 = a1, b2 = b1;
+                map_compose(a2, b2, sa[tid], sb[tid]);
+                sa[tid] = a2;
+                sb[tid] = b2;
             }
             __syncthreads();
         }
-        int64_t before = pool;
+        int64_t pl = pool;
         if(tid > 0) {
             const int64_t v = pool + sa[tid - 1];
-            before = v > sb[tid - 1] ? v : sb[tid - 1];
+            pl = v > sb[tid - 1] ? v : sb[tid - 1];
         }
-        int32_t granted = 0, cap = m.max_nnz, keep = 0;
-        if(sparse && cnt > m.max_nnz) {
-            const int64_t e = (int64_t) cnt - m.max_nnz;
-            granted = (int32_t) (e < before ? e : before);
-        }
-        if(sparse) {
-            cap = m.max_nnz + granted;
-            keep = cnt < cap ? cnt : cap;
-        }
-        const int64_t nix = sparse ? keep : 0;
-        const int64_t nval = sparse ? keep : (int64_t) h * w;
-        sx[tid] = nix; sv[tid] = nval;
+        const int64_t tot_a = sa[SCAN_TPB - 1], tot_b = sb[SCAN_TPB - 1];
         __syncthreads();
-        for(int o = 1; o < 1024; o <<= 1) {
+        // per item: granted space and output sizes
+        int32_t granted[SCAN_IPT], keep[SCAN_IPT];
+        int64_t nix_t = 0, nval_t = 0;
+#pragma unroll
+        for(int k = 0; k < SCAN_IPT; ++k) {
+            granted[k] = 0;
+            keep[k] = 0;
+            if(mm[k].format == FMT_SPARSE) {
+                if(cnt[k] > mm[k].max_nnz) {
+                    const int64_t e = (int64_t) cnt[k] - mm[k].max_nnz;
+                    granted[k] = (int32_t) (e < pl ? e : pl);
+                }
+                const int32_t cap = mm[k].max_nnz + granted[k];
+                keep[k] = cnt[k] < cap ? cnt[k] : cap;
+                int64_t pa = 0, pb = 0;
+                map_compose(pa, pb, (int64_t) mm[k].max_nnz - cnt[k], 0);
+                const int64_t v = pl + pa;
+                pl = v > pb ? v : pb;
+                nix_t += keep[k];
+                nval_t += keep[k];
+            } else if(mm[k].format == FMT_DENSE) {
+                const int h = bs[mm[k].block_row + 1] - bs[mm[k].block_row];
+                const int w = bs[mm[k].block_col + 1] - bs[mm[k].block_col];
+                nval_t += (int64_t) h * w;
+            }
+        }
+        // exclusive scan of the output sizes
+        sa[tid] = nix_t; sb[tid] = nval_t;
+        __syncthreads();
+        for(int o = 1; o < SCAN_TPB; o <<= 1) {
             int64_t x1 = 0, v1 = 0;
-            if(tid >= o) { x1 = sx[tid - o]; v1 = sv[tid - o]; }
+            if(tid >= o) { x1 = sa[tid - o]; v1 = sb[tid - o]; }
             __syncthreads();
-            sx[tid] += x1; sv[tid] += v1;
+            sa[tid] += x1; sb[tid] += v1;
             __syncthreads();
         }
-        if(q < lr.size) {
-            BlockMeta o = m;
-            o.nnz = sparse ? keep : 0;
-            o.max_nnz = cap;
-            if(sparse && cnt > m.max_nnz) o.got_extra = granted;
-            o.col_ptr = (int32_t) (oix + sx[tid] - nix);
-            o.val_ptr = (int32_t) (oval + sv[tid] - nval);
-            bout[m.id] = o;
-            overflow[q] = (uint8_t) (sparse && cnt > cap);
-        }
-        // carry to the next chunk: pool through all maps, offset totals
-        const int64_t tot_a = sa[1023], tot_b = sb[1023];
-        const int64_t tot_x = sx[1023], tot_v = sv[1023];
+        int64_t px = oix + sa[tid] - nix_t, pv = oval + sb[tid] - nval_t;
+        const int64_t tot_x = sa[SCAN_TPB - 1], tot_v = sb[SCAN_TPB - 1];
         __syncthreads();
+#pragma unroll
+        for(int k = 0; k < SCAN_IPT; ++k) {
+            const int q = c0 + tid * SCAN_IPT + k;
+            if(mm[k].format < 0) continue;
+            BlockMeta o = mm[k];
+            const bool sp = (o.format == FMT_SPARSE);
+            int64_t nv = keep[k];
+            if(!sp) {
+                const int h = bs[o.block_row + 1] - bs[o.block_row];
+                const int w = bs[o.block_col + 1] - bs[o.block_col];
+                nv = (int64_t) h * w;
+            }
+            o.nnz = sp ? keep[k] : 0;
+            if(sp) {
+                o.max_nnz = mm[k].max_nnz + granted[k];
+                if(cnt[k] > mm[k].max_nnz) o.got_extra = granted[k];
+            }
+            o.col_ptr = (int32_t) px;
+            o.val_ptr = (int32_t) pv;
+            px += sp ? keep[k] : 0;
+            pv += nv;
+            bout[o.id] = o;
+            overflow[q] = (uint8_t) (sp && cnt[k] > o.max_nnz);
+        }
         pool = (pool + tot_a > tot_b) ? (pool + tot_a) : tot_b;
         oix += tot_x;
         oval += tot_v;
@@ -615,8 +858,9 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
     hipStream_t st = (hipStream_t) stream;
     if(r.npiv > 0) k_pivots<VT><<<r.npiv, 64, 0, st>>>(a, r);
     if(r.size > r.npiv) k_offdiag<VT><<<r.size - r.npiv, 64, 0, st>>>(a, r);
-    if(r.npiv > 0) k_rowdrop<VT><<<r.npiv, 64, 0, st>>>(a, r);
-    k_scan_sizes<<<1, 1024, 0, st>>>(a.bin, a.bout, a.lvlb_idx, a.bs, a.drop_cnt, a.overflow,
+    if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
+    k_dropcount<VT><<<(r.size + 255) / 256, 256, 0, st>>>(a, r);
+    k_scan_sizes<<<1, SCAN_TPB, 0, st>>>(a.bin, a.bout, a.lvlb_idx, a.bs, a.drop_cnt, a.overflow,
         a.carry, r);
     k_store<VT><<<r.size, 64, 0, st>>>(a, r);
 }

@@ -24,6 +24,7 @@ struct FactorArgs {
     const int32_t* lvlb_idx = nullptr;    // level block list
     const int32_t* block_in_lvl = nullptr;
     const int32_t* lvl_rows = nullptr;
+    const int32_t* rfo = nullptr;         // row_first_block_offset
     const int32_t* ix_in = nullptr;       // input stores
     const VT* val_in = nullptr;
     int32_t* ix_out = nullptr;            // output stores

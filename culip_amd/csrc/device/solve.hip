@@ -182,47 +182,80 @@ __device__ __forceinline__ T seg_scan(T v, int sg)
     return v;
 }
 
-// one level: one wave per block row; y = x_i - sum(off-diag) ; x_i = Uinv y
+// task descriptors of the block rows in level order, two int4 each:
+// {fr, h, off-diagonal begin, end}, {inverse begin, end, 0, 0}
+__global__ void k_plan_tasks(int32_t nb, const int32_t* lvl_rows, const int32_t* bs,
+    const int32_t* o_ptr, const int32_t* i_ptr, int4* task)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= nb) return;
+    const int i = lvl_rows[t];
+    const int fr = bs[i], h = bs[i + 1] - fr;
+    task[2 * t] = make_int4(fr, h, o_ptr[fr], o_ptr[fr + h]);
+    task[2 * t + 1] = make_int4(i_ptr[fr], i_ptr[fr + h], 0, 0);
+}
+
+constexpr int TRSV_PF = 4;   // 64-entry chunks whose loads are issued together
+
+// one level: one wave per block row; y = x_i - sum(off-diag) ; x_i = Uinv y.
+// All entry loads of up to TRSV_PF chunks are issued before the x gathers;
+// the chunks' segmented sums are then applied to y in chunk order.
 template<typename VT>
-__global__ __launch_bounds__(64) void k_trsv_level(const int32_t* rows, const int32_t* bs,
-    const int32_t* o_ptr, const int32_t* o_col, const VT* o_val, const uint8_t* o_lr,
-    const int32_t* i_ptr, const int32_t* i_col, const VT* i_val, const uint8_t* i_lr,
-    double* x)
+__global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32_t* o_col,
+    const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
+    const uint8_t* i_lr, double* x)
 {
     __shared__ VT y[33];
     __shared__ VT res[33];
     const int lane = lane_id();
-    const int i = rows[blockIdx.x];
-    const int fr = bs[i];
-    const int h = bs[i + 1] - fr;
-    if(lane < h) { y[lane] = (VT) x[fr + lane]; res[lane] = 0; }
-    wave_sync();
-    const int s = o_ptr[fr], e_end = o_ptr[fr + h];
-    for(int e0 = s; e0 < e_end; e0 += 64) {
-        const int e = e0 + lane;
-        VT v = 0;
-        int sg = 64;
-        if(e < e_end) {
-            v = o_val[e] * (VT) x[o_col[e]];
-            sg = o_lr[e];
-        }
-        v = seg_scan(v, sg);
-        const int nsg = __shfl_down(sg, 1, 64);
-        if(sg < 64 && (lane == 63 || nsg != sg)) y[sg] -= v;
-        wave_sync();
+    const int4 t0 = task[2 * blockIdx.x];
+    const int4 t1 = task[2 * blockIdx.x + 1];
+    const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
+    const double xo = (lane < h) ? x[fr + lane] : 0.0;
+    int icol = fr;
+    VT ival = 0;
+    int isg = 64;
+    if(ib + lane < ie) {
+        icol = i_col[ib + lane];
+        ival = i_val[ib + lane];
+        isg = i_lr[ib + lane];
     }
-    const int s2 = i_ptr[fr], e2 = i_ptr[fr + h];
-    for(int e0 = s2; e0 < e2; e0 += 64) {
-        const int e = e0 + lane;
-        VT v = 0;
-        int sg = 64;
-        if(e < e2) {
-            v = i_val[e] * y[i_col[e] - fr];
-            sg = i_lr[e];
+    if(lane < h) { y[lane] = (VT) xo; res[lane] = 0; }
+    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
+        int c[TRSV_PF], sg[TRSV_PF];
+        VT v[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            const int e = e0 + 64 * k + lane;
+            c[k] = 0;
+            v[k] = 0;
+            sg[k] = 64;
+            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
         }
-        v = seg_scan(v, sg);
-        const int nsg = __shfl_down(sg, 1, 64);
-        if(sg < 64 && (lane == 63 || nsg != sg)) res[sg] += v;
+        VT xv[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? (VT) x[c[k]] : (VT) 0;
+        wave_sync();
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            if(e0 + 64 * k >= oe) break;
+            VT p = seg_scan(v[k] * xv[k], sg[k]);
+            const int nsg = __shfl_down(sg[k], 1, 64);
+            if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
+            wave_sync();
+        }
+    }
+    wave_sync();
+    for(int e0 = ib; e0 < ie; e0 += 64) {
+        if(e0 != ib) {
+            const int e = e0 + lane;
+            isg = 64;
+            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+        }
+        VT p = (isg < 64) ? ival * y[icol - fr] : (VT) 0;
+        p = seg_scan(p, isg);
+        const int nsg = __shfl_down(isg, 1, 64);
+        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
         wave_sync();
     }
     if(lane < h) x[fr + lane] = (double) res[lane];
@@ -269,6 +302,9 @@ void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT*, SolvePlan
     hipStream_t st = (hipStream_t) stream;
     k_plan_rows<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
     k_plan_cols<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
+    const unsigned g = (unsigned) ((t.nb + 255) / 256);
+    k_plan_tasks<<<g, 256, 0, st>>>(t.nb, t.lvl_rows, t.bs, p.uo_ptr, p.ui_ptr, p.task_u);
+    k_plan_tasks<<<g, 256, 0, st>>>(t.nb, t.lvl_rows, t.bs, p.to_ptr, p.ti_ptr, p.task_t);
 }
 
 void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, void* stream)
@@ -283,8 +319,8 @@ void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* str
     for(int l = t.num_levels - 1; l >= 0; --l) {
         const int lo = t.lvl_off_h[l], cnt = t.lvl_off_h[l + 1] - lo;
         if(cnt > 0)
-            k_trsv_level<VT><<<cnt, 64, 0, st>>>(t.lvl_rows + lo, t.bs, p.uo_ptr, p.uo_col,
-                p.uo_val, p.uo_lr, p.ui_ptr, p.ui_col, p.ui_val, p.ui_lr, x);
+            k_trsv_level<VT><<<cnt, 64, 0, st>>>(p.task_u + 2 * lo, p.uo_col, p.uo_val, p.uo_lr,
+                p.ui_col, p.ui_val, p.ui_lr, x);
     }
 }
 
@@ -295,8 +331,8 @@ void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* s
     for(int l = 0; l < t.num_levels; ++l) {
         const int lo = t.lvl_off_h[l], cnt = t.lvl_off_h[l + 1] - lo;
         if(cnt > 0)
-            k_trsv_level<VT><<<cnt, 64, 0, st>>>(t.lvl_rows + lo, t.bs, p.to_ptr, p.to_col,
-                p.to_val, p.to_lr, p.ti_ptr, p.ti_col, p.ti_val, p.ti_lr, x);
+            k_trsv_level<VT><<<cnt, 64, 0, st>>>(p.task_t + 2 * lo, p.to_col, p.to_val, p.to_lr,
+                p.ti_col, p.ti_val, p.ti_lr, x);
     }
 }
 

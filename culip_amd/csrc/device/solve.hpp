@@ -1,6 +1,8 @@
 // Block triangular solves with the BILDLt factor on the device.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "factor.hpp"
@@ -19,6 +21,8 @@ struct SolvePlan {
     int32_t* ui_ptr; int32_t* ui_col; VT* ui_val; uint8_t* ui_lr;
     int32_t* to_ptr; int32_t* to_col; VT* to_val; uint8_t* to_lr;
     int32_t* ti_ptr; int32_t* ti_col; VT* ti_val; uint8_t* ti_lr;
+    int4* task_u;   // per block row in level order: {fr, h, uo range}, {ui range}
+    int4* task_t;   // the same for the U' solve (to, ti)
 };
 
 struct SolveTopo {

@@ -458,6 +458,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     HIPCHK(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
     int32_t it = 0;
     const int period = opt.true_residual_period;
+    double prev_nr = -1.0;
     while(it < opt.max_it) {
         const int k0 = it % 3, k1 = (it + 1) % 3, km1 = (it + 2) % 3, km2 = (it + 1) % 3;
         k_alpha<<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
@@ -475,14 +476,24 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
         cur = nxt;
         HIPCHK(hipGetLastError());
 
-        if(period > 0 && (it + 1) % period == 0) {
-            // true solution and residual of the current iterate
-            k_dsolve<<<nb, 256, 0, st>>>(n, w.xt, xout, ops.tri, ops.is_piv);
-            if(!ops.tri) k_copy<<<nb, 256, 0, st>>>(n, w.xt, xout);
-            if(ops.right) ops.right(xout, st);
-            double nr, nbn;
-            residual_norms(ops, w, b, xout, st, nr, nbn);
-            if(nr <= gate_abs) { gate_met = true; ++it; break; }
+        if(period > 0) {
+            // true-residual gate: the recurrence runs on past the reference's
+            // quasi-residual test; it stops when the FP64 true residual meets
+            // the tolerance, and hands over to a refinement restart on a
+            // Lanczos breakdown or when the true residual stagnates
+            if((it + 1) % period == 0) {
+                k_dsolve<<<nb, 256, 0, st>>>(n, w.xt, xout, ops.tri, ops.is_piv);
+                if(ops.right) ops.right(xout, st);
+                double nr, nbn;
+                residual_norms(ops, w, b, xout, st, nr, nbn);
+                if(nr <= gate_abs) { gate_met = true; ++it; break; }
+                HIPCHK(hipMemcpy(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+                const bool stalled = prev_nr > 0 && nr > 0.9 * prev_nr;
+                prev_nr = nr;
+                if(w.host_flags[0] || stalled) { ++it; break; }
+            }
+            ++it;
+            continue;
         }
         // throttled convergence check (:309-331)
         if(it > 0 && (it % 4) == 0) {
@@ -534,7 +545,9 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
     residual_norms(ops, w, b, x, st, nr, nbn);
     double rr = nbn > 0 ? nr / nbn : nr;
     // iterative refinement: solve A d = r with a fresh recurrence, x += d,
-    // while the FP64 true residual misses the tolerance
+    // while the FP64 true residual misses the tolerance. A correction that
+    // does not lower the true residual is undone and ends the refinement,
+    // so x is never worse than the best iterate seen.
     int restarts = 0;
     while(rr > opt.tol && restarts < opt.max_restarts && total_it < opt.max_it) {
         k_copy<<<nb, 256, 0, st>>>(n, x, w.xacc);
@@ -546,9 +559,15 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
         sqmr_run(ops, w, w.rhs, x, o2, r2, st, total_it, g2, gate_abs);
         k_axpy1<<<nb, 256, 0, st>>>(n, w.xacc, x);
         residual_norms(ops, w, b, x, st, nr, nbn);
-        rr = nbn > 0 ? nr / nbn : nr;
-        res.residual = r2.residual;
+        const double rr_new = nbn > 0 ? nr / nbn : nr;
         ++restarts;
+        if(!(rr_new < rr)) {
+            k_copy<<<nb, 256, 0, st>>>(n, w.xacc, x);
+            HIPCHK(hipStreamSynchronize(st));
+            break;
+        }
+        rr = rr_new;
+        res.residual = r2.residual;
     }
     res.iterations = total_it;
     res.restarts = restarts;

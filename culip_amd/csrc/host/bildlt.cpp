@@ -37,6 +37,7 @@ void Bildlt::upload_pattern()
     upload(d_lvlb_idx, s.lvlb_idx);
     upload(d_block_in_lvl, s.block_in_lvl);
     upload(d_lvl_rows, s.lvl_rows);
+    upload(d_rfo, s.row_first_block_offset);
     upload(d_ix_in, s.ix_store);
     upload(d_value_slot, s.value_slot);
 
@@ -58,6 +59,7 @@ void Bildlt::upload_pattern()
     d_piv_loc.ensure(sizeof(int32_t) * s.nb);
     d_carry.ensure(sizeof(int32_t) * 4);
     for(int q = 0; q < 4; ++q) d_ptr[q].ensure(sizeof(int32_t) * (size_t) (n + 1));
+    for(int q = 0; q < 2; ++q) d_task[q].ensure(sizeof(int32_t) * 8 * (size_t) s.nb);
     d_totals.ensure(sizeof(int32_t) * 4);
     uploaded = true;
 }
@@ -81,6 +83,7 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.lvlb_idx = d_lvlb_idx.as<int32_t>();
     a.block_in_lvl = d_block_in_lvl.as<int32_t>();
     a.lvl_rows = d_lvl_rows.as<int32_t>();
+    a.rfo = d_rfo.as<int32_t>();
     a.ix_in = d_ix_in.as<int32_t>();
     a.val_in = d_val_in.as<VT>();
     a.ix_out = d_ix_out.as<int32_t>();
@@ -113,6 +116,8 @@ SolvePlan<VT> Bildlt::plan() const
         *vals[q] = d_val[q].as<VT>();
         *lrs[q] = d_lr[q].as<uint8_t>();
     }
+    p.task_u = d_task[0].as<int4>();
+    p.task_t = d_task[1].as<int4>();
     return p;
 }
 
@@ -232,7 +237,7 @@ int64_t Bildlt::device_bytes() const
             &d_upd_ki, &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in,
             &d_value_slot, &d_vals, &d_val_in, &d_bout, &d_ix_out, &d_val_out, &d_tri, &d_rn,
             &d_if_val, &d_if_idx, &d_if_cnt, &d_rn_part, &d_drop_cnt, &d_overflow, &d_piv_loc,
-            &d_carry, &d_totals})
+            &d_carry, &d_totals, &d_task[0], &d_task[1]})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
         b += (int64_t) (d_ptr[q].bytes + d_col[q].bytes + d_val[q].bytes + d_lr[q].bytes);

@@ -9,6 +9,7 @@
 //   2. H off-diagonal pairs: repeat {i, j, value} until n*o/2 distinct pairs
 //   3. A: for each column j: c distinct rows, then c values
 //   4. rhs, N(0,1) for all n + m rows
+//   5. barrier weights (if requested), i = 0..n-1
 #include "kkt.hpp"
 
 #include <algorithm>
@@ -144,6 +145,14 @@ Kkt* kkt_generate(const culip_kkt_params& p)
     for(int32_t i = 0; i < n; ++i)
         for(int32_t q = k->K.row_ptr[i]; q < k->K.row_ptr[i + 1]; ++q)
             if(k->K.col[q] == i) k->diag_pos[i] = q;
+    k->h0 = hd;
+    // 5. barrier term
+    if(p.barrier_hi > 0) {
+        CULIP_REQUIRE(p.barrier_lo > 0 && p.barrier_hi >= p.barrier_lo,
+            "kkt: barrier range must be positive");
+        const double a = std::log(p.barrier_lo), b = std::log(p.barrier_hi);
+        for(int32_t i = 0; i < n; ++i) k->K.val[k->diag_pos[i]] += std::exp(a + rng.u01() * (b - a));
+    }
     return k;
 }
 
@@ -152,10 +161,8 @@ void kkt_barrier_values(const Kkt& k, double wlo, double whi, uint64_t seed, dou
     CULIP_REQUIRE(wlo > 0 && whi >= wlo, "barrier weights must be positive");
     std::memcpy(out, k.K.val.data(), sizeof(double) * k.K.val.size());
     Rng rng(seed);
-    for(int32_t i = 0; i < k.p.n; ++i) {
-        const double w = std::exp(std::log(wlo) + rng.u01() * (std::log(whi) - std::log(wlo)));
-        out[k.diag_pos[i]] *= w;
-    }
+    const double a = std::log(wlo), b = std::log(whi);
+    for(int32_t i = 0; i < k.p.n; ++i) out[k.diag_pos[i]] = k.h0[i] + std::exp(a + rng.u01() * (b - a));
 }
 
 // ---------------------------------------------------------------------------

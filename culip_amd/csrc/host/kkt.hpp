@@ -10,6 +10,7 @@ struct Kkt {
     HostCsr K;                       // full symmetric, sorted columns
     std::vector<double> rhs;
     std::vector<int64_t> diag_pos;   // entry of H_ii in K
+    std::vector<double> h0;          // H diagonal before any barrier term
 };
 
 struct Prep {

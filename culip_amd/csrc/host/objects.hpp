@@ -63,12 +63,12 @@ struct Bildlt {
 
     // static (per pattern)
     DevBuf d_bs, d_is_piv, d_bin, d_csr_off, d_csc_off, d_csc_idx, d_upd_off, d_upd_ki, d_upd_kj,
-        d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot;
+        d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot, d_rfo;
     // numeric
     DevBuf d_vals, d_val_in, d_bout, d_ix_out, d_val_out, d_tri, d_rn;
     DevBuf d_if_val, d_if_idx, d_if_cnt, d_rn_part, d_drop_cnt, d_overflow, d_piv_loc, d_carry;
     // solve layout
-    DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4];
+    DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
     DevBuf d_totals;
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
 

@@ -66,15 +66,20 @@ typedef struct {
     double h_offdiag_std;       /* std of H off-diagonal values */
     double delta;               /* K(2,2) = -delta I */
     uint64_t seed;
+    /* Barrier term: barrier_hi > 0 adds w_i, log-uniform in
+     * [barrier_lo, barrier_hi], to H_ii (H = H0 + X^-1 Z of an interior-
+     * point method), drawn after the right-hand side. 0: none. */
+    double barrier_lo, barrier_hi;
 } culip_kkt_params;
 
 typedef struct culip_kkt culip_kkt;
 culip_status culip_kkt_generate(const culip_kkt_params * p, culip_kkt ** out);
 /* Full symmetric CSR of K (sorted columns) and the N(0,1) right-hand side. */
 culip_status culip_kkt_view(const culip_kkt * k, culip_csr * K, const double ** rhs);
-/* Serving refactor input: H_ii <- H_ii * w_i, w_i log-uniform in
- * [w_lo, w_hi] drawn from `seed`; writes the new values of K (same pattern)
- * into vals_out (nnz doubles). */
+/* Serving refactor input: fresh barrier weights, H_ii <- H0_ii + w_i with
+ * w_i log-uniform in [w_lo, w_hi] drawn from `seed` (replacing any barrier
+ * term of the generated system); writes the new values of K (same
+ * pattern) into vals_out (nnz doubles). */
 culip_status culip_kkt_barrier_values(const culip_kkt * k, double w_lo,
     double w_hi, uint64_t seed, double * vals_out);
 void culip_kkt_destroy(culip_kkt * k);
@@ -146,7 +151,16 @@ culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
 int32_t culip_bildlt_n(const culip_bildlt * h);
 
 /* Host views of the symbolic structures and (after compute) of the factor
- * downloaded from the device. Names: see DESIGN of the symbolic dump. */
+ * downloaded from the device; valid until the next call on h.
+ * int names: block_starts, is_piv, coarse_csr_off, coarse_csr_col,
+ *   coarse_csc_off, coarse_csc_idx, a_block_nnz, lvl_off, lvl_rows,
+ *   row_level, lvlb_off, lvlb_idx, block_in_lvl, row_first_block_offset,
+ *   solve_off, solve_idx, blocks (9 int32 per block: id, format, nnz,
+ *   max_nnz, got_extra, block_row, block_col, col_ptr, val_ptr), ix_store,
+ *   upd_off, upd_ki, upd_kj, scalars, and after compute fblocks, fix_store,
+ *   carry, plan_sizes.
+ * real names: val_store; after compute fval_store, tri (3 per row:
+ *   D(i-1,i), D(i,i), D(i+1,i)), row_norms. */
 culip_status culip_bildlt_get_int(culip_bildlt * h, const char * name,
     const int32_t ** ptr, int64_t * len);
 culip_status culip_bildlt_get_real(culip_bildlt * h, const char * name,

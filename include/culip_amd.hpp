@@ -1,0 +1,176 @@
+/*
+ * C++ classes over the C-ABI of culip_amd.h with the names, argument
+ * meaning and call order of the reference's C++ interface for the BILDLt +
+ * SQMR path, so a caller of the reference changes types and includes, not
+ * its call sequence:
+ *
+ *   BlockiLDLt<T>     libs/la/preconditioner/block_ildlt.cuh:188-311
+ *   CSRMatrixSPMV     libs/la/spmv.cuh:36-66
+ *   AugmentedMatrixSPMV libs/la/spmv.cuh:116-149
+ *   SQMR              libs/la/iterative_solvers/sqmr.cuh:21-72
+ *
+ * Differences: matrices are culip_csr views of host arrays (the reference's
+ * csr_matrix_t on the host), vectors are device arrays of doubles whatever T
+ * is (T selects the factor's storage and arithmetic), the GPU handle is a
+ * HIP stream, and failures throw culip_amd::Error instead of raising
+ * SIGSEGV (libs/utils/types.cuh:32).
+ */
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "culip_amd.h"
+
+namespace culip_amd {
+
+struct Error : std::runtime_error {
+    culip_status status;
+    Error(culip_status s, const char* what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(culip_status s)
+{
+    if(s != CULIP_OK) throw Error(s, culip_last_error());
+}
+
+/* Preconditioner<T> with solve_left/middle/right (preconditioner.cuh). */
+template<typename T>
+class BlockiLDLt {
+    static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+        "BlockiLDLt<T>: T is double or float");
+
+public:
+    /* block_ildlt.impl.cuh:44-79 */
+    BlockiLDLt(void* stream, const culip_csr* h_A, int32_t num_pivs, const int32_t* piv_starts,
+        int32_t num_blocks, const int32_t* block_starts)
+        : stream_(stream)
+    {
+        check(culip_bildlt_create(h_A, num_pivs, piv_starts, num_blocks, block_starts,
+            std::is_same<T, float>::value ? CULIP_PREC_F32 : CULIP_PREC_F64, &h_));
+    }
+    ~BlockiLDLt() { culip_bildlt_destroy(h_); }
+    BlockiLDLt(const BlockiLDLt&) = delete;
+    BlockiLDLt& operator=(const BlockiLDLt&) = delete;
+
+    int32_t n() const { return culip_bildlt_n(h_); }
+
+    /* block_ildlt.impl.cuh:103-146; includes solve_analysis(). The
+     * reference returns a Manteuffel shift that its static path leaves 0. */
+    T compute(int32_t fill_level, T fill_factor, T threshold)
+    {
+        check(culip_bildlt_compute(h_, fill_level, fill_factor, threshold, stream_));
+        return T(0);
+    }
+
+    /* New values of the same pattern (nnz doubles in h_A's entry order). */
+    void refactor(const double* vals, bool vals_on_device, T threshold)
+    {
+        check(culip_bildlt_refactor(h_, vals, vals_on_device ? 1 : 0, threshold, stream_));
+    }
+
+    /* solve_analysis() is part of compute(); kept so callers compile. */
+    void solve_analysis() {}
+
+    bool is_left() const { return true; }
+    bool is_middle() const { return true; }
+    bool is_right() const { return true; }
+
+    /* block_ildlt.impl.cuh:183-246; b, x: n device doubles. */
+    void solve_left(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_bildlt_solve(h_, CULIP_SOLVE_LEFT, transpose, b, x, stream_));
+    }
+    void solve_middle(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_bildlt_solve(h_, CULIP_SOLVE_MIDDLE, transpose, b, x, stream_));
+    }
+    void solve_right(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_bildlt_solve(h_, CULIP_SOLVE_RIGHT, transpose, b, x, stream_));
+    }
+
+    const culip_bildlt* handle() const { return h_; }
+    culip_bildlt* handle() { return h_; }
+
+private:
+    void* stream_;
+    culip_bildlt* h_ = nullptr;
+};
+
+/* SPMV<T> (spmv.cuh:18-31): y = alpha op(A) x + beta y on device doubles. */
+class SPMV {
+public:
+    virtual ~SPMV() { culip_spmv_destroy(op_); }
+    SPMV(const SPMV&) = delete;
+    SPMV& operator=(const SPMV&) = delete;
+    int32_t m() const { return culip_spmv_m(op_); }
+    int32_t n() const { return culip_spmv_n(op_); }
+    void multiply(const double* x, double* y, double alpha = 1.0, double beta = 0.0,
+        bool transpose = false) const
+    {
+        check(culip_spmv_multiply(op_, x, y, alpha, beta, transpose, stream_));
+    }
+    const culip_spmv* handle() const { return op_; }
+
+protected:
+    explicit SPMV(void* stream) : stream_(stream) {}
+    void* stream_;
+    culip_spmv* op_ = nullptr;
+};
+
+/* CSRMatrixSPMV (spmv.cuh:36-66); copies the host CSR to the device. */
+class CSRMatrixSPMV : public SPMV {
+public:
+    CSRMatrixSPMV(void* stream, const culip_csr* h_A) : SPMV(stream)
+    {
+        check(culip_spmv_create_csr(h_A, &op_));
+    }
+};
+
+/* AugmentedMatrixSPMV (spmv.cuh:116-149): [diag(d) A'; A delta I(ix)]. */
+class AugmentedMatrixSPMV : public SPMV {
+public:
+    AugmentedMatrixSPMV(void* stream, const culip_csr* h_A, const double* d, double delta,
+        int32_t n_ix = 0, const int32_t* ix = nullptr)
+        : SPMV(stream)
+    {
+        check(culip_spmv_create_kkt(h_A, d, delta, n_ix, ix, &op_));
+    }
+};
+
+/* SQMR<T> (sqmr.cuh:21-72). The reference's ctor takes (spmv, tol, max_it,
+ * preconditioner); true_residual_period = 0 keeps its stopping test. */
+class SQMR {
+public:
+    template<typename T>
+    SQMR(void* stream, const SPMV* spmv, double tol, int32_t max_it,
+        const BlockiLDLt<T>* prec = nullptr, int32_t true_residual_period = 0,
+        int32_t max_restarts = 0)
+        : stream_(stream)
+    {
+        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts};
+        check(culip_sqmr_create(spmv->handle(), prec ? prec->handle() : nullptr, &o, &s_));
+    }
+    ~SQMR() { culip_sqmr_destroy(s_); }
+    SQMR(const SQMR&) = delete;
+    SQMR& operator=(const SQMR&) = delete;
+
+    /* sqmr.impl.cuh:109-355: residual and iterations as the reference
+     * reports them; the full result (true residual, restarts) via last(). */
+    void solve(const double* b, double* x, double& residual, int32_t& iterations)
+    {
+        check(culip_sqmr_solve(s_, b, x, &last_, stream_));
+        residual = last_.residual;
+        iterations = last_.iterations;
+    }
+    const culip_sqmr_result& last() const { return last_; }
+
+private:
+    void* stream_;
+    culip_sqmr* s_ = nullptr;
+    culip_sqmr_result last_{};
+};
+
+}  // namespace culip_amd

@@ -63,16 +63,38 @@ def test_generator_is_deterministic_and_symmetric():
     assert abs(off.std() - 0.1) < 0.02
 
 
-def test_barrier_values_rescale_h_diagonal_only():
+def test_barrier_values_change_h_diagonal_only():
     k = ca.KKT(300, 100, 4, 3, seed=1)
     v = k.barrier_values(1e-4, 1e4, seed=3)
     K0 = sp.csr_matrix((k.val, k.col, k.row_ptr))
     K1 = sp.csr_matrix((v, k.col, k.row_ptr))
     D = (K1 - K0)
-    assert (D - sp.diags(D.diagonal())).nnz == 0 or abs(D - sp.diags(D.diagonal())).max() == 0
-    w = K1.diagonal()[:300] / K0.diagonal()[:300]
-    assert w.min() >= 1e-4 and w.max() <= 1e4
+    assert abs(D - sp.diags(D.diagonal())).max() == 0
+    w = K1.diagonal()[:300] - K0.diagonal()[:300]
+    assert w.min() >= 1e-4 * (1 - 1e-12) and w.max() <= 1e4 * (1 + 1e-12)
+    # log-uniform: about half the weights below the geometric midpoint 1
+    assert 0.4 < np.mean(w < 1.0) < 0.6
     assert np.array_equal(K1.diagonal()[300:], K0.diagonal()[300:])
+    # a second draw replaces the barrier term instead of compounding it
+    k2 = ca.KKT(300, 100, 4, 3, seed=1, barrier=(1e-6, 1e6))
+    v2 = k2.barrier_values(1e-4, 1e4, seed=3)
+    assert np.array_equal(v2, v)
+
+
+def test_generator_barrier_term():
+    k0 = ca.KKT(500, 200, 4, 3, seed=4, delta=1e-10)
+    k1 = ca.KKT(500, 200, 4, 3, seed=4, delta=1e-10, barrier=(1e-6, 1e6))
+    assert np.array_equal(k0.rhs, k1.rhs) and np.array_equal(k0.col, k1.col)
+    K0 = sp.csr_matrix((k0.val, k0.col, k0.row_ptr))
+    K1 = sp.csr_matrix((k1.val, k1.col, k1.row_ptr))
+    D = K1 - K0
+    w = D.diagonal()[:500]
+    assert abs(D - sp.diags(D.diagonal())).max() == 0 and not D.diagonal()[500:].any()
+    lw = np.log10(w)
+    assert lw.min() >= -6 - 1e-9 and lw.max() <= 6 + 1e-9
+    assert abs(lw.mean()) < 0.6 and 2.9 < lw.std() < 4.0   # U(-6, 6): std 3.46
+    # H = H0 + diag(w) stays positive definite
+    assert np.linalg.eigvalsh(K1[:500, :500].toarray()).min() > 0
 
 
 @pytest.mark.parametrize("order", [ca.ORDER_NATURAL, ca.ORDER_BLOCK_COLOR])
@@ -107,3 +129,16 @@ def test_matrix_market_reader(golden):
     assert all((np.diff(ci[rp[i]:rp[i + 1]]) >= 0).all() for i in range(62))
     p = ca.read_dense_vector(f"{golden}/sqmr_small_prec_p.mtx")
     assert sorted(p.astype(int)) == list(range(62))
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "kkt_driver")
+
+
+def test_cpp_driver_builds_against_the_class_header():
+    """tests/cpp/kkt_driver.cpp (the reference driver's call sequence over
+    include/culip_amd.hpp) is built with the library and links to it; with no
+    arguments it prints its usage without touching a device."""
+    import subprocess
+    assert os.path.exists(DRIVER), "run __graft_entry__.build() first"
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
