@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT,
-                   ORDER_NATURAL, ORDER_BLOCK_COLOR, check, lib)
+                   ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_FLAGGED, check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
            "SQMR", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
@@ -167,6 +167,11 @@ class BlockiLDLt:
             v = _f64(vals)
             check(lib().culip_bildlt_refactor(self._h, v.ctypes.data_as(C.c_void_p), 0,
                                               threshold, _stream(stream)))
+
+    def set_solve_algorithm(self, algorithm):
+        """'levels' (one launch per level) or 'flagged' (one launch per solve)."""
+        a = {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED}.get(algorithm, algorithm)
+        check(lib().culip_bildlt_set_solve_algorithm(self._h, a))
 
     def _solve(self, side, b, x, transpose, stream):
         check(lib().culip_bildlt_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),

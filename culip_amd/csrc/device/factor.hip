@@ -10,9 +10,10 @@
 //               a dense tile or as a compact (index, value) list of the
 //               entries with |v| > 1e-10
 //   k_rownorm   one wave per block row: row sums of squares in a fixed order
-//   k_dropcount one lane per sparse block: entries surviving the threshold
-//   k_scan_sizes one workgroup: leftover-space pool, granted extra space and
-//               output offsets for the level's blocks in level-list order
+//   k_level_maps, k_level_grant, k_level_write
+//               chunked scans in level-list order: entries surviving the
+//               threshold, leftover-space pool, granted extra space, output
+//               offsets and the factored metadata
 //   k_store     one wave per block: compacted final storage
 //
 // The dependency blocks of a tile are walked with their metadata fetched
@@ -86,15 +87,16 @@ __device__ __forceinline__ Dep load_dep(const BlockMeta* bout, const int32_t* bs
     return d;
 }
 
+// src is uniform across the wave: the result lives in scalar registers
 __device__ __forceinline__ Dep bcast(const Dep& d, int src)
 {
     Dep o;
-    o.fmt = wave_bcast(d.fmt, src);
-    o.nnz = wave_bcast(d.nnz, src);
-    o.cp = wave_bcast(d.cp, src);
-    o.vp = wave_bcast(d.vp, src);
-    o.rb = wave_bcast(d.rb, src);
-    o.rk = wave_bcast(d.rk, src);
+    o.fmt = __builtin_amdgcn_readlane(d.fmt, src);
+    o.nnz = __builtin_amdgcn_readlane(d.nnz, src);
+    o.cp = __builtin_amdgcn_readlane(d.cp, src);
+    o.vp = __builtin_amdgcn_readlane(d.vp, src);
+    o.rb = __builtin_amdgcn_readlane(d.rb, src);
+    o.rk = __builtin_amdgcn_readlane(d.rk, src);
     return o;
 }
 
@@ -390,7 +392,14 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
         else step_2x2(s.T, tri, p, len);
         p += order;
     }
-    if(lane < len) a.rn[fr + lane] = row_ssq(s.T, lane, len);
+    if(lane < len) {
+        a.rn[fr + lane] = row_ssq(s.T, lane, len);
+        // nonzero columns right of the diagonal, for the U_ii' solves
+        uint32_t um = 0;
+        for(int c = lane + 1; c < len; ++c)
+            if(s.T[32 * lane + c] != (VT) 0) um |= 1u << c;
+        a.piv_umask[(int64_t) q * 32 + lane] = um;
+    }
     VT* out = a.if_val + (int64_t) q * 1024;
     for(int e = lane; e < len * len; e += 64) {
         const int r = e / len, c = e - r * len;
@@ -403,13 +412,28 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
 }
 
 // k_update_scale_offdiagonal_blocks (kernels :1628-1803) with
-// d_left_solve_with_D_Ut (:1489-1570)
+// d_left_solve_with_D_Ut (:1489-1570): general form, all dependency blocks
+// staged as tiles; takes the positions k_offdiag leaves to it.
 template<typename VT>
-__global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+__device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& lr, int q,
+    OffLds<VT>& s);
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_offdiag_general(FactorArgs<VT> a, LevelRange lr)
 {
     __shared__ OffLds<VT> s;
+    const int ndef = a.defer_cnt[lr.level];
+    for(int t = blockIdx.x; t < ndef; t += gridDim.x) {
+        offdiag_general_one(a, lr, a.defer_list[t], s);
+        __syncthreads();
+    }
+}
+
+template<typename VT>
+__device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& lr, int q,
+    OffLds<VT>& s)
+{
     const int lane = lane_id();
-    const int q = lr.npiv + blockIdx.x;
     const int b = a.lvlb_idx[lr.lo + q];
     const BlockMeta m = a.bin[b];
     const int i = m.block_row, jb = m.block_col;
@@ -551,6 +575,253 @@ __global__ __launch_bounds__(64) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Off-diagonal blocks whose right factors U_kj are sparse with at most 64
+// entries (every block at level-0 fill of the KKT workloads): the output
+// tile lives in registers (lane j + 32 c0 holds rows c0, c0 + 2, ... of
+// column j), U_ki is staged in the one LDS tile, and U_kj becomes a list of
+// its nonzero entries ordered by (column, row), so a wave needs about 10 KB
+// of LDS instead of 25 KB. Arithmetic and order are those of the general
+// form. A block with a larger right factor is handed to k_offdiag_general.
+
+template<typename VT>
+struct alignas(16) OffLdsSmall {
+    VT X[1024];
+    VT cs[32], cd[32], cb[32];
+    VT rv[64];
+    int32_t cst[33];
+    uint8_t rl[64];
+};
+
+__device__ __forceinline__ Dep dep_meta(const BlockMeta* bout, int blk, int rb, int rk)
+{
+    const BlockMeta m = bout[blk];
+    return Dep{m.format, m.nnz, m.col_ptr, m.val_ptr, rb, rk};
+}
+
+// nonzero entries of a small sparse block (width w, one per lane) sorted by
+// (column, row) into rl / rv; cst[c] = entries in columns < c, c = 0..32
+template<typename VT>
+__device__ void list_build(VT* rv, uint8_t* rl, int32_t* cst, const Dep& d, const Pre<VT>& p,
+    int w)
+{
+    const int lane = lane_id();
+    constexpr int NONE = 0x7fffffff;
+    int key = NONE, r = 0;
+    if(lane < d.nnz && p.v != (VT) 0) {
+        r = p.ix / w;
+        key = (p.ix - r * w) * 32 + r;
+    }
+    int rank = 0, below = 0;
+    for(int f = 0; f < d.nnz; ++f) {
+        const int kf = wave_bcast(key, f);
+        rank += (kf < key);
+        below += (kf != NONE) && ((kf >> 5) < lane);
+    }
+    if(key != NONE) {
+        rl[rank] = (uint8_t) r;
+        rv[rank] = p.v;
+    }
+    if(lane <= 32) cst[lane] = below;
+}
+
+// T[c][j] -= sum_l R[l][j] * Lt[c][l] over the listed rows l of column j,
+// ascending, Lt as in product()
+template<typename VT>
+__device__ void product_list(VT (&T)[16], const VT* A, const VT* rv, const uint8_t* rl,
+    const int32_t* cst, const VT* cs, const VT* cd, const VT* cb, int lh, int rh)
+{
+    const int lane = lane_id();
+    const int j = lane & 31;
+    const int c0 = lane >> 5;
+    const int k0 = (j < rh) ? cst[j] : 0;
+    const int k1 = (j < rh) ? cst[j + 1] : 0;
+    if(k0 < k1) {
+        // one output row at a time keeps a single accumulator live
+#pragma unroll
+        for(int t = 0; t < 16; ++t) {
+            const int c = c0 + 2 * t;
+            if(c < lh) {
+                VT acc = 0;
+                for(int k = k0; k < k1; ++k) {
+                    const int l = rl[k];
+                    const VT r = rv[k];
+                    const VT left = (l == 0) ? (VT) 0 : A[32 * (l - 1) + c];
+                    const VT mid = A[32 * l + c];
+                    const VT right = (l == 31) ? (VT) 0 : A[32 * (l + 1) + c];
+                    const VT lt = fma(cb[l], right, fma(cd[l], mid, cs[l] * left));
+                    acc = fma(r, lt, acc);
+                }
+                T[t] -= acc;
+            }
+        }
+    }
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ OffLdsSmall<VT> s;
+    const int lane = lane_id();
+    const int q = lr.npiv + blockIdx.x;
+    const int64_t p = (int64_t) lr.lo + q;
+    const int4 t0 = ((const int4*) a.off_task)[3 * p];
+    const int4 t1 = ((const int4*) a.off_task)[3 * p + 1];
+    const int4 t2 = ((const int4*) a.off_task)[3 * p + 2];
+    const int fr = t0.y;
+    const int hgt = t0.z & 255, wid = (t0.z >> 8) & 255;
+    const bool all1x1 = ((t0.z >> 16) & 1) != 0;
+    const int pq = t0.w;
+    const int np = t2.y;
+    const int4* pairs = (const int4*) a.upd4 + t2.x;
+    const int j = lane & 31, c0 = lane >> 5;
+
+    // input block A_ij into the register tile
+    tile_zero(s.X);
+    wave_sync();
+    if(t1.x == FMT_DENSE) {
+#pragma unroll
+        for(int t = 0; t < 16; ++t) {
+            const int idx = 64 * t + lane, r = idx >> 5, c = idx & 31;
+            if(r < hgt && c < wid) s.X[idx] = a.val_in[(int64_t) t1.w + r * wid + c];
+        }
+    } else {
+        for(int e = lane; e < t1.y; e += 64) {
+            const int ix = a.ix_in[(int64_t) t1.z + e];
+            const int r = ix / wid, c = ix - r * wid;
+            s.X[32 * r + c] = a.val_in[(int64_t) t1.w + e];
+        }
+    }
+    wave_sync();
+    VT T[16];
+#pragma unroll
+    for(int t = 0; t < 16; ++t) T[t] = s.X[32 * (c0 + 2 * t) + j];
+    wave_sync();
+    tile_zero(s.X);
+    wave_sync();
+
+    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order
+    for(int e0 = 0; e0 < np; e0 += 64) {
+        Dep mi{}, mj{};
+        if(e0 + lane < np) {
+            const int4 pr = pairs[e0 + lane];
+            mi = dep_meta(a.bout, pr.x, pr.z, pr.w);
+            mj = dep_meta(a.bout, pr.y, pr.z, pr.w);
+        }
+        if(wave_ballot(e0 + lane < np && !small_sparse<VT>(mj))) {
+            if(lane == 0) a.defer_list[atomicAdd(a.defer_cnt + lr.level, 1)] = q;
+            return;
+        }
+        const int cnt = min(64, np - e0);
+        Dep ci = bcast(mi, 0), cj = bcast(mj, 0);
+        Pre<VT> pi = prefetch(ci, a.ix_out, a.val_out, a.tri, true);
+        Pre<VT> pj = prefetch(cj, a.ix_out, a.val_out, a.tri, false);
+        for(int d = 0; d < cnt; ++d) {
+            Dep ni = ci, nj = cj;
+            Pre<VT> qi = pi, qj = pj;
+            if(d + 1 < cnt) {
+                ni = bcast(mi, d + 1);
+                nj = bcast(mj, d + 1);
+                qi = prefetch(ni, a.ix_out, a.val_out, a.tri, true);
+                qj = prefetch(nj, a.ix_out, a.val_out, a.tri, false);
+            }
+            stage(s.X, (uint32_t*) nullptr, ci, pi, hgt, a.ix_out, a.val_out);
+            list_build(s.rv, s.rl, s.cst, cj, pj, wid);
+            stage_d(s.cs, s.cd, s.cb, pi);
+            wave_sync();
+            product_list(T, s.X, s.rv, s.rl, s.cst, s.cs, s.cd, s.cb, hgt, wid);
+            wave_sync();
+            unstage(s.X, (uint32_t*) nullptr, ci, pi, hgt);
+            wave_sync();
+            ci = ni; cj = nj; pi = qi; pj = qj;
+        }
+    }
+
+    // the updated tile back to LDS, row c, column j
+#pragma unroll
+    for(int t = 0; t < 16; ++t) s.X[32 * (c0 + 2 * t) + j] = T[t];
+    wave_sync();
+
+    // U_ii' forward substitution in place, column jc per lane (lanes >= 32
+    // idle), over the nonzero columns of the pivot rows (exact no-ops skipped)
+    const VT* piv = a.if_val + (int64_t) pq * 1024;
+    const uint32_t um_row = (lane < hgt) ? a.piv_umask[(int64_t) pq * 32 + lane] : 0u;
+    const int jc = lane;
+    VT* col = s.X + jc;
+    for(int ii = 0; ii < hgt; ++ii) {
+        const uint32_t um = (uint32_t) __builtin_amdgcn_readlane((int) um_row, ii);
+        if(um && jc < wid) {
+            const VT rv = col[32 * ii];
+            if(rv != (VT) 0) {
+                const VT pv = rv / piv[ii * hgt + ii];
+                for(uint32_t bits = um; bits; bits &= bits - 1) {
+                    const int l = __ffs(bits) - 1;
+                    col[32 * l] = fma(-piv[ii * hgt + l], pv, col[32 * l]);
+                }
+            }
+        }
+    }
+    // D solve in place (d_left_solve_with_D_Ut, kernels :1533-1566); a row
+    // is written only after both rows it reads have been read
+    const double* D = a.tri + 3 * (int64_t) fr;
+    const int32_t* ip = a.is_piv + fr;
+    if(jc < wid) {
+        if(all1x1) {
+            for(int l = 0; l < hgt; ++l) col[32 * l] = (VT) ((double) col[32 * l] / D[3 * l + 1]);
+        } else {
+            for(int l = 0; l < hgt; ++l) {
+                const bool p1 = ip[l] != 0;
+                const bool p2 = (l + 1 < hgt) && !ip[l + 1];
+                const double rl = (double) col[32 * l];
+                const double right = (l < 31) ? (double) col[32 * (l + 1)] : rl;
+                const double d11 = D[3 * l + 1];
+                const double d12 = D[3 * l + 2];
+                double d22 = (l + 1 < hgt) ? D[3 * (l + 1) + 1] : 0.0;
+                if(l == 31) d22 = d11;
+                const double r = sqrt(fma(d11, d11, d12 * d12));
+                const double c = d11 / r, sn = d12 / r;
+                double x1 = fma(c, rl, sn * right);
+                double x2 = fma(-sn, rl, c * right);
+                const double t11 = fma(c, d11, sn * d12);
+                const double t12 = fma(c, d12, sn * d22);
+                const double t22 = fma(-sn, d12, c * d22);
+                x2 = p2 ? (x2 / t22) : 0.0;
+                x1 = fma(-t12, x2, x1) / t11;
+                if(p1) col[32 * l] = (VT) x1;
+                if(p2) col[32 * (l + 1)] = (VT) x2;
+            }
+        }
+    }
+    wave_sync();
+    if(lane < hgt) a.rn_part[(int64_t) q * 32 + lane] = row_ssq(s.X, lane, wid);
+
+    VT* oval = a.if_val + (int64_t) q * 1024;
+    if(t1.x == FMT_DENSE) {
+        for(int e = lane; e < hgt * wid; e += 64) {
+            const int r = e / wid, c = e - r * wid;
+            oval[e] = s.X[32 * r + c];
+        }
+        if(lane == 0) a.if_cnt[q] = hgt * wid;
+    } else {
+        uint16_t* oix = a.if_idx + (int64_t) q * 1024;
+        int base = 0;
+#pragma unroll
+        for(int t = 0; t < 16; ++t) {
+            const int idx = 64 * t + lane, r = idx >> 5, c = idx & 31;
+            const VT v = s.X[idx];
+            const bool pred = r < hgt && c < wid && fabs((double) v) > NZ_EPS;
+            const uint64_t msk = wave_ballot(pred);
+            if(pred) {
+                const int pos = base + lanes_below(msk);
+                oix[pos] = (uint16_t) idx;
+                oval[pos] = v;
+            }
+            base += __popcll(msk);
+        }
+        if(lane == 0) a.if_cnt[q] = base;
+    }
+}
+
 template<typename VT>
 __device__ __forceinline__ VT drop_norm(VT rn) { return (VT) sqrtf((float) rn); }
 
@@ -583,162 +854,212 @@ __global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
     a.rn[fr + lane] = acc;
 }
 
-// entries of each sparse block that survive |v| >= tau * sqrtf(norm)
+// entries of a sparse position that survive |v| >= tau * sqrtf(norm)
 // (kernels :1919-1946)
 template<typename VT>
-__global__ __launch_bounds__(256) void k_dropcount(FactorArgs<VT> a, LevelRange lr)
+__device__ int drop_survivors(const FactorArgs<VT>& a, int q, int fr)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if(q >= lr.size) return;
-    const BlockMeta m = a.bin[a.lvlb_idx[lr.lo + q]];
-    if(m.format != FMT_SPARSE) return;
-    const int fr = a.bs[m.block_row];
     const int cnt = a.if_cnt[q];
     const VT* vals = a.if_val + (int64_t) q * 1024;
     const uint16_t* ixs = a.if_idx + (int64_t) q * 1024;
     int keep = 0;
     for(int e = 0; e < cnt; ++e) {
-        const VT v = vals[e];
         const VT nrm = drop_norm(a.rn[fr + (ixs[e] >> 5)]);
-        keep += !(fabs(v) < a.threshold * nrm);
+        keep += !(fabs(vals[e]) < a.threshold * nrm);
     }
-    a.drop_cnt[q] = keep;
+    return keep;
 }
 
-// leftover pool, granted extra space and output offsets, in level-list
-// order (kernels :1948-2008 serialised). Pool after a sparse block:
-// max(pool + max_nnz - nnz, 0); composition of such maps is a scan.
-constexpr int SCAN_TPB = 1024;
-constexpr int SCAN_IPT = 8;
-
-__device__ __forceinline__ void map_compose(int64_t& A, int64_t& B, int64_t a, int64_t b)
+// Leftover pool, granted extra space and output offsets of a level's blocks
+// in level-list order (kernels :1948-2008 with the atomics serialised in
+// that order). The pool after a sparse block is max(pool + max_nnz - kept,
+// 0); such maps p -> max(p + a, b) compose associatively, so the pool
+// sequence is a scan: chunk maps (k_level_maps), pools and grants
+// (k_level_grant), output offsets (k_level_write).
+struct PoolMap {
+    int64_t a, b;
+    __device__ static PoolMap shfl_up(const PoolMap& m, int o)
+    {
+        return {__shfl_up(m.a, o, WAVE), __shfl_up(m.b, o, WAVE)};
+    }
+};
+struct MapOp {
+    // first x, then y
+    __device__ PoolMap operator()(const PoolMap& x, const PoolMap& y) const
+    {
+        const int64_t nb = x.b + y.a;
+        return {x.a + y.a, nb > y.b ? nb : y.b};
+    }
+};
+__device__ __forceinline__ int64_t map_apply(const PoolMap& m, int64_t p)
 {
-    // first (A, B), then (a, b):  p -> max(max(p + A, B) + a, b)
-    A = A + a;
-    const int64_t nb = B + a;
-    B = nb > b ? nb : b;
+    const int64_t v = p + m.a;
+    return v > m.b ? v : m.b;
+}
+struct Sizes {
+    int64_t x, v;   // index and value entries
+    __device__ static Sizes shfl_up(const Sizes& s, int o)
+    {
+        return {__shfl_up(s.x, o, WAVE), __shfl_up(s.v, o, WAVE)};
+    }
+};
+struct SumOp {
+    __device__ Sizes operator()(const Sizes& p, const Sizes& q) const { return {p.x + q.x, p.v + q.v}; }
+};
+
+constexpr int SC_TPB = 256;
+constexpr int SC_IPT = SCAN_CHUNK / SC_TPB;
+constexpr int SC_NW = SC_TPB / WAVE;
+constexpr int32_t CAP_NONE = -0x7fffffff;
+
+// in-order combination of load(0 .. count-1) over the workgroup
+template<typename V, typename Op, typename Load>
+__device__ V block_prefix(int count, Load load, Op op, const V& id, V* sh)
+{
+    const int per = (count + SC_TPB - 1) / SC_TPB;
+    V v = id;
+    for(int k = 0; k < per; ++k) {
+        const int i = (int) threadIdx.x * per + k;
+        if(i < count) v = op(v, load(i));
+    }
+    V tot;
+    (void) block_exclusive_scan<V, Op, SC_NW>(v, op, id, sh, &tot);
+    return tot;
 }
 
-__global__ __launch_bounds__(SCAN_TPB) void k_scan_sizes(const BlockMeta* bin, BlockMeta* bout,
-    const int32_t* lvlb_idx, const int32_t* bs, const int32_t* drop_cnt, uint8_t* overflow,
-    int32_t* carry, LevelRange lr)
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void k_level_maps(FactorArgs<VT> a, LevelRange lr)
 {
-    __shared__ int64_t sa[SCAN_TPB], sb[SCAN_TPB];
-    const int tid = threadIdx.x;
-    int64_t pool = carry[0];
-    int64_t oix = carry[1], oval = carry[2];
-    for(int c0 = 0; c0 < lr.size; c0 += SCAN_TPB * SCAN_IPT) {
-        BlockMeta mm[SCAN_IPT];
-        int32_t cnt[SCAN_IPT];
-        int64_t A = 0, B = 0;
+    __shared__ PoolMap sh[SC_NW];
+    const int c = blockIdx.x;
+    PoolMap m{0, 0};
 #pragma unroll
-        for(int k = 0; k < SCAN_IPT; ++k) {
-            const int q = c0 + tid * SCAN_IPT + k;
-            cnt[k] = 0;
-            if(q < lr.size) {
-                mm[k] = bin[lvlb_idx[lr.lo + q]];
-                if(mm[k].format == FMT_SPARSE) {
-                    cnt[k] = drop_cnt[q];
-                    map_compose(A, B, (int64_t) mm[k].max_nnz - cnt[k], 0);
-                }
-            } else {
-                mm[k].format = -1;
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        if(q < lr.size) {
+            const int cap = a.lq_cap[lr.lo + q];
+            if(cap >= 0) {
+                const int cnt = drop_survivors(a, q, a.lq_fr[lr.lo + q]);
+                a.drop_cnt[q] = cnt;
+                m = MapOp()(m, PoolMap{(int64_t) cap - cnt, 0});
             }
         }
-        // exclusive scan of the per-thread maps
-        sa[tid] = A; sb[tid] = B;
-        __syncthreads();
-        for(int o = 1; o < SCAN_TPB; o <<= 1) {
-            int64_t a1 = 0, b1 = 0;
-            const bool has = tid >= o;
-            if(has) { a1 = sa[tid - o]; b1 = sb[tid - o]; }
-            __syncthreads();
-            if(has) {
-                int64_t a2 = a1, b2 = b1;
-                map_compose(a2, b2, sa[tid], sb[tid]);
-                sa[tid] = a2;
-                sb[tid] = b2;
-            }
-            __syncthreads();
-        }
-        int64_t pl = pool;
-        if(tid > 0) {
-            const int64_t v = pool + sa[tid - 1];
-            pl = v > sb[tid - 1] ? v : sb[tid - 1];
-        }
-        const int64_t tot_a = sa[SCAN_TPB - 1], tot_b = sb[SCAN_TPB - 1];
-        __syncthreads();
-        // per item: granted space and output sizes
-        int32_t granted[SCAN_IPT], keep[SCAN_IPT];
-        int64_t nix_t = 0, nval_t = 0;
-#pragma unroll
-        for(int k = 0; k < SCAN_IPT; ++k) {
-            granted[k] = 0;
-            keep[k] = 0;
-            if(mm[k].format == FMT_SPARSE) {
-                if(cnt[k] > mm[k].max_nnz) {
-                    const int64_t e = (int64_t) cnt[k] - mm[k].max_nnz;
-                    granted[k] = (int32_t) (e < pl ? e : pl);
-                }
-                const int32_t cap = mm[k].max_nnz + granted[k];
-                keep[k] = cnt[k] < cap ? cnt[k] : cap;
-                int64_t pa = 0, pb = 0;
-                map_compose(pa, pb, (int64_t) mm[k].max_nnz - cnt[k], 0);
-                const int64_t v = pl + pa;
-                pl = v > pb ? v : pb;
-                nix_t += keep[k];
-                nval_t += keep[k];
-            } else if(mm[k].format == FMT_DENSE) {
-                const int h = bs[mm[k].block_row + 1] - bs[mm[k].block_row];
-                const int w = bs[mm[k].block_col + 1] - bs[mm[k].block_col];
-                nval_t += (int64_t) h * w;
-            }
-        }
-        // exclusive scan of the output sizes
-        sa[tid] = nix_t; sb[tid] = nval_t;
-        __syncthreads();
-        for(int o = 1; o < SCAN_TPB; o <<= 1) {
-            int64_t x1 = 0, v1 = 0;
-            if(tid >= o) { x1 = sa[tid - o]; v1 = sb[tid - o]; }
-            __syncthreads();
-            sa[tid] += x1; sb[tid] += v1;
-            __syncthreads();
-        }
-        int64_t px = oix + sa[tid] - nix_t, pv = oval + sb[tid] - nval_t;
-        const int64_t tot_x = sa[SCAN_TPB - 1], tot_v = sb[SCAN_TPB - 1];
-        __syncthreads();
-#pragma unroll
-        for(int k = 0; k < SCAN_IPT; ++k) {
-            const int q = c0 + tid * SCAN_IPT + k;
-            if(mm[k].format < 0) continue;
-            BlockMeta o = mm[k];
-            const bool sp = (o.format == FMT_SPARSE);
-            int64_t nv = keep[k];
-            if(!sp) {
-                const int h = bs[o.block_row + 1] - bs[o.block_row];
-                const int w = bs[o.block_col + 1] - bs[o.block_col];
-                nv = (int64_t) h * w;
-            }
-            o.nnz = sp ? keep[k] : 0;
-            if(sp) {
-                o.max_nnz = mm[k].max_nnz + granted[k];
-                if(cnt[k] > mm[k].max_nnz) o.got_extra = granted[k];
-            }
-            o.col_ptr = (int32_t) px;
-            o.val_ptr = (int32_t) pv;
-            px += sp ? keep[k] : 0;
-            pv += nv;
-            bout[o.id] = o;
-            overflow[q] = (uint8_t) (sp && cnt[k] > o.max_nnz);
-        }
-        pool = (pool + tot_a > tot_b) ? (pool + tot_a) : tot_b;
-        oix += tot_x;
-        oval += tot_v;
     }
-    if(tid == 0) {
-        carry[0] = (int32_t) pool;
-        carry[1] = (int32_t) oix;
-        carry[2] = (int32_t) oval;
+    PoolMap tot;
+    (void) block_exclusive_scan<PoolMap, MapOp, SC_NW>(m, MapOp(), PoolMap{0, 0}, sh, &tot);
+    if(threadIdx.x == 0) {
+        a.chunk[4 * c + 0] = tot.a;
+        a.chunk[4 * c + 1] = tot.b;
+    }
+}
+
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void k_level_grant(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ PoolMap shm[SC_NW];
+    __shared__ Sizes shs[SC_NW];
+    const int c = blockIdx.x;
+    const int32_t* cin = a.carry + 4 * (lr.level & 1);
+    int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
+    const int64_t* ch = a.chunk;
+    const PoolMap before = block_prefix<PoolMap>(c,
+        [ch](int i) { return PoolMap{ch[4 * i + 0], ch[4 * i + 1]}; }, MapOp(), PoolMap{0, 0}, shm);
+    const int64_t pool_c = map_apply(before, cin[0]);
+    int cap[SC_IPT], cnt[SC_IPT];
+    PoolMap m{0, 0};
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        cap[k] = (q < lr.size) ? a.lq_cap[lr.lo + q] : CAP_NONE;
+        cnt[k] = (cap[k] >= 0) ? a.drop_cnt[q] : 0;
+        if(cap[k] >= 0) m = MapOp()(m, PoolMap{(int64_t) cap[k] - cnt[k], 0});
+    }
+    PoolMap tot;
+    const PoolMap ex = block_exclusive_scan<PoolMap, MapOp, SC_NW>(m, MapOp(), PoolMap{0, 0}, shm, &tot);
+    int64_t pl = map_apply(ex, pool_c);
+    Sizes sz{0, 0};
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        if(cap[k] >= 0) {
+            int64_t g = 0;
+            if(cnt[k] > cap[k]) {
+                const int64_t e = (int64_t) cnt[k] - cap[k];
+                g = e < pl ? e : pl;
+            }
+            const int64_t room = cap[k] + g;
+            const int32_t kp = (int32_t) (cnt[k] < room ? cnt[k] : room);
+            const int64_t np = pl + cap[k] - cnt[k];
+            pl = np > 0 ? np : 0;
+            a.keep[q] = kp;
+            a.granted[q] = (int32_t) g;
+            sz.x += kp;
+            sz.v += kp;
+        } else if(cap[k] != CAP_NONE) {
+            sz.v += -1 - cap[k];
+        }
+    }
+    Sizes stot;
+    (void) block_exclusive_scan<Sizes, SumOp, SC_NW>(sz, SumOp(), Sizes{0, 0}, shs, &stot);
+    if(threadIdx.x == 0) {
+        a.chunk[4 * c + 2] = stot.x;
+        a.chunk[4 * c + 3] = stot.v;
+        if(c == (int) gridDim.x - 1) cout[0] = (int32_t) map_apply(tot, pool_c);
+    }
+}
+
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void k_level_write(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ Sizes shs[SC_NW];
+    const int c = blockIdx.x;
+    const int32_t* cin = a.carry + 4 * (lr.level & 1);
+    int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
+    const int64_t* ch = a.chunk;
+    const Sizes before = block_prefix<Sizes>(c,
+        [ch](int i) { return Sizes{ch[4 * i + 2], ch[4 * i + 3]}; }, SumOp(), Sizes{0, 0}, shs);
+    int cap[SC_IPT], kp[SC_IPT];
+    Sizes sz{0, 0};
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        cap[k] = (q < lr.size) ? a.lq_cap[lr.lo + q] : CAP_NONE;
+        kp[k] = (cap[k] >= 0) ? a.keep[q] : 0;
+        if(cap[k] >= 0) { sz.x += kp[k]; sz.v += kp[k]; }
+        else if(cap[k] != CAP_NONE) sz.v += -1 - cap[k];
+    }
+    Sizes stot;
+    const Sizes ex = block_exclusive_scan<Sizes, SumOp, SC_NW>(sz, SumOp(), Sizes{0, 0}, shs, &stot);
+    int64_t px = cin[1] + before.x + ex.x, pv = cin[2] + before.v + ex.v;
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        if(cap[k] == CAP_NONE) continue;
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        const int b = a.lvlb_idx[lr.lo + q];
+        BlockMeta o = a.bin[b];
+        int64_t nv;
+        if(cap[k] >= 0) {
+            const int g = a.granted[q];
+            const int cn = a.drop_cnt[q];
+            o.nnz = kp[k];
+            o.max_nnz = cap[k] + g;
+            if(cn > cap[k]) o.got_extra = g;
+            a.overflow[q] = (uint8_t) (cn > cap[k] + g);
+            nv = kp[k];
+        } else {
+            o.nnz = 0;
+            a.overflow[q] = 0;
+            nv = -1 - cap[k];
+        }
+        o.col_ptr = (int32_t) px;
+        o.val_ptr = (int32_t) pv;
+        px += (cap[k] >= 0) ? kp[k] : 0;
+        pv += nv;
+        a.bout[o.id] = o;
+    }
+    if(threadIdx.x == 0 && c == (int) gridDim.x - 1) {
+        cout[1] = (int32_t) (cin[1] + before.x + stot.x);
+        cout[2] = (int32_t) (cin[2] + before.v + stot.v);
     }
 }
 
@@ -818,7 +1139,7 @@ __global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
 template<typename VT>
 __global__ void k_zero_carry(int32_t* carry)
 {
-    if(threadIdx.x < 4) carry[threadIdx.x] = 0;
+    if(threadIdx.x < 8) carry[threadIdx.x] = 0;
 }
 
 template<typename VT>
@@ -849,7 +1170,11 @@ void gather_values(const double* vals, const int64_t* slot, int64_t nnz, VT* val
 template<typename VT>
 void factor_begin(const FactorArgs<VT>& a, void* stream)
 {
-    k_zero_carry<VT><<<1, 64, 0, (hipStream_t) stream>>>(a.carry);
+    hipStream_t st = (hipStream_t) stream;
+    k_zero_carry<VT><<<1, 64, 0, st>>>(a.carry);
+    if(a.num_levels > 0)
+        k_zero<int32_t><<<(unsigned) ((a.num_levels + 255) / 256), 256, 0, st>>>(a.defer_cnt,
+            a.num_levels);
 }
 
 template<typename VT>
@@ -857,11 +1182,16 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
     if(r.npiv > 0) k_pivots<VT><<<r.npiv, 64, 0, st>>>(a, r);
-    if(r.size > r.npiv) k_offdiag<VT><<<r.size - r.npiv, 64, 0, st>>>(a, r);
+    if(r.size > r.npiv) {
+        const int noff = r.size - r.npiv;
+        k_offdiag<VT><<<noff, 64, 0, st>>>(a, r);
+        k_offdiag_general<VT><<<noff < 2048 ? noff : 2048, 64, 0, st>>>(a, r);
+    }
     if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
-    k_dropcount<VT><<<(r.size + 255) / 256, 256, 0, st>>>(a, r);
-    k_scan_sizes<<<1, SCAN_TPB, 0, st>>>(a.bin, a.bout, a.lvlb_idx, a.bs, a.drop_cnt, a.overflow,
-        a.carry, r);
+    const unsigned nch = (unsigned) ((r.size + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    k_level_maps<VT><<<nch, SC_TPB, 0, st>>>(a, r);
+    k_level_grant<VT><<<nch, SC_TPB, 0, st>>>(a, r);
+    k_level_write<VT><<<nch, SC_TPB, 0, st>>>(a, r);
     k_store<VT><<<r.size, 64, 0, st>>>(a, r);
 }
 

@@ -37,17 +37,40 @@ struct FactorArgs {
     int32_t* if_cnt = nullptr;
     VT* rn_part = nullptr;                // 32 per position
     int32_t* drop_cnt = nullptr;
+    int32_t* keep = nullptr;              // entries kept per position
+    int32_t* granted = nullptr;           // leftover space granted per position
     uint8_t* overflow = nullptr;
     int32_t* piv_loc = nullptr;           // nb
-    int32_t* carry = nullptr;             // pool, out_ix, out_val
+    // static, in level-list order: capacity of a sparse block (max_nnz >= 0)
+    // or -1 - rows * cols for a dense block; first fine row of its block row
+    const int32_t* lq_cap = nullptr;
+    const int32_t* lq_fr = nullptr;
+    int64_t* chunk = nullptr;             // per scan chunk: pool map (2), sizes (2)
+    // static, per level-list position of an off-diagonal block, 12 int32:
+    // {block, first row, rows | cols << 8 | all-1x1 << 16, pivot position},
+    // {input format, nnz, col_ptr, val_ptr}, {first pair, pairs, 0, 0}
+    const int32_t* off_task = nullptr;
+    // static, per Schur pair: {block (k, i), block (k, j), first row of k, rows of k}
+    const int32_t* upd4 = nullptr;
+    uint32_t* piv_umask = nullptr;        // 32 per pivot position: nonzero columns > row
+    int32_t* defer_list = nullptr;        // positions left to the general kernel
+    int32_t* defer_cnt = nullptr;         // one counter per level
+    int32_t num_levels = 0;
+    // two sets of {pool, out_ix, out_val, 0}; level l reads set l & 1 and
+    // writes set (l + 1) & 1
+    int32_t* carry = nullptr;
     VT threshold = 0;
 };
+
+// positions per chunk of the level scans (256 threads x 8)
+constexpr int SCAN_CHUNK = 2048;
 
 struct LevelRange {
     int32_t lo;        // offset of the level in the level block list
     int32_t npiv;      // pivot blocks (first in the level)
     int32_t size;      // blocks in the level
     int32_t row_lo;    // offset of the level in lvl_rows
+    int32_t level;     // level index
 };
 
 // Host entry points (factor.hip). All launches go to `stream`.

@@ -143,29 +143,64 @@ __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out
 }
 
 // inclusive scan of a[1..n] in place (a[0] = 0), one workgroup, in chunks
-__global__ __launch_bounds__(1024) void k_scan_inplace(int32_t* a, int32_t n, int32_t* total)
+// prefix sums of a[1..n] in place, in chunks of PS_TPB * PS_IPT entries:
+// chunk totals first, then each chunk adds the totals before it
+constexpr int PS_TPB = 256;
+constexpr int PS_IPT = 8;
+constexpr int PS_CHUNK = PS_TPB * PS_IPT;
+
+struct ISum {
+    int32_t s;
+    __device__ static ISum shfl_up(const ISum& v, int o) { return {__shfl_up(v.s, o, WAVE)}; }
+};
+struct ISumOp {
+    __device__ ISum operator()(const ISum& p, const ISum& q) const { return {p.s + q.s}; }
+};
+
+__global__ __launch_bounds__(PS_TPB) void k_chunk_sums(const int32_t* a, int32_t n, int32_t* tmp)
 {
-    __shared__ int32_t s[1024];
-    const int tid = threadIdx.x;
-    int32_t carry = 0;
-    if(tid == 0) a[0] = 0;
-    for(int32_t c0 = 1; c0 <= n; c0 += 1024) {
-        const int32_t i = c0 + tid;
-        int32_t v = (i <= n) ? a[i] : 0;
-        s[tid] = v;
-        __syncthreads();
-        for(int o = 1; o < 1024; o <<= 1) {
-            const int32_t u = (tid >= o) ? s[tid - o] : 0;
-            __syncthreads();
-            s[tid] += u;
-            __syncthreads();
-        }
-        if(i <= n) a[i] = carry + s[tid];
-        const int32_t tot = s[1023];
-        __syncthreads();
-        carry += tot;
+    __shared__ ISum sh[PS_TPB / WAVE];
+    int32_t v = 0;
+#pragma unroll
+    for(int k = 0; k < PS_IPT; ++k) {
+        const int64_t i = 1 + (int64_t) blockIdx.x * PS_CHUNK + (int64_t) threadIdx.x * PS_IPT + k;
+        if(i <= n) v += a[i];
     }
-    if(tid == 0 && total) *total = carry;
+    ISum tot;
+    (void) block_exclusive_scan<ISum, ISumOp, PS_TPB / WAVE>(ISum{v}, ISumOp(), ISum{0}, sh, &tot);
+    if(threadIdx.x == 0) tmp[blockIdx.x] = tot.s;
+}
+
+__global__ __launch_bounds__(PS_TPB) void k_chunk_scan(int32_t* a, int32_t n, const int32_t* tmp,
+    int32_t* total)
+{
+    __shared__ ISum sh[PS_TPB / WAVE];
+    const int c = blockIdx.x;
+    // totals of the chunks before this one
+    int32_t pre = 0;
+    for(int q = threadIdx.x; q < c; q += PS_TPB) pre += tmp[q];
+    ISum ptot;
+    (void) block_exclusive_scan<ISum, ISumOp, PS_TPB / WAVE>(ISum{pre}, ISumOp(), ISum{0}, sh, &ptot);
+    int32_t v[PS_IPT];
+    int32_t s = 0;
+    const int64_t i0 = 1 + (int64_t) c * PS_CHUNK + (int64_t) threadIdx.x * PS_IPT;
+#pragma unroll
+    for(int k = 0; k < PS_IPT; ++k) {
+        v[k] = (i0 + k <= n) ? a[i0 + k] : 0;
+        s += v[k];
+    }
+    ISum tot;
+    const ISum ex = block_exclusive_scan<ISum, ISumOp, PS_TPB / WAVE>(ISum{s}, ISumOp(), ISum{0}, sh, &tot);
+    int32_t run = ptot.s + ex.s;
+#pragma unroll
+    for(int k = 0; k < PS_IPT; ++k) {
+        run += v[k];
+        if(i0 + k <= n) a[i0 + k] = run;
+    }
+    if(threadIdx.x == 0) {
+        if(c == 0) a[0] = 0;
+        if(c == (int) gridDim.x - 1 && total) *total = ptot.s + tot.s;
+    }
 }
 
 // segmented inclusive sum over lanes; segments are runs of equal sg
@@ -261,6 +296,103 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
     if(lane < h) x[fr + lane] = (double) res[lane];
 }
 
+// Whole triangular solve in one launch: workgroup w takes the block row at
+// position w of the level order (position nb - 1 - w for the U solve), so
+// every block row it depends on has a smaller workgroup index. Before it
+// reads x of another block row it waits for that row's flag to carry this
+// solve's epoch; after writing its own rows it publishes its flag with
+// release ordering at agent scope (the L2 of each XCD is separate).
+// Workgroups are dispatched in index order, so the lowest unfinished one
+// always has its inputs ready and the solve drains. Every wait is bounded:
+// past SPIN_LIMIT polls the wave records an error in *err and goes on.
+constexpr int SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ void wait_flag(const int32_t* f, int32_t epoch, int32_t* err)
+{
+    int spins = 0;
+    while(__hip_atomic_load(const_cast<int32_t*>(f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+        epoch) {
+        __builtin_amdgcn_s_sleep(2);
+        if(++spins > SPIN_LIMIT) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t nb, int32_t reverse,
+    const int32_t* o_col, const VT* o_val, const uint8_t* o_lr, const int32_t* i_col,
+    const VT* i_val, const uint8_t* i_lr, const int32_t* row_block, int32_t* flag, int32_t epoch,
+    int32_t* err, double* x)
+{
+    __shared__ VT y[33];
+    __shared__ VT res[33];
+    const int lane = lane_id();
+    const int pos = reverse ? (nb - 1 - (int) blockIdx.x) : (int) blockIdx.x;
+    const int4 t0 = task[2 * pos];
+    const int4 t1 = task[2 * pos + 1];
+    const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
+    const double xo = (lane < h) ? x[fr + lane] : 0.0;
+    int icol = fr;
+    VT ival = 0;
+    int isg = 64;
+    if(ib + lane < ie) {
+        icol = i_col[ib + lane];
+        ival = i_val[ib + lane];
+        isg = i_lr[ib + lane];
+    }
+    if(lane < h) { y[lane] = (VT) xo; res[lane] = 0; }
+    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
+        int c[TRSV_PF], sg[TRSV_PF];
+        VT v[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            const int e = e0 + 64 * k + lane;
+            c[k] = 0;
+            v[k] = 0;
+            sg[k] = 64;
+            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
+        }
+        int dep[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) dep[k] = (sg[k] < 64) ? row_block[c[k]] : -1;
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k)
+            if(dep[k] >= 0) wait_flag(flag + dep[k], epoch, err);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        VT xv[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? (VT) x[c[k]] : (VT) 0;
+        wave_sync();
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            if(e0 + 64 * k >= oe) break;
+            VT p = seg_scan(v[k] * xv[k], sg[k]);
+            const int nsg = __shfl_down(sg[k], 1, 64);
+            if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
+            wave_sync();
+        }
+    }
+    wave_sync();
+    for(int e0 = ib; e0 < ie; e0 += 64) {
+        if(e0 != ib) {
+            const int e = e0 + lane;
+            isg = 64;
+            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+        }
+        VT p = (isg < 64) ? ival * y[icol - fr] : (VT) 0;
+        p = seg_scan(p, isg);
+        const int nsg = __shfl_down(isg, 1, 64);
+        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
+        wave_sync();
+    }
+    if(lane < h) x[fr + lane] = (double) res[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if(lane == 0)
+        __hip_atomic_store(flag + row_block[fr], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ void k_solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b,
     double* x)
 {
@@ -307,9 +439,18 @@ void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT*, SolvePlan
     k_plan_tasks<<<g, 256, 0, st>>>(t.nb, t.lvl_rows, t.bs, p.to_ptr, p.ti_ptr, p.task_t);
 }
 
-void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, void* stream)
+size_t scan_tmp_ints(int32_t n) { return (size_t) ((n + PS_CHUNK - 1) / PS_CHUNK) + 1; }
+
+void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, int32_t* tmp, void* stream)
 {
-    k_scan_inplace<<<1, 1024, 0, (hipStream_t) stream>>>(ptr, n, total_dev);
+    hipStream_t st = (hipStream_t) stream;
+    const unsigned nch = (unsigned) ((n + PS_CHUNK - 1) / PS_CHUNK);
+    if(nch == 0) {
+        k_chunk_scan<<<1, PS_TPB, 0, st>>>(ptr, n, tmp, total_dev);
+        return;
+    }
+    k_chunk_sums<<<nch, PS_TPB, 0, st>>>(ptr, n, tmp);
+    k_chunk_scan<<<nch, PS_TPB, 0, st>>>(ptr, n, tmp, total_dev);
 }
 
 template<typename VT>
@@ -346,6 +487,28 @@ template void solve_plan_count<double>(const SolveTopo&, const double*, double*,
 template void solve_plan_count<float>(const SolveTopo&, const float*, float*, SolvePlan<float>&, void*);
 template void solve_plan_fill<double>(const SolveTopo&, const double*, const double*, SolvePlan<double>&, void*);
 template void solve_plan_fill<float>(const SolveTopo&, const float*, const float*, SolvePlan<float>&, void*);
+template<typename VT>
+void trsv_upper_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+    void* stream)
+{
+    if(t.nb <= 0) return;
+    k_trsv_flagged<VT><<<t.nb, 64, 0, (hipStream_t) stream>>>(p.task_u, t.nb, 1, p.uo_col, p.uo_val,
+        p.uo_lr, p.ui_col, p.ui_val, p.ui_lr, t.row_block, t.flag_u, epoch, t.err, x);
+}
+
+template<typename VT>
+void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+    void* stream)
+{
+    if(t.nb <= 0) return;
+    k_trsv_flagged<VT><<<t.nb, 64, 0, (hipStream_t) stream>>>(p.task_t, t.nb, 0, p.to_col, p.to_val,
+        p.to_lr, p.ti_col, p.ti_val, p.ti_lr, t.row_block, t.flag_t, epoch, t.err, x);
+}
+
+template void trsv_upper_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
+template void trsv_upper_flagged<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
+template void trsv_upper_t_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
+template void trsv_upper_t_flagged<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
 template void trsv_upper<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
 template void trsv_upper<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
 template void trsv_upper_t<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);

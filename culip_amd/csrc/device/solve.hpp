@@ -37,6 +37,11 @@ struct SolveTopo {
     const int32_t* ix_out;
     const double* tri;
     const int32_t* is_piv;
+    // single-launch solves
+    const int32_t* row_block;   // fine row -> block row
+    int32_t* flag_u;            // per block row: epoch of its last U solve
+    int32_t* flag_t;            // per block row: epoch of its last U' solve
+    int32_t* err;               // set when a dependency wait gave up
 };
 
 // Counting pass of the layout: fills the four count arrays (n entries each,
@@ -49,14 +54,25 @@ template<typename VT>
 void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT* dinv, SolvePlan<VT>& p,
     void* stream);
 
-// exclusive prefix sum of cnt[1..n] into ptr[0..n] (ptr[0] = 0), in place
-void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, void* stream);
+// exclusive prefix sum of cnt[1..n] into ptr[0..n] (ptr[0] = 0), in place;
+// tmp holds scan_tmp_ints(n) ints of scratch
+size_t scan_tmp_ints(int32_t n);
+void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, int32_t* tmp, void* stream);
 
 // x <- U \ x (in place); x <- U' \ x; x <- D \ x.
 template<typename VT>
 void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
 template<typename VT>
 void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
+// The same solves as one launch each, block rows ordered by level and
+// synchronised through per-block-row flags; epoch must differ from the
+// previous call's on the same flags.
+template<typename VT>
+void trsv_upper_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+    void* stream);
+template<typename VT>
+void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+    void* stream);
 void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
     void* stream);
 

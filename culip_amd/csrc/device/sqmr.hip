@@ -458,7 +458,9 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     HIPCHK(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
     int32_t it = 0;
     const int period = opt.true_residual_period;
-    double prev_nr = -1.0;
+    constexpr int STALL_PERIODS = 5;
+    double best_nr = -1.0;
+    int stall = 0;
     while(it < opt.max_it) {
         const int k0 = it % 3, k1 = (it + 1) % 3, km1 = (it + 2) % 3, km2 = (it + 1) % 3;
         k_alpha<<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
@@ -480,7 +482,10 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
             // true-residual gate: the recurrence runs on past the reference's
             // quasi-residual test; it stops when the FP64 true residual meets
             // the tolerance, and hands over to a refinement restart on a
-            // Lanczos breakdown or when the true residual stagnates
+            // Lanczos breakdown or when the best true residual has not halved
+            // for STALL_PERIODS checks (the recurrence has drifted). The
+            // quasi-residual test is not used here: with a large ||A||
+            // estimate it fires long before the true residual is small.
             if((it + 1) % period == 0) {
                 k_dsolve<<<nb, 256, 0, st>>>(n, w.xt, xout, ops.tri, ops.is_piv);
                 if(ops.right) ops.right(xout, st);
@@ -488,9 +493,14 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
                 residual_norms(ops, w, b, xout, st, nr, nbn);
                 if(nr <= gate_abs) { gate_met = true; ++it; break; }
                 HIPCHK(hipMemcpy(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
-                const bool stalled = prev_nr > 0 && nr > 0.9 * prev_nr;
-                prev_nr = nr;
-                if(w.host_flags[0] || stalled) { ++it; break; }
+                if(best_nr < 0 || nr < 0.5 * best_nr) {
+                    best_nr = nr;
+                    stall = 0;
+                } else {
+                    ++stall;
+                }
+                const bool far_enough = ops.restart_rel > 0 && nr <= ops.restart_rel * nbn;
+                if(w.host_flags[0] || stall >= STALL_PERIODS || far_enough) { ++it; break; }
             }
             ++it;
             continue;

@@ -48,4 +48,34 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
+// Ordered scans over the threads of a workgroup for an associative (not
+// necessarily commutative) operation on a value type V with members that
+// __shfl can move. Op: V op(const V& earlier, const V& later), id: identity.
+// Returns the exclusive prefix of this thread; *total receives the
+// combination over all threads. NW = waves per workgroup; sh holds NW values.
+template<typename V, typename Op, int NW>
+__device__ __forceinline__ V block_exclusive_scan(V v, Op op, const V& id, V* sh, V* total)
+{
+    const int lane = lane_id();
+    const int wv = (int) (threadIdx.x / WAVE);
+    V inc = v;
+#pragma unroll
+    for(int o = 1; o < WAVE; o <<= 1) {
+        V u = V::shfl_up(inc, o);
+        if(lane >= o) inc = op(u, inc);
+    }
+    if(lane == WAVE - 1) sh[wv] = inc;
+    __syncthreads();
+    V pre = id;
+    for(int q = 0; q < wv; ++q) pre = op(pre, sh[q]);
+    V all = id;
+#pragma unroll
+    for(int q = 0; q < NW; ++q) all = op(all, sh[q]);
+    V ex = V::shfl_up(inc, 1);
+    if(lane == 0) ex = id;
+    __syncthreads();
+    if(total) *total = all;
+    return op(pre, ex);
+}
+
 }  // namespace culip

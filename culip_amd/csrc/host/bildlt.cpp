@@ -16,13 +16,24 @@ void Bildlt::symbolic(int32_t fill_level, double fill_factor)
 {
     symbolic_analyze(sym, A, piv_starts, block_starts, fill_level, fill_factor, f32());
     have_sym = true;
+    pat->uploaded = false;
     uploaded = false;
     factored = false;
     host_int.clear();
     host_real.clear();
 }
 
-void Bildlt::upload_pattern()
+int64_t BildltPattern::device_bytes() const
+{
+    int64_t b = 0;
+    for(const DevBuf* d : {&d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off,
+            &d_upd_ki, &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in,
+            &d_value_slot, &d_rfo, &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4, &d_row_block})
+        b += (int64_t) d->bytes;
+    return b;
+}
+
+void BildltPattern::upload_static()
 {
     const Symbolic& s = sym;
     upload(d_bs, s.block_starts);
@@ -42,7 +53,78 @@ void Bildlt::upload_pattern()
     upload(d_value_slot, s.value_slot);
 
     const int32_t n = s.n, K = s.coarse_nnz;
+    // per level-list position: sparse capacity or -1 - rows * cols (dense),
+    // and the first fine row of the block row
+    {
+        std::vector<int32_t> cap(K), fr(K);
+        for(int32_t p = 0; p < K; ++p) {
+            const BlockMeta& m = s.blocks[s.lvlb_idx[p]];
+            const int32_t h = s.block_starts[m.block_row + 1] - s.block_starts[m.block_row];
+            const int32_t w = s.block_starts[m.block_col + 1] - s.block_starts[m.block_col];
+            cap[p] = (m.format == FMT_SPARSE) ? m.max_nnz : -1 - h * w;
+            fr[p] = s.block_starts[m.block_row];
+        }
+        upload(d_lq_cap, cap);
+        upload(d_lq_fr, fr);
+    }
+    // off-diagonal task descriptors (12 int32 per level-list position) and
+    // Schur pairs with the rows of their block k
+    {
+        CULIP_REQUIRE(s.upd_off[K] < (int64_t) INT32_MAX, "factor: too many Schur pairs");
+        std::vector<int32_t> diag_id(s.nb, -1);
+        for(int32_t b = 0; b < K; ++b)
+            if(s.blocks[b].block_row == s.blocks[b].block_col) diag_id[s.blocks[b].block_row] = b;
+        std::vector<int32_t> task((size_t) 12 * K, 0);
+        for(int32_t l = 0; l < s.num_levels; ++l) {
+            const int32_t lo = s.lvlb_off[l], hi = s.lvlb_off[l + 1];
+            const int32_t npiv = s.lvl_off[l + 1] - s.lvl_off[l];
+            for(int32_t p = lo + npiv; p < hi; ++p) {
+                const int32_t b = s.lvlb_idx[p];
+                const BlockMeta& m = s.blocks[b];
+                const int32_t i = m.block_row, jb = m.block_col;
+                const int32_t fr = s.block_starts[i];
+                const int32_t h = s.block_starts[i + 1] - fr;
+                const int32_t w = s.block_starts[jb + 1] - s.block_starts[jb];
+                bool all1 = true;
+                for(int32_t r = fr; r < fr + h; ++r) all1 = all1 && s.is_piv[r] != 0;
+                CULIP_REQUIRE(diag_id[i] >= 0, "factor: block row without a pivot block");
+                const int32_t pq = s.block_in_lvl[diag_id[i]] - lo;
+                CULIP_REQUIRE(pq >= 0 && pq < npiv, "factor: pivot block outside its level");
+                int32_t* t = &task[(size_t) 12 * p];
+                t[0] = b; t[1] = fr; t[2] = h | (w << 8) | ((all1 ? 1 : 0) << 16); t[3] = pq;
+                t[4] = m.format; t[5] = m.nnz; t[6] = m.col_ptr; t[7] = m.val_ptr;
+                t[8] = (int32_t) s.upd_off[b];
+                t[9] = (int32_t) (s.upd_off[b + 1] - s.upd_off[b]);
+            }
+        }
+        upload(d_off_task, task);
+        const size_t npairs = s.upd_ki.size();
+        std::vector<int32_t> u4(4 * npairs);
+        for(size_t u = 0; u < npairs; ++u) {
+            const int32_t k = s.blocks[s.upd_ki[u]].block_row;
+            u4[4 * u + 0] = s.upd_ki[u];
+            u4[4 * u + 1] = s.upd_kj[u];
+            u4[4 * u + 2] = s.block_starts[k];
+            u4[4 * u + 3] = s.block_starts[k + 1] - s.block_starts[k];
+        }
+        upload(d_upd4, u4);
+    }
+    upload(d_row_block, s.row_block);
+    uploaded = true;
+}
+
+void Bildlt::upload_pattern()
+{
+    if(!pat->uploaded) pat->upload_static();
+    const Symbolic& s = sym;
+    const int32_t n = s.n, K = s.coarse_nnz;
     const int64_t mlb = std::max<int64_t>(1, s.max_level_blocks);
+    d_keep.ensure(sizeof(int32_t) * mlb);
+    d_granted.ensure(sizeof(int32_t) * mlb);
+    d_piv_umask.ensure(sizeof(uint32_t) * 32 * mlb);
+    d_defer_list.ensure(sizeof(int32_t) * mlb);
+    d_defer_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
+    d_chunk.ensure(sizeof(int64_t) * 4 * (size_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK));
     d_vals.ensure(sizeof(double) * std::max<int64_t>(1, A.nnz()));
     d_val_in.ensure(vt() * std::max<int64_t>(1, s.val_space));
     d_bout.ensure(sizeof(BlockMeta) * std::max(1, K));
@@ -57,10 +139,17 @@ void Bildlt::upload_pattern()
     d_drop_cnt.ensure(sizeof(int32_t) * mlb);
     d_overflow.ensure(mlb);
     d_piv_loc.ensure(sizeof(int32_t) * s.nb);
-    d_carry.ensure(sizeof(int32_t) * 4);
+    d_carry.ensure(sizeof(int32_t) * 8);
     for(int q = 0; q < 4; ++q) d_ptr[q].ensure(sizeof(int32_t) * (size_t) (n + 1));
     for(int q = 0; q < 2; ++q) d_task[q].ensure(sizeof(int32_t) * 8 * (size_t) s.nb);
     d_totals.ensure(sizeof(int32_t) * 4);
+    d_scan_tmp.ensure(sizeof(int32_t) * scan_tmp_ints(n));
+    d_flag_u.ensure(sizeof(int32_t) * std::max(1, s.nb));
+    d_flag_t.ensure(sizeof(int32_t) * std::max(1, s.nb));
+    d_trsv_err.ensure(sizeof(int32_t));
+    HIP_CHECK(hipMemset(d_flag_u.p, 0, d_flag_u.bytes));
+    HIP_CHECK(hipMemset(d_flag_t.p, 0, d_flag_t.bytes));
+    HIP_CHECK(hipMemset(d_trsv_err.p, 0, d_trsv_err.bytes));
     uploaded = true;
 }
 
@@ -95,8 +184,19 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.if_cnt = d_if_cnt.as<int32_t>();
     a.rn_part = d_rn_part.as<VT>();
     a.drop_cnt = d_drop_cnt.as<int32_t>();
+    a.keep = d_keep.as<int32_t>();
+    a.granted = d_granted.as<int32_t>();
     a.overflow = d_overflow.as<uint8_t>();
     a.piv_loc = d_piv_loc.as<int32_t>();
+    a.lq_cap = d_lq_cap.as<int32_t>();
+    a.lq_fr = d_lq_fr.as<int32_t>();
+    a.chunk = d_chunk.as<int64_t>();
+    a.off_task = d_off_task.as<int32_t>();
+    a.upd4 = d_upd4.as<int32_t>();
+    a.piv_umask = d_piv_umask.as<uint32_t>();
+    a.defer_list = d_defer_list.as<int32_t>();
+    a.defer_cnt = d_defer_cnt.as<int32_t>();
+    a.num_levels = sym.num_levels;
     a.carry = d_carry.as<int32_t>();
     a.threshold = (VT) threshold;
     return a;
@@ -137,6 +237,10 @@ SolveTopo Bildlt::topo() const
     t.ix_out = d_ix_out.as<int32_t>();
     t.tri = d_tri.as<double>();
     t.is_piv = d_is_piv.as<int32_t>();
+    t.row_block = d_row_block.as<int32_t>();
+    t.flag_u = d_flag_u.as<int32_t>();
+    t.flag_t = d_flag_t.as<int32_t>();
+    t.err = d_trsv_err.as<int32_t>();
     return t;
 }
 
@@ -154,6 +258,7 @@ void Bildlt::factor_impl(hipStream_t st)
         r.size = s.lvlb_off[l + 1] - r.lo;
         r.npiv = s.lvl_off[l + 1] - s.lvl_off[l];
         r.row_lo = s.lvl_off[l];
+        r.level = l;
         factor_level<VT>(a, r, st);
     }
     HIP_CHECK(hipGetLastError());
@@ -163,7 +268,8 @@ void Bildlt::factor_impl(hipStream_t st)
     SolvePlan<VT> p = plan<VT>();
     solve_plan_count<VT>(t, a.val_out, nullptr, p, st);
     int32_t* tot = d_totals.as<int32_t>();
-    for(int q = 0; q < 4; ++q) scan_counts(d_ptr[q].as<int32_t>(), s.n, tot + q, st);
+    for(int q = 0; q < 4; ++q)
+        scan_counts(d_ptr[q].as<int32_t>(), s.n, tot + q, d_scan_tmp.as<int32_t>(), st);
     int32_t htot[4];
     HIP_CHECK(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -203,14 +309,35 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
 
 void Bildlt::trsv_u(double* x, hipStream_t st) const
 {
+    if(solve_algo == CULIP_TRSV_FLAGGED) {
+        const int32_t e = ++epoch_u;
+        if(f32()) trsv_upper_flagged<float>(topo(), plan<float>(), x, e, st);
+        else trsv_upper_flagged<double>(topo(), plan<double>(), x, e, st);
+        return;
+    }
     if(f32()) trsv_upper<float>(topo(), plan<float>(), x, st);
     else trsv_upper<double>(topo(), plan<double>(), x, st);
 }
 
 void Bildlt::trsv_ut(double* x, hipStream_t st) const
 {
+    if(solve_algo == CULIP_TRSV_FLAGGED) {
+        const int32_t e = ++epoch_t;
+        if(f32()) trsv_upper_t_flagged<float>(topo(), plan<float>(), x, e, st);
+        else trsv_upper_t_flagged<double>(topo(), plan<double>(), x, e, st);
+        return;
+    }
     if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
     else trsv_upper_t<double>(topo(), plan<double>(), x, st);
+}
+
+void Bildlt::check_solve_error(hipStream_t st) const
+{
+    if(!uploaded) return;
+    int32_t e = 0;
+    HIP_CHECK(hipMemcpyAsync(&e, d_trsv_err.p, sizeof(e), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    if(e) fail(CULIP_ERR_HIP, "triangular solve: a dependency wait timed out");
 }
 
 void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
@@ -230,27 +357,31 @@ void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
     else trsv_ut(x, st);
 }
 
-int64_t Bildlt::device_bytes() const
+int64_t Bildlt::numeric_bytes() const
 {
     int64_t b = 0;
-    for(const DevBuf* d : {&d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off,
-            &d_upd_ki, &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in,
-            &d_value_slot, &d_vals, &d_val_in, &d_bout, &d_ix_out, &d_val_out, &d_tri, &d_rn,
+    for(const DevBuf* d : {&d_vals, &d_val_in, &d_bout, &d_ix_out, &d_val_out, &d_tri, &d_rn,
             &d_if_val, &d_if_idx, &d_if_cnt, &d_rn_part, &d_drop_cnt, &d_overflow, &d_piv_loc,
-            &d_carry, &d_totals, &d_task[0], &d_task[1]})
+            &d_carry, &d_totals, &d_task[0], &d_task[1], &d_keep, &d_granted, &d_chunk,
+            &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_flag_u, &d_flag_t,
+            &d_trsv_err})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
         b += (int64_t) (d_ptr[q].bytes + d_col[q].bytes + d_val[q].bytes + d_lr[q].bytes);
     return b;
 }
 
+int64_t Bildlt::device_bytes() const { return pat->device_bytes() + numeric_bytes(); }
+
 void Bildlt::download_factor()
 {
     if(!factored) fail(CULIP_ERR_STATE, "factor has not been computed");
     if(host_int.count("fblocks")) return;
     HIP_CHECK(hipDeviceSynchronize());
+    // final {pool, out_ix, out_val} set after the last level
     int32_t carry[4];
-    HIP_CHECK(hipMemcpy(carry, d_carry.p, sizeof(carry), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(carry, d_carry.as<int32_t>() + 4 * (sym.num_levels & 1), sizeof(carry),
+        hipMemcpyDeviceToHost));
     const int32_t K = sym.coarse_nnz;
     std::vector<int32_t> fb((size_t) K * 9);
     HIP_CHECK(hipMemcpy(fb.data(), d_bout.p, sizeof(BlockMeta) * K, hipMemcpyDeviceToHost));
@@ -288,10 +419,60 @@ void Spmv::multiply(const double* x, double* y, double alpha, double beta, bool 
     else csr_spmv(A, x, y, alpha, beta, st);
 }
 
+void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h)
+{
+    upload(br, h.row_ptr);
+    upload(bc, h.col);
+    upload(bv, h.val);
+    d.m = h.m;
+    d.n = h.n;
+    d.nnz = h.nnz();
+    d.row_ptr = br.as<int32_t>();
+    d.col = bc.as<int32_t>();
+    d.val = bv.as<double>();
+}
+
 Sqmr::~Sqmr()
 {
     if(host_flags) (void) hipHostFree(host_flags);
     if(host_scal) (void) hipHostFree(host_scal);
+}
+
+void Sqmr::init(const Spmv* op, const Bildlt* p, const culip_sqmr_opts& o)
+{
+    A = op;
+    prec = p;
+    opts = o;
+    const int32_t n = op->m;
+    work.ensure(sqmr_work_bytes(n));
+    if(!host_flags)
+        HIP_CHECK(hipHostMalloc((void**) &host_flags, 8 * sizeof(int32_t), hipHostMallocDefault));
+    if(!host_scal)
+        HIP_CHECK(hipHostMalloc((void**) &host_scal, 64 * sizeof(double), hipHostMallocDefault));
+    sqmr_work_bind(w, n, work.as<char>(), host_flags, host_scal);
+}
+
+void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t st)
+{
+    if(prec && !prec->factored) fail(CULIP_ERR_STATE, "sqmr: preconditioner has not been computed");
+    SqmrOps ops;
+    ops.n = A->m;
+    const Spmv* op = A;
+    ops.spmv = [op](const double* xx, double* yy, void* s) {
+        op->multiply(xx, yy, 1.0, 0.0, false, (hipStream_t) s);
+    };
+    if(prec) {
+        const Bildlt* P = prec;
+        ops.left = [P](double* xx, void* s) { P->trsv_ut(xx, (hipStream_t) s); };
+        ops.right = [P](double* xx, void* s) { P->trsv_u(xx, (hipStream_t) s); };
+        ops.tri = P->d_tri.as<double>();
+        ops.is_piv = P->d_is_piv.as<int32_t>();
+        // an FP32 factor applied in FP32 limits one recurrence to a few
+        // digits of true-residual reduction
+        if(P->f32()) ops.restart_rel = 1e-5;
+    }
+    sqmr_solve(ops, w, b, x, opts, res, st);
+    if(prec) prec->check_solve_error(st);
 }
 
 }  // namespace culip

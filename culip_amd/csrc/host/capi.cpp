@@ -281,6 +281,7 @@ culip_status culip_bildlt_create(const culip_csr* hA, int32_t num_pivs, const in
             h->b.A = to_host(hA);
             CULIP_REQUIRE(h->b.A.m == h->b.A.n, "bildlt: matrix must be square");
             h->b.prec = prec;
+            h->b.pat->prec = prec;
             h->b.piv_starts.assign(piv_starts, piv_starts + num_pivs);
             h->b.block_starts.assign(block_starts, block_starts + num_blocks);
         } catch(...) { delete h; throw; }
@@ -332,6 +333,16 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
 }
 
 int32_t culip_bildlt_n(const culip_bildlt* h) { return h ? h->b.n() : -1; }
+
+culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algorithm algorithm)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null handle");
+        CULIP_REQUIRE(algorithm == CULIP_TRSV_LEVELS || algorithm == CULIP_TRSV_FLAGGED,
+            "bildlt: unknown solve algorithm");
+        h->b.solve_algo = algorithm;
+    });
+}
 
 int64_t culip_bildlt_device_bytes(const culip_bildlt* h) { return h ? h->b.device_bytes() : 0; }
 
@@ -425,19 +436,6 @@ culip_status culip_bildlt_get_real(culip_bildlt* h, const char* name, const doub
 void culip_bildlt_destroy(culip_bildlt* h) { delete h; }
 
 // ---- SPMV -----------------------------------------------------------------
-
-static void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h)
-{
-    upload(br, h.row_ptr);
-    upload(bc, h.col);
-    upload(bv, h.val);
-    d.m = h.m;
-    d.n = h.n;
-    d.nnz = h.nnz();
-    d.row_ptr = br.as<int32_t>();
-    d.col = bc.as<int32_t>();
-    d.val = bv.as<double>();
-}
 
 static HostCsr transpose_csr(const HostCsr& a)
 {
@@ -551,15 +549,7 @@ culip_status culip_sqmr_create(const culip_spmv* A, const culip_bildlt* prec,
         device_present();
         auto* s = new culip_sqmr;
         try {
-            Sqmr& q = s->q;
-            q.A = &A->s;
-            q.prec = prec ? &prec->b : nullptr;
-            q.opts = *opts;
-            const int32_t n = A->s.m;
-            q.work.ensure(sqmr_work_bytes(n));
-            HIP_CHECK(hipHostMalloc((void**) &q.host_flags, 8 * sizeof(int32_t), hipHostMallocDefault));
-            HIP_CHECK(hipHostMalloc((void**) &q.host_scal, 64 * sizeof(double), hipHostMallocDefault));
-            sqmr_work_bind(q.w, n, q.work.as<char>(), q.host_flags, q.host_scal);
+            s->q.init(&A->s, prec ? &prec->b : nullptr, *opts);
         } catch(...) { delete s; throw; }
         *out = s;
     });
@@ -570,22 +560,7 @@ culip_status culip_sqmr_solve(culip_sqmr* s, const double* b, double* x, culip_s
 {
     return guarded([&] {
         CULIP_REQUIRE(s && b && x && res, "sqmr: null argument");
-        Sqmr& q = s->q;
-        if(q.prec) CULIP_REQUIRE(q.prec->factored, "sqmr: preconditioner has not been computed");
-        SqmrOps ops;
-        ops.n = q.A->m;
-        const Spmv* A = q.A;
-        ops.spmv = [A](const double* xx, double* yy, void* st) {
-            A->multiply(xx, yy, 1.0, 0.0, false, (hipStream_t) st);
-        };
-        if(q.prec) {
-            const Bildlt* P = q.prec;
-            ops.left = [P](double* xx, void* st) { P->trsv_ut(xx, (hipStream_t) st); };
-            ops.right = [P](double* xx, void* st) { P->trsv_u(xx, (hipStream_t) st); };
-            ops.tri = P->d_tri.as<double>();
-            ops.is_piv = P->d_is_piv.as<int32_t>();
-        }
-        sqmr_solve(ops, q.w, b, x, q.opts, *res, stream);
+        s->q.run(b, x, *res, (hipStream_t) stream);
     });
 }
 

@@ -51,41 +51,88 @@ void upload(DevBuf& d, const std::vector<T>& h)
     if(!h.empty()) HIP_CHECK(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
 
-struct Bildlt {
-    culip_prec prec = CULIP_PREC_F64;
+// Per sparsity pattern: the matrix as given, its symbolic analysis and the
+// static device structures derived from it. Shared by every numeric
+// instance of the pattern (one per concurrent factorization).
+struct BildltPattern {
     HostCsr A;
     std::vector<int32_t> piv_starts, block_starts;
     Symbolic sym;
     bool have_sym = false;
     bool uploaded = false;
+    culip_prec prec = CULIP_PREC_F64;
+    DevBuf d_bs, d_is_piv, d_bin, d_csr_off, d_csc_off, d_csc_idx, d_upd_off, d_upd_ki, d_upd_kj,
+        d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot, d_rfo;
+    DevBuf d_lq_cap, d_lq_fr, d_off_task, d_upd4, d_row_block;
+    void upload_static();
+    int64_t device_bytes() const;
+};
+
+struct Bildlt {
+    std::shared_ptr<BildltPattern> pat;
+    // the pattern's members under their own names
+    HostCsr& A;
+    std::vector<int32_t>& piv_starts;
+    std::vector<int32_t>& block_starts;
+    Symbolic& sym;
+    bool& have_sym;
+    DevBuf &d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off, &d_upd_ki,
+        &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in, &d_value_slot, &d_rfo;
+    DevBuf &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4, &d_row_block;
+
+    culip_prec prec = CULIP_PREC_F64;
+    bool uploaded = false;   // this instance's numeric buffers exist
     bool factored = false;
     double threshold = 0.0;
 
-    // static (per pattern)
-    DevBuf d_bs, d_is_piv, d_bin, d_csr_off, d_csc_off, d_csc_idx, d_upd_off, d_upd_ki, d_upd_kj,
-        d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot, d_rfo;
-    // numeric
+    // numeric (per instance)
     DevBuf d_vals, d_val_in, d_bout, d_ix_out, d_val_out, d_tri, d_rn;
     DevBuf d_if_val, d_if_idx, d_if_cnt, d_rn_part, d_drop_cnt, d_overflow, d_piv_loc, d_carry;
+    DevBuf d_keep, d_granted, d_chunk;
+    DevBuf d_piv_umask, d_defer_list, d_defer_cnt;
     // solve layout
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
-    DevBuf d_totals;
+    DevBuf d_totals, d_scan_tmp;
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
+    // single-launch triangular solves: flags, wait error, epochs
+    DevBuf d_flag_u, d_flag_t, d_trsv_err;
+    culip_solve_algorithm solve_algo = CULIP_TRSV_LEVELS;
+    mutable int32_t epoch_u = 0, epoch_t = 0;
 
     // host copies for inspection
     std::map<std::string, std::vector<int32_t>> host_int;
     std::map<std::string, std::vector<double>> host_real;
+
+    Bildlt() : Bildlt(std::make_shared<BildltPattern>()) {}
+    explicit Bildlt(std::shared_ptr<BildltPattern> p)
+        : pat(p), A(p->A), piv_starts(p->piv_starts), block_starts(p->block_starts), sym(p->sym),
+          have_sym(p->have_sym), d_bs(p->d_bs), d_is_piv(p->d_is_piv), d_bin(p->d_bin),
+          d_csr_off(p->d_csr_off), d_csc_off(p->d_csc_off), d_csc_idx(p->d_csc_idx),
+          d_upd_off(p->d_upd_off), d_upd_ki(p->d_upd_ki), d_upd_kj(p->d_upd_kj),
+          d_lvlb_idx(p->d_lvlb_idx), d_block_in_lvl(p->d_block_in_lvl), d_lvl_rows(p->d_lvl_rows),
+          d_ix_in(p->d_ix_in), d_value_slot(p->d_value_slot), d_rfo(p->d_rfo),
+          d_lq_cap(p->d_lq_cap), d_lq_fr(p->d_lq_fr), d_off_task(p->d_off_task),
+          d_upd4(p->d_upd4), d_row_block(p->d_row_block), prec(p->prec)
+    {
+    }
+    Bildlt(const Bildlt&) = delete;
+    Bildlt& operator=(const Bildlt&) = delete;
 
     int32_t n() const { return A.m; }
     bool f32() const { return prec == CULIP_PREC_F32; }
     size_t vt() const { return f32() ? sizeof(float) : sizeof(double); }
 
     void symbolic(int32_t fill_level, double fill_factor);
+    // static structures of the pattern (once) and this instance's buffers
     void upload_pattern();
+    // device bytes of this instance's numeric buffers only
+    int64_t numeric_bytes() const;
     void factor(const double* vals, bool vals_on_device, double thr, hipStream_t st);
     void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
     void trsv_u(double* x, hipStream_t st) const;
     void trsv_ut(double* x, hipStream_t st) const;
+    // fails if a single-launch solve gave up waiting (synchronises st)
+    void check_solve_error(hipStream_t st) const;
     SolveTopo topo() const;
     int64_t device_bytes() const;
     void download_factor();
@@ -107,6 +154,9 @@ struct Spmv {
         hipStream_t st) const;
 };
 
+// device copy of a host CSR matrix (row_ptr, col, val buffers)
+void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h);
+
 struct Sqmr {
     const Spmv* A = nullptr;
     const Bildlt* prec = nullptr;
@@ -115,7 +165,15 @@ struct Sqmr {
     int32_t* host_flags = nullptr;
     double* host_scal = nullptr;
     SqmrWork w;
+    Sqmr() = default;
+    Sqmr(const Sqmr&) = delete;
+    Sqmr& operator=(const Sqmr&) = delete;
     ~Sqmr();
+    // binds the operator and preconditioner (borrowed) and allocates the
+    // work vectors
+    void init(const Spmv* op, const Bildlt* p, const culip_sqmr_opts& o);
+    // SQMR::solve on device vectors; synchronises st
+    void run(const double* b, double* x, culip_sqmr_result& res, hipStream_t st);
 };
 
 }  // namespace culip

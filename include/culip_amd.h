@@ -150,6 +150,17 @@ culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
 
 int32_t culip_bildlt_n(const culip_bildlt * h);
 
+/* set_solve_algorithm (block_ildlt.cuh:226-228): schedule of the block
+ * triangular solves. LEVELS (default): one launch per level. FLAGGED: one
+ * launch per solve, block rows in level order, each waiting on per-row
+ * flags of the rows it reads (slower on gfx950 at the kkt-400k size: the
+ * agent-scope release/acquire per block row costs more than the level
+ * launches save). Both compute the same values in the same order. Solves
+ * with one object must be ordered on one stream. */
+typedef enum { CULIP_TRSV_LEVELS = 0, CULIP_TRSV_FLAGGED = 1 } culip_solve_algorithm;
+culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
+    culip_solve_algorithm algorithm);
+
 /* Host views of the symbolic structures and (after compute) of the factor
  * downloaded from the device; valid until the next call on h.
  * int names: block_starts, is_piv, coarse_csr_off, coarse_csr_col,

@@ -245,3 +245,27 @@ def test_cpp_driver(hip_device, prec):
     import json
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["true_rel_residual"] <= 1e-8 and out["max_err"] < 1e-5, out
+
+
+@pytest.mark.parametrize("t", ["d", "f"])
+def test_flagged_solves_equal_level_solves(hip_device, t):
+    """The single-launch (flag-synchronised) triangular solves give the same
+    bits as the per-level launches, on repeated calls (epochs advance)."""
+    import torch
+    k, p = _kkt(4000, 1600, 16, ca.ORDER_BLOCK_COLOR)
+    g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts,
+                      "f32" if t == "f" else "f64")
+    g.compute(0, 1.0, 1e-4)
+    b = torch.tensor(np.random.default_rng(9).normal(size=p.row_ptr.size - 1), device=hip_device)
+    outs = {}
+    for algo in ("levels", "flagged"):
+        g.set_solve_algorithm(algo)
+        for side in ("left", "right"):
+            for _ in range(3):
+                x = torch.zeros_like(b)
+                getattr(g, f"solve_{side}")(b, x)
+                torch.cuda.synchronize()
+                outs.setdefault(side, []).append(x.cpu().numpy())
+    for side, xs in outs.items():
+        for x in xs[1:]:
+            assert np.array_equal(x, xs[0]), side
