@@ -37,8 +37,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="kkt-400k", choices=sorted(WORKLOADS))
-    ap.add_argument("--prec", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--workload", default="kkt-400k",
+                    choices=sorted(WORKLOADS) + ["serve", "batch"])
+    ap.add_argument("--prec", default=None, choices=["f64", "f32"],
+                    help="factor precision (default f64; f32 for serve and batch)")
+    # serve / batch
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--patterns", type=int, default=64)
+    ap.add_argument("--n-lo", type=int, default=5000)
+    ap.add_argument("--n-hi", type=int, default=200000)
+    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="offered requests/s (0: 85%% of the warm-up throughput)")
+    ap.add_argument("--max-block", type=int, default=32)
+    ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--order", default="color", choices=["color", "natural"])
     ap.add_argument("--fill-factor", type=float, default=1.0)
     ap.add_argument("--threshold", type=float, default=1e-4)
@@ -77,8 +89,8 @@ def cpu_baseline(wl, order, seed, fill_factor, threshold):
     b = kkt.rhs[prep.perm]
     o = ol.OracleBildlt(prep.row_ptr, prep.col, prep.val, prep.piv_starts, prep.block_starts,
                         "d")
+    o.symbolic(0, fill_factor)   # cached per pattern, as on the GPU path: not timed
     t0 = time.perf_counter()
-    o.symbolic(0, fill_factor)
     o.factor(threshold)
     t1 = time.perf_counter()
     x, r = ol.sqmr_bildlt(prep.row_ptr, prep.col, prep.val, o, b, 1e-8, 1000)
@@ -87,9 +99,211 @@ def cpu_baseline(wl, order, seed, fill_factor, threshold):
     ratio = nnz_full / prep.col.size
     t_solve = (t2 - t0) * ratio
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle full solve of the same generator at n={n}, m={m} (seed {seed}): "
-                      f"symbolic+factor {t1 - t0:.2f}s, SQMR {r.iterations} iterations "
-                      f"{t2 - t1:.2f}s; scaled by nnz ratio {ratio:.2f} to the workload"}
+            "sample": f"oracle factor + SQMR (reference stopping test) of the same generator "
+                      f"at n={n}, m={m} (seed {seed}): factor {t1 - t0:.2f}s, SQMR "
+                      f"{r.iterations} iterations {t2 - t1:.2f}s; scaled by nnz ratio "
+                      f"{ratio:.2f} to the workload"}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _pattern(ca, n, seed, B, order):
+    k = ca.KKT(n, int(0.4 * n), 4, 3, seed=seed)
+    p = ca.Prep(k.row_ptr, k.col, k.val, n, B, order)
+    return k, p
+
+
+def bench_serve(a):
+    """Poisson stream of refactor + solve requests over resident patterns
+    (n ~ U[5k, 200k], m = 0.4 n); each request rescales the barrier part of
+    the H diagonal (H_ii = H0_ii + w_i, w_i log-uniform in [1e-4, 1e4])."""
+    import concurrent.futures as cf
+    import threading
+    import culip_amd as ca
+    rng = np.random.default_rng(a.seed)
+    sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
+    order = ca.ORDER_BLOCK_COLOR
+    srv = ca.Server(slots=a.slots, prec=a.prec)
+    nsets = 2
+
+    def make(i):
+        k, p = _pattern(ca, int(sizes[i]), 1000 + i, a.max_block, order)
+        pid = srv.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+        vals = [p.permute_values(k.barrier_values(1e-4, 1e4, seed=10 * i + s))
+                for s in range(nsets)]
+        return pid, vals, k.rhs[p.perm].copy(), p.col.size
+
+    t_setup = time.perf_counter()
+    with cf.ThreadPoolExecutor(16) as ex:
+        pats = list(ex.map(make, range(a.patterns)))
+    t_setup = time.perf_counter() - t_setup
+    log(f"{len(pats)} patterns registered in {t_setup:.1f}s")
+    picks = rng.integers(0, a.patterns, size=a.requests + 2 * a.patterns)
+    sets = rng.integers(0, nsets, size=picks.size)
+
+    def run(order_ix, gaps):
+        ids = [None] * len(order_ix)
+
+        def feeder():
+            t = time.perf_counter()
+            for q, j in enumerate(order_ix):
+                if gaps is not None:
+                    t += gaps[q]
+                    d = t - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                pid, vals, rhs, _ = pats[picks[j]]
+                ids[q] = srv.submit(pid, vals[sets[j]], rhs)
+
+        th = threading.Thread(target=feeder)
+        t0 = time.perf_counter()
+        th.start()
+        out = []
+        for q in range(len(order_ix)):
+            while ids[q] is None:
+                time.sleep(0.0005)
+            out.append(srv.wait(ids[q]))
+            if (q + 1) % 100 == 0:
+                log(f"{q + 1} requests done")
+        th.join()
+        return out, time.perf_counter() - t0
+
+    # warm-up: every pattern once per slot set-up, untimed; its throughput
+    # calibrates the offered load of the timed stream
+    warm, t_warm = run(list(range(a.requests, a.requests + 2 * a.patterns)), None)
+    thr = len(warm) / t_warm
+    rate = a.rate if a.rate > 0 else 0.85 * thr
+    gaps = rng.exponential(1.0 / rate, size=a.requests)
+    res, elapsed = run(list(range(a.requests)), gaps)
+    lat = np.array([r[2]["latency"] for r in res]) * 1e3
+    que = np.array([r[2]["queued"] for r in res]) * 1e3
+    its = np.array([r[1]["iterations"] for r in res])
+    trr = max(r[1]["true_rel_residual"] for r in res)
+    first = min(r[2]["submit"] for r in res)
+    last = max(r[2]["submit"] + r[2]["latency"] for r in res)
+    st = srv.stats()
+    srv.close()
+    return {
+        "metric": "KKT solves/sec to rel. residual 1e-8 (FP32-factor vs FP64); p50/p99 solve latency",
+        "value": len(res) / (last - first), "unit": "solves/s", "n_gpus": 1,
+        "steps": len(res), "warmup": len(warm), "ms_per_step": (last - first) / len(res) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.prec == "f64" else "f32-factor/f64-sqmr", "data": "synthetic",
+        "config": {"workload": "serve", "patterns": a.patterns, "n_range": [a.n_lo, a.n_hi],
+                   "m": "0.4n", "max_block": a.max_block, "slots": a.slots,
+                   "requests": len(res), "offered_rate_per_s": rate,
+                   "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "parallelism": "none"},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "queue_p50_ms": float(np.percentile(que, 50)),
+        "service_p50_ms": float(np.percentile(lat - que, 50)),
+        "sqmr_iterations_mean": float(its.mean()), "max_true_rel_residual": trr,
+        "warmup_solves_per_s": thr, "setup_s": t_setup,
+        "device_bytes_peak": st["peak"], "device_budget": st["budget"],
+        "instances": st["instances"],
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_batch(a):
+    """512 independent systems (n ~ U[1k, 40k], m = 0.4 n, seeds 0..511)
+    solved as one batch on `slots` concurrent streams with the block size
+    picked per size bucket by timing {8, 16, 32}; beside it the plain FP64
+    path, one system at a time, on the same systems."""
+    import concurrent.futures as cf
+    import culip_amd as ca
+    rng = np.random.default_rng(a.seed)
+    sizes = rng.integers(1000, 40001, size=a.systems)
+    order = ca.ORDER_BLOCK_COLOR
+    buckets = [(1000, 10000), (10000, 20000), (20000, 30000), (30000, 40001)]
+
+    def bucket(n):
+        return next(i for i, (lo, hi) in enumerate(buckets) if lo <= n < hi)
+
+    # autotuner: per bucket, the block size with the shortest refactor +
+    # solve on the bucket's first system (FP32 factor)
+    import torch
+    dev = torch.device("cuda:0")
+    tuned, tune_ms = {}, {}
+    for bi in range(len(buckets)):
+        idx = [i for i in range(a.systems) if bucket(sizes[i]) == bi]
+        if not idx:
+            continue
+        i = idx[0]
+        best = None
+        for B in (8, 16, 32):
+            k, p = _pattern(ca, int(sizes[i]), i, B, order)
+            g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, a.prec)
+            g.compute(0, 1.0, 1e-4)
+            A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+            q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
+            b = torch.tensor(k.rhs[p.perm], device=dev)
+            x = torch.zeros_like(b)
+            dv = torch.tensor(p.val, device=dev)
+            for _ in range(2):
+                g.refactor(dv, 1e-4)
+                q.solve(b, x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                g.refactor(dv, 1e-4)
+                q.solve(b, x)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / 3 * 1e3
+            tune_ms[f"{buckets[bi][0]}-{buckets[bi][1] - 1}:B{B}"] = ms
+            if best is None or ms < best[0]:
+                best = (ms, B)
+        tuned[bi] = best[1]
+        log(f"autotune bucket {buckets[bi]}: block size {best[1]}")
+
+    def make(i):
+        k, p = _pattern(ca, int(sizes[i]), i, tuned[bucket(sizes[i])], order)
+        return p, k.rhs[p.perm].copy()
+
+    with cf.ThreadPoolExecutor(16) as ex:
+        systems = list(ex.map(make, range(a.systems)))
+    log(f"{len(systems)} systems generated")
+
+    def solve_all(slots, prec):
+        srv = ca.Server(slots=slots, prec=prec)
+        with cf.ThreadPoolExecutor(16) as ex:
+            pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,
+                                                         s[0].piv_starts, s[0].block_starts),
+                               systems))
+        # first pass creates the per-slot instances; the second is timed
+        for rep in range(2):
+            t0 = time.perf_counter()
+            ids = [srv.submit(pid, s[0].val, s[1]) for pid, s in zip(pids, systems)]
+            out = [srv.wait(r) for r in ids]
+            el = time.perf_counter() - t0
+            log(f"batch pass {rep} ({slots} slots, {prec}): {len(out) / el:.1f} solves/s")
+        srv.close()
+        return out, el
+
+    res, el = solve_all(a.slots, a.prec)
+    base, el_base = solve_all(1, "f64")
+    trr = max(r[1]["true_rel_residual"] for r in res)
+    lat = np.array([r[2]["latency"] for r in res]) * 1e3
+    return {
+        "metric": "KKT solves/sec to rel. residual 1e-8 (FP32-factor vs FP64); p50/p99 solve latency",
+        "value": len(res) / el, "unit": "solves/s", "n_gpus": 1, "steps": len(res),
+        "warmup": len(res), "ms_per_step": el / len(res) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.prec == "f64" else "f32-factor/f64-sqmr", "data": "synthetic",
+        "config": {"workload": "batch", "systems": a.systems, "n_range": [1000, 40000],
+                   "m": "0.4n", "seeds": f"0..{a.systems - 1}", "slots": a.slots,
+                   "max_block_by_bucket": {f"{buckets[b][0]}-{buckets[b][1] - 1}": tuned[b]
+                                           for b in tuned},
+                   "tol": 1e-8, "parallelism": "none"},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "max_true_rel_residual": trr,
+        "plain_f64_per_system_solves_per_s": len(base) / el_base,
+        "speedup_vs_plain_f64_per_system": (len(res) / el) / (len(base) / el_base),
+        "plain_max_true_rel_residual": max(r[1]["true_rel_residual"] for r in base),
+        "autotune_ms": tune_ms,
+        "roofline": None, "cpu_baseline": None,
+    }
 
 
 def main():
@@ -102,6 +316,14 @@ def main():
     import torch
     import culip_amd as ca
 
+    if a.workload in ("serve", "batch"):
+        if a.prec is None:
+            a.prec = "f32"
+        out = bench_serve(a) if a.workload == "serve" else bench_batch(a)
+        print(json.dumps(out))
+        return
+    if a.prec is None:
+        a.prec = "f64"
     wl = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
     kkt = ca.KKT(wl["n"], wl["m"], wl["o"], wl["c"], h_diag=wl["h_diag"], h_diag_log=wl["log"],
@@ -167,6 +389,31 @@ def main():
                "launches": int(sc[6])}
     dominant, secondary = (rl_trsv, rl_spmv) if per_it_trsv >= per_it_spmv else (rl_spmv, rl_trsv)
 
+    # the same step with the FP32 factor (FP64 SQMR, FP64 true residual)
+    other = None
+    if a.prec == "f64":
+        g32 = ca.BlockiLDLt(prep.row_ptr, prep.col, prep.val, prep.piv_starts,
+                            prep.block_starts, "f32")
+        g32.symbolic(0, a.fill_factor)
+        s32 = ca.SQMR(A, 1e-8, 1000, g32, true_residual_period=20, max_restarts=10)
+        x32 = torch.zeros_like(x)
+        for _ in range(a.warmup):
+            g32.refactor(dvals, a.threshold)
+            s32.solve(db, x32)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter()
+        for _ in range(a.steps):
+            g32.refactor(dvals, a.threshold)
+            s32.solve(db, x32)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t32
+        r32 = s32.result
+        other = {"value": a.steps / t32, "ms_per_step": t32 / a.steps * 1e3,
+                 "sqmr_iterations": r32["iterations"], "refinement_restarts": r32["restarts"],
+                 "true_rel_residual": r32["true_rel_residual"],
+                 "max_abs_diff_vs_f64_solution": float((x32 - x).abs().max())}
+        del g32, s32
+
     cpu = None
     if not a.no_cpu_baseline:
         try:
@@ -203,6 +450,8 @@ def main():
         "roofline_secondary": secondary,
         "cpu_baseline": cpu,
     }
+    if other is not None:
+        out["f32_factor"] = other
     print(json.dumps(out))
 
 

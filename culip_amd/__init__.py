@@ -17,7 +17,7 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
                    ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_FLAGGED, check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
-           "SQMR", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
+           "SQMR", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
            "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR"]
 
 
@@ -276,6 +276,71 @@ class SQMR:
         if getattr(self, "_h", None):
             lib().culip_sqmr_destroy(self._h)
             self._h = None
+
+
+class Server:
+    """Request loop for KKT solves over resident sparsity patterns.
+
+    add_pattern(K, piv_starts, block_starts) -> pattern id (symbolic analysis
+    and upload, once); submit(pattern, vals, rhs) -> request id (host arrays,
+    copied); wait(request) -> (x, result, timing, slot) with timing =
+    {queued, service, latency, submit} seconds."""
+
+    def __init__(self, slots=4, prec="f32", hbm_budget=0, fill_level=0, fill_factor=1.0,
+                 threshold=1e-4, tol=1e-8, max_it=1000, true_residual_period=20,
+                 max_restarts=10):
+        pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
+        o = _lib.ServerOpts(slots, int(hbm_budget), pr, fill_level, fill_factor, threshold,
+                            _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts))
+        h = C.c_void_p()
+        check(lib().culip_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._n = {}
+        self._req_n = {}
+
+    def add_pattern(self, row_ptr, col, val, piv_starts, block_starts):
+        s, keep = csr_struct(row_ptr, col, val)
+        ps, bs = _i32(piv_starts), _i32(block_starts)
+        p = C.c_int32()
+        check(lib().culip_server_add_pattern(self._h, C.byref(s), len(ps), _ptr(ps, C.c_int32),
+                                             len(bs), _ptr(bs, C.c_int32), C.byref(p)))
+        self._n[p.value] = len(row_ptr) - 1
+        return p.value
+
+    def submit(self, pattern, vals, rhs):
+        v, b = _f64(vals), _f64(rhs)
+        r = C.c_int64()
+        check(lib().culip_server_submit(self._h, pattern, _ptr(v, C.c_double),
+                                        _ptr(b, C.c_double), C.byref(r)))
+        self._req_n[r.value] = b.size
+        return r.value
+
+    def wait(self, request):
+        x = np.empty(self._req_n.pop(request))
+        res = _lib.SqmrResult()
+        t = np.empty(4)
+        slot = C.c_int32()
+        check(lib().culip_server_wait(self._h, request, _ptr(x, C.c_double), C.byref(res),
+                                      _ptr(t, C.c_double), C.byref(slot)))
+        result = {"residual": res.residual, "iterations": res.iterations,
+                  "converged": bool(res.converged), "restarts": res.restarts,
+                  "true_rel_residual": res.true_rel_residual}
+        return x, result, {"queued": t[0], "service": t[1], "latency": t[2],
+                           "submit": t[3]}, slot.value
+
+    def stats(self):
+        out = np.zeros(5, np.int64)
+        check(lib().culip_server_stats(self._h, _ptr(out, C.c_int64)))
+        return {"used": int(out[0]), "peak": int(out[1]), "budget": int(out[2]),
+                "instances": int(out[3]), "completed": int(out[4])}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().culip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def read_matrix_csr(path):

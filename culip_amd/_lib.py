@@ -50,6 +50,12 @@ class SqmrResult(C.Structure):
                 ("true_rel_residual", C.c_double)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("prec", C.c_int),
+                ("fill_level", C.c_int32), ("fill_factor", C.c_double),
+                ("threshold", C.c_double), ("sqmr", SqmrOpts)]
+
+
 _lib = None
 
 # (name, restype, argtypes)
@@ -106,6 +112,17 @@ _SIGS = [
     ("culip_sqmr_solve", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SqmrResult),
                                    C.c_void_p]),
     ("culip_sqmr_destroy", None, [C.c_void_p]),
+    ("culip_server_create", C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    ("culip_server_add_pattern", C.c_int, [C.c_void_p, C.POINTER(Csr), C.c_int32,
+                                           C.POINTER(C.c_int32), C.c_int32,
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("culip_server_submit", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    ("culip_server_wait", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double),
+                                    C.POINTER(SqmrResult), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int32)]),
+    ("culip_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("culip_server_destroy", None, [C.c_void_p]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -321,13 +321,48 @@ __device__ VT row_ssq(const VT* T, int r, int w)
     return s;
 }
 
+// dependency blocks staged per round of the pivot update; the first tile
+// holds the pivot tile for the factorization afterwards
+constexpr int PIV_ROUND = 4;
+
 template<typename VT>
 struct alignas(16) PivLds {
-    VT T[1024];
-    VT R[1024];
-    VT cs[32], cd[32], cb[32];
-    uint32_t mask[32];
+    VT X[PIV_ROUND][1024];
+    VT cs[PIV_ROUND][32], cd[PIV_ROUND][32], cb[PIV_ROUND][32];
+    uint32_t mask[PIV_ROUND][32];
 };
+
+// T (registers, rows c0 + 2t of column j) -= A' D A for a staged tile A
+// whose nonzero rows per column are in mask; the arithmetic and order of
+// product() with R = A, one output row at a time
+template<typename VT>
+__device__ void product_regs(VT (&T)[16], const VT* A, const uint32_t* mask, const VT* cs,
+    const VT* cd, const VT* cb, int len)
+{
+    const int lane = lane_id();
+    const int j = lane & 31;
+    const int c0 = lane >> 5;
+    const uint32_t mk = (j < len) ? mask[j] : 0u;
+    if(mk) {
+#pragma unroll
+        for(int t = 0; t < 16; ++t) {
+            const int c = c0 + 2 * t;
+            if(c < len) {
+                VT acc = 0;
+                for(uint32_t bits = mk; bits; bits &= bits - 1) {
+                    const int l = __ffs(bits) - 1;
+                    const VT r = A[32 * l + j];
+                    const VT left = (l == 0) ? (VT) 0 : A[32 * (l - 1) + c];
+                    const VT mid = A[32 * l + c];
+                    const VT right = (l == 31) ? (VT) 0 : A[32 * (l + 1) + c];
+                    const VT lt = fma(cb[l], right, fma(cd[l], mid, cs[l] * left));
+                    acc = fma(r, lt, acc);
+                }
+                T[t] -= acc;
+            }
+        }
+    }
+}
 
 template<typename VT>
 struct alignas(16) OffLds {
@@ -355,55 +390,92 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
     const int len = a.bs[i + 1] - fr;
     const int cb0 = a.csc_off[i];
     const int nd = a.csc_off[i + 1] - 1 - cb0;
-    tile_zero(s.R);
-    if(lane < 32) s.mask[lane] = 0;
-    tile_load(s.T, m, a.bs, a.ix_in, a.val_in);
-    // updates from the blocks (k, i) above the diagonal, in CSC order
+    const int j = lane & 31, cq = lane >> 5;
+    // the pivot tile in registers (rows cq + 2t of column j)
+    VT* T0 = s.X[0];
+    tile_load(T0, m, a.bs, a.ix_in, a.val_in);
+    VT T[16];
+#pragma unroll
+    for(int t = 0; t < 16; ++t) T[t] = T0[32 * (cq + 2 * t) + j];
+    wave_sync();
+#pragma unroll
+    for(int u = 0; u < PIV_ROUND; ++u) tile_zero(s.X[u]);
+    if(lane < 32) {
+#pragma unroll
+        for(int u = 0; u < PIV_ROUND; ++u) s.mask[u][lane] = 0;
+    }
+    wave_sync();
+    // updates from the blocks (k, i) above the diagonal, in CSC order:
+    // PIV_ROUND blocks staged per round, their products subtracted in order
     for(int c0 = 0; c0 < nd; c0 += 64) {
         Dep mine{};
         if(c0 + lane < nd) mine = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + lane]);
         const int cnt = min(64, nd - c0);
-        Dep cur = bcast(mine, 0);
-        Pre<VT> pc = prefetch(cur, a.ix_out, a.val_out, a.tri, true);
-        for(int d = 0; d < cnt; ++d) {
-            Dep nxt = cur;
-            Pre<VT> pn = pc;
-            if(d + 1 < cnt) {
-                nxt = bcast(mine, d + 1);
-                pn = prefetch(nxt, a.ix_out, a.val_out, a.tri, true);
+        Dep cur[PIV_ROUND];
+        Pre<VT> pc[PIV_ROUND];
+#pragma unroll
+        for(int u = 0; u < PIV_ROUND; ++u) {
+            cur[u] = bcast(mine, u < cnt ? u : 0);
+            if(u < cnt) pc[u] = prefetch(cur[u], a.ix_out, a.val_out, a.tri, true);
+        }
+        for(int d0 = 0; d0 < cnt; d0 += PIV_ROUND) {
+            const int nr = min(PIV_ROUND, cnt - d0);
+            Dep nxt[PIV_ROUND];
+            Pre<VT> pn[PIV_ROUND];
+#pragma unroll
+            for(int u = 0; u < PIV_ROUND; ++u) {
+                const int dn = d0 + PIV_ROUND + u;
+                nxt[u] = bcast(mine, dn < cnt ? dn : 0);
+                if(dn < cnt) pn[u] = prefetch(nxt[u], a.ix_out, a.val_out, a.tri, true);
             }
-            stage(s.R, s.mask, cur, pc, len, a.ix_out, a.val_out);
-            stage_d(s.cs, s.cd, s.cb, pc);
+#pragma unroll
+            for(int u = 0; u < PIV_ROUND; ++u) {
+                if(u < nr) {
+                    stage(s.X[u], s.mask[u], cur[u], pc[u], len, a.ix_out, a.val_out);
+                    stage_d(s.cs[u], s.cd[u], s.cb[u], pc[u]);
+                }
+            }
             wave_sync();
-            product(s.T, s.R, s.R, s.mask, s.cs, s.cd, s.cb, len, len);
+#pragma unroll
+            for(int u = 0; u < PIV_ROUND; ++u)
+                if(u < nr) product_regs(T, s.X[u], s.mask[u], s.cs[u], s.cd[u], s.cb[u], len);
             wave_sync();
-            unstage(s.R, s.mask, cur, pc, len);
+#pragma unroll
+            for(int u = 0; u < PIV_ROUND; ++u)
+                if(u < nr) unstage(s.X[u], s.mask[u], cur[u], pc[u], len);
             wave_sync();
-            cur = nxt;
-            pc = pn;
+#pragma unroll
+            for(int u = 0; u < PIV_ROUND; ++u) {
+                cur[u] = nxt[u];
+                pc[u] = pn[u];
+            }
         }
     }
+    // the updated tile back to LDS (tile 0 is zero again after unstaging)
+#pragma unroll
+    for(int t = 0; t < 16; ++t) T0[32 * (cq + 2 * t) + j] = T[t];
+    wave_sync();
     // d_factor_pivot_factorize with static pivots (kernels :956-1083)
     double* tri = a.tri + 3 * (int64_t) fr;
     int p = 0;
     while(p < len) {
         const int order = (p < len - 1 && !a.is_piv[fr + p + 1]) ? 2 : 1;
-        if(order == 1) step_1x1(s.T, tri, p, len);
-        else step_2x2(s.T, tri, p, len);
+        if(order == 1) step_1x1(T0, tri, p, len);
+        else step_2x2(T0, tri, p, len);
         p += order;
     }
     if(lane < len) {
-        a.rn[fr + lane] = row_ssq(s.T, lane, len);
+        a.rn[fr + lane] = row_ssq(T0, lane, len);
         // nonzero columns right of the diagonal, for the U_ii' solves
         uint32_t um = 0;
         for(int c = lane + 1; c < len; ++c)
-            if(s.T[32 * lane + c] != (VT) 0) um |= 1u << c;
+            if(T0[32 * lane + c] != (VT) 0) um |= 1u << c;
         a.piv_umask[(int64_t) q * 32 + lane] = um;
     }
     VT* out = a.if_val + (int64_t) q * 1024;
     for(int e = lane; e < len * len; e += 64) {
         const int r = e / len, c = e - r * len;
-        out[e] = s.T[32 * r + c];
+        out[e] = T0[32 * r + c];
     }
     if(lane == 0) {
         a.if_cnt[q] = len * len;

@@ -9,6 +9,11 @@
 // is one launch: one wave per block row gathers its rows with coalesced loads
 // and a segmented wave reduction (fixed order, no atomics), then applies
 // U_ii^-1. k_solve_D follows helper_kernels.impl.cuh:720-772.
+// Factor entries are read in their storage precision (FP32 for an FP32
+// factor) and every product and sum runs in FP64: the preconditioner stays
+// one fixed linear operator to FP64 rounding, which SQMR's short
+// recurrences need (FP32 sums make it vary by ~1e-7 from apply to apply and
+// cost the recurrence most of its progress below that level).
 #include "solve.hpp"
 #include "wave.hpp"
 
@@ -41,6 +46,38 @@ __device__ void diag_inverse(const VT* U, int h, double* X)
     wave_sync();
 }
 
+// up to 64 factored blocks staged one per lane: metadata, and the entries
+// of sparse blocks with at most PLAN_E of them
+constexpr int PLAN_E = 8;
+
+template<typename VT>
+struct PlanStage {
+    int32_t fmt[64], nnz[64], cp[64], vp[64], off[64], ext[64];
+    uint8_t small[64];
+    int16_t ix[64 * PLAN_E];
+    VT v[64 * PLAN_E];
+};
+
+// off / ext: first fine index and size of the block's other dimension
+template<typename VT>
+__device__ void plan_stage(PlanStage<VT>& sg, int k, const BlockMeta& m, int off, int ext,
+    const int32_t* ix_out, const VT* val_out)
+{
+    sg.fmt[k] = m.format;
+    sg.nnz[k] = m.nnz;
+    sg.cp[k] = m.col_ptr;
+    sg.vp[k] = m.val_ptr;
+    sg.off[k] = off;
+    sg.ext[k] = ext;
+    const bool sm = m.format == FMT_SPARSE && m.nnz <= PLAN_E;
+    sg.small[k] = sm ? 1 : 0;
+    if(sm)
+        for(int e = 0; e < m.nnz; ++e) {
+            sg.ix[PLAN_E * k + e] = (int16_t) ix_out[(int64_t) m.col_ptr + e];
+            sg.v[PLAN_E * k + e] = val_out[(int64_t) m.val_ptr + e];
+        }
+}
+
 template<typename VT, bool FILL>
 __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
 {
@@ -70,76 +107,118 @@ __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out
         }
         if(!FILL) { p.ui_ptr[fr + lane + 1] = cr; p.ti_ptr[fr + lane + 1] = cc; }
     }
-    // off-diagonal blocks of block row i (uo), by block id
-    if(lane < h) {
-        int cnt = 0;
-        const int32_t base = FILL ? p.uo_ptr[fr + lane] : 0;
-        for(int b = t.csr_off[i] + 1; b < t.csr_off[i + 1]; ++b) {
-            const BlockMeta m = t.bout[b];
-            const int fc = t.bs[m.block_col];
-            const int w = t.bs[m.block_col + 1] - fc;
-            const VT* v = val_out + m.val_ptr;
-            if(m.format == FMT_DENSE) {
-                for(int c = 0; c < w; ++c) {
-                    const VT x = v[lane * w + c];
-                    if(x != (VT) 0) {
-                        if(FILL) { p.uo_col[base + cnt] = fc + c; p.uo_val[base + cnt] = x; p.uo_lr[base + cnt] = (uint8_t) lane; }
+    // off-diagonal blocks of block row i (uo), by block id; 64 blocks at a
+    // time are staged (metadata and small entry lists) one per lane, then
+    // walked in order by the lanes of the fine rows
+    __shared__ PlanStage<VT> sg;
+    int cnt = 0;
+    const int32_t base = (FILL && lane < h) ? p.uo_ptr[fr + lane] : 0;
+    const int b0 = t.csr_off[i] + 1, b1 = t.csr_off[i + 1];
+    for(int c0 = b0; c0 < b1; c0 += 64) {
+        const int nblk = min(64, b1 - c0);
+        wave_sync();
+        if(lane < nblk) {
+            const BlockMeta m = t.bout[c0 + lane];
+            plan_stage(sg, lane, m, t.bs[m.block_col], t.bs[m.block_col + 1] - t.bs[m.block_col],
+                t.ix_out, val_out);
+        }
+        wave_sync();
+        for(int k = 0; k < nblk; ++k) {
+            const int fc = sg.off[k], w = sg.ext[k];
+            if(sg.small[k]) {
+                for(int e = 0; e < sg.nnz[k]; ++e) {
+                    const int q = sg.ix[PLAN_E * k + e];
+                    const int r = q / w;
+                    if(r == lane) {
+                        if(FILL) { p.uo_col[base + cnt] = fc + (q - r * w); p.uo_val[base + cnt] = sg.v[PLAN_E * k + e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
                         ++cnt;
                     }
                 }
-            } else {
-                const int32_t* ix = t.ix_out + m.col_ptr;
-                for(int e = 0; e < m.nnz; ++e) {
-                    const int q = ix[e];
-                    const int r = q / w;
-                    if(r == lane) {
-                        if(FILL) { p.uo_col[base + cnt] = fc + (q - r * w); p.uo_val[base + cnt] = v[e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
-                        ++cnt;
+            } else if(lane < h) {
+                const VT* v = val_out + sg.vp[k];
+                if(sg.fmt[k] == FMT_DENSE) {
+                    for(int c = 0; c < w; ++c) {
+                        const VT x = v[lane * w + c];
+                        if(x != (VT) 0) {
+                            if(FILL) { p.uo_col[base + cnt] = fc + c; p.uo_val[base + cnt] = x; p.uo_lr[base + cnt] = (uint8_t) lane; }
+                            ++cnt;
+                        }
+                    }
+                } else {
+                    const int32_t* ix = t.ix_out + sg.cp[k];
+                    for(int e = 0; e < sg.nnz[k]; ++e) {
+                        const int q = ix[e];
+                        const int r = q / w;
+                        if(r == lane) {
+                            if(FILL) { p.uo_col[base + cnt] = fc + (q - r * w); p.uo_val[base + cnt] = v[e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
+                            ++cnt;
+                        }
                     }
                 }
             }
         }
-        if(!FILL) p.uo_ptr[fr + lane + 1] = cnt;
     }
+    if(!FILL && lane < h) p.uo_ptr[fr + lane + 1] = cnt;
 }
 
 template<typename VT, bool FILL>
 __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
 {
+    __shared__ PlanStage<VT> sg;
     const int lane = lane_id();
     const int j = blockIdx.x;
     const int fc = t.bs[j];
     const int w = t.bs[j + 1] - fc;
-    if(lane >= w) return;
     int cnt = 0;
-    const int32_t base = FILL ? p.to_ptr[fc + lane] : 0;
-    // blocks of column j above the diagonal, ascending block row
-    for(int q = t.csc_off[j]; q < t.csc_off[j + 1] - 1; ++q) {
-        const BlockMeta m = t.bout[t.csc_idx[q]];
-        const int fr = t.bs[m.block_row];
-        const int h = t.bs[m.block_row + 1] - fr;
-        const VT* v = val_out + m.val_ptr;
-        if(m.format == FMT_DENSE) {
-            for(int r = 0; r < h; ++r) {
-                const VT x = v[r * w + lane];
-                if(x != (VT) 0) {
-                    if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = x; p.to_lr[base + cnt] = (uint8_t) lane; }
-                    ++cnt;
+    const int32_t base = (FILL && lane < w) ? p.to_ptr[fc + lane] : 0;
+    // blocks of column j above the diagonal, ascending block row, staged 64
+    // at a time as in k_plan_rows
+    const int q0 = t.csc_off[j], q1 = t.csc_off[j + 1] - 1;
+    for(int c0 = q0; c0 < q1; c0 += 64) {
+        const int nblk = min(64, q1 - c0);
+        wave_sync();
+        if(lane < nblk) {
+            const BlockMeta m = t.bout[t.csc_idx[c0 + lane]];
+            plan_stage(sg, lane, m, t.bs[m.block_row], t.bs[m.block_row + 1] - t.bs[m.block_row],
+                t.ix_out, val_out);
+        }
+        wave_sync();
+        for(int k = 0; k < nblk; ++k) {
+            const int fr = sg.off[k], h = sg.ext[k];
+            if(sg.small[k]) {
+                for(int e = 0; e < sg.nnz[k]; ++e) {
+                    const int qq = sg.ix[PLAN_E * k + e];
+                    const int r = qq / w;
+                    if(qq - r * w == lane) {
+                        if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = sg.v[PLAN_E * k + e]; p.to_lr[base + cnt] = (uint8_t) lane; }
+                        ++cnt;
+                    }
                 }
-            }
-        } else {
-            const int32_t* ix = t.ix_out + m.col_ptr;
-            for(int e = 0; e < m.nnz; ++e) {
-                const int qq = ix[e];
-                const int r = qq / w;
-                if(qq - r * w == lane) {
-                    if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = v[e]; p.to_lr[base + cnt] = (uint8_t) lane; }
-                    ++cnt;
+            } else if(lane < w) {
+                const VT* v = val_out + sg.vp[k];
+                if(sg.fmt[k] == FMT_DENSE) {
+                    for(int r = 0; r < h; ++r) {
+                        const VT x = v[r * w + lane];
+                        if(x != (VT) 0) {
+                            if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = x; p.to_lr[base + cnt] = (uint8_t) lane; }
+                            ++cnt;
+                        }
+                    }
+                } else {
+                    const int32_t* ix = t.ix_out + sg.cp[k];
+                    for(int e = 0; e < sg.nnz[k]; ++e) {
+                        const int qq = ix[e];
+                        const int r = qq / w;
+                        if(qq - r * w == lane) {
+                            if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = v[e]; p.to_lr[base + cnt] = (uint8_t) lane; }
+                            ++cnt;
+                        }
+                    }
                 }
             }
         }
     }
-    if(!FILL) p.to_ptr[fc + lane + 1] = cnt;
+    if(!FILL && lane < w) p.to_ptr[fc + lane + 1] = cnt;
 }
 
 // inclusive scan of a[1..n] in place (a[0] = 0), one workgroup, in chunks
@@ -240,8 +319,8 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
     const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
     const uint8_t* i_lr, double* x)
 {
-    __shared__ VT y[33];
-    __shared__ VT res[33];
+    __shared__ double y[33];     // FP64 accumulation for both factor precisions
+    __shared__ double res[33];
     const int lane = lane_id();
     const int4 t0 = task[2 * blockIdx.x];
     const int4 t1 = task[2 * blockIdx.x + 1];
@@ -255,7 +334,7 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
         ival = i_val[ib + lane];
         isg = i_lr[ib + lane];
     }
-    if(lane < h) { y[lane] = (VT) xo; res[lane] = 0; }
+    if(lane < h) { y[lane] = xo; res[lane] = 0; }
     for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
         int c[TRSV_PF], sg[TRSV_PF];
         VT v[TRSV_PF];
@@ -267,14 +346,14 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
             sg[k] = 64;
             if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
         }
-        VT xv[TRSV_PF];
+        double xv[TRSV_PF];
 #pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? (VT) x[c[k]] : (VT) 0;
+        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? x[c[k]] : 0.0;
         wave_sync();
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
             if(e0 + 64 * k >= oe) break;
-            VT p = seg_scan(v[k] * xv[k], sg[k]);
+            double p = seg_scan((double) v[k] * xv[k], sg[k]);
             const int nsg = __shfl_down(sg[k], 1, 64);
             if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
             wave_sync();
@@ -287,7 +366,7 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
             isg = 64;
             if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
         }
-        VT p = (isg < 64) ? ival * y[icol - fr] : (VT) 0;
+        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
         p = seg_scan(p, isg);
         const int nsg = __shfl_down(isg, 1, 64);
         if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
@@ -326,8 +405,8 @@ __global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t n
     const VT* i_val, const uint8_t* i_lr, const int32_t* row_block, int32_t* flag, int32_t epoch,
     int32_t* err, double* x)
 {
-    __shared__ VT y[33];
-    __shared__ VT res[33];
+    __shared__ double y[33];     // FP64 accumulation for both factor precisions
+    __shared__ double res[33];
     const int lane = lane_id();
     const int pos = reverse ? (nb - 1 - (int) blockIdx.x) : (int) blockIdx.x;
     const int4 t0 = task[2 * pos];
@@ -342,7 +421,7 @@ __global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t n
         ival = i_val[ib + lane];
         isg = i_lr[ib + lane];
     }
-    if(lane < h) { y[lane] = (VT) xo; res[lane] = 0; }
+    if(lane < h) { y[lane] = xo; res[lane] = 0; }
     for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
         int c[TRSV_PF], sg[TRSV_PF];
         VT v[TRSV_PF];
@@ -360,15 +439,19 @@ __global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t n
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k)
             if(dep[k] >= 0) wait_flag(flag + dep[k], epoch, err);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        VT xv[TRSV_PF];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // x of other block rows: agent-coherent loads (no cache-wide
+        // invalidation per block row)
+        double xv[TRSV_PF];
 #pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? (VT) x[c[k]] : (VT) 0;
+        for(int k = 0; k < TRSV_PF; ++k)
+            xv[k] = (sg[k] < 64) ?
+                __hip_atomic_load(x + c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
         wave_sync();
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
             if(e0 + 64 * k >= oe) break;
-            VT p = seg_scan(v[k] * xv[k], sg[k]);
+            double p = seg_scan((double) v[k] * xv[k], sg[k]);
             const int nsg = __shfl_down(sg[k], 1, 64);
             if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
             wave_sync();
@@ -381,14 +464,17 @@ __global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t n
             isg = 64;
             if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
         }
-        VT p = (isg < 64) ? ival * y[icol - fr] : (VT) 0;
+        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
         p = seg_scan(p, isg);
         const int nsg = __shfl_down(isg, 1, 64);
         if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
         wave_sync();
     }
-    if(lane < h) x[fr + lane] = (double) res[lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // agent-coherent stores of this block row, completed before the flag
+    if(lane < h)
+        __hip_atomic_store(x + fr + lane, (double) res[lane], __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if(lane == 0)
         __hip_atomic_store(flag + row_block[fr], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -52,7 +52,7 @@ void BildltPattern::upload_static()
     upload(d_ix_in, s.ix_store);
     upload(d_value_slot, s.value_slot);
 
-    const int32_t n = s.n, K = s.coarse_nnz;
+    const int32_t K = s.coarse_nnz;
     // per level-list position: sparse capacity or -1 - rows * cols (dense),
     // and the first fine row of the block row
     {
@@ -372,6 +372,33 @@ int64_t Bildlt::numeric_bytes() const
 }
 
 int64_t Bildlt::device_bytes() const { return pat->device_bytes() + numeric_bytes(); }
+
+int64_t Bildlt::numeric_bytes_estimate(bool with_layout) const
+{
+    const Symbolic& s = sym;
+    const int64_t n = s.n, nb = s.nb, K = s.coarse_nnz, v = (int64_t) vt();
+    const int64_t mlb = std::max<int64_t>(1, s.max_level_blocks);
+    auto r16 = [](int64_t b) { return std::max<int64_t>(16, b); };
+    int64_t b = 0;
+    b += r16(4 * mlb) * 2 + r16(128 * mlb) + r16(4 * mlb) + r16(4 * std::max(1, s.num_levels));
+    b += r16(32 * ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK));
+    b += r16(8 * (int64_t) s.value_slot.size()) + 2 * r16(v * s.val_space) + r16(36 * K);
+    b += r16(4 * s.ix_space) + 24 * n + v * n;
+    b += v * 1024 * mlb + 2048 * mlb + 4 * mlb + v * 32 * mlb + 4 * mlb + mlb;
+    b += 4 * nb + 32 + 4 * 4 * (n + 1) + 2 * 32 * nb + 16 + 4 * ((int64_t) scan_tmp_ints(s.n));
+    b += 8 * nb + 4;
+    if(with_layout) {
+        // U and U' rows: each stored entry once per layout, plus the pivot
+        // block inverses (at most rows^2 per block row, twice)
+        int64_t inv = 0;
+        for(int32_t i = 0; i < s.nb; ++i) {
+            const int64_t h = s.block_starts[i + 1] - s.block_starts[i];
+            inv += h * h;
+        }
+        b += (2 * s.val_space + 2 * inv) * (4 + v + 1);
+    }
+    return b;
+}
 
 void Bildlt::download_factor()
 {

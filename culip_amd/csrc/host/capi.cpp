@@ -9,7 +9,9 @@
 #include <sstream>
 
 #include "kkt.hpp"
+#include "serve.hpp"
 
+struct culip_server { std::unique_ptr<culip::Server> s; };
 struct culip_kkt { culip::Kkt* k; };
 struct culip_prep { culip::Prep* p; };
 struct culip_host_matrix { culip::HostCsr a; };
@@ -565,5 +567,62 @@ culip_status culip_sqmr_solve(culip_sqmr* s, const double* b, double* x, culip_s
 }
 
 void culip_sqmr_destroy(culip_sqmr* s) { delete s; }
+
+// ---- request loop --------------------------------------------------------
+
+culip_status culip_server_create(const culip_server_opts* o, culip_server** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(o && out, "server: null argument");
+        device_present();
+        auto* h = new culip_server;
+        try {
+            h->s = std::make_unique<Server>(*o);
+        } catch(...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+culip_status culip_server_add_pattern(culip_server* s, const culip_csr* K, int32_t num_pivs,
+    const int32_t* piv_starts, int32_t num_blocks, const int32_t* block_starts, int32_t* pattern)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s && K && pattern, "server: null argument");
+        CULIP_REQUIRE(num_pivs > 0 && piv_starts, "server: pivot starts required");
+        CULIP_REQUIRE(num_blocks > 0 && block_starts, "server: block starts required");
+        HostCsr h = to_host(K);
+        CULIP_REQUIRE(h.m == h.n, "server: matrix must be square");
+        *pattern = s->s->add_pattern(h, std::vector<int32_t>(piv_starts, piv_starts + num_pivs),
+            std::vector<int32_t>(block_starts, block_starts + num_blocks));
+    });
+}
+
+culip_status culip_server_submit(culip_server* s, int32_t pattern, const double* vals,
+    const double* rhs, int64_t* request)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s && request, "server: null argument");
+        *request = s->s->submit(pattern, vals, rhs);
+    });
+}
+
+culip_status culip_server_wait(culip_server* s, int64_t request, double* x, culip_sqmr_result* res,
+    double* timing, int32_t* slot)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s, "server: null handle");
+        s->s->wait(request, x, res, timing, slot);
+    });
+}
+
+culip_status culip_server_stats(const culip_server* s, int64_t* out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s && out, "server: null argument");
+        s->s->stats(out);
+    });
+}
+
+void culip_server_destroy(culip_server* s) { delete s; }
 
 }  // extern "C"

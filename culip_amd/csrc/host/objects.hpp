@@ -127,6 +127,10 @@ struct Bildlt {
     void upload_pattern();
     // device bytes of this instance's numeric buffers only
     int64_t numeric_bytes() const;
+    // the same before allocation, from the symbolic analysis; the solve
+    // layout (sized by the factor's entries) is bounded from above when
+    // with_layout is set and left out otherwise
+    int64_t numeric_bytes_estimate(bool with_layout) const;
     void factor(const double* vals, bool vals_on_device, double thr, hipStream_t st);
     void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
     void trsv_u(double* x, hipStream_t st) const;

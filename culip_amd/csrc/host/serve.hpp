@@ -35,7 +35,9 @@ struct ServePattern {
     int32_t N = 0;
     int64_t nnz = 0;
     int64_t static_bytes = 0;
-    int64_t inst_bytes = 0;   // measured on the first instance, estimated before
+    int64_t inst_bytes = 0;   // measured on the first instance
+    int64_t est_max = 0;      // before that: upper bound of an instance
+    int64_t est_min = 0;      // and its part without the solve layout
 };
 
 struct ServeInstance {
@@ -45,6 +47,7 @@ struct ServeInstance {
     DevBuf rhs, x;
     int64_t bytes = 0;
     uint64_t last_use = 0;
+    bool busy = false;        // a request is running on it
 };
 
 struct ServeRequest {
@@ -81,7 +84,6 @@ private:
 
     void worker(Slot* s);
     ServeInstance* instance(Slot* s, int32_t pattern, std::unique_lock<std::mutex>& lk);
-    void release_idle(Slot* s, int32_t keep_pattern, std::unique_lock<std::mutex>& lk);
     void run(ServeInstance* in, ServeRequest* r, hipStream_t st);
     double now() const;
 
@@ -95,6 +97,7 @@ private:
     int64_t next_id_ = 1;
     int64_t budget_ = 0, used_ = 0, peak_ = 0, completed_ = 0, instances_ = 0;
     int32_t mem_waiters_ = 0;
+    int32_t creating_ = 0;     // instances being allocated outside the lock
     uint64_t tick_ = 0;
     bool stop_ = false;
     std::vector<std::unique_ptr<Slot>> slots_;

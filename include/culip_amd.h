@@ -239,6 +239,47 @@ culip_status culip_sqmr_solve(culip_sqmr * s, const double * b, double * x,
     culip_sqmr_result * res, void * stream);
 void culip_sqmr_destroy(culip_sqmr * s);
 
+/* ------------------------------------------------------------------------ */
+/* Request loop for KKT solves (no counterpart in the reference, which
+ * solves one system per process): sparsity patterns are registered once
+ * (symbolic analysis and static device structures stay resident), requests
+ * carry new values of K and a right-hand side and are served by `slots`
+ * worker threads with one HIP stream each, so solves that arrive while
+ * others run start as soon as a slot and device memory are free and
+ * overlap on the device. Per request: BlockiLDLt refactor + SQMR.       */
+
+typedef struct {
+    int32_t slots;              /* concurrent solves, 1..64 */
+    int64_t hbm_budget;         /* device bytes for patterns and per-slot
+                                   instances; 0: 90% of free memory */
+    culip_prec prec;            /* factor precision (SQMR is FP64) */
+    int32_t fill_level;
+    double fill_factor;
+    double threshold;
+    culip_sqmr_opts sqmr;
+} culip_server_opts;
+
+typedef struct culip_server culip_server;
+culip_status culip_server_create(const culip_server_opts * o, culip_server ** out);
+/* Symbolic analysis and upload of a pattern: K full symmetric CSR (its
+ * values are not used), ordering and blocking as for culip_bildlt_create.
+ * Thread-safe; CULIP_ERR_BUDGET if the static data does not fit. */
+culip_status culip_server_add_pattern(culip_server * s, const culip_csr * K,
+    int32_t num_pivs, const int32_t * piv_starts, int32_t num_blocks,
+    const int32_t * block_starts, int32_t * pattern);
+/* Queues a solve: vals = nnz values of K in the pattern's entry order,
+ * rhs = N doubles (host; copied before return). */
+culip_status culip_server_submit(culip_server * s, int32_t pattern,
+    const double * vals, const double * rhs, int64_t * request);
+/* Waits for a request; x: N doubles (host, may be NULL). timing (may be
+ * NULL): {queued, service, latency, submit time} in seconds. slot: the
+ * worker that served it (may be NULL). The request is forgotten after. */
+culip_status culip_server_wait(culip_server * s, int64_t request, double * x,
+    culip_sqmr_result * res, double * timing, int32_t * slot);
+/* out[5]: device bytes in use, peak, budget, instances, completed. */
+culip_status culip_server_stats(const culip_server * s, int64_t * out);
+void culip_server_destroy(culip_server * s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,3 +142,14 @@ def test_cpp_driver_builds_against_the_class_header():
     assert os.path.exists(DRIVER), "run __graft_entry__.build() first"
     r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_server_needs_a_device():
+    """Without a HIP device the request loop refuses to start with a status
+    code (no device work is attempted)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(ca.CulipError) as e:
+        ca.Server(slots=2)
+    assert e.value.status == 8
