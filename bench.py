@@ -56,7 +56,12 @@ def parse():
     ap.add_argument("--threshold", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one step and one calibration copy, for rocprofv3 --pmc runs")
     return ap.parse_args()
+
+
+PROBE_COPY_DOUBLES = 1 << 25   # 256 MiB read + 256 MiB written
 
 
 def event_time(fn, reps, stream=None):
@@ -348,6 +353,19 @@ def main():
     def step():
         g.refactor(dvals, a.threshold)
         solver.solve(db, x)
+
+    if a.pmc_probe:
+        # for hardware-counter runs: one step, then a plain device copy of
+        # known size (the counters' bytes-per-KiB calibration)
+        step()
+        src = torch.ones(PROBE_COPY_DOUBLES, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": a.workload, "copy_bytes_read": 8 * PROBE_COPY_DOUBLES,
+                          "copy_bytes_written": 8 * PROBE_COPY_DOUBLES,
+                          "sqmr_iterations": solver.result["iterations"]}))
+        return
 
     for _ in range(a.warmup):
         step()

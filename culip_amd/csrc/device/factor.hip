@@ -731,11 +731,10 @@ __device__ void product_list(VT (&T)[16], const VT* A, const VT* rv, const uint8
 }
 
 template<typename VT>
-__global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+__device__ void offdiag_small_one(const FactorArgs<VT>& a, const LevelRange& lr, int q,
+    OffLdsSmall<VT>& s)
 {
-    __shared__ OffLdsSmall<VT> s;
     const int lane = lane_id();
-    const int q = lr.npiv + blockIdx.x;
     const int64_t p = (int64_t) lr.lo + q;
     const int4 t0 = ((const int4*) a.off_task)[3 * p];
     const int4 t1 = ((const int4*) a.off_task)[3 * p + 1];
@@ -892,6 +891,264 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
         }
         if(lane == 0) a.if_cnt[q] = base;
     }
+}
+
+// the positions k_offdiag_tiny leaves to the wave form, grid-stride
+template<typename VT>
+__global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ OffLdsSmall<VT> s;
+    const int nw = a.wave_cnt[lr.level];
+    for(int t = blockIdx.x; t < nw; t += gridDim.x) {
+        offdiag_small_one(a, lr, a.wave_list[t], s);
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One lane per off-diagonal block for the smallest blocks (every block of the
+// KKT workloads at level-0 fill: 1-2 entries, 1-2 Schur pairs). Static
+// eligibility (task bit): sparse input with at most TINY_E entries, at most
+// TINY_P pairs, 1x1 pivots in the block row. At run time every dependency
+// must be sparse with at most TINY_E entries and the tile must stay within
+// TINY_T nonzeros; otherwise the position goes to the wave form before
+// anything is written. The tile is a list of (row * 32 + col, value) in LDS,
+// one column of the list arrays per lane; every value is formed by the same
+// operations in the same order as in the wave form (terms that are exact
+// zeros in the wave form are skipped).
+constexpr int TINY_E = 8;
+constexpr int TINY_P = 8;
+constexpr int TINY_T = 32;
+constexpr int TINY_TPB = 64;
+
+template<typename VT>
+struct TinyLds {
+    VT v[TINY_T][TINY_TPB];
+    int16_t ix[TINY_T][TINY_TPB];
+};
+
+template<typename VT>
+__device__ __forceinline__ void to_wave_form(const FactorArgs<VT>& a, const LevelRange& lr, int q)
+{
+    a.wave_list[atomicAdd(a.wave_cnt + lr.level, 1)] = q;
+}
+
+template<typename VT>
+__global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ TinyLds<VT> s;
+    const int tid = threadIdx.x;
+    const int q = lr.npiv + (int) blockIdx.x * TINY_TPB + tid;
+    if(q >= lr.size) return;
+    const int64_t p = (int64_t) lr.lo + q;
+    const int4 t0 = ((const int4*) a.off_task)[3 * p];
+    const int4 t1 = ((const int4*) a.off_task)[3 * p + 1];
+    const int4 t2 = ((const int4*) a.off_task)[3 * p + 2];
+    if(!(t2.z & 1)) { to_wave_form(a, lr, q); return; }
+    const int fr = t0.y;
+    const int hgt = t0.z & 255, wid = (t0.z >> 8) & 255;
+    const int pq = t0.w;
+    const int np = t2.y;
+    const int4* pairs = (const int4*) a.upd4 + t2.x;
+    VT* tv = &s.v[0][tid];
+    int16_t* ti = &s.ix[0][tid];
+    auto V = [&](int k) -> VT& { return tv[k * TINY_TPB]; };
+    auto I = [&](int k) -> int16_t& { return ti[k * TINY_TPB]; };
+    int n = 0;
+    auto find = [&](int idx) {
+        for(int k = 0; k < n; ++k)
+            if(I(k) == idx) return k;
+        return -1;
+    };
+    // input entries
+    for(int e = 0; e < t1.y; ++e) {
+        const int ix = a.ix_in[(int64_t) t1.z + e];
+        const int r = ix / wid, c = ix - r * wid;
+        I(n) = (int16_t) (32 * r + c);
+        V(n) = a.val_in[(int64_t) t1.w + e];
+        ++n;
+    }
+    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order
+    for(int pr = 0; pr < np; ++pr) {
+        const int4 pp = pairs[pr];
+        const BlockMeta mi = a.bout[pp.x];
+        const BlockMeta mj = a.bout[pp.y];
+        if(mi.format != FMT_SPARSE || mi.nnz > TINY_E || mj.format != FMT_SPARSE ||
+            mj.nnz > TINY_E) {
+            to_wave_form(a, lr, q);
+            return;
+        }
+        const int rb = pp.z;
+        // U_ki: rows l of block k, columns c (rows of block i), width hgt
+        int al[TINY_E], ac[TINY_E];
+        VT av[TINY_E];
+#pragma unroll
+        for(int e = 0; e < TINY_E; ++e) {
+            al[e] = -4; ac[e] = -1; av[e] = 0;
+            if(e < mi.nnz) {
+                const int ix = a.ix_out[(int64_t) mi.col_ptr + e];
+                al[e] = ix / hgt;
+                ac[e] = ix - al[e] * hgt;
+                av[e] = a.val_out[(int64_t) mi.val_ptr + e];
+            }
+        }
+        // U_kj: rows l, columns j, width wid; nonzero entries by (column, row)
+        int rl[TINY_E], rj[TINY_E];
+        VT rv[TINY_E];
+#pragma unroll
+        for(int e = 0; e < TINY_E; ++e) {
+            rl[e] = 0; rj[e] = 64; rv[e] = 0;
+            if(e < mj.nnz) {
+                const int ix = a.ix_out[(int64_t) mj.col_ptr + e];
+                const VT v = a.val_out[(int64_t) mj.val_ptr + e];
+                if(v != (VT) 0) {
+                    rl[e] = ix / wid;
+                    rj[e] = ix - rl[e] * wid;
+                    rv[e] = v;
+                }
+            }
+        }
+        int key[TINY_E], rank[TINY_E];
+#pragma unroll
+        for(int e = 0; e < TINY_E; ++e) key[e] = rj[e] * 32 + rl[e];
+#pragma unroll
+        for(int e = 0; e < TINY_E; ++e) {
+            int rk = 0;
+#pragma unroll
+            for(int f = 0; f < TINY_E; ++f) rk += (key[f] < key[e]) || (key[f] == key[e] && f < e);
+            rank[e] = rk;
+        }
+        // distinct columns of U_ki
+        int dc[TINY_E];
+        int ndc = 0;
+#pragma unroll
+        for(int e = 0; e < TINY_E; ++e) {
+            bool seen = ac[e] < 0;
+#pragma unroll
+            for(int f = 0; f < e; ++f) seen = seen || (ac[f] == ac[e]);
+            dc[e] = -1;
+            if(!seen) dc[e] = ac[e];
+        }
+        // columns j in ascending order; within a column, rows ascending
+        VT acc[TINY_E];
+        int curj = -1;
+        for(int rk = 0; rk <= TINY_E; ++rk) {
+            int e = -1;
+#pragma unroll
+            for(int f = 0; f < TINY_E; ++f)
+                if(rank[f] == rk) e = f;
+            const int j = (rk < TINY_E && e >= 0) ? rj[e] : 64;
+            if(j != curj && curj >= 0 && curj < 64) {
+                // column curj done: T[c][curj] -= acc[c]
+#pragma unroll
+                for(int f = 0; f < TINY_E; ++f) {
+                    if(dc[f] < 0) continue;
+                    const int idx = 32 * dc[f] + curj;
+                    int k = find(idx);
+                    if(k < 0) {
+                        if(acc[f] == (VT) 0) continue;
+                        if(n == TINY_T) { to_wave_form(a, lr, q); return; }
+                        k = n++;
+                        I(k) = (int16_t) idx;
+                        V(k) = 0;
+                    }
+                    V(k) -= acc[f];
+                }
+            }
+            if(j >= 64) break;
+            if(j != curj) {
+                curj = j;
+#pragma unroll
+                for(int f = 0; f < TINY_E; ++f) acc[f] = 0;
+            }
+            const int l = rl[e];
+            const VT r = rv[e];
+            const double* td = a.tri + 3 * (int64_t) (rb + l);
+            const VT sup = (VT) td[0], dia = (VT) td[1], sub = (VT) td[2];
+#pragma unroll
+            for(int f = 0; f < TINY_E; ++f) {
+                if(dc[f] < 0) continue;
+                const int c = dc[f];
+                VT left = 0, mid = 0, right = 0;
+#pragma unroll
+                for(int g = 0; g < TINY_E; ++g) {
+                    if(ac[g] != c) continue;
+                    if(al[g] == l - 1) left = av[g];
+                    if(al[g] == l) mid = av[g];
+                    if(al[g] == l + 1) right = av[g];
+                }
+                const VT lt = fma(sub, right, fma(dia, mid, sup * left));
+                acc[f] = fma(r, lt, acc[f]);
+            }
+        }
+    }
+    // U_ii' forward substitution over the rows in ascending order
+    const VT* piv = a.if_val + (int64_t) pq * 1024;
+    for(int ii = 0; ii < hgt; ++ii) {
+        bool any = false;
+        for(int k = 0; k < n; ++k) any = any || ((I(k) >> 5) == ii && V(k) != (VT) 0);
+        if(!any) continue;
+        const uint32_t um = a.piv_umask[(int64_t) pq * 32 + ii];
+        if(!um) continue;
+        const VT d = piv[ii * hgt + ii];
+        const int n0 = n;
+        for(int k = 0; k < n0; ++k) {
+            if((I(k) >> 5) != ii || V(k) == (VT) 0) continue;
+            const int jc = I(k) & 31;
+            const VT pv = V(k) / d;
+            for(uint32_t bits = um; bits; bits &= bits - 1) {
+                const int l = __ffs(bits) - 1;
+                const int idx = 32 * l + jc;
+                int m = find(idx);
+                if(m < 0) {
+                    if(n == TINY_T) { to_wave_form(a, lr, q); return; }
+                    m = n++;
+                    I(m) = (int16_t) idx;
+                    V(m) = 0;
+                }
+                V(m) = fma(-piv[ii * hgt + l], pv, V(m));
+            }
+        }
+    }
+    // D solve (1x1 pivots): the Givens-form solve reduces to a division
+    const double* D = a.tri + 3 * (int64_t) fr;
+    for(int k = 0; k < n; ++k) V(k) = (VT) ((double) V(k) / D[3 * (I(k) >> 5) + 1]);
+    // tile order (row-major)
+    for(int k = 1; k < n; ++k) {
+        const int16_t ik = I(k);
+        const VT vk = V(k);
+        int m = k - 1;
+        while(m >= 0 && I(m) > ik) {
+            I(m + 1) = I(m);
+            V(m + 1) = V(m);
+            --m;
+        }
+        I(m + 1) = ik;
+        V(m + 1) = vk;
+    }
+    // row partial sums of squares (columns ascending) and the compact list
+    VT* rp = a.rn_part + (int64_t) q * 32;
+    int k = 0;
+    for(int r = 0; r < hgt; ++r) {
+        VT ssq = 0;
+        while(k < n && (I(k) >> 5) == r) {
+            ssq = fma(V(k), V(k), ssq);
+            ++k;
+        }
+        rp[r] = ssq;
+    }
+    VT* oval = a.if_val + (int64_t) q * 1024;
+    uint16_t* oix = a.if_idx + (int64_t) q * 1024;
+    int cnt = 0;
+    for(int m = 0; m < n; ++m) {
+        const VT v = V(m);
+        if(fabs((double) v) > NZ_EPS) {
+            oix[cnt] = (uint16_t) I(m);
+            oval[cnt] = v;
+            ++cnt;
+        }
+    }
+    a.if_cnt[q] = cnt;
 }
 
 template<typename VT>
@@ -1135,15 +1392,13 @@ __global__ __launch_bounds__(SC_TPB) void k_level_write(FactorArgs<VT> a, LevelR
     }
 }
 
-// final compacted storage (kernels :2011-2073, d_save_block :85-173)
+// final compacted storage (kernels :2011-2073, d_save_block :85-173), one
+// wave per block
 template<typename VT>
-__global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
+__device__ void store_one(const FactorArgs<VT>& a, const LevelRange& lr, int q, VT* nrm, VT* key,
+    uint16_t* kix)
 {
-    __shared__ VT nrm[32];
-    __shared__ VT key[1024];
-    __shared__ uint16_t kix[1024];
     const int lane = lane_id();
-    const int q = blockIdx.x;
     const int b = a.lvlb_idx[lr.lo + q];
     const BlockMeta mo = a.bout[b];
     const int fr = a.bs[mo.block_row];
@@ -1208,6 +1463,83 @@ __global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
     }
 }
 
+// the positions k_store_tiny leaves to the wave form, grid-stride
+template<typename VT>
+__global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ VT nrm[32];
+    __shared__ VT key[1024];
+    __shared__ uint16_t kix[1024];
+    const int ns = a.store_cnt[lr.level];
+    for(int t = blockIdx.x; t < ns; t += gridDim.x) {
+        store_one(a, lr, a.store_list[t], nrm, key, kix);
+        wave_sync();
+    }
+}
+
+// one lane per sparse block with at most STORE_TINY pre-drop entries, the
+// same selection and order as store_one; the rest go to the wave form
+constexpr int STORE_TINY = 16;
+
+template<typename VT>
+__global__ __launch_bounds__(256) void k_store_tiny(FactorArgs<VT> a, LevelRange lr)
+{
+    const int q = (int) blockIdx.x * 256 + (int) threadIdx.x;
+    if(q >= lr.size) return;
+    const int cap = a.lq_cap[lr.lo + q];
+    const int cnt = a.if_cnt[q];
+    if(cap < 0 || cnt > STORE_TINY) {
+        a.store_list[atomicAdd(a.store_cnt + lr.level, 1)] = q;
+        return;
+    }
+    const int b = a.lvlb_idx[lr.lo + q];
+    const BlockMeta mo = a.bout[b];
+    const int fr = a.lq_fr[lr.lo + q];
+    const int w = a.bs[mo.block_col + 1] - a.bs[mo.block_col];
+    const VT* vals = a.if_val + (int64_t) q * 1024;
+    const uint16_t* ixs = a.if_idx + (int64_t) q * 1024;
+    VT* out_v = a.val_out + mo.val_ptr;
+    int32_t* out_i = a.ix_out + mo.col_ptr;
+    VT key[STORE_TINY];
+    int ixk[STORE_TINY];
+#pragma unroll
+    for(int e = 0; e < STORE_TINY; ++e) {
+        key[e] = (VT) -1;
+        ixk[e] = 0;
+        if(e < cnt) {
+            const VT v = vals[e];
+            const int ix = ixs[e];
+            const bool k = !(fabs(v) < a.threshold * drop_norm(a.rn[fr + (ix >> 5)]));
+            key[e] = k ? (VT) fabs(v) : (VT) -1;
+            ixk[e] = ix;
+        }
+    }
+    if(!a.overflow[q]) {
+        int pos = 0;
+#pragma unroll
+        for(int e = 0; e < STORE_TINY; ++e) {
+            if(key[e] >= (VT) 0) {
+                out_i[pos] = (ixk[e] >> 5) * w + (ixk[e] & 31);
+                out_v[pos] = vals[e];
+                ++pos;
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for(int e = 0; e < STORE_TINY; ++e) {
+        if(key[e] < (VT) 0) continue;
+        int rank = 0;
+#pragma unroll
+        for(int f = 0; f < STORE_TINY; ++f)
+            rank += (f < cnt) && ((key[f] > key[e]) || (key[f] == key[e] && ixk[f] < ixk[e]));
+        if(rank < mo.max_nnz) {
+            out_i[rank] = (ixk[e] >> 5) * w + (ixk[e] & 31);
+            out_v[rank] = vals[e];
+        }
+    }
+}
+
 template<typename VT>
 __global__ void k_zero_carry(int32_t* carry)
 {
@@ -1244,9 +1576,12 @@ void factor_begin(const FactorArgs<VT>& a, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
     k_zero_carry<VT><<<1, 64, 0, st>>>(a.carry);
-    if(a.num_levels > 0)
-        k_zero<int32_t><<<(unsigned) ((a.num_levels + 255) / 256), 256, 0, st>>>(a.defer_cnt,
-            a.num_levels);
+    if(a.num_levels > 0) {
+        const unsigned g = (unsigned) ((a.num_levels + 255) / 256);
+        k_zero<int32_t><<<g, 256, 0, st>>>(a.defer_cnt, a.num_levels);
+        k_zero<int32_t><<<g, 256, 0, st>>>(a.wave_cnt, a.num_levels);
+        k_zero<int32_t><<<g, 256, 0, st>>>(a.store_cnt, a.num_levels);
+    }
 }
 
 template<typename VT>
@@ -1256,15 +1591,18 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
     if(r.npiv > 0) k_pivots<VT><<<r.npiv, 64, 0, st>>>(a, r);
     if(r.size > r.npiv) {
         const int noff = r.size - r.npiv;
-        k_offdiag<VT><<<noff, 64, 0, st>>>(a, r);
-        k_offdiag_general<VT><<<noff < 2048 ? noff : 2048, 64, 0, st>>>(a, r);
+        const int grid = noff < 2048 ? noff : 2048;
+        k_offdiag_tiny<VT><<<(noff + TINY_TPB - 1) / TINY_TPB, TINY_TPB, 0, st>>>(a, r);
+        k_offdiag<VT><<<grid, 64, 0, st>>>(a, r);
+        k_offdiag_general<VT><<<grid, 64, 0, st>>>(a, r);
     }
     if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
     const unsigned nch = (unsigned) ((r.size + SCAN_CHUNK - 1) / SCAN_CHUNK);
     k_level_maps<VT><<<nch, SC_TPB, 0, st>>>(a, r);
     k_level_grant<VT><<<nch, SC_TPB, 0, st>>>(a, r);
     k_level_write<VT><<<nch, SC_TPB, 0, st>>>(a, r);
-    k_store<VT><<<r.size, 64, 0, st>>>(a, r);
+    k_store_tiny<VT><<<(r.size + 255) / 256, 256, 0, st>>>(a, r);
+    k_store<VT><<<r.size < 2048 ? r.size : 2048, 64, 0, st>>>(a, r);
 }
 
 template void gather_values<double>(const double*, const int64_t*, int64_t, double*, int64_t, void*);

@@ -95,6 +95,9 @@ void BildltPattern::upload_static()
                 t[4] = m.format; t[5] = m.nnz; t[6] = m.col_ptr; t[7] = m.val_ptr;
                 t[8] = (int32_t) s.upd_off[b];
                 t[9] = (int32_t) (s.upd_off[b + 1] - s.upd_off[b]);
+                // one-lane kernel: sparse input of at most 8 entries, at
+                // most 8 Schur pairs, 1x1 pivots in the block row
+                t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= 8 && all1) ? 1 : 0;
             }
         }
         upload(d_off_task, task);
@@ -124,6 +127,10 @@ void Bildlt::upload_pattern()
     d_piv_umask.ensure(sizeof(uint32_t) * 32 * mlb);
     d_defer_list.ensure(sizeof(int32_t) * mlb);
     d_defer_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
+    d_wave_list.ensure(sizeof(int32_t) * mlb);
+    d_wave_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
+    d_store_list.ensure(sizeof(int32_t) * mlb);
+    d_store_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
     d_chunk.ensure(sizeof(int64_t) * 4 * (size_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK));
     d_vals.ensure(sizeof(double) * std::max<int64_t>(1, A.nnz()));
     d_val_in.ensure(vt() * std::max<int64_t>(1, s.val_space));
@@ -196,6 +203,10 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.piv_umask = d_piv_umask.as<uint32_t>();
     a.defer_list = d_defer_list.as<int32_t>();
     a.defer_cnt = d_defer_cnt.as<int32_t>();
+    a.wave_list = d_wave_list.as<int32_t>();
+    a.wave_cnt = d_wave_cnt.as<int32_t>();
+    a.store_list = d_store_list.as<int32_t>();
+    a.store_cnt = d_store_cnt.as<int32_t>();
     a.num_levels = sym.num_levels;
     a.carry = d_carry.as<int32_t>();
     a.threshold = (VT) threshold;
@@ -363,7 +374,9 @@ int64_t Bildlt::numeric_bytes() const
     for(const DevBuf* d : {&d_vals, &d_val_in, &d_bout, &d_ix_out, &d_val_out, &d_tri, &d_rn,
             &d_if_val, &d_if_idx, &d_if_cnt, &d_rn_part, &d_drop_cnt, &d_overflow, &d_piv_loc,
             &d_carry, &d_totals, &d_task[0], &d_task[1], &d_keep, &d_granted, &d_chunk,
-            &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_flag_u, &d_flag_t,
+            &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_wave_list, &d_wave_cnt,
+            &d_store_list, &d_store_cnt,
+            &d_flag_u, &d_flag_t,
             &d_trsv_err})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
@@ -387,6 +400,7 @@ int64_t Bildlt::numeric_bytes_estimate(bool with_layout) const
     b += v * 1024 * mlb + 2048 * mlb + 4 * mlb + v * 32 * mlb + 4 * mlb + mlb;
     b += 4 * nb + 32 + 4 * 4 * (n + 1) + 2 * 32 * nb + 16 + 4 * ((int64_t) scan_tmp_ints(s.n));
     b += 8 * nb + 4;
+    b += 2 * r16(4 * mlb) + 2 * r16(4 * std::max(1, s.num_levels));   // wave / store lists
     if(with_layout) {
         // U and U' rows: each stored entry once per layout, plus the pivot
         // block inverses (at most rows^2 per block row, twice)

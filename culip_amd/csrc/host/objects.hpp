@@ -89,7 +89,8 @@ struct Bildlt {
     DevBuf d_vals, d_val_in, d_bout, d_ix_out, d_val_out, d_tri, d_rn;
     DevBuf d_if_val, d_if_idx, d_if_cnt, d_rn_part, d_drop_cnt, d_overflow, d_piv_loc, d_carry;
     DevBuf d_keep, d_granted, d_chunk;
-    DevBuf d_piv_umask, d_defer_list, d_defer_cnt;
+    DevBuf d_piv_umask, d_defer_list, d_defer_cnt, d_wave_list, d_wave_cnt, d_store_list,
+        d_store_cnt;
     // solve layout
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
     DevBuf d_totals, d_scan_tmp;

@@ -173,4 +173,39 @@ private:
     culip_sqmr_result last_{};
 };
 
+/* Request loop (no reference counterpart): resident patterns, requests with
+ * new values of K and a right-hand side, concurrent solves on `slots`
+ * streams, admission by device-memory budget. Host arrays throughout. */
+class Server {
+public:
+    explicit Server(const culip_server_opts& o) { check(culip_server_create(&o, &s_)); }
+    ~Server() { culip_server_destroy(s_); }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+
+    int32_t add_pattern(const culip_csr* K, int32_t num_pivs, const int32_t* piv_starts,
+        int32_t num_blocks, const int32_t* block_starts)
+    {
+        int32_t p = -1;
+        check(culip_server_add_pattern(s_, K, num_pivs, piv_starts, num_blocks, block_starts, &p));
+        return p;
+    }
+    int64_t submit(int32_t pattern, const double* vals, const double* rhs)
+    {
+        int64_t r = -1;
+        check(culip_server_submit(s_, pattern, vals, rhs, &r));
+        return r;
+    }
+    /* timing: {queued, service, latency, submit time} seconds (may be null) */
+    culip_sqmr_result wait(int64_t request, double* x, double* timing = nullptr)
+    {
+        culip_sqmr_result r{};
+        check(culip_server_wait(s_, request, x, &r, timing, nullptr));
+        return r;
+    }
+
+private:
+    culip_server* s_ = nullptr;
+};
+
 }  // namespace culip_amd
