@@ -458,7 +458,10 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     HIPCHK(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
     int32_t it = 0;
     const int period = opt.true_residual_period;
-    constexpr int STALL_PERIODS = 5;
+    // long plateaus of the true residual are normal on barrier-conditioned
+    // systems (a few hundred iterations); restarting inside one throws the
+    // Krylov space away
+    constexpr int STALL_PERIODS = 25;
     double best_nr = -1.0;
     int stall = 0;
     while(it < opt.max_it) {
@@ -554,6 +557,12 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
     double nr, nbn;
     residual_norms(ops, w, b, x, st, nr, nbn);
     double rr = nbn > 0 ? nr / nbn : nr;
+    if((opt.true_residual_period > 0 || opt.max_restarts > 0) && nbn > 0 && !(rr <= 1.0)) {
+        // worse than the zero start: keep x = 0 (residual b)
+        k_fill<<<nb, 256, 0, st>>>(n, x, 0.0);
+        k_copy<<<nb, 256, 0, st>>>(n, b, w.rt);
+        rr = 1.0;
+    }
     // iterative refinement: solve A d = r with a fresh recurrence, x += d,
     // while the FP64 true residual misses the tolerance. A correction that
     // does not lower the true residual is undone and ends the refinement,

@@ -508,9 +508,6 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         ops.right = [P](double* xx, void* s) { P->trsv_u(xx, (hipStream_t) s); };
         ops.tri = P->d_tri.as<double>();
         ops.is_piv = P->d_is_piv.as<int32_t>();
-        // an FP32 factor applied in FP32 limits one recurrence to a few
-        // digits of true-residual reduction
-        if(P->f32()) ops.restart_rel = 1e-5;
     }
     sqmr_solve(ops, w, b, x, opts, res, st);
     if(prec) prec->check_solve_error(st);
