@@ -906,8 +906,9 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
 }
 
 // ---------------------------------------------------------------------------
-// One lane per off-diagonal block for the smallest blocks (every block of the
-// KKT workloads at level-0 fill: 1-2 entries, 1-2 Schur pairs). Static
+// One lane per off-diagonal block for small blocks (at level-0 fill of the
+// KKT workloads: 1-2 entries each, up to ~60 Schur pairs, ~15 on average;
+// a wave then works on 64 blocks instead of one). Static
 // eligibility (task bit): sparse input with at most TINY_E entries, at most
 // TINY_P pairs, 1x1 pivots in the block row. At run time every dependency
 // must be sparse with at most TINY_E entries and the tile must stay within
@@ -917,7 +918,7 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
 // operations in the same order as in the wave form (terms that are exact
 // zeros in the wave form are skipped).
 constexpr int TINY_E = 8;
-constexpr int TINY_P = 8;
+constexpr int TINY_P = 64;   // the host sets the eligibility bit with this bound
 constexpr int TINY_T = 32;
 constexpr int TINY_TPB = 64;
 
@@ -968,11 +969,22 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         V(n) = a.val_in[(int64_t) t1.w + e];
         ++n;
     }
-    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order
+    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order. Loads run two
+    // pairs ahead: the pair record of pair pr + 2 and the metadata of pair
+    // pr + 1 are in flight while pair pr is computed.
+    int4 pp_b = (np > 0) ? pairs[0] : make_int4(0, 0, 0, 0);
+    int4 pp_a = (np > 1) ? pairs[1] : make_int4(0, 0, 0, 0);
+    BlockMeta mi_b = a.bout[pp_b.x], mj_b = a.bout[pp_b.y];
     for(int pr = 0; pr < np; ++pr) {
-        const int4 pp = pairs[pr];
-        const BlockMeta mi = a.bout[pp.x];
-        const BlockMeta mj = a.bout[pp.y];
+        const int4 pp = pp_b;
+        const BlockMeta mi = mi_b;
+        const BlockMeta mj = mj_b;
+        if(pr + 1 < np) {
+            pp_b = pp_a;
+            mi_b = a.bout[pp_b.x];
+            mj_b = a.bout[pp_b.y];
+            if(pr + 2 < np) pp_a = pairs[pr + 2];
+        }
         if(mi.format != FMT_SPARSE || mi.nnz > TINY_E || mj.format != FMT_SPARSE ||
             mj.nnz > TINY_E) {
             to_wave_form(a, lr, q);
