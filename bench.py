@@ -51,6 +51,8 @@ def parse():
                     help="offered requests/s (0: 85%% of the warm-up throughput)")
     ap.add_argument("--max-block", type=int, default=32)
     ap.add_argument("--systems", type=int, default=512)
+    ap.add_argument("--max-it", type=int, default=0,
+                    help="SQMR iteration budget per request (0: 3000 for serve)")
     ap.add_argument("--order", default="color", choices=["color", "natural"])
     ap.add_argument("--fill-factor", type=float, default=1.0)
     ap.add_argument("--threshold", type=float, default=1e-4)
@@ -132,7 +134,7 @@ def bench_serve(a):
     rng = np.random.default_rng(a.seed)
     sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
     order = ca.ORDER_BLOCK_COLOR
-    srv = ca.Server(slots=a.slots, prec=a.prec)
+    srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000)
     nsets = 2
 
     def make(i):
@@ -201,7 +203,8 @@ def bench_serve(a):
         "config": {"workload": "serve", "patterns": a.patterns, "n_range": [a.n_lo, a.n_hi],
                    "m": "0.4n", "max_block": a.max_block, "slots": a.slots,
                    "requests": len(res), "offered_rate_per_s": rate,
-                   "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "parallelism": "none"},
+                   "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "max_it": a.max_it or 3000,
+                   "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "queue_p50_ms": float(np.percentile(que, 50)),
         "service_p50_ms": float(np.percentile(lat - que, 50)),

@@ -344,23 +344,28 @@ __device__ void product_regs(VT (&T)[16], const VT* A, const uint32_t* mask, con
     const int c0 = lane >> 5;
     const uint32_t mk = (j < len) ? mask[j] : 0u;
     if(mk) {
+        // one pass over the nonzero rows with sixteen independent
+        // accumulators (the loads of a row feed all of them)
+        VT acc[16];
 #pragma unroll
-        for(int t = 0; t < 16; ++t) {
-            const int c = c0 + 2 * t;
-            if(c < len) {
-                VT acc = 0;
-                for(uint32_t bits = mk; bits; bits &= bits - 1) {
-                    const int l = __ffs(bits) - 1;
-                    const VT r = A[32 * l + j];
-                    const VT left = (l == 0) ? (VT) 0 : A[32 * (l - 1) + c];
-                    const VT mid = A[32 * l + c];
-                    const VT right = (l == 31) ? (VT) 0 : A[32 * (l + 1) + c];
-                    const VT lt = fma(cb[l], right, fma(cd[l], mid, cs[l] * left));
-                    acc = fma(r, lt, acc);
-                }
-                T[t] -= acc;
+        for(int t = 0; t < 16; ++t) acc[t] = 0;
+        for(uint32_t bits = mk; bits; bits &= bits - 1) {
+            const int l = __ffs(bits) - 1;
+            const VT r = A[32 * l + j];
+            const VT sup = cs[l], dia = cd[l], sub = cb[l];
+#pragma unroll
+            for(int t = 0; t < 16; ++t) {
+                const int c = c0 + 2 * t;
+                const VT left = (l == 0) ? (VT) 0 : A[32 * (l - 1) + c];
+                const VT mid = A[32 * l + c];
+                const VT right = (l == 31) ? (VT) 0 : A[32 * (l + 1) + c];
+                const VT lt = fma(sub, right, fma(dia, mid, sup * left));
+                acc[t] = fma(r, lt, acc[t]);
             }
         }
+#pragma unroll
+        for(int t = 0; t < 16; ++t)
+            if(c0 + 2 * t < len) T[t] -= acc[t];
     }
 }
 

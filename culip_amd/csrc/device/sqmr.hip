@@ -241,8 +241,11 @@ __global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, c
 }
 
 // k_sqmr_test (:89-107) and the state update; one workgroup
+// brel < 0: the reference's breakdown test |delta| <= tol; brel >= 0:
+// |delta| <= brel * ||A||est, scale-free (the delta of a badly scaled but
+// healthy recurrence can fall below an absolute 1e-8)
 __global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, const double* part,
-    int32_t* flags, double tol)
+    int32_t* flags, double tol, double brel)
 {
     __shared__ double sh[16];
     double S[NSCAL];
@@ -252,7 +255,8 @@ __global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, con
         const int k1 = (k + 1) % 3;
         S[S_NORMX] = nx;
         S[S_GAMMA] = S[S_S + k1] * S[S_GAMMA];
-        flags[0] |= (fabs(S[S_DELTA + k1]) <= tol);
+        const double dl = fabs(S[S_DELTA + k1]);
+        flags[0] |= (brel < 0) ? (dl <= tol) : (dl <= brel * S[S_NORMA]);
         flags[1] |= (S[S_GAMMA] / (S[S_NORMA] * S[S_NORMX] + S[S_NORMPB]) <= tol);
         S[S_ITER] = (double) k;
         for(int q = 0; q < NSCAL; ++q) scal[q] = S[q];
@@ -477,7 +481,8 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
         k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
         k_pupdate<<<G, TPB, 0, st>>>(n, it, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
             w.scal, w.part);
-        k_scalars<<<1, TPB, 0, st>>>(it, G, w.scal, w.part, w.flags, opt.tol);
+        k_scalars<<<1, TPB, 0, st>>>(it, G, w.scal, w.part, w.flags, opt.tol,
+            period > 0 ? 1e-14 : -1.0);
         cur = nxt;
         HIPCHK(hipGetLastError());
 
