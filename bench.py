@@ -58,8 +58,10 @@ def parse():
     ap.add_argument("--threshold", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--solve-algorithm", default="levels", choices=["levels", "flagged"],
+    ap.add_argument("--solve-algorithm", default="levels",
+                    choices=["levels", "flagged", "jacobi"],
                     help="triangular-solve schedule (set_solve_algorithm)")
+    ap.add_argument("--jacobi-sweeps", type=int, default=0)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one step and one calibration copy, for rocprofv3 --pmc runs")
     return ap.parse_args()
@@ -356,7 +358,7 @@ def main():
     g.symbolic(0, a.fill_factor)
     t_sym = time.perf_counter() - t_sym0
     sc = g.get_int("scalars")
-    g.set_solve_algorithm(a.solve_algorithm)
+    g.set_solve_algorithm(a.solve_algorithm, a.jacobi_sweeps)
     A = ca.CSRMatrixSPMV(prep.row_ptr, prep.col, prep.val)
     solver = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
     dvals = torch.tensor(prep.val, device=dev)
@@ -414,11 +416,22 @@ def main():
     rl_spmv = {"bound": "hbm", "kernel": "k_csr_spmv", "achieved": bytes_spmv / ms_spmv / 1e6,
                "peak": peak, "unit": "GB/s", "frac": bytes_spmv / ms_spmv / 1e6 / peak,
                "traffic": None, "bytes_per_launch": bytes_spmv, "ms_per_launch": ms_spmv}
-    rl_trsv = {"bound": "hbm", "kernel": "k_trsv_level (one U solve, all levels)",
+    launches = int(sc[6])
+    label = "k_trsv_level (one U solve, all levels)"
+    if a.solve_algorithm == "jacobi":
+        # sweeps after the first read every entry, the first only U_ii^-1
+        launches = a.jacobi_sweeps
+        first = int(ps[1]) * (sv + 4) + 2 * N * 8 + (N + 1) * 4
+        bytes_trsv = first + (a.jacobi_sweeps - 1) * (bytes_trsv + N * 8)
+        label = f"k_jacobi_sweep (one U solve, {a.jacobi_sweeps} sweeps)"
+    elif a.solve_algorithm == "flagged":
+        launches = 1
+        label = "k_trsv_flagged (one U solve)"
+    rl_trsv = {"bound": "hbm", "kernel": label,
                "achieved": bytes_trsv / ms_trsv / 1e6, "peak": peak, "unit": "GB/s",
                "frac": bytes_trsv / ms_trsv / 1e6 / peak, "traffic": None,
                "bytes_per_launch": bytes_trsv, "ms_per_launch": ms_trsv,
-               "launches": int(sc[6])}
+               "launches": launches}
     dominant, secondary = (rl_trsv, rl_spmv) if per_it_trsv >= per_it_spmv else (rl_spmv, rl_trsv)
 
     # the same step with the FP32 factor (FP64 SQMR, FP64 true residual)
@@ -427,6 +440,7 @@ def main():
         g32 = ca.BlockiLDLt(prep.row_ptr, prep.col, prep.val, prep.piv_starts,
                             prep.block_starts, "f32")
         g32.symbolic(0, a.fill_factor)
+        g32.set_solve_algorithm(a.solve_algorithm, a.jacobi_sweeps)
         s32 = ca.SQMR(A, 1e-8, 1000, g32, true_residual_period=20, max_restarts=10)
         x32 = torch.zeros_like(x)
         for _ in range(a.warmup):
@@ -471,7 +485,7 @@ def main():
                    "max_block": wl["B"], "fill_level": 0, "fill_factor": a.fill_factor,
                    "drop_tol": a.threshold, "ordering": a.order, "levels": int(sc[6]),
                    "coarse_blocks": int(sc[5]), "tol": 1e-8, "parallelism": "none",
-                   "solve_algorithm": a.solve_algorithm},
+                   "solve_algorithm": a.solve_algorithm, "jacobi_sweeps": a.jacobi_sweeps},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "factor_ms_mean": float(np.mean(fac) * 1e3),

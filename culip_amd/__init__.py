@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT,
-                   ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_FLAGGED, check, lib)
+                   ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_FLAGGED, TRSV_JACOBI,
+                   check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
            "SQMR", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
@@ -168,10 +169,13 @@ class BlockiLDLt:
             check(lib().culip_bildlt_refactor(self._h, v.ctypes.data_as(C.c_void_p), 0,
                                               threshold, _stream(stream)))
 
-    def set_solve_algorithm(self, algorithm):
-        """'levels' (one launch per level) or 'flagged' (one launch per solve)."""
-        a = {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED}.get(algorithm, algorithm)
-        check(lib().culip_bildlt_set_solve_algorithm(self._h, a))
+    def set_solve_algorithm(self, algorithm, num_jacobi_sweeps=0):
+        """set_solve_algorithm: 'levels' (one launch per level), 'flagged'
+        (one launch per solve) or 'jacobi' (num_jacobi_sweeps block-Jacobi
+        sweeps)."""
+        a = {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED,
+             "jacobi": TRSV_JACOBI}.get(algorithm, algorithm)
+        check(lib().culip_bildlt_set_solve_algorithm(self._h, a, num_jacobi_sweeps))
 
     def _solve(self, side, b, x, transpose, stream):
         check(lib().culip_bildlt_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),

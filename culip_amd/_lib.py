@@ -16,7 +16,7 @@ STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOMEM", 4: "STATE", 5: "NUMERIC",
 PREC_F64, PREC_F32 = 0, 1
 SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT = 0, 1, 2
 ORDER_NATURAL, ORDER_BLOCK_COLOR = 0, 1
-TRSV_LEVELS, TRSV_FLAGGED = 0, 1
+TRSV_LEVELS, TRSV_FLAGGED, TRSV_JACOBI = 0, 1, 2
 
 
 class CulipError(RuntimeError):
@@ -91,7 +91,7 @@ _SIGS = [
     ("culip_bildlt_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p]),
     ("culip_bildlt_n", C.c_int32, [C.c_void_p]),
-    ("culip_bildlt_set_solve_algorithm", C.c_int, [C.c_void_p, C.c_int]),
+    ("culip_bildlt_set_solve_algorithm", C.c_int, [C.c_void_p, C.c_int, C.c_int32]),
     ("culip_bildlt_get_int", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.POINTER(C.c_int32)),
                                        C.POINTER(C.c_int64)]),
     ("culip_bildlt_get_real", C.c_int, [C.c_void_p, C.c_char_p,

@@ -375,6 +375,73 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
     if(lane < h) x[fr + lane] = (double) res[lane];
 }
 
+// One block-Jacobi sweep over all block rows (solve_U_jacobi /
+// solve_Ut_jacobi, block_ildlt.impl.cuh:2454-2493, 2637-2676, with
+// k_solve_block_stripe_level): x_out_i = U_ii^-1 (b_i - sum_j U_ij x_in_j).
+// x_in == nullptr is the first sweep from x_0 = 0. The per-row sums run in
+// the same order as in k_trsv_level, so once x_in is the exact solution the
+// sweep reproduces the level-scheduled solve bit for bit.
+template<typename VT>
+__global__ __launch_bounds__(64) void k_jacobi_sweep(const int4* task, const int32_t* o_col,
+    const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
+    const uint8_t* i_lr, const double* b, const double* xin, double* xout)
+{
+    __shared__ double y[33];
+    __shared__ double res[33];
+    const int lane = lane_id();
+    const int4 t0 = task[2 * blockIdx.x];
+    const int4 t1 = task[2 * blockIdx.x + 1];
+    const int fr = t0.x, h = t0.y, ob = t0.z, oe = xin ? t0.w : t0.z, ib = t1.x, ie = t1.y;
+    const double bo = (lane < h) ? b[fr + lane] : 0.0;
+    int icol = fr;
+    VT ival = 0;
+    int isg = 64;
+    if(ib + lane < ie) {
+        icol = i_col[ib + lane];
+        ival = i_val[ib + lane];
+        isg = i_lr[ib + lane];
+    }
+    if(lane < h) { y[lane] = bo; res[lane] = 0; }
+    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
+        int c[TRSV_PF], sg[TRSV_PF];
+        VT v[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            const int e = e0 + 64 * k + lane;
+            c[k] = 0;
+            v[k] = 0;
+            sg[k] = 64;
+            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
+        }
+        double xv[TRSV_PF];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? xin[c[k]] : 0.0;
+        wave_sync();
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            if(e0 + 64 * k >= oe) break;
+            double p = seg_scan((double) v[k] * xv[k], sg[k]);
+            const int nsg = __shfl_down(sg[k], 1, 64);
+            if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
+            wave_sync();
+        }
+    }
+    wave_sync();
+    for(int e0 = ib; e0 < ie; e0 += 64) {
+        if(e0 != ib) {
+            const int e = e0 + lane;
+            isg = 64;
+            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+        }
+        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
+        p = seg_scan(p, isg);
+        const int nsg = __shfl_down(isg, 1, 64);
+        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
+        wave_sync();
+    }
+    if(lane < h) xout[fr + lane] = res[lane];
+}
+
 // Whole triangular solve in one launch: workgroup w takes the block row at
 // position w of the level order (position nb - 1 - w for the U solve), so
 // every block row it depends on has a smaller workgroup index. Before it
@@ -591,6 +658,29 @@ void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x,
         p.to_lr, p.ti_col, p.ti_val, p.ti_lr, t.row_block, t.flag_t, epoch, t.err, x);
 }
 
+template<typename VT>
+void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, double* x,
+    int32_t sweeps, double* b_tmp, double* x_a, double* x_b, void* stream)
+{
+    if(t.nb <= 0 || sweeps <= 0) return;
+    hipStream_t st = (hipStream_t) stream;
+    const int4* task = transpose ? p.task_t : p.task_u;
+    const int32_t* oc = transpose ? p.to_col : p.uo_col;
+    const VT* ov = transpose ? p.to_val : p.uo_val;
+    const uint8_t* ol = transpose ? p.to_lr : p.uo_lr;
+    const int32_t* ic = transpose ? p.ti_col : p.ui_col;
+    const VT* iv = transpose ? p.ti_val : p.ui_val;
+    const uint8_t* il = transpose ? p.ti_lr : p.ui_lr;
+    const double* in = nullptr;
+    for(int32_t s = 0; s < sweeps; ++s) {
+        double* out = (s == sweeps - 1) ? x : ((s & 1) ? x_b : x_a);
+        k_jacobi_sweep<VT><<<t.nb, 64, 0, st>>>(task, oc, ov, ol, ic, iv, il, b_tmp, in, out);
+        in = out;
+    }
+}
+
+template void jacobi_sweeps<double>(const SolveTopo&, const SolvePlan<double>&, bool, double*, int32_t, double*, double*, double*, void*);
+template void jacobi_sweeps<float>(const SolveTopo&, const SolvePlan<float>&, bool, double*, int32_t, double*, double*, double*, void*);
 template void trsv_upper_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
 template void trsv_upper_flagged<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
 template void trsv_upper_t_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);

@@ -73,6 +73,11 @@ void trsv_upper_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, i
 template<typename VT>
 void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
     void* stream);
+// x <- approximately U \ b (U' \ b when transpose) by `sweeps` block-Jacobi
+// sweeps from 0; b_tmp holds b (n doubles), x_a / x_b are scratch vectors
+template<typename VT>
+void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, double* x,
+    int32_t sweeps, double* b_tmp, double* x_a, double* x_b, void* stream);
 void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
     void* stream);
 

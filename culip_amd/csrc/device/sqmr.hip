@@ -466,7 +466,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     // systems (a few hundred iterations); restarting inside one throws the
     // Krylov space away
     constexpr int STALL_PERIODS = 25;
-    double best_nr = -1.0;
+    double best_nr = -1.0, gam_at_best = 0.0;
     int stall = 0;
     while(it < opt.max_it) {
         const int k0 = it % 3, k1 = (it + 1) % 3, km1 = (it + 2) % 3, km2 = (it + 1) % 3;
@@ -501,14 +501,24 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
                 residual_norms(ops, w, b, xout, st, nr, nbn);
                 if(nr <= gate_abs) { gate_met = true; ++it; break; }
                 HIPCHK(hipMemcpy(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+                double gam = 0;
+                HIPCHK(hipMemcpy(&gam, w.scal + S_GAMMA, sizeof(double), hipMemcpyDeviceToHost));
+                gam = fabs(gam);
                 if(best_nr < 0 || nr < 0.5 * best_nr) {
                     best_nr = nr;
+                    gam_at_best = gam;
                     stall = 0;
                 } else {
                     ++stall;
                 }
+                // drift: the recurrence's own residual estimate has fallen a
+                // thousandfold since the true residual last halved
+                const bool drifted = stall > 0 && gam < 1e-3 * gam_at_best;
                 const bool far_enough = ops.restart_rel > 0 && nr <= ops.restart_rel * nbn;
-                if(w.host_flags[0] || stall >= STALL_PERIODS || far_enough) { ++it; break; }
+                if(w.host_flags[0] || stall >= STALL_PERIODS || drifted || far_enough) {
+                    ++it;
+                    break;
+                }
             }
             ++it;
             continue;

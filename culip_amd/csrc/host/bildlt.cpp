@@ -318,8 +318,26 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
     host_real.erase("row_norms");
 }
 
+void Bildlt::jacobi(double* x, bool transpose, hipStream_t st) const
+{
+    const size_t bytes = sizeof(double) * (size_t) std::max(1, n());
+    DevBuf* jb = const_cast<DevBuf*>(d_jac);
+    for(int q = 0; q < 3; ++q) jb[q].ensure(bytes);
+    HIP_CHECK(hipMemcpyAsync(jb[0].p, x, sizeof(double) * (size_t) n(), hipMemcpyDeviceToDevice, st));
+    if(f32())
+        ::culip::jacobi_sweeps<float>(topo(), plan<float>(), transpose, x, jacobi_sweeps,
+            jb[0].as<double>(), jb[1].as<double>(), jb[2].as<double>(), st);
+    else
+        ::culip::jacobi_sweeps<double>(topo(), plan<double>(), transpose, x, jacobi_sweeps,
+            jb[0].as<double>(), jb[1].as<double>(), jb[2].as<double>(), st);
+}
+
 void Bildlt::trsv_u(double* x, hipStream_t st) const
 {
+    if(solve_algo == CULIP_TRSV_JACOBI) {
+        jacobi(x, false, st);
+        return;
+    }
     if(solve_algo == CULIP_TRSV_FLAGGED) {
         const int32_t e = ++epoch_u;
         if(f32()) trsv_upper_flagged<float>(topo(), plan<float>(), x, e, st);
@@ -332,6 +350,10 @@ void Bildlt::trsv_u(double* x, hipStream_t st) const
 
 void Bildlt::trsv_ut(double* x, hipStream_t st) const
 {
+    if(solve_algo == CULIP_TRSV_JACOBI) {
+        jacobi(x, true, st);
+        return;
+    }
     if(solve_algo == CULIP_TRSV_FLAGGED) {
         const int32_t e = ++epoch_t;
         if(f32()) trsv_upper_t_flagged<float>(topo(), plan<float>(), x, e, st);

@@ -336,13 +336,18 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
 
 int32_t culip_bildlt_n(const culip_bildlt* h) { return h ? h->b.n() : -1; }
 
-culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algorithm algorithm)
+culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algorithm algorithm,
+    int32_t num_jacobi_sweeps)
 {
     return guarded([&] {
         CULIP_REQUIRE(h, "bildlt: null handle");
-        CULIP_REQUIRE(algorithm == CULIP_TRSV_LEVELS || algorithm == CULIP_TRSV_FLAGGED,
+        CULIP_REQUIRE(algorithm == CULIP_TRSV_LEVELS || algorithm == CULIP_TRSV_FLAGGED ||
+                algorithm == CULIP_TRSV_JACOBI,
             "bildlt: unknown solve algorithm");
+        CULIP_REQUIRE(algorithm != CULIP_TRSV_JACOBI || num_jacobi_sweeps >= 1,
+            "bildlt: block-Jacobi solves need at least one sweep");
         h->b.solve_algo = algorithm;
+        h->b.jacobi_sweeps = num_jacobi_sweeps;
     });
 }
 

@@ -98,7 +98,9 @@ struct Bildlt {
     // single-launch triangular solves: flags, wait error, epochs
     DevBuf d_flag_u, d_flag_t, d_trsv_err;
     culip_solve_algorithm solve_algo = CULIP_TRSV_LEVELS;
+    int32_t jacobi_sweeps = 0;
     mutable int32_t epoch_u = 0, epoch_t = 0;
+    DevBuf d_jac[3];          // block-Jacobi right-hand side and iterates
 
     // host copies for inspection
     std::map<std::string, std::vector<int32_t>> host_int;
@@ -136,6 +138,7 @@ struct Bildlt {
     void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
     void trsv_u(double* x, hipStream_t st) const;
     void trsv_ut(double* x, hipStream_t st) const;
+    void jacobi(double* x, bool transpose, hipStream_t st) const;
     // fails if a single-launch solve gave up waiting (synchronises st)
     void check_solve_error(hipStream_t st) const;
     SolveTopo topo() const;

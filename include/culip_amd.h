@@ -150,16 +150,24 @@ culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
 
 int32_t culip_bildlt_n(const culip_bildlt * h);
 
-/* set_solve_algorithm (block_ildlt.cuh:226-228): schedule of the block
- * triangular solves. LEVELS (default): one launch per level. FLAGGED: one
- * launch per solve, block rows in level order, each waiting on per-row
- * flags of the rows it reads (slower on gfx950 at the kkt-400k size: the
- * agent-scope release/acquire per block row costs more than the level
- * launches save). Both compute the same values in the same order. Solves
- * with one object must be ordered on one stream. */
-typedef enum { CULIP_TRSV_LEVELS = 0, CULIP_TRSV_FLAGGED = 1 } culip_solve_algorithm;
+/* set_solve_algorithm(algorithm, num_jacobi_sweeps) (block_ildlt.cuh:
+ * 226-228): how the block triangular solves run. LEVELS (default,
+ * SOLVE_BLOCK): one launch per level. FLAGGED: one launch per solve, block
+ * rows in level order, each waiting on per-row flags of the rows it reads
+ * (slower on gfx950 at the kkt-400k size). LEVELS and FLAGGED compute the
+ * same values in the same order. JACOBI (SOLVE_JACOBI,
+ * block_ildlt.impl.cuh:2454-2493): num_jacobi_sweeps >= 1 block-Jacobi
+ * sweeps from x = 0, x <- U_ii^-1 (b - U_off x), one launch each — an
+ * approximate solve that is a fixed linear operator (exact once the sweeps
+ * reach the number of levels). Solves with one object must be ordered on
+ * one stream. */
+typedef enum {
+    CULIP_TRSV_LEVELS = 0,
+    CULIP_TRSV_FLAGGED = 1,
+    CULIP_TRSV_JACOBI = 2
+} culip_solve_algorithm;
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
-    culip_solve_algorithm algorithm);
+    culip_solve_algorithm algorithm, int32_t num_jacobi_sweeps);
 
 /* Host views of the symbolic structures and (after compute) of the factor
  * downloaded from the device; valid until the next call on h.
