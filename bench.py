@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--threshold", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP32-factor and block-Jacobi sub-runs")
     ap.add_argument("--solve-algorithm", default="levels",
                     choices=["levels", "flagged", "jacobi"],
                     help="triangular-solve schedule (set_solve_algorithm)")
@@ -434,31 +436,38 @@ def main():
                "launches": launches}
     dominant, secondary = (rl_trsv, rl_spmv) if per_it_trsv >= per_it_spmv else (rl_spmv, rl_trsv)
 
-    # the same step with the FP32 factor (FP64 SQMR, FP64 true residual)
-    other = None
-    if a.prec == "f64":
-        g32 = ca.BlockiLDLt(prep.row_ptr, prep.col, prep.val, prep.piv_starts,
-                            prep.block_starts, "f32")
-        g32.symbolic(0, a.fill_factor)
-        g32.set_solve_algorithm(a.solve_algorithm, a.jacobi_sweeps)
-        s32 = ca.SQMR(A, 1e-8, 1000, g32, true_residual_period=20, max_restarts=10)
-        x32 = torch.zeros_like(x)
+    def variant(prec, algorithm, sweeps):
+        """The same step with another factor precision or solve schedule."""
+        gv = ca.BlockiLDLt(prep.row_ptr, prep.col, prep.val, prep.piv_starts,
+                           prep.block_starts, prec)
+        gv.symbolic(0, a.fill_factor)
+        gv.set_solve_algorithm(algorithm, sweeps)
+        sv_ = ca.SQMR(A, 1e-8, 1000, gv, true_residual_period=20, max_restarts=10)
+        xv = torch.zeros_like(x)
         for _ in range(a.warmup):
-            g32.refactor(dvals, a.threshold)
-            s32.solve(db, x32)
+            gv.refactor(dvals, a.threshold)
+            sv_.solve(db, xv)
         torch.cuda.synchronize()
-        t32 = time.perf_counter()
+        tv = time.perf_counter()
         for _ in range(a.steps):
-            g32.refactor(dvals, a.threshold)
-            s32.solve(db, x32)
+            gv.refactor(dvals, a.threshold)
+            sv_.solve(db, xv)
         torch.cuda.synchronize()
-        t32 = time.perf_counter() - t32
-        r32 = s32.result
-        other = {"value": a.steps / t32, "ms_per_step": t32 / a.steps * 1e3,
-                 "sqmr_iterations": r32["iterations"], "refinement_restarts": r32["restarts"],
-                 "true_rel_residual": r32["true_rel_residual"],
-                 "max_abs_diff_vs_f64_solution": float((x32 - x).abs().max())}
-        del g32, s32
+        tv = time.perf_counter() - tv
+        rv = sv_.result
+        return {"value": a.steps / tv, "ms_per_step": tv / a.steps * 1e3, "prec": prec,
+                "solve_algorithm": algorithm, "jacobi_sweeps": sweeps,
+                "sqmr_iterations": rv["iterations"], "refinement_restarts": rv["restarts"],
+                "true_rel_residual": rv["true_rel_residual"],
+                "max_abs_diff_vs_main_solution": float((xv - x).abs().max())}
+
+    # FP32 factor (FP64 SQMR, FP64 true residual); block-Jacobi solves
+    other = jac = None
+    if a.prec == "f64" and not a.no_variants:
+        other = variant("f32", a.solve_algorithm, a.jacobi_sweeps)
+        if a.solve_algorithm == "levels":
+            log("variant: jacobi solves, 2 sweeps")
+            jac = variant("f64", "jacobi", 2)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -499,6 +508,8 @@ def main():
     }
     if other is not None:
         out["f32_factor"] = other
+    if jac is not None:
+        out["jacobi_solves"] = jac
     print(json.dumps(out))
 
 

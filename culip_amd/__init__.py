@@ -4,6 +4,7 @@ Python mirror of the reference's C++ interfaces (dthuerck/culip):
   BlockiLDLt   libs/la/preconditioner/block_ildlt.cuh:188-311
   CSRMatrixSPMV, AugmentedMatrixSPMV   libs/la/spmv.cuh:36-149
   SQMR         libs/la/iterative_solvers/sqmr.cuh:21-72
+  ImportLDUPreconditioner   libs/la/preconditioner/import_ldu_preconditioner.cuh:36-125
   read_matrix_csr / read_dense_vector  libs/test/test.h
 over the C-ABI of libculip_amd.so. Device vectors are torch CUDA (HIP)
 float64 tensors; torch only provides device memory and streams.
@@ -18,7 +19,7 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
                    check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
-           "SQMR", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
+           "SQMR", "ImportLDUPreconditioner", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
            "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR"]
 
 
@@ -255,6 +256,75 @@ class AugmentedMatrixSPMV(_Spmv):
         self._h = h
 
 
+class ImportLDUPreconditioner:
+    """ImportLDUPreconditioner<T> (import_ldu_preconditioner.cuh:36-125):
+    M = inv(S_r) P_r' L D R P_c' inv(S_c) with R = L' (U None) or U.
+
+    L, U: (row_ptr, col, val) host CSR; tri / is_piv: the tridiagonal D and
+    its pivot-start flags (None: no D); exact_solve False applies 10 Jacobi
+    sweeps instead of the substitution; prec 'f32' stores L and U in FP32."""
+
+    def __init__(self, L, tri=None, is_piv=None, U=None, exact_solve=True, prec="f64"):
+        ls, keep_l = csr_struct(*L)
+        us, keep_u = csr_struct(*U) if U is not None else (None, None)
+        self._keep = (keep_l, keep_u)
+        t = _f64(tri) if tri is not None else None
+        p = _i32(is_piv) if is_piv is not None else None
+        pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
+        h = C.c_void_p()
+        check(lib().culip_ldu_create(C.byref(ls), C.byref(us) if us is not None else None,
+                                     _ptr(t, C.c_double) if t is not None else None,
+                                     _ptr(p, C.c_int32) if p is not None else None,
+                                     1 if exact_solve else 0, pr, C.byref(h)))
+        self._h = h
+        self.N = len(L[0]) - 1
+
+    def n(self):
+        return lib().culip_ldu_n(self._h)
+
+    def is_left(self):
+        return True
+
+    def is_middle(self):
+        return bool(lib().culip_ldu_is_middle(self._h))
+
+    def is_right(self):
+        return True
+
+    def set_permutation(self, p_r, p_c=None):
+        """Push convention: entry i moves to p[i]; None removes."""
+        if p_r is None:
+            check(lib().culip_ldu_set_permutation(self._h, None, None))
+            return
+        a, b = _i32(p_r), _i32(p_c if p_c is not None else p_r)
+        check(lib().culip_ldu_set_permutation(self._h, _ptr(a, C.c_int32), _ptr(b, C.c_int32)))
+
+    def set_scaling(self, s_r, s_c=None):
+        if s_r is None:
+            check(lib().culip_ldu_set_scaling(self._h, None, None))
+            return
+        a, b = _f64(s_r), _f64(s_c if s_c is not None else s_r)
+        check(lib().culip_ldu_set_scaling(self._h, _ptr(a, C.c_double), _ptr(b, C.c_double)))
+
+    def _solve(self, side, b, x, transpose, stream):
+        check(lib().culip_ldu_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),
+                                    _stream(stream)))
+
+    def solve_left(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_LEFT, b, x, transpose, stream)
+
+    def solve_middle(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_MIDDLE, b, x, transpose, stream)
+
+    def solve_right(self, b, x, transpose=False, stream=None):
+        self._solve(SOLVE_RIGHT, b, x, transpose, stream)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().culip_ldu_destroy(self._h)
+            self._h = None
+
+
 class SQMR:
     """SQMR<double>(spmv, tol, max_it, prec); solve(b, x) -> (residual, iterations)."""
 
@@ -263,8 +333,11 @@ class SQMR:
         self._spmv, self._prec = spmv, prec
         o = _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts)
         h = C.c_void_p()
-        check(lib().culip_sqmr_create(spmv._h, prec._h if prec is not None else None,
-                                      C.byref(o), C.byref(h)))
+        if isinstance(prec, ImportLDUPreconditioner):
+            check(lib().culip_sqmr_create_ldu(spmv._h, prec._h, C.byref(o), C.byref(h)))
+        else:
+            check(lib().culip_sqmr_create(spmv._h, prec._h if prec is not None else None,
+                                          C.byref(o), C.byref(h)))
         self._h = h
         self.result = None
 

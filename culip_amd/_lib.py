@@ -112,6 +112,20 @@ _SIGS = [
     ("culip_sqmr_solve", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SqmrResult),
                                    C.c_void_p]),
     ("culip_sqmr_destroy", None, [C.c_void_p]),
+    ("culip_ldu_create", C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int32), C.c_int32, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    ("culip_ldu_set_permutation", C.c_int, [C.c_void_p, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32)]),
+    ("culip_ldu_set_scaling", C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]),
+    ("culip_ldu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    ("culip_ldu_n", C.c_int32, [C.c_void_p]),
+    ("culip_ldu_is_middle", C.c_int32, [C.c_void_p]),
+    ("culip_ldu_destroy", None, [C.c_void_p]),
+    ("culip_sqmr_create_ldu", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SqmrOpts),
+                                        C.POINTER(C.c_void_p)]),
     ("culip_server_create", C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     ("culip_server_add_pattern", C.c_int, [C.c_void_p, C.POINTER(Csr), C.c_int32,
                                            C.POINTER(C.c_int32), C.c_int32,

@@ -922,8 +922,7 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
 // one column of the list arrays per lane; every value is formed by the same
 // operations in the same order as in the wave form (terms that are exact
 // zeros in the wave form are skipped).
-constexpr int TINY_E = 8;
-constexpr int TINY_P = 64;   // the host sets the eligibility bit with this bound
+constexpr int TINY_E = 8;    // with at most 64 Schur pairs, set by the host
 constexpr int TINY_T = 32;
 constexpr int TINY_TPB = 64;
 
@@ -1037,7 +1036,6 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         }
         // distinct columns of U_ki
         int dc[TINY_E];
-        int ndc = 0;
 #pragma unroll
         for(int e = 0; e < TINY_E; ++e) {
             bool seen = ac[e] < 0;

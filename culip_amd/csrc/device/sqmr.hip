@@ -115,10 +115,13 @@ __device__ void sym_ortho(double a, double b, double& c, double& s, double& r)
     }
 }
 
-// scalar part of one iteration (sqmr.impl.cuh:192-283), on a private copy
-__device__ void scalar_step(double* S, int k, double alpha, double delta1, double beta1)
+// scalar part of one iteration (sqmr.impl.cuh:192-283), on a private copy;
+// K0 = k mod 3 is a template argument so that every index into S is a
+// constant and S stays in registers
+template<int K0>
+__device__ __forceinline__ void scalar_step(double* S, double alpha, double delta1, double beta1)
 {
-    const int k0 = k % 3, k1 = (k + 1) % 3, km1 = (k + 2) % 3;
+    constexpr int k0 = K0, k1 = (K0 + 1) % 3, km1 = (K0 + 2) % 3;
     S[S_ALPHA] = alpha;
     S[S_DELTA + k1] = delta1;
     S[S_BETA + k1] = beta1;
@@ -204,25 +207,27 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
     if(threadIdx.x == 0) { part[8 * blockIdx.x + 3] = num; part[8 * blockIdx.x + 4] = den; }
 }
 
-__device__ void iteration_scalars(double* S, const double* scal, const double* part, int G,
-    int k, double* sh)
+template<int K0>
+__device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
+    const double* part, int G, double* sh)
 {
+#pragma unroll
     for(int q = 0; q < NSCAL; ++q) S[q] = scal[q];
     const double alpha = reduce_part(part, G, 8, 0, sh) / reduce_part(part, G, 8, 1, sh);
     const double delta = sqrt(reduce_part(part, G, 8, 2, sh));
     const double beta = reduce_part(part, G, 8, 3, sh) / reduce_part(part, G, 8, 4, sh);
-    scalar_step(S, k, alpha, delta, beta);
+    scalar_step<K0>(S, alpha, delta, beta);
 }
 
-__global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
-    double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
-    double* part)
+template<int K0>
+__device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk, double* pk,
+    const double* pkm1, const double* pkm2, double* x, const double* scal, double* part)
 {
     __shared__ double sh[16];
     double S[NSCAL];
     const int G = gridDim.x;
-    iteration_scalars(S, scal, part, G, k, sh);
-    const double ib = 1.0 / S[S_BETA + (k + 1) % 3];
+    iteration_scalars<K0>(S, scal, part, G, sh);
+    const double ib = 1.0 / S[S_BETA + (K0 + 1) % 3];
     const double nbrr = -S[S_BRR], neps = -S[S_EPS], ir = 1.0 / S[S_RHO], tau = S[S_TAU];
     double ss = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
@@ -240,26 +245,49 @@ __global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, c
     if(threadIdx.x == 0) part[8 * blockIdx.x + 5] = ss;
 }
 
+__global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
+    double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
+    double* part)
+{
+    switch(k % 3) {
+    case 0: pupdate<0>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
+    case 1: pupdate<1>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
+    default: pupdate<2>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
+    }
+}
+
 // k_sqmr_test (:89-107) and the state update; one workgroup
 // brel < 0: the reference's breakdown test |delta| <= tol; brel >= 0:
 // |delta| <= brel * ||A||est, scale-free (the delta of a badly scaled but
 // healthy recurrence can fall below an absolute 1e-8)
-__global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, const double* part,
+template<int K0>
+__device__ __forceinline__ void scalars(int k, int G, double* scal, const double* part,
     int32_t* flags, double tol, double brel)
 {
     __shared__ double sh[16];
     double S[NSCAL];
-    iteration_scalars(S, scal, part, G, k, sh);
+    iteration_scalars<K0>(S, scal, part, G, sh);
     const double nx = sqrt(reduce_part(part, G, 8, 5, sh));
     if(threadIdx.x == 0) {
-        const int k1 = (k + 1) % 3;
+        constexpr int k1 = (K0 + 1) % 3;
         S[S_NORMX] = nx;
         S[S_GAMMA] = S[S_S + k1] * S[S_GAMMA];
         const double dl = fabs(S[S_DELTA + k1]);
         flags[0] |= (brel < 0) ? (dl <= tol) : (dl <= brel * S[S_NORMA]);
         flags[1] |= (S[S_GAMMA] / (S[S_NORMA] * S[S_NORMX] + S[S_NORMPB]) <= tol);
         S[S_ITER] = (double) k;
+#pragma unroll
         for(int q = 0; q < NSCAL; ++q) scal[q] = S[q];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, const double* part,
+    int32_t* flags, double tol, double brel)
+{
+    switch(k % 3) {
+    case 0: scalars<0>(k, G, scal, part, flags, tol, brel); break;
+    case 1: scalars<1>(k, G, scal, part, flags, tol, brel); break;
+    default: scalars<2>(k, G, scal, part, flags, tol, brel); break;
     }
 }
 
@@ -490,8 +518,10 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
             // true-residual gate: the recurrence runs on past the reference's
             // quasi-residual test; it stops when the FP64 true residual meets
             // the tolerance, and hands over to a refinement restart on a
-            // Lanczos breakdown or when the best true residual has not halved
-            // for STALL_PERIODS checks (the recurrence has drifted). The
+            // Lanczos breakdown, when the best true residual has not halved
+            // for STALL_PERIODS checks, or when the quasi-residual has fallen
+            // a thousandfold since the true residual last halved (in both
+            // cases the recurrence has drifted from the true residual). The
             // quasi-residual test is not used here: with a large ||A||
             // estimate it fires long before the true residual is small.
             if((it + 1) % period == 0) {

@@ -501,7 +501,7 @@ Sqmr::~Sqmr()
     if(host_scal) (void) hipHostFree(host_scal);
 }
 
-void Sqmr::init(const Spmv* op, const Bildlt* p, const culip_sqmr_opts& o)
+void Sqmr::init(const Spmv* op, const Precond* p, const culip_sqmr_opts& o)
 {
     A = op;
     prec = p;
@@ -517,7 +517,7 @@ void Sqmr::init(const Spmv* op, const Bildlt* p, const culip_sqmr_opts& o)
 
 void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t st)
 {
-    if(prec && !prec->factored) fail(CULIP_ERR_STATE, "sqmr: preconditioner has not been computed");
+    if(prec && !prec->ready()) fail(CULIP_ERR_STATE, "sqmr: preconditioner has not been computed");
     SqmrOps ops;
     ops.n = A->m;
     const Spmv* op = A;
@@ -525,14 +525,14 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         op->multiply(xx, yy, 1.0, 0.0, false, (hipStream_t) s);
     };
     if(prec) {
-        const Bildlt* P = prec;
-        ops.left = [P](double* xx, void* s) { P->trsv_ut(xx, (hipStream_t) s); };
-        ops.right = [P](double* xx, void* s) { P->trsv_u(xx, (hipStream_t) s); };
-        ops.tri = P->d_tri.as<double>();
-        ops.is_piv = P->d_is_piv.as<int32_t>();
+        const Precond* P = prec;
+        ops.left = [P](double* xx, void* s) { P->left(xx, (hipStream_t) s); };
+        ops.right = [P](double* xx, void* s) { P->right(xx, (hipStream_t) s); };
+        ops.tri = P->tri();
+        ops.is_piv = P->is_piv();
     }
     sqmr_solve(ops, w, b, x, opts, res, st);
-    if(prec) prec->check_solve_error(st);
+    if(prec) prec->after_solve(st);
 }
 
 }  // namespace culip

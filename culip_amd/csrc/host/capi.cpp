@@ -18,6 +18,7 @@ struct culip_host_matrix { culip::HostCsr a; };
 struct culip_bildlt { culip::Bildlt b; };
 struct culip_spmv { culip::Spmv s; };
 struct culip_sqmr { culip::Sqmr q; };
+struct culip_ldu { culip::Ldu l; };
 
 using namespace culip;
 
@@ -328,7 +329,6 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
         CULIP_REQUIRE(h && b && x, "bildlt: null argument");
         CULIP_REQUIRE(side == CULIP_SOLVE_LEFT || side == CULIP_SOLVE_MIDDLE || side == CULIP_SOLVE_RIGHT,
             "bildlt: unknown side");
-        CULIP_REQUIRE(side == CULIP_SOLVE_MIDDLE || b != x || true, "");
         h->b.solve(side, transpose != 0, b, x, (hipStream_t) stream);
         HIP_CHECK(hipGetLastError());
     });
@@ -572,6 +572,81 @@ culip_status culip_sqmr_solve(culip_sqmr* s, const double* b, double* x, culip_s
 }
 
 void culip_sqmr_destroy(culip_sqmr* s) { delete s; }
+
+// ---- imported factor -------------------------------------------------------
+
+culip_status culip_ldu_create(const culip_csr* L, const culip_csr* U, const double* tri,
+    const int32_t* is_piv, int32_t exact_solve, culip_prec prec, culip_ldu** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(L && out, "ldu: null argument");
+        CULIP_REQUIRE(prec == CULIP_PREC_F64 || prec == CULIP_PREC_F32, "ldu: unknown precision");
+        HostCsr l = to_host(L);
+        HostCsr u;
+        if(U) u = to_host(U);
+        CULIP_REQUIRE(l.m == l.n, "ldu: L must be square");
+        CULIP_REQUIRE(!U || (u.m == l.m && u.n == l.n), "ldu: U must match L");
+        CULIP_REQUIRE((tri == nullptr) == (is_piv == nullptr),
+            "ldu: D needs both the tridiagonal and the pivot flags");
+        device_present();
+        auto* h = new culip_ldu;
+        try {
+            h->l.create(l, U ? &u : nullptr, tri, is_piv, exact_solve != 0, prec);
+        } catch(...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+culip_status culip_ldu_set_permutation(culip_ldu* h, const int32_t* p_r, const int32_t* p_c)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "ldu: null handle");
+        h->l.set_permutation(p_r, p_c);
+    });
+}
+
+culip_status culip_ldu_set_scaling(culip_ldu* h, const double* s_r, const double* s_c)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "ldu: null handle");
+        h->l.set_scaling(s_r, s_c);
+    });
+}
+
+culip_status culip_ldu_solve(const culip_ldu* h, culip_side side, int32_t transpose,
+    const double* b, double* x, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && b && x, "ldu: null argument");
+        CULIP_REQUIRE(side == CULIP_SOLVE_LEFT || side == CULIP_SOLVE_MIDDLE || side == CULIP_SOLVE_RIGHT,
+            "ldu: unknown side");
+        h->l.solve(side, transpose != 0, b, x, (hipStream_t) stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int32_t culip_ldu_n(const culip_ldu* h) { return h ? h->l.n : -1; }
+
+int32_t culip_ldu_is_middle(const culip_ldu* h) { return h && h->l.has_d ? 1 : 0; }
+
+void culip_ldu_destroy(culip_ldu* h) { delete h; }
+
+culip_status culip_sqmr_create_ldu(const culip_spmv* A, const culip_ldu* prec,
+    const culip_sqmr_opts* opts, culip_sqmr** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(A && prec && opts && out, "sqmr: null argument");
+        CULIP_REQUIRE(A->s.m == A->s.n, "sqmr: operator must be square");
+        CULIP_REQUIRE(prec->l.n == A->s.m, "sqmr: preconditioner size mismatch");
+        CULIP_REQUIRE(opts->max_it >= 0 && opts->tol >= 0, "sqmr: bad options");
+        device_present();
+        auto* s = new culip_sqmr;
+        try {
+            s->q.init(&A->s, &prec->l, *opts);
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
 
 // ---- request loop --------------------------------------------------------
 

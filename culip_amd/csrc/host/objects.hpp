@@ -51,6 +51,21 @@ void upload(DevBuf& d, const std::vector<T>& h)
     if(!h.empty()) HIP_CHECK(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
 
+// What SQMR needs of a preconditioner M = Pl D Pr (Preconditioner<T>,
+// preconditioner.cuh:18-43): in-place left and right solves and D as a
+// tridiagonal vector with pivot-start flags (NULL: identity).
+struct Precond {
+    virtual ~Precond() = default;
+    virtual int32_t size() const = 0;
+    virtual bool ready() const = 0;
+    virtual void left(double* x, hipStream_t st) const = 0;    // x = Pl \ x
+    virtual void right(double* x, hipStream_t st) const = 0;   // x = Pr \ x
+    virtual const double* tri() const = 0;
+    virtual const int32_t* is_piv() const = 0;
+    // after a solve: report errors the device recorded (may synchronise st)
+    virtual void after_solve(hipStream_t) const {}
+};
+
 // Per sparsity pattern: the matrix as given, its symbolic analysis and the
 // static device structures derived from it. Shared by every numeric
 // instance of the pattern (one per concurrent factorization).
@@ -68,7 +83,7 @@ struct BildltPattern {
     int64_t device_bytes() const;
 };
 
-struct Bildlt {
+struct Bildlt final : Precond {
     std::shared_ptr<BildltPattern> pat;
     // the pattern's members under their own names
     HostCsr& A;
@@ -148,6 +163,61 @@ struct Bildlt {
     template<typename VT> FactorArgs<VT> factor_args() const;
     template<typename VT> SolvePlan<VT> plan() const;
     template<typename VT> void factor_impl(hipStream_t st);
+
+    // M = U' D U: left U^-T, right U^-1
+    int32_t size() const override { return n(); }
+    bool ready() const override { return factored; }
+    void left(double* x, hipStream_t st) const override { trsv_ut(x, st); }
+    void right(double* x, hipStream_t st) const override { trsv_u(x, st); }
+    const double* tri() const override { return d_tri.as<double>(); }
+    const int32_t* is_piv() const override { return d_is_piv.as<int32_t>(); }
+    void after_solve(hipStream_t st) const override { check_solve_error(st); }
+};
+
+// ImportLDUPreconditioner (import_ldu_preconditioner.cuh:36-125): an
+// imported factor M = inv(S_r) P_r' L D R P_c' inv(S_c) with R = L' (the
+// symmetric constructors) or U, each triangular operand resident with its
+// level schedule; the transposed operands are stored explicitly so every
+// substitution reads rows.
+struct Ldu final : Precond {
+    struct Operand {
+        int32_t levels = 0;
+        bool lower = true;
+        std::vector<int32_t> lvl_off;   // host: levels + 1
+        DevBuf row_ptr, col, val, diag_pos, order;
+    };
+    int32_t n = 0;
+    culip_prec prec = CULIP_PREC_F64;
+    bool symmetric = true, has_d = false, exact = true;
+    Operand L, Lt, U, Ut;
+    DevBuf d_inv_l, d_inv_u;            // approximate solves: 1 / diagonal
+    DevBuf d_tri, d_is_piv;
+    DevBuf d_pr, d_pc, d_sr, d_sc;      // permutation / scaling (optional)
+    bool has_p = false, has_s = false;
+    DevBuf d_t1, d_t2;                  // work vectors (solves are not reentrant)
+    static constexpr int32_t APPROX_SWEEPS = 10;   // import_ldu_preconditioner.impl.cuh:223
+
+    void create(const HostCsr& l, const HostCsr* u, const double* tri, const int32_t* is_piv,
+        bool exact_solve, culip_prec p);
+    void set_permutation(const int32_t* pr, const int32_t* pc);
+    void set_scaling(const double* sr, const double* sc);
+    void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
+
+    int32_t size() const override { return n; }
+    bool ready() const override { return true; }
+    void left(double* x, hipStream_t st) const override { solve(CULIP_SOLVE_LEFT, false, x, x, st); }
+    void right(double* x, hipStream_t st) const override
+    {
+        solve(CULIP_SOLVE_RIGHT, false, x, x, st);
+    }
+    const double* tri() const override { return has_d ? d_tri.as<double>() : nullptr; }
+    const int32_t* is_piv() const override { return has_d ? d_is_piv.as<int32_t>() : nullptr; }
+    int64_t device_bytes() const;
+
+private:
+    void tri_apply(const Operand& o, const DevBuf& inv, const double* in, double* out,
+        hipStream_t st) const;
+    void ps(bool left_side, bool push, const double* b, double* x, hipStream_t st) const;
 };
 
 struct Spmv {
@@ -167,7 +237,7 @@ void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h)
 
 struct Sqmr {
     const Spmv* A = nullptr;
-    const Bildlt* prec = nullptr;
+    const Precond* prec = nullptr;
     culip_sqmr_opts opts{};
     DevBuf work;
     int32_t* host_flags = nullptr;
@@ -179,7 +249,7 @@ struct Sqmr {
     ~Sqmr();
     // binds the operator and preconditioner (borrowed) and allocates the
     // work vectors
-    void init(const Spmv* op, const Bildlt* p, const culip_sqmr_opts& o);
+    void init(const Spmv* op, const Precond* p, const culip_sqmr_opts& o);
     // SQMR::solve on device vectors; synchronises st
     void run(const double* b, double* x, culip_sqmr_result& res, hipStream_t st);
 };

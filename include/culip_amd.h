@@ -248,6 +248,46 @@ culip_status culip_sqmr_solve(culip_sqmr * s, const double * b, double * x,
 void culip_sqmr_destroy(culip_sqmr * s);
 
 /* ------------------------------------------------------------------------ */
+/* ImportLDUPreconditioner (libs/la/preconditioner/import_ldu_preconditioner.cuh:
+ * 36-125): an imported factor M = inv(S_r) P_r' L D R P_c' inv(S_c). */
+
+typedef struct culip_ldu culip_ldu;
+
+/* The three ctors (import_ldu_preconditioner.impl.cuh:19-130) in one:
+ * L (host CSR, n x n; its lower triangle with a non-unit diagonal is used)
+ * and U (NULL: R = L', the symmetric ctors; else its upper triangle);
+ * tri / is_piv (3n doubles / n ints, host; both NULL: no D) in the layout
+ * of the middle solve (helper_kernels.impl.cuh:720-772). exact_solve = 0
+ * applies 10 Jacobi sweeps instead of the exact substitution
+ * (sparse_la.impl.cuh:79-126). prec = F32 stores L and U in FP32 (sums in
+ * FP64). Everything is copied to the device; a row without a diagonal
+ * entry divides by 1. */
+culip_status culip_ldu_create(const culip_csr * L, const culip_csr * U, const double * tri,
+    const int32_t * is_piv, int32_t exact_solve, culip_prec prec, culip_ldu ** out);
+
+/* set_permutation (:142-155): n-entry host arrays, push convention (entry
+ * i moves to p[i]); both NULL removes the permutation. */
+culip_status culip_ldu_set_permutation(culip_ldu * h, const int32_t * p_r, const int32_t * p_c);
+
+/* set_scaling (:157-170): componentwise factors, host; both NULL removes. */
+culip_status culip_ldu_set_scaling(culip_ldu * h, const double * s_r, const double * s_c);
+
+/* solve_left / solve_middle / solve_right (:197-330). b, x: n device
+ * doubles (b == x allowed). Asynchronous on `stream`; one handle's solves
+ * share its work vectors, so they must not overlap. */
+culip_status culip_ldu_solve(const culip_ldu * h, culip_side side, int32_t transpose,
+    const double * b, double * x, void * stream);
+int32_t culip_ldu_n(const culip_ldu * h);
+/* is_middle (:180-186): 1 when D was given */
+int32_t culip_ldu_is_middle(const culip_ldu * h);
+void culip_ldu_destroy(culip_ldu * h);
+
+/* SQMR ctor with an imported factor as the preconditioner
+ * (sqmr.impl.cuh:19-56, Preconditioner<T> = ImportLDUPreconditioner<T>). */
+culip_status culip_sqmr_create_ldu(const culip_spmv * A, const culip_ldu * prec,
+    const culip_sqmr_opts * opts, culip_sqmr ** out);
+
+/* ------------------------------------------------------------------------ */
 /* Request loop for KKT solves (no counterpart in the reference, which
  * solves one system per process): sparsity patterns are registered once
  * (symbolic analysis and static device structures stay resident), requests

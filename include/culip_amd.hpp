@@ -35,6 +35,9 @@ inline void check(culip_status s)
     if(s != CULIP_OK) throw Error(s, culip_last_error());
 }
 
+/* block_ildlt.cuh:24-30, same names and values. */
+enum SOLVE_ALGORITHM { SOLVE_STRIPE = 0, SOLVE_BLOCK = 1, SOLVE_SCALAR = 2, SOLVE_JACOBI = 3 };
+
 /* Preconditioner<T> with solve_left/middle/right (preconditioner.cuh). */
 template<typename T>
 class BlockiLDLt {
@@ -72,6 +75,16 @@ public:
 
     /* solve_analysis() is part of compute(); kept so callers compile. */
     void solve_analysis() {}
+
+    /* block_ildlt.impl.cuh:253-260. STRIPE, BLOCK and SCALAR all apply
+     * the exact U^-1 (resp. U^-T); here they share the level-scheduled
+     * solve. JACOBI runs num_jacobi_sweeps block-Jacobi sweeps from x = 0. */
+    void set_solve_algorithm(SOLVE_ALGORITHM algorithm, int32_t num_jacobi_sweeps)
+    {
+        check(culip_bildlt_set_solve_algorithm(h_,
+            algorithm == SOLVE_JACOBI ? CULIP_TRSV_JACOBI : CULIP_TRSV_LEVELS,
+            num_jacobi_sweeps));
+    }
 
     bool is_left() const { return true; }
     bool is_middle() const { return true; }

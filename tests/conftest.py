@@ -63,3 +63,39 @@ def hip_device():
     if not torch.cuda.is_available():
         pytest.fail("gpu test run without a HIP device")
     return torch.device("cuda:0")
+
+
+def _read(path):
+    import culip_amd as ca
+    return ca.read_matrix_csr(path)
+
+
+def ldu_problem(golden, prefix):
+    """A sqmr_{small,big} fixture: A, L, tridiagonal D + is_piv, p, s."""
+    import culip_amd as ca
+    ar, ac, av, an = _read(f"{golden}/{prefix}_A.mtx")
+    lr, lc, lv, ln = _read(f"{golden}/{prefix}_prec_L.mtx")
+    dr, dc, dv, dn = _read(f"{golden}/{prefix}_prec_D.mtx")
+    p = ca.read_dense_vector(f"{golden}/{prefix}_prec_p.mtx").astype(np.int32)
+    s = ca.read_dense_vector(f"{golden}/{prefix}_prec_s.mtx")
+    n = an
+    # D to tridiagonal + is_piv (test_sqmr.cc:103-152)
+    tri = np.zeros(3 * n)
+    is_piv = np.ones(n, np.int32)
+    i = 0
+    while i < n:
+        ln_i = dr[i + 1] - dr[i]
+        if ln_i == 1:
+            if i > 0:
+                tri[3 * (i - 1) + 2] = 0
+            tri[3 * i + 1] = dv[dr[i]]
+            if i < n - 1:
+                tri[3 * (i + 1)] = 0
+            i += 1
+        else:
+            iv, ipv = dv[dr[i]:dr[i + 1]], dv[dr[i + 1]:dr[i + 2]]
+            tri[3 * i], tri[3 * i + 1], tri[3 * i + 2] = 0, iv[0], iv[1]
+            tri[3 * (i + 1)], tri[3 * (i + 1) + 1], tri[3 * (i + 1) + 2] = ipv[0], ipv[1], 0
+            is_piv[i + 1] = 0
+            i += 2
+    return (ar, ac, av), (lr, lc, lv), tri, is_piv, p, s, n

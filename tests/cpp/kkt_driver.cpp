@@ -54,6 +54,7 @@ static int run(int32_t n, int32_t m)
     {
         BlockiLDLt<T> blocky(nullptr, &Kp, np, ps, nb, bs);
         blocky.compute(1, T(1.0), T(1e-4));
+        blocky.set_solve_algorithm(SOLVE_BLOCK, 0);
         blocky.solve_analysis();
 
         const int32_t N = Kp.m;

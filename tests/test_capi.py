@@ -153,3 +153,22 @@ def test_server_needs_a_device():
     with pytest.raises(ca.CulipError) as e:
         ca.Server(slots=2)
     assert e.value.status == 8
+
+
+def test_imported_factor_checks_inputs_before_the_device():
+    """ImportLDUPreconditioner: malformed inputs are rejected on the host
+    with INVALID; well-formed ones need a device (NO_DEVICE here)."""
+    import torch
+    L = (np.array([0, 1, 3], np.int32), np.array([0, 0, 1], np.int32), np.array([2.0, 1.0, 3.0]))
+    with pytest.raises(ca.CulipError) as e:
+        ca.ImportLDUPreconditioner(L, tri=np.zeros(6))          # D without pivot flags
+    assert e.value.status == 1
+    rect = (np.array([0, 1, 2], np.int32), np.array([0, 2], np.int32), np.array([1.0, 1.0]))
+    with pytest.raises(ca.CulipError) as e:
+        ca.ImportLDUPreconditioner((rect[0], rect[1], rect[2]), U=L)
+    assert e.value.status == 1                                   # column out of range
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(ca.CulipError) as e:
+        ca.ImportLDUPreconditioner(L)
+    assert e.value.status == 8

@@ -11,6 +11,7 @@ import scipy.sparse as sp
 
 import culip_amd as ca
 import oracle_lib as ol
+from conftest import ldu_problem
 
 
 def _csr(path):
@@ -50,39 +51,10 @@ def test_kkt_adapter_alpha_beta_delta_ix():
     np.testing.assert_allclose(out, 2.0 * K @ x - 0.5 * y0, rtol=1e-13, atol=1e-13)
 
 
-def _ldu_problem(golden, prefix):
-    ar, ac, av, an = _csr(f"{golden}/{prefix}_A.mtx")
-    lr, lc, lv, ln = _csr(f"{golden}/{prefix}_prec_L.mtx")
-    dr, dc, dv, dn = _csr(f"{golden}/{prefix}_prec_D.mtx")
-    p = ca.read_dense_vector(f"{golden}/{prefix}_prec_p.mtx").astype(np.int32)
-    s = ca.read_dense_vector(f"{golden}/{prefix}_prec_s.mtx")
-    n = an
-    # D to tridiagonal + is_piv (test_sqmr.cc:103-152)
-    tri = np.zeros(3 * n)
-    is_piv = np.ones(n, np.int32)
-    i = 0
-    while i < n:
-        ln_i = dr[i + 1] - dr[i]
-        if ln_i == 1:
-            if i > 0:
-                tri[3 * (i - 1) + 2] = 0
-            tri[3 * i + 1] = dv[dr[i]]
-            if i < n - 1:
-                tri[3 * (i + 1)] = 0
-            i += 1
-        else:
-            iv, ipv = dv[dr[i]:dr[i + 1]], dv[dr[i + 1]:dr[i + 2]]
-            tri[3 * i], tri[3 * i + 1], tri[3 * i + 2] = 0, iv[0], iv[1]
-            tri[3 * (i + 1)], tri[3 * (i + 1) + 1], tri[3 * (i + 1) + 2] = ipv[0], ipv[1], 0
-            is_piv[i + 1] = 0
-            i += 2
-    return (ar, ac, av), (lr, lc, lv), tri, is_piv, p, s, n
-
-
 @pytest.mark.parametrize("prefix,precondition,two_by_two", [
     ("sqmr_small", False, 0), ("sqmr_small", True, 0), ("sqmr_big", True, 15)])
 def test_sqmr_reference_fixtures(golden, prefix, precondition, two_by_two):
-    (ar, ac, av), L, tri, is_piv, p, s, n = _ldu_problem(golden, prefix)
+    (ar, ac, av), L, tri, is_piv, p, s, n = ldu_problem(golden, prefix)
     assert (is_piv == 0).sum() == two_by_two
     x_true = np.ones(n)
     A = sp.csr_matrix((av, ac, ar), shape=(n, n))
