@@ -71,6 +71,15 @@ def parse():
 
 PROBE_COPY_DOUBLES = 1 << 25   # 256 MiB read + 256 MiB written
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one run each, `--pmc-probe`), scaled by what the same counters
+# read for the probe's device copy of known size (reads x2.000, writes
+# x1.000); kept in profiles/kkt400k_hbm_traffic.md. Configurations without
+# such a run report null.
+MEASURED_TRAFFIC = {
+    ("kkt-400k", "f64", "color", "levels"): {"k_trsv_level": 322.1e6, "k_csr_spmv": 269.8e6},
+}
+
 
 def event_time(fn, reps, stream=None):
     """Average device duration of fn() in ms, HIP events on torch's stream."""
@@ -434,6 +443,12 @@ def main():
                "frac": bytes_trsv / ms_trsv / 1e6 / peak, "traffic": None,
                "bytes_per_launch": bytes_trsv, "ms_per_launch": ms_trsv,
                "launches": launches}
+    tr = MEASURED_TRAFFIC.get((a.workload, a.prec, a.order, a.solve_algorithm))
+    if tr is not None:
+        rl_trsv["traffic"] = tr["k_trsv_level"]
+        rl_spmv["traffic"] = tr["k_csr_spmv"]
+        for r in (rl_trsv, rl_spmv):
+            r["traffic_source"] = "profiles/kkt400k_hbm_traffic.md"
     dominant, secondary = (rl_trsv, rl_spmv) if per_it_trsv >= per_it_spmv else (rl_spmv, rl_trsv)
 
     def variant(prec, algorithm, sweeps):

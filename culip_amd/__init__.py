@@ -10,6 +10,7 @@ over the C-ABI of libculip_amd.so. Device vectors are torch CUDA (HIP)
 float64 tensors; torch only provides device memory and streams.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -177,6 +178,13 @@ class BlockiLDLt:
         a = {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED,
              "jacobi": TRSV_JACOBI}.get(algorithm, algorithm)
         check(lib().culip_bildlt_set_solve_algorithm(self._h, a, num_jacobi_sweeps))
+
+    def export_factorized(self, folder):
+        """Writes out_U.mtx, out_D.mtx, out_P.mtx under folder; returns the
+        achieved fill factor (2 nnz(U) + nnz(D)) / nnz(A)."""
+        f = C.c_double()
+        check(lib().culip_bildlt_export_factorized(self._h, os.fsencode(folder), C.byref(f)))
+        return f.value
 
     def _solve(self, side, b, x, transpose, stream):
         check(lib().culip_bildlt_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),

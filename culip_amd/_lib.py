@@ -92,6 +92,8 @@ _SIGS = [
                                      C.c_void_p]),
     ("culip_bildlt_n", C.c_int32, [C.c_void_p]),
     ("culip_bildlt_set_solve_algorithm", C.c_int, [C.c_void_p, C.c_int, C.c_int32]),
+    ("culip_bildlt_export_factorized", C.c_int, [C.c_void_p, C.c_char_p,
+                                                 C.POINTER(C.c_double)]),
     ("culip_bildlt_get_int", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.POINTER(C.c_int32)),
                                        C.POINTER(C.c_int64)]),
     ("culip_bildlt_get_real", C.c_int, [C.c_void_p, C.c_char_p,

@@ -17,9 +17,11 @@
 //   k_store     one wave per block: compacted final storage
 //
 // The dependency blocks of a tile are walked with their metadata fetched
-// lane-parallel up front and each block's entries fetched one step ahead
-// into registers, so a dependency costs LDS work rather than a chain of
-// dependent global loads. The update U' D U is applied over the nonzero rows
+// lane-parallel up front. In k_pivots a block whose nonzero entries share
+// one column (one entry, on sparse KKT patterns) has its whole update -- one
+// diagonal element -- formed by its own lane, so the serial walk over the
+// block column costs a few scalar instructions per block; other blocks are
+// staged as tiles. The update U' D U is applied over the nonzero rows
 // of the right factor only, with D applied on the fly: every term the
 // reference adds is added, in the same order with the same fused
 // multiply-adds as the CPU oracle's restatement; the skipped terms are exact
@@ -369,6 +371,102 @@ __device__ void product_regs(VT (&T)[16], const VT* A, const uint32_t* mask, con
     }
 }
 
+// the tile element T[c][c] held by this lane (rows cq + 2t of column j)
+template<typename VT>
+__device__ __forceinline__ VT diag_get(const VT (&T)[16], bool own, int t0)
+{
+    VT v = (VT) 0;
+#pragma unroll
+    for(int t = 0; t < 16; ++t)
+        if(own && t == t0) v = T[t];
+    return v;
+}
+
+template<typename VT>
+__device__ __forceinline__ void diag_put(VT (&T)[16], bool own, int t0, VT v)
+{
+#pragma unroll
+    for(int t = 0; t < 16; ++t)
+        if(own && t == t0) T[t] = v;
+}
+
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const unsigned long long b = (unsigned long long) __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) b, src);
+    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (b >> 32), src);
+    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
+
+__device__ __forceinline__ float lane_value(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// nonzero entries of a dependency block taken by the one-column form
+constexpr int COL_E = 4;
+
+// A dependency block of width w whose nonzero entries all lie in one column
+// c updates only T[c][c] in product_regs; this forms that update in one
+// lane with the same operations in the same order (rows ascending, the
+// three-term D product, fma accumulation). Returns false for any other
+// block; c = -1 when the block has no nonzero entry.
+template<typename VT>
+__device__ __forceinline__ bool column_update(const Dep& d, int w, const int32_t* ixs,
+    const VT* vals, const double* tri, int& c_out, VT& v_out)
+{
+    c_out = -1;
+    v_out = (VT) 0;
+    if(d.fmt != FMT_SPARSE || d.nnz > COL_E) return false;
+    constexpr int NONE = 1 << 20;
+    int r[COL_E];
+    VT v[COL_E];
+    int col = -1;
+    bool one_col = true;
+#pragma unroll
+    for(int e = 0; e < COL_E; ++e) {
+        r[e] = NONE;
+        v[e] = (VT) 0;
+        if(e < d.nnz) {
+            const int ix = ixs[(int64_t) d.cp + e];
+            const VT x = vals[(int64_t) d.vp + e];
+            if(x != (VT) 0) {
+                const int rr = ix / w, cc = ix - rr * w;
+                one_col = one_col && (col < 0 || cc == col);
+                col = cc;
+                r[e] = rr;
+                v[e] = x;
+            }
+        }
+    }
+    if(!one_col) return false;
+    if(col < 0) return true;
+    // rows ascending (sorting network on four)
+    auto cswap = [&](int p, int q) {
+        if(r[q] < r[p]) {
+            const int tr = r[p]; r[p] = r[q]; r[q] = tr;
+            const VT tv = v[p]; v[p] = v[q]; v[q] = tv;
+        }
+    };
+    cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+    VT acc = (VT) 0;
+#pragma unroll
+    for(int k = 0; k < COL_E; ++k) {
+        if(r[k] != NONE) {
+            const int l = r[k];
+            const double* t = tri + 3 * (int64_t) (d.rb + l);
+            const VT sup = (VT) t[0], dia = (VT) t[1], sub = (VT) t[2];
+            const VT left = (k > 0 && l > 0 && r[k - 1] == l - 1) ? v[k - 1] : (VT) 0;
+            const VT right = (k + 1 < COL_E && l < 31 && r[k + 1] == l + 1) ? v[k + 1] : (VT) 0;
+            const VT lt = fma(sub, right, fma(dia, v[k], sup * left));
+            acc = fma(v[k], lt, acc);
+        }
+    }
+    c_out = col;
+    v_out = acc;
+    return true;
+}
+
 template<typename VT>
 struct alignas(16) OffLds {
     VT T[1024];
@@ -410,51 +508,69 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
         for(int u = 0; u < PIV_ROUND; ++u) s.mask[u][lane] = 0;
     }
     wave_sync();
-    // updates from the blocks (k, i) above the diagonal, in CSC order:
-    // PIV_ROUND blocks staged per round, their products subtracted in order
+    // updates from the blocks (k, i) above the diagonal, in CSC order, 64
+    // blocks per chunk. A block whose nonzero entries lie in one column c
+    // (nearly all of them on sparse KKT patterns) changes only T[c][c]; its
+    // lane forms that value and the walk subtracts it from the diagonal
+    // register of the owning lane. Runs of other blocks are staged as tiles,
+    // PIV_ROUND per round, and go through product_regs.
+    const bool owns_diag = ((j & 1) == cq) && j < len;
+    const int tdiag = (j - cq) >> 1;
     for(int c0 = 0; c0 < nd; c0 += 64) {
         Dep mine{};
-        if(c0 + lane < nd) mine = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + lane]);
         const int cnt = min(64, nd - c0);
-        Dep cur[PIV_ROUND];
-        Pre<VT> pc[PIV_ROUND];
-#pragma unroll
-        for(int u = 0; u < PIV_ROUND; ++u) {
-            cur[u] = bcast(mine, u < cnt ? u : 0);
-            if(u < cnt) pc[u] = prefetch(cur[u], a.ix_out, a.val_out, a.tri, true);
-        }
-        for(int d0 = 0; d0 < cnt; d0 += PIV_ROUND) {
-            const int nr = min(PIV_ROUND, cnt - d0);
-            Dep nxt[PIV_ROUND];
-            Pre<VT> pn[PIV_ROUND];
-#pragma unroll
-            for(int u = 0; u < PIV_ROUND; ++u) {
-                const int dn = d0 + PIV_ROUND + u;
-                nxt[u] = bcast(mine, dn < cnt ? dn : 0);
-                if(dn < cnt) pn[u] = prefetch(nxt[u], a.ix_out, a.val_out, a.tri, true);
-            }
-#pragma unroll
-            for(int u = 0; u < PIV_ROUND; ++u) {
-                if(u < nr) {
-                    stage(s.X[u], s.mask[u], cur[u], pc[u], len, a.ix_out, a.val_out);
-                    stage_d(s.cs[u], s.cd[u], s.cb[u], pc[u]);
+        if(lane < cnt) mine = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + lane]);
+        int uc = -1;
+        VT uv = (VT) 0;
+        const bool simple = lane < cnt && column_update(mine, len, a.ix_out, a.val_out, a.tri,
+            uc, uv);
+        const uint64_t smask = wave_ballot(simple);
+        VT dg = diag_get(T, owns_diag, tdiag);
+        int d = 0;
+        while(d < cnt) {
+            if((smask >> d) & 1ull) {
+                const int c = __builtin_amdgcn_readlane(uc, d);
+                if(c >= 0) {
+                    const VT v = lane_value(uv, d);
+                    if(owns_diag && j == c) dg -= v;
                 }
+                ++d;
+                continue;
             }
-            wave_sync();
+            // a run of staged blocks [d, d1)
+            const uint64_t later = smask >> d;
+            const int d1 = later ? min(cnt, d + __ffsll((long long) later) - 1) : cnt;
+            diag_put(T, owns_diag, tdiag, dg);
+            for(int r0 = d; r0 < d1; r0 += PIV_ROUND) {
+                const int nr = min(PIV_ROUND, d1 - r0);
+                Dep cur[PIV_ROUND];
+                Pre<VT> pc[PIV_ROUND];
 #pragma unroll
-            for(int u = 0; u < PIV_ROUND; ++u)
-                if(u < nr) product_regs(T, s.X[u], s.mask[u], s.cs[u], s.cd[u], s.cb[u], len);
-            wave_sync();
+                for(int u = 0; u < PIV_ROUND; ++u) {
+                    cur[u] = bcast(mine, u < nr ? r0 + u : r0);
+                    if(u < nr) pc[u] = prefetch(cur[u], a.ix_out, a.val_out, a.tri, true);
+                }
 #pragma unroll
-            for(int u = 0; u < PIV_ROUND; ++u)
-                if(u < nr) unstage(s.X[u], s.mask[u], cur[u], pc[u], len);
-            wave_sync();
+                for(int u = 0; u < PIV_ROUND; ++u) {
+                    if(u < nr) {
+                        stage(s.X[u], s.mask[u], cur[u], pc[u], len, a.ix_out, a.val_out);
+                        stage_d(s.cs[u], s.cd[u], s.cb[u], pc[u]);
+                    }
+                }
+                wave_sync();
 #pragma unroll
-            for(int u = 0; u < PIV_ROUND; ++u) {
-                cur[u] = nxt[u];
-                pc[u] = pn[u];
+                for(int u = 0; u < PIV_ROUND; ++u)
+                    if(u < nr) product_regs(T, s.X[u], s.mask[u], s.cs[u], s.cd[u], s.cb[u], len);
+                wave_sync();
+#pragma unroll
+                for(int u = 0; u < PIV_ROUND; ++u)
+                    if(u < nr) unstage(s.X[u], s.mask[u], cur[u], pc[u], len);
+                wave_sync();
             }
+            dg = diag_get(T, owns_diag, tdiag);
+            d = d1;
         }
+        diag_put(T, owns_diag, tdiag, dg);
     }
     // the updated tile back to LDS (tile 0 is zero again after unstaging)
 #pragma unroll

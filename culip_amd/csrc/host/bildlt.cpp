@@ -3,6 +3,7 @@
 #include "objects.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 
 namespace culip {
@@ -470,6 +471,108 @@ void Bildlt::download_factor()
     std::vector<double> tri(3 * (size_t) n());
     HIP_CHECK(hipMemcpy(tri.data(), d_tri.p, sizeof(double) * tri.size(), hipMemcpyDeviceToHost));
     host_real["tri"] = std::move(tri);
+}
+
+// export_factorized (block_ildlt.impl.cuh:1700-1916): U as fine CSR (the
+// nonzero entries of every factored block, columns sorted per row), the
+// tridiagonal D and the permutation (identity for static pivots), each
+// written as a MatrixMarket coordinate file (test.impl.h:480-545).
+static void write_mm(const std::string& path, int32_t m, const std::vector<int32_t>& rp,
+    const std::vector<int32_t>& ci, const std::vector<double>& cv)
+{
+    FILE* f = std::fopen(path.c_str(), "w");
+    if(!f) fail(CULIP_ERR_INVALID, "export: cannot write " + path);
+    std::fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n");
+    std::fprintf(f, "%d %d %d\n", m, m, rp[m]);
+    for(int32_t i = 0; i < m; ++i)
+        for(int32_t q = rp[i]; q < rp[i + 1]; ++q)
+            std::fprintf(f, "%d %d %.12g\n", i + 1, ci[q] + 1, cv[q]);
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    if(!ok) fail(CULIP_ERR_INVALID, "export: write error on " + path);
+}
+
+double Bildlt::export_factorized(const std::string& folder)
+{
+    download_factor();
+    const int32_t N = n();
+    const std::vector<int32_t>& fb = host_int["fblocks"];
+    const std::vector<int32_t>& fix = host_int["fix_store"];
+    const std::vector<double>& fv = host_real["fval_store"];
+    const std::vector<double>& tri = host_real["tri"];
+    const std::vector<int32_t>& bs = block_starts;
+    const int32_t K = (int32_t) (fb.size() / 9);
+    // U: count, offsets, fill, sort
+    std::vector<int32_t> rp(N + 1, 0);
+    auto each = [&](auto&& visit) {
+        for(int32_t b = 0; b < K; ++b) {
+            const BlockMeta& m = *reinterpret_cast<const BlockMeta*>(&fb[9 * (size_t) b]);
+            const int32_t fi = bs[m.block_row], fj = bs[m.block_col];
+            const int32_t h = bs[m.block_row + 1] - fi, w = bs[m.block_col + 1] - fj;
+            if(m.format == FMT_DENSE) {
+                for(int32_t r = 0; r < h; ++r)
+                    for(int32_t c = 0; c < w; ++c) {
+                        const double v = fv[(size_t) m.val_ptr + r * w + c];
+                        if(v != 0.0) visit(fi + r, fj + c, v);
+                    }
+            } else {
+                for(int32_t e = 0; e < m.nnz; ++e) {
+                    const int32_t ix = fix[(size_t) m.col_ptr + e];
+                    const double v = fv[(size_t) m.val_ptr + e];
+                    if(v != 0.0) visit(fi + ix / w, fj + ix % w, v);
+                }
+            }
+        }
+    };
+    each([&](int32_t r, int32_t, double) { ++rp[r + 1]; });
+    for(int32_t i = 0; i < N; ++i) rp[i + 1] += rp[i];
+    std::vector<int32_t> ci(rp[N]), next(rp.begin(), rp.end() - 1);
+    std::vector<double> cv(rp[N]);
+    each([&](int32_t r, int32_t c, double v) {
+        const int32_t q = next[r]++;
+        ci[q] = c;
+        cv[q] = v;
+    });
+    std::vector<std::pair<int32_t, double>> row;
+    for(int32_t i = 0; i < N; ++i) {
+        row.clear();
+        for(int32_t q = rp[i]; q < rp[i + 1]; ++q) row.emplace_back(ci[q], cv[q]);
+        std::sort(row.begin(), row.end(),
+            [](const std::pair<int32_t, double>& x, const std::pair<int32_t, double>& y) {
+                return x.first < y.first;
+            });
+        for(size_t k = 0; k < row.size(); ++k) {
+            ci[rp[i] + k] = row[k].first;
+            cv[rp[i] + k] = row[k].second;
+        }
+    }
+    write_mm(folder + "/out_U.mtx", N, rp, ci, cv);
+    const int64_t nnz_u = rp[N];
+    // D: per row its sub-diagonal neighbour above, diagonal, neighbour below,
+    // the off-diagonal ones only when nonzero; rows in order, columns sorted
+    std::vector<int32_t> drp(N + 1, 0), dci;
+    std::vector<double> dcv;
+    for(int32_t i = 0; i < N; ++i) {
+        // row i holds D(i, i-1) = tri[3 (i-1) + 2], D(i, i), D(i, i+1) = tri[3 (i+1)]
+        if(i > 0 && tri[3 * (size_t) (i - 1) + 2] != 0.0) {
+            dci.push_back(i - 1);
+            dcv.push_back(tri[3 * (size_t) (i - 1) + 2]);
+        }
+        dci.push_back(i);
+        dcv.push_back(tri[3 * (size_t) i + 1]);
+        if(i < N - 1 && tri[3 * (size_t) (i + 1)] != 0.0) {
+            dci.push_back(i + 1);
+            dcv.push_back(tri[3 * (size_t) (i + 1)]);
+        }
+        drp[i + 1] = (int32_t) dci.size();
+    }
+    write_mm(folder + "/out_D.mtx", N, drp, dci, dcv);
+    // P: the identity of static pivoting
+    std::vector<int32_t> prp(N + 1), pci(N);
+    for(int32_t i = 0; i <= N; ++i) prp[i] = i;
+    for(int32_t i = 0; i < N; ++i) pci[i] = i;
+    write_mm(folder + "/out_P.mtx", N, prp, pci, std::vector<double>(N, 1.0));
+    return (2.0 * (double) nnz_u + (double) dci.size()) / (double) A.nnz();
 }
 
 // ---------------------------------------------------------------------------

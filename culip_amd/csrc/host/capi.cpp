@@ -336,6 +336,16 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
 
 int32_t culip_bildlt_n(const culip_bildlt* h) { return h ? h->b.n() : -1; }
 
+culip_status culip_bildlt_export_factorized(culip_bildlt* h, const char* folder,
+    double* fill_factor)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && folder, "bildlt: null argument");
+        const double f = h->b.export_factorized(folder);
+        if(fill_factor) *fill_factor = f;
+    });
+}
+
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algorithm algorithm,
     int32_t num_jacobi_sweeps)
 {

@@ -159,6 +159,9 @@ struct Bildlt final : Precond {
     SolveTopo topo() const;
     int64_t device_bytes() const;
     void download_factor();
+    // out_U.mtx, out_D.mtx, out_P.mtx under folder; returns the achieved
+    // fill factor (2 nnz(U) + nnz(D)) / nnz(A)
+    double export_factorized(const std::string& folder);
 
     template<typename VT> FactorArgs<VT> factor_args() const;
     template<typename VT> SolvePlan<VT> plan() const;

@@ -169,6 +169,16 @@ typedef enum {
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
     culip_solve_algorithm algorithm, int32_t num_jacobi_sweeps);
 
+/* export_factorized(folder) (block_ildlt.impl.cuh:1700-1916): writes
+ * folder/out_U.mtx (the nonzero entries of every factored block as one fine
+ * CSR matrix), folder/out_D.mtx (the tridiagonal D) and folder/out_P.mtx
+ * (the identity of static pivoting) in the MatrixMarket form of
+ * test.impl.h:480-545 ("%d %d %.12g" lines, 1-based). fill_factor (may be
+ * NULL) receives (2 nnz(U) + nnz(D)) / nnz(A), the figure the reference
+ * prints. Needs a computed factor. */
+culip_status culip_bildlt_export_factorized(culip_bildlt * h, const char * folder,
+    double * fill_factor);
+
 /* Host views of the symbolic structures and (after compute) of the factor
  * downloaded from the device; valid until the next call on h.
  * int names: block_starts, is_piv, coarse_csr_off, coarse_csr_col,

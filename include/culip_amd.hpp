@@ -8,6 +8,7 @@
  *   CSRMatrixSPMV     libs/la/spmv.cuh:36-66
  *   AugmentedMatrixSPMV libs/la/spmv.cuh:116-149
  *   SQMR              libs/la/iterative_solvers/sqmr.cuh:21-72
+ *   ImportLDUPreconditioner libs/la/preconditioner/import_ldu_preconditioner.cuh:36-125
  *
  * Differences: matrices are culip_csr views of host arrays (the reference's
  * csr_matrix_t on the host), vectors are device arrays of doubles whatever T
@@ -76,6 +77,14 @@ public:
     /* solve_analysis() is part of compute(); kept so callers compile. */
     void solve_analysis() {}
 
+    /* block_ildlt.impl.cuh:1700-1916; returns the achieved fill factor. */
+    double export_factorized(const char* folder)
+    {
+        double f = 0.0;
+        check(culip_bildlt_export_factorized(h_, folder, &f));
+        return f;
+    }
+
     /* block_ildlt.impl.cuh:253-260. STRIPE, BLOCK and SCALAR all apply
      * the exact U^-1 (resp. U^-T); here they share the level-scheduled
      * solve. JACOBI runs num_jacobi_sweeps block-Jacobi sweeps from x = 0. */
@@ -110,6 +119,72 @@ public:
 private:
     void* stream_;
     culip_bildlt* h_ = nullptr;
+};
+
+/* ImportLDUPreconditioner<T> (import_ldu_preconditioner.cuh:36-125). The
+ * three ctors: (L, U), (L) and (L, D, D_is_piv); D is the tridiagonal
+ * vector (3n) and its pivot-start flags, host arrays. */
+template<typename T>
+class ImportLDUPreconditioner {
+    static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+        "ImportLDUPreconditioner<T>: T is double or float");
+
+public:
+    ImportLDUPreconditioner(void* stream, const culip_csr* L, const culip_csr* U,
+        bool exact_solve = true)
+        : stream_(stream)
+    {
+        check(culip_ldu_create(L, U, nullptr, nullptr, exact_solve ? 1 : 0, prec(), &h_));
+    }
+    ImportLDUPreconditioner(void* stream, const culip_csr* L, bool exact_solve = true)
+        : stream_(stream)
+    {
+        check(culip_ldu_create(L, nullptr, nullptr, nullptr, exact_solve ? 1 : 0, prec(), &h_));
+    }
+    ImportLDUPreconditioner(void* stream, const culip_csr* L, const double* D,
+        const int32_t* D_is_piv, bool exact_solve = true)
+        : stream_(stream)
+    {
+        check(culip_ldu_create(L, nullptr, D, D_is_piv, exact_solve ? 1 : 0, prec(), &h_));
+    }
+    ~ImportLDUPreconditioner() { culip_ldu_destroy(h_); }
+    ImportLDUPreconditioner(const ImportLDUPreconditioner&) = delete;
+    ImportLDUPreconditioner& operator=(const ImportLDUPreconditioner&) = delete;
+
+    /* push convention: entry i moves to P[i] (host arrays) */
+    void set_permutation(const int32_t* P_r, const int32_t* P_c)
+    {
+        check(culip_ldu_set_permutation(h_, P_r, P_c));
+    }
+    void set_scaling(const double* S_r, const double* S_c)
+    {
+        check(culip_ldu_set_scaling(h_, S_r, S_c));
+    }
+
+    int32_t n() const { return culip_ldu_n(h_); }
+    bool is_left() const { return true; }
+    bool is_middle() const { return culip_ldu_is_middle(h_) != 0; }
+    bool is_right() const { return true; }
+
+    void solve_left(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_ldu_solve(h_, CULIP_SOLVE_LEFT, transpose, b, x, stream_));
+    }
+    void solve_middle(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_ldu_solve(h_, CULIP_SOLVE_MIDDLE, transpose, b, x, stream_));
+    }
+    void solve_right(const double* b, double* x, bool transpose = false) const
+    {
+        check(culip_ldu_solve(h_, CULIP_SOLVE_RIGHT, transpose, b, x, stream_));
+    }
+
+    const culip_ldu* handle() const { return h_; }
+
+private:
+    static culip_prec prec() { return std::is_same<T, float>::value ? CULIP_PREC_F32 : CULIP_PREC_F64; }
+    void* stream_;
+    culip_ldu* h_ = nullptr;
 };
 
 /* SPMV<T> (spmv.cuh:18-31): y = alpha op(A) x + beta y on device doubles. */
@@ -165,6 +240,16 @@ public:
     {
         culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts};
         check(culip_sqmr_create(spmv->handle(), prec ? prec->handle() : nullptr, &o, &s_));
+    }
+    template<typename T>
+    SQMR(void* stream, const SPMV* spmv, double tol, int32_t max_it,
+        const ImportLDUPreconditioner<T>* prec, int32_t true_residual_period = 0,
+        int32_t max_restarts = 0)
+        : stream_(stream)
+    {
+        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts};
+        if(prec) check(culip_sqmr_create_ldu(spmv->handle(), prec->handle(), &o, &s_));
+        else check(culip_sqmr_create(spmv->handle(), nullptr, &o, &s_));
     }
     ~SQMR() { culip_sqmr_destroy(s_); }
     SQMR(const SQMR&) = delete;

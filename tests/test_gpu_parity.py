@@ -306,3 +306,56 @@ def test_jacobi_solves(hip_device):
         x1 = Dm @ b
         x2 = Dm @ (b - off @ x1)
         np.testing.assert_allclose(out[("jacobi", 2, side)], x2, rtol=1e-10, atol=1e-12)
+
+
+def _read_mm_coo(path):
+    with open(path) as f:
+        assert f.readline().startswith("%%MatrixMarket matrix coordinate real general")
+        m, n, nnz = map(int, f.readline().split())
+        rows = [l.split() for l in f if l.strip()]
+    assert len(rows) == nnz
+    i = np.array([int(r[0]) - 1 for r in rows])
+    j = np.array([int(r[1]) - 1 for r in rows])
+    v = np.array([float(r[2]) for r in rows])
+    return m, n, i, j, v
+
+
+@pytest.mark.parametrize("case", ["kkt", "single_block"])
+def test_export_factorized(hip_device, tmp_path, case):
+    """export_factorized (block_ildlt.impl.cuh:1700-1916): out_U holds every
+    nonzero entry of the factored blocks (row-major, columns sorted), out_D
+    the tridiagonal D, out_P the identity; values as "%.12g". On a single
+    dense block with no dropping the factor is a complete LDL', so
+    U' D U reproduces A."""
+    if case == "kkt":
+        k, p = _kkt(2000, 800, 16, ca.ORDER_BLOCK_COLOR)
+        csr, piv, bs = (p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts
+        thr = 1e-4
+    else:
+        csr, piv, bs, _ = sym_problem(n_blocks=1, bmax=32, density=1.0, seed=5)
+        thr = 0.0
+    g, o = _pair(csr, piv, bs, "d", thr=thr)
+    n = csr[0].size - 1
+    fill = g.export_factorized(str(tmp_path))
+    # U against the oracle's factor (bitwise equal to the device's)
+    U = _dense_U(o, n) if case == "kkt" else None
+    m, nn, i, j, v = _read_mm_coo(tmp_path / "out_U.mtx")
+    assert m == nn == n
+    order = np.lexsort((j, i))
+    assert np.array_equal(order, np.arange(i.size)), "rows, then columns ascending"
+    Ue = np.zeros((n, n))
+    Ue[i, j] = v
+    if U is not None:
+        assert (U != 0).sum() == i.size
+        np.testing.assert_allclose(Ue, U, rtol=1e-11, atol=0)
+    m, nn, di, dj, dv = _read_mm_coo(tmp_path / "out_D.mtx")
+    D = np.zeros((n, n))
+    D[di, dj] = dv
+    np.testing.assert_allclose(D, _dense_D(o, n), rtol=1e-11, atol=0)
+    m, nn, pi, pj, pv = _read_mm_coo(tmp_path / "out_P.mtx")
+    assert np.array_equal(pi, np.arange(n)) and np.array_equal(pj, np.arange(n))
+    assert (pv == 1.0).all()
+    assert fill == pytest.approx((2 * i.size + di.size) / csr[1].size, rel=1e-15)
+    if case == "single_block":
+        A = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n)).toarray()
+        np.testing.assert_allclose(Ue.T @ D @ Ue, A, rtol=0, atol=1e-9 * np.abs(A).max())
