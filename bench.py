@@ -233,8 +233,9 @@ def bench_serve(a):
 def bench_batch(a):
     """512 independent systems (n ~ U[1k, 40k], m = 0.4 n, seeds 0..511)
     solved as one batch on `slots` concurrent streams with the block size
-    picked per size bucket by timing {8, 16, 32}; beside it the plain FP64
-    path, one system at a time, on the same systems."""
+    ({8, 16, 32}) and the triangular-solve schedule picked per size bucket by
+    timing; beside it the plain FP64 path (level-scheduled solves), one
+    system at a time, on the same systems."""
     import concurrent.futures as cf
     import culip_amd as ca
     rng = np.random.default_rng(a.seed)
@@ -245,10 +246,13 @@ def bench_batch(a):
     def bucket(n):
         return next(i for i, (lo, hi) in enumerate(buckets) if lo <= n < hi)
 
-    # autotuner: per bucket, the block size with the shortest refactor +
-    # solve on the bucket's first system (FP32 factor)
+    # autotuner: per bucket, the block size and triangular-solve schedule
+    # (level launches, or block-Jacobi with 2 or 3 sweeps) with the shortest
+    # refactor + solve to the true-residual tolerance on the bucket's first
+    # system (FP32 factor)
     import torch
     dev = torch.device("cuda:0")
+    scheds = [("levels", 0), ("jacobi", 2), ("jacobi", 3)]
     tuned, tune_ms = {}, {}
     for bi in range(len(buckets)):
         idx = [i for i in range(a.systems) if bucket(sizes[i]) == bi]
@@ -261,40 +265,49 @@ def bench_batch(a):
             g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, a.prec)
             g.compute(0, 1.0, 1e-4)
             A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
-            q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
             b = torch.tensor(k.rhs[p.perm], device=dev)
-            x = torch.zeros_like(b)
             dv = torch.tensor(p.val, device=dev)
-            for _ in range(2):
-                g.refactor(dv, 1e-4)
-                q.solve(b, x)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(3):
-                g.refactor(dv, 1e-4)
-                q.solve(b, x)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) / 3 * 1e3
-            tune_ms[f"{buckets[bi][0]}-{buckets[bi][1] - 1}:B{B}"] = ms
-            if best is None or ms < best[0]:
-                best = (ms, B)
-        tuned[bi] = best[1]
-        log(f"autotune bucket {buckets[bi]}: block size {best[1]}")
+            for algo, sw in scheds:
+                g.set_solve_algorithm(algo, sw)
+                q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
+                x = torch.zeros_like(b)
+                for _ in range(2):
+                    g.refactor(dv, 1e-4)
+                    q.solve(b, x)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    g.refactor(dv, 1e-4)
+                    q.solve(b, x)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) / 3 * 1e3
+                if q.result["true_rel_residual"] > 1e-8:
+                    ms = float("inf")
+                name = algo if algo == "levels" else f"{algo}{sw}"
+                tune_ms[f"{buckets[bi][0]}-{buckets[bi][1] - 1}:B{B}:{name}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, B, algo, sw)
+        tuned[bi] = best[1:]
+        log(f"autotune bucket {buckets[bi]}: block size {best[1]}, {best[2]} {best[3]}")
 
     def make(i):
-        k, p = _pattern(ca, int(sizes[i]), i, tuned[bucket(sizes[i])], order)
+        k, p = _pattern(ca, int(sizes[i]), i, tuned[bucket(sizes[i])][0], order)
         return p, k.rhs[p.perm].copy()
 
     with cf.ThreadPoolExecutor(16) as ex:
         systems = list(ex.map(make, range(a.systems)))
     log(f"{len(systems)} systems generated")
 
-    def solve_all(slots, prec):
+    def solve_all(slots, prec, tuned_schedule):
         srv = ca.Server(slots=slots, prec=prec)
         with cf.ThreadPoolExecutor(16) as ex:
             pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,
                                                          s[0].piv_starts, s[0].block_starts),
                                systems))
+        if tuned_schedule:
+            for i, pid in enumerate(pids):
+                _, algo, sw = tuned[bucket(sizes[i])]
+                srv.set_pattern_solve(pid, algo, sw)
         # first pass creates the per-slot instances; the second is timed
         for rep in range(2):
             t0 = time.perf_counter()
@@ -305,8 +318,10 @@ def bench_batch(a):
         srv.close()
         return out, el
 
-    res, el = solve_all(a.slots, a.prec)
-    base, el_base = solve_all(1, "f64")
+    res, el = solve_all(a.slots, a.prec, True)
+    # the plain path: FP64 factor, level-scheduled solves (the reference
+    # driver's SOLVE_BLOCK), one system at a time
+    base, el_base = solve_all(1, "f64", False)
     trr = max(r[1]["true_rel_residual"] for r in res)
     lat = np.array([r[2]["latency"] for r in res]) * 1e3
     return {
@@ -317,8 +332,12 @@ def bench_batch(a):
         "dtype": "f64" if a.prec == "f64" else "f32-factor/f64-sqmr", "data": "synthetic",
         "config": {"workload": "batch", "systems": a.systems, "n_range": [1000, 40000],
                    "m": "0.4n", "seeds": f"0..{a.systems - 1}", "slots": a.slots,
-                   "max_block_by_bucket": {f"{buckets[b][0]}-{buckets[b][1] - 1}": tuned[b]
+                   "max_block_by_bucket": {f"{buckets[b][0]}-{buckets[b][1] - 1}": tuned[b][0]
                                            for b in tuned},
+                   "solve_schedule_by_bucket": {
+                       f"{buckets[b][0]}-{buckets[b][1] - 1}":
+                           tuned[b][1] if tuned[b][1] == "levels" else f"{tuned[b][1]}{tuned[b][2]}"
+                       for b in tuned},
                    "tol": 1e-8, "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_true_rel_residual": trr,

@@ -2,7 +2,7 @@
 
 Python mirror of the reference's C++ interfaces (dthuerck/culip):
   BlockiLDLt   libs/la/preconditioner/block_ildlt.cuh:188-311
-  CSRMatrixSPMV, AugmentedMatrixSPMV   libs/la/spmv.cuh:36-149
+  CSRMatrixSPMV, NormalMatrixSPMV, AugmentedMatrixSPMV   libs/la/spmv.cuh:36-149
   SQMR         libs/la/iterative_solvers/sqmr.cuh:21-72
   ImportLDUPreconditioner   libs/la/preconditioner/import_ldu_preconditioner.cuh:36-125
   read_matrix_csr / read_dense_vector  libs/test/test.h
@@ -20,7 +20,7 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
                    check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
-           "SQMR", "ImportLDUPreconditioner", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
+           "SQMR", "ImportLDUPreconditioner", "NormalMatrixSPMV", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
            "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR"]
 
 
@@ -34,6 +34,11 @@ def _f64(a):
 
 def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _solve_algorithm(a):
+    """'levels' | 'flagged' | 'jacobi' (or the CULIP_TRSV_* value)."""
+    return {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED, "jacobi": TRSV_JACOBI}.get(a, a)
 
 
 def csr_struct(row_ptr, col, val, n=None):
@@ -175,9 +180,8 @@ class BlockiLDLt:
         """set_solve_algorithm: 'levels' (one launch per level), 'flagged'
         (one launch per solve) or 'jacobi' (num_jacobi_sweeps block-Jacobi
         sweeps)."""
-        a = {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED,
-             "jacobi": TRSV_JACOBI}.get(algorithm, algorithm)
-        check(lib().culip_bildlt_set_solve_algorithm(self._h, a, num_jacobi_sweeps))
+        check(lib().culip_bildlt_set_solve_algorithm(self._h, _solve_algorithm(algorithm),
+                                                     num_jacobi_sweeps))
 
     def export_factorized(self, folder):
         """Writes out_U.mtx, out_D.mtx, out_P.mtx under folder; returns the
@@ -261,6 +265,21 @@ class AugmentedMatrixSPMV(_Spmv):
         check(lib().culip_spmv_create_kkt(C.byref(s), _ptr(dd, C.c_double), delta, len(ixa),
                                           _ptr(ixa, C.c_int32) if len(ixa) else None,
                                           C.byref(h)))
+        self._h = h
+
+
+class NormalMatrixSPMV(_Spmv):
+    """N = A diag(d)^-1 A' + delta*I(ix) (spmv.cuh:74-107), dimension m."""
+
+    def __init__(self, a_row_ptr, a_col, a_val, a_ncols, d, delta=0.0, ix=None):
+        s, self._keep = csr_struct(a_row_ptr, a_col, a_val, a_ncols)
+        dd = _f64(d)
+        h = C.c_void_p()
+        ixa = _i32(ix if ix is not None else [])
+        check(lib().culip_spmv_create_normal(C.byref(s), _ptr(dd, C.c_double), delta,
+                                             len(ixa),
+                                             _ptr(ixa, C.c_int32) if len(ixa) else None,
+                                             C.byref(h)))
         self._h = h
 
 
@@ -373,10 +392,11 @@ class Server:
 
     def __init__(self, slots=4, prec="f32", hbm_budget=0, fill_level=0, fill_factor=1.0,
                  threshold=1e-4, tol=1e-8, max_it=1000, true_residual_period=20,
-                 max_restarts=10):
+                 max_restarts=10, solve_algorithm="levels", jacobi_sweeps=0):
         pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
         o = _lib.ServerOpts(slots, int(hbm_budget), pr, fill_level, fill_factor, threshold,
-                            _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts))
+                            _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts),
+                            _solve_algorithm(solve_algorithm), jacobi_sweeps)
         h = C.c_void_p()
         check(lib().culip_server_create(C.byref(o), C.byref(h)))
         self._h = h
@@ -391,6 +411,11 @@ class Server:
                                              len(bs), _ptr(bs, C.c_int32), C.byref(p)))
         self._n[p.value] = len(row_ptr) - 1
         return p.value
+
+    def set_pattern_solve(self, pattern, algorithm, jacobi_sweeps=0):
+        """Triangular-solve schedule of one pattern, before its first submit."""
+        check(lib().culip_server_set_pattern_solve(self._h, pattern,
+                                                   _solve_algorithm(algorithm), jacobi_sweeps))
 
     def submit(self, pattern, vals, rhs):
         v, b = _f64(vals), _f64(rhs)

@@ -53,7 +53,8 @@ class SqmrResult(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("prec", C.c_int),
                 ("fill_level", C.c_int32), ("fill_factor", C.c_double),
-                ("threshold", C.c_double), ("sqmr", SqmrOpts)]
+                ("threshold", C.c_double), ("sqmr", SqmrOpts),
+                ("solve_algorithm", C.c_int), ("jacobi_sweeps", C.c_int32)]
 
 
 _lib = None
@@ -103,6 +104,9 @@ _SIGS = [
     ("culip_spmv_create_csr", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_void_p)]),
     ("culip_spmv_create_kkt", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_double), C.c_double,
                                         C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]),
+    ("culip_spmv_create_normal", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_double), C.c_double,
+                                           C.c_int32, C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_void_p)]),
     ("culip_spmv_update_values", C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     ("culip_spmv_multiply", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                       C.c_double, C.c_int32, C.c_void_p]),
@@ -132,6 +136,7 @@ _SIGS = [
     ("culip_server_add_pattern", C.c_int, [C.c_void_p, C.POINTER(Csr), C.c_int32,
                                            C.POINTER(C.c_int32), C.c_int32,
                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("culip_server_set_pattern_solve", C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.c_int32]),
     ("culip_server_submit", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("culip_server_wait", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_double),

@@ -1,5 +1,6 @@
-// CSR SpMV (cuSPARSE csrmv semantics, sparse_la_instances.cu:130-232) and
-// the matrix-free KKT operator (AugmentedMatrixSPMV, spmv.impl.cuh:362-427).
+// CSR SpMV (cuSPARSE csrmv semantics, sparse_la_instances.cu:130-232), the
+// matrix-free KKT operator (AugmentedMatrixSPMV, spmv.impl.cuh:362-427) and
+// the normal-equations operator (NormalMatrixSPMV, spmv.impl.cuh:122-265).
 // y = alpha * A x + beta * y. Eight lanes per row, coalesced entry loads,
 // fixed-order reduction (xor tree within the eight lanes).
 #include "spmv.hpp"
@@ -74,7 +75,52 @@ __global__ __launch_bounds__(256) void k_kkt_dual(int32_t n, int32_t m, const do
     }
 }
 
+// normal equations: t = (A' x) ./ d, then y = alpha (A t + dix .* x) + beta y
+__global__ __launch_bounds__(256) void k_normal_inner(int32_t n, const int32_t* atp,
+    const int32_t* ati, const double* atv, const double* d, const double* x, double* t)
+{
+    const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row = tid / LPR;
+    double acc = 0.0;
+    if(row < n)
+        for(int32_t q = atp[row] + sub; q < atp[row + 1]; q += LPR) acc = fma(atv[q], x[ati[q]], acc);
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if(row < n && sub == 0) t[row] = acc / d[row];
+}
+
+__global__ __launch_bounds__(256) void k_normal_outer(int32_t m, const int32_t* ap,
+    const int32_t* ai, const double* av, const double* dix, const double* t, const double* x,
+    double* y, double alpha, double beta)
+{
+    const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row = tid / LPR;
+    double acc = 0.0;
+    if(row < m)
+        for(int32_t q = ap[row] + sub; q < ap[row + 1]; q += LPR) acc = fma(av[q], t[ai[q]], acc);
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    if(row < m && sub == 0) {
+        const double b = fma(dix[row], x[row], acc);
+        y[row] = (beta == 0.0) ? alpha * b : fma(beta, y[row], alpha * b);
+    }
+}
+
 }  // namespace
+
+void normal_spmv(const DevNormal& N, const double* x, double* y, double alpha, double beta,
+    void* stream)
+{
+    hipStream_t st = (hipStream_t) stream;
+    if(N.n > 0)
+        k_normal_inner<<<(unsigned) (((int64_t) N.n * LPR + 255) / 256), 256, 0, st>>>(N.n,
+            N.At.row_ptr, N.At.col, N.At.val, N.d, x, N.t);
+    if(N.m > 0)
+        k_normal_outer<<<(unsigned) (((int64_t) N.m * LPR + 255) / 256), 256, 0, st>>>(N.m,
+            N.A.row_ptr, N.A.col, N.A.val, N.dix, N.t, x, y, alpha, beta);
+}
 
 void csr_spmv(const DevCsr& A, const double* x, double* y, double alpha, double beta,
     void* stream)

@@ -21,7 +21,19 @@ struct DevKkt {
     double* dix = nullptr;  // m
 };
 
+// N = A diag(d)^-1 A' + diag(dix), m x m (NormalMatrixSPMV,
+// spmv.impl.cuh:122-265); t: n doubles of work.
+struct DevNormal {
+    int32_t m = 0, n = 0;
+    DevCsr A, At;
+    double* d = nullptr;    // n
+    double* dix = nullptr;  // m
+    double* t = nullptr;    // n
+};
+
 void csr_spmv(const DevCsr& A, const double* x, double* y, double alpha, double beta,
+    void* stream);
+void normal_spmv(const DevNormal& N, const double* x, double* y, double alpha, double beta,
     void* stream);
 void kkt_spmv(const DevKkt& K, const double* x, double* y, double alpha, double beta,
     void* stream);

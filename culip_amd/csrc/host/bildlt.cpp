@@ -319,6 +319,18 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
     host_real.erase("row_norms");
 }
 
+void Bildlt::set_solve(culip_solve_algorithm a, int32_t sweeps)
+{
+    if(a != CULIP_TRSV_LEVELS && a != CULIP_TRSV_FLAGGED && a != CULIP_TRSV_JACOBI)
+        fail(CULIP_ERR_INVALID, "bildlt: unknown solve algorithm");
+    if(a == CULIP_TRSV_JACOBI && sweeps < 1)
+        fail(CULIP_ERR_INVALID, "bildlt: block-Jacobi solves need at least one sweep");
+    solve_algo = a;
+    jacobi_sweeps = sweeps;
+    if(a == CULIP_TRSV_JACOBI)
+        for(int q = 0; q < 3; ++q) d_jac[q].ensure(sizeof(double) * (size_t) std::max(1, n()));
+}
+
 void Bildlt::jacobi(double* x, bool transpose, hipStream_t st) const
 {
     const size_t bytes = sizeof(double) * (size_t) std::max(1, n());
@@ -400,7 +412,7 @@ int64_t Bildlt::numeric_bytes() const
             &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_wave_list, &d_wave_cnt,
             &d_store_list, &d_store_cnt,
             &d_flag_u, &d_flag_t,
-            &d_trsv_err})
+            &d_trsv_err, &d_jac[0], &d_jac[1], &d_jac[2]})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
         b += (int64_t) (d_ptr[q].bytes + d_col[q].bytes + d_val[q].bytes + d_lr[q].bytes);
@@ -434,6 +446,7 @@ int64_t Bildlt::numeric_bytes_estimate(bool with_layout) const
         }
         b += (2 * s.val_space + 2 * inv) * (4 + v + 1);
     }
+    if(solve_algo == CULIP_TRSV_JACOBI) b += 3 * 8 * n;   // block-Jacobi vectors
     return b;
 }
 
@@ -500,7 +513,7 @@ double Bildlt::export_factorized(const std::string& folder)
     const std::vector<int32_t>& fix = host_int["fix_store"];
     const std::vector<double>& fv = host_real["fval_store"];
     const std::vector<double>& tri = host_real["tri"];
-    const std::vector<int32_t>& bs = block_starts;
+    const std::vector<int32_t>& bs = sym.block_starts;   // nb + 1, with the sentinel N
     const int32_t K = (int32_t) (fb.size() / 9);
     // U: count, offsets, fill, sort
     std::vector<int32_t> rp(N + 1, 0);
@@ -581,6 +594,8 @@ void Spmv::multiply(const double* x, double* y, double alpha, double beta, bool 
     hipStream_t st) const
 {
     if(kkt) { kkt_spmv(K, x, y, alpha, beta, st); return; }
+    // N is symmetric: the transpose flag is ignored (spmv.impl.cuh:230)
+    if(normal) { normal_spmv(N, x, y, alpha, beta, st); return; }
     if(transpose && have_at) csr_spmv(At, x, y, alpha, beta, st);
     else csr_spmv(A, x, y, alpha, beta, st);
 }

@@ -356,8 +356,8 @@ culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algor
             "bildlt: unknown solve algorithm");
         CULIP_REQUIRE(algorithm != CULIP_TRSV_JACOBI || num_jacobi_sweeps >= 1,
             "bildlt: block-Jacobi solves need at least one sweep");
-        h->b.solve_algo = algorithm;
-        h->b.jacobi_sweeps = num_jacobi_sweeps;
+        if(algorithm == CULIP_TRSV_JACOBI) device_present();
+        h->b.set_solve(algorithm, num_jacobi_sweeps);
     });
 }
 
@@ -528,11 +528,44 @@ culip_status culip_spmv_create_kkt(const culip_csr* hA, const double* d, double 
     });
 }
 
+culip_status culip_spmv_create_normal(const culip_csr* hA, const double* d, double delta,
+    int32_t n_ix, const int32_t* ix, culip_spmv** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out && d, "spmv: null argument");
+        HostCsr a = to_host(hA);
+        std::vector<double> dix(a.m, 0.0);
+        for(int32_t q = 0; q < n_ix; ++q) {
+            CULIP_REQUIRE(ix && ix[q] >= 0 && ix[q] < a.m, "spmv: ix out of range");
+            dix[ix[q]] += delta;
+        }
+        device_present();
+        auto* s = new culip_spmv;
+        try {
+            Spmv& o = s->s;
+            o.normal = true;
+            o.N.m = a.m;
+            o.N.n = a.n;
+            upload_csr(o.N.A, o.b_arow, o.b_acol, o.b_aval, a);
+            upload_csr(o.N.At, o.b_atrow, o.b_atcol, o.b_atval, transpose_csr(a));
+            upload(o.b_d, std::vector<double>(d, d + a.n));
+            upload(o.b_dix, dix);
+            o.b_t.ensure(sizeof(double) * (size_t) std::max(a.n, 1));
+            o.N.d = o.b_d.as<double>();
+            o.N.dix = o.b_dix.as<double>();
+            o.N.t = o.b_t.as<double>();
+            o.m = o.n = a.m;
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
+
 culip_status culip_spmv_update_values(culip_spmv* op, const double* vals, int32_t on_device,
     void* stream)
 {
     return guarded([&] {
-        CULIP_REQUIRE(op && vals && !op->s.kkt, "spmv: values can be replaced on CSR operators only");
+        CULIP_REQUIRE(op && vals && !op->s.kkt && !op->s.normal,
+            "spmv: values can be replaced on CSR operators only");
         HIP_CHECK(hipMemcpyAsync(op->s.A.val, vals, sizeof(double) * (size_t) op->s.A.nnz,
             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, (hipStream_t) stream));
         CULIP_REQUIRE(!op->s.have_at, "spmv: value update of a non-symmetric operator");
@@ -684,6 +717,15 @@ culip_status culip_server_add_pattern(culip_server* s, const culip_csr* K, int32
         CULIP_REQUIRE(h.m == h.n, "server: matrix must be square");
         *pattern = s->s->add_pattern(h, std::vector<int32_t>(piv_starts, piv_starts + num_pivs),
             std::vector<int32_t>(block_starts, block_starts + num_blocks));
+    });
+}
+
+culip_status culip_server_set_pattern_solve(culip_server* s, int32_t pattern,
+    culip_solve_algorithm algorithm, int32_t jacobi_sweeps)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s, "server: null handle");
+        s->s->set_pattern_solve(pattern, algorithm, jacobi_sweeps);
     });
 }
 

@@ -154,6 +154,8 @@ struct Bildlt final : Precond {
     void trsv_u(double* x, hipStream_t st) const;
     void trsv_ut(double* x, hipStream_t st) const;
     void jacobi(double* x, bool transpose, hipStream_t st) const;
+    // set_solve_algorithm; allocates the block-Jacobi vectors when needed
+    void set_solve(culip_solve_algorithm a, int32_t sweeps);
     // fails if a single-launch solve gave up waiting (synchronises st)
     void check_solve_error(hipStream_t st) const;
     SolveTopo topo() const;
@@ -225,6 +227,9 @@ private:
 
 struct Spmv {
     bool kkt = false;
+    bool normal = false;
+    DevNormal N;
+    DevBuf b_t;             // normal equations: work vector (not reentrant)
     DevCsr A, At;           // At only for a non-symmetric CSR transpose product
     bool have_at = false;
     DevKkt K;

@@ -12,6 +12,9 @@ Server::Server(const culip_server_opts& o) : o_(o)
     CULIP_REQUIRE(o.prec == CULIP_PREC_F64 || o.prec == CULIP_PREC_F32, "server: unknown precision");
     CULIP_REQUIRE(o.fill_level >= 0 && o.fill_factor >= 1.0, "server: bad fill parameters");
     CULIP_REQUIRE(o.sqmr.max_it >= 0 && o.sqmr.tol >= 0, "server: bad SQMR options");
+    CULIP_REQUIRE(o.solve_algorithm == CULIP_TRSV_LEVELS || o.solve_algorithm == CULIP_TRSV_FLAGGED ||
+            (o.solve_algorithm == CULIP_TRSV_JACOBI && o.jacobi_sweeps >= 1),
+        "server: bad solve algorithm");
     HIP_CHECK(hipGetDevice(&device_));
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
@@ -58,12 +61,12 @@ int32_t Server::add_pattern(const HostCsr& K, const std::vector<int32_t>& piv_st
     P->pat->piv_starts = piv_starts;
     P->pat->block_starts = block_starts;
     P->pat->prec = o_.prec;
+    P->algo = o_.solve_algorithm;
+    P->sweeps = o_.jacobi_sweeps;
     {
         Bildlt tmp(P->pat);
         tmp.symbolic(o_.fill_level, o_.fill_factor);
-        const int64_t vec = (int64_t) sqmr_work_bytes(K.m) + 2 * 8 * (int64_t) K.m + 512;
-        P->est_max = tmp.numeric_bytes_estimate(true) + vec;
-        P->est_min = tmp.numeric_bytes_estimate(false) + vec;
+        estimate(*P);
     }
     P->pat->upload_static();
     upload(P->k_row, K.row_ptr);
@@ -87,6 +90,32 @@ int32_t Server::pattern_n(int32_t p) const
     return patterns_[p]->N;
 }
 
+// device bytes of one instance of P: an upper bound, and the part without
+// the solve layout (the pattern's symbolic analysis must exist)
+void Server::estimate(ServePattern& P) const
+{
+    Bildlt tmp(P.pat);
+    tmp.solve_algo = P.algo;
+    const int64_t vec = (int64_t) sqmr_work_bytes(P.pat->A.m) + 2 * 8 * (int64_t) P.pat->A.m + 512;
+    P.est_max = tmp.numeric_bytes_estimate(true) + vec;
+    P.est_min = tmp.numeric_bytes_estimate(false) + vec;
+}
+
+void Server::set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t sweeps)
+{
+    CULIP_REQUIRE(a == CULIP_TRSV_LEVELS || a == CULIP_TRSV_FLAGGED ||
+            (a == CULIP_TRSV_JACOBI && sweeps >= 1),
+        "server: bad solve algorithm");
+    std::lock_guard<std::mutex> g(mu_);
+    CULIP_REQUIRE(pattern >= 0 && pattern < (int32_t) patterns_.size(), "server: unknown pattern");
+    ServePattern& P = *patterns_[pattern];
+    if(P.submitted > 0)
+        fail(CULIP_ERR_STATE, "server: the solve schedule is fixed once a request was submitted");
+    P.algo = a;
+    P.sweeps = sweeps;
+    estimate(P);
+}
+
 int64_t Server::submit(int32_t pattern, const double* vals, const double* rhs)
 {
     CULIP_REQUIRE(vals && rhs, "server: null argument");
@@ -103,6 +132,7 @@ int64_t Server::submit(int32_t pattern, const double* vals, const double* rhs)
     r->vals.assign(vals, vals + nnz);
     r->rhs.assign(rhs, rhs + N);
     std::lock_guard<std::mutex> g(mu_);
+    ++patterns_[pattern]->submitted;
     r->id = next_id_++;
     r->t_submit = now();
     const int64_t id = r->id;
@@ -209,6 +239,7 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
         in = std::make_unique<ServeInstance>();
         in->f = std::make_unique<Bildlt>(P.pat);
         in->f->upload_pattern();
+        in->f->set_solve(P.algo, P.sweeps);
         in->op.kkt = false;
         in->op.m = in->op.n = P.N;
         in->op.A.m = in->op.A.n = P.N;

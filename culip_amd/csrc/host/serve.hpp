@@ -38,6 +38,9 @@ struct ServePattern {
     int64_t inst_bytes = 0;   // measured on the first instance
     int64_t est_max = 0;      // before that: upper bound of an instance
     int64_t est_min = 0;      // and its part without the solve layout
+    culip_solve_algorithm algo = CULIP_TRSV_LEVELS;
+    int32_t sweeps = 0;
+    int64_t submitted = 0;    // requests queued for it so far
 };
 
 struct ServeInstance {
@@ -69,6 +72,7 @@ public:
     int32_t add_pattern(const HostCsr& K, const std::vector<int32_t>& piv_starts,
         const std::vector<int32_t>& block_starts);
     int64_t submit(int32_t pattern, const double* vals, const double* rhs);
+    void set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t sweeps);
     // blocks until the request is done; copies x (N doubles) if x != nullptr
     void wait(int64_t id, double* x, culip_sqmr_result* res, double* timing, int32_t* slot);
     void stats(int64_t* out) const;
@@ -86,6 +90,7 @@ private:
     ServeInstance* instance(Slot* s, int32_t pattern, std::unique_lock<std::mutex>& lk);
     void run(ServeInstance* in, ServeRequest* r, hipStream_t st);
     double now() const;
+    void estimate(ServePattern& P) const;
 
     culip_server_opts o_;
     int device_ = 0;

@@ -210,6 +210,14 @@ culip_status culip_spmv_create_csr(const culip_csr * hA, culip_spmv ** out);
  * return A->m, spmv.impl.cuh:319-337; fixed here). */
 culip_status culip_spmv_create_kkt(const culip_csr * hA, const double * d,
     double delta, int32_t n_ix, const int32_t * ix, culip_spmv ** out);
+/* NormalMatrixSPMV (spmv.cuh:74-107, spmv.impl.cuh:122-265): the m x m
+ * operator N = A diag(d)^-1 A' + delta I on the rows in ix, from host A
+ * (m x n) and d (n). The reference adds delta * x[k] at row ix[k]
+ * (k_add_app); here row ix[k] gets delta * x[ix[k]], which is the same when
+ * ix lists rows in order from 0 or is empty. transpose is ignored (N is
+ * symmetric). */
+culip_status culip_spmv_create_normal(const culip_csr * A, const double * d, double delta,
+    int32_t n_ix, const int32_t * ix, culip_spmv ** out);
 /* Replace the values of a CSR operator (same pattern), host array. */
 culip_status culip_spmv_update_values(culip_spmv * op, const double * vals,
     int32_t vals_on_device, void * stream);
@@ -315,6 +323,10 @@ typedef struct {
     double fill_factor;
     double threshold;
     culip_sqmr_opts sqmr;
+    /* triangular-solve schedule of every pattern unless set per pattern
+     * (culip_server_set_pattern_solve); as culip_bildlt_set_solve_algorithm */
+    culip_solve_algorithm solve_algorithm;
+    int32_t jacobi_sweeps;
 } culip_server_opts;
 
 typedef struct culip_server culip_server;
@@ -325,6 +337,10 @@ culip_status culip_server_create(const culip_server_opts * o, culip_server ** ou
 culip_status culip_server_add_pattern(culip_server * s, const culip_csr * K,
     int32_t num_pivs, const int32_t * piv_starts, int32_t num_blocks,
     const int32_t * block_starts, int32_t * pattern);
+/* The triangular-solve schedule of one pattern (default: the server's);
+ * only before the pattern's first submit (CULIP_ERR_STATE after). */
+culip_status culip_server_set_pattern_solve(culip_server * s, int32_t pattern,
+    culip_solve_algorithm algorithm, int32_t jacobi_sweeps);
 /* Queues a solve: vals = nnz values of K in the pattern's entry order,
  * rhs = N doubles (host; copied before return). */
 culip_status culip_server_submit(culip_server * s, int32_t pattern,

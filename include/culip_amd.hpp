@@ -7,6 +7,7 @@
  *   BlockiLDLt<T>     libs/la/preconditioner/block_ildlt.cuh:188-311
  *   CSRMatrixSPMV     libs/la/spmv.cuh:36-66
  *   AugmentedMatrixSPMV libs/la/spmv.cuh:116-149
+ *   NormalMatrixSPMV  libs/la/spmv.cuh:74-107
  *   SQMR              libs/la/iterative_solvers/sqmr.cuh:21-72
  *   ImportLDUPreconditioner libs/la/preconditioner/import_ldu_preconditioner.cuh:36-125
  *
@@ -228,6 +229,17 @@ public:
     }
 };
 
+/* NormalMatrixSPMV (spmv.cuh:74-107): A diag(d)^-1 A' + delta I(ix). */
+class NormalMatrixSPMV : public SPMV {
+public:
+    NormalMatrixSPMV(void* stream, const culip_csr* h_A, const double* d, double delta = 0.0,
+        int32_t n_ix = 0, const int32_t* ix = nullptr)
+        : SPMV(stream)
+    {
+        check(culip_spmv_create_normal(h_A, d, delta, n_ix, ix, &op_));
+    }
+};
+
 /* SQMR<T> (sqmr.cuh:21-72). The reference's ctor takes (spmv, tol, max_it,
  * preconditioner); true_residual_period = 0 keeps its stopping test. */
 class SQMR {
@@ -287,6 +299,11 @@ public:
         int32_t p = -1;
         check(culip_server_add_pattern(s_, K, num_pivs, piv_starts, num_blocks, block_starts, &p));
         return p;
+    }
+    /* before the pattern's first submit */
+    void set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t jacobi_sweeps)
+    {
+        check(culip_server_set_pattern_solve(s_, pattern, a, jacobi_sweeps));
     }
     int64_t submit(int32_t pattern, const double* vals, const double* rhs)
     {

@@ -180,6 +180,39 @@ def test_spmv_matches_fixture(hip_device, golden):
     np.testing.assert_allclose(Aug2.cpu().numpy(), ref, rtol=1e-10, atol=1e-10)
 
 
+def test_normal_spmv_matches_fixture(hip_device, golden):
+    """NormalMatrixSPMV vs tests/la/data/spmv_N.mtx (test_spmv.cc:170-189),
+    with alpha / beta and a delta term on every row."""
+    import torch
+    ar, ac, av, an = ca.read_matrix_csr(f"{golden}/spmv_A.mtx")
+    d = ca.read_dense_vector(f"{golden}/spmv_d.mtx")
+    nr, nc, nv, nn = ca.read_matrix_csr(f"{golden}/spmv_N.mtx")
+    m = ar.size - 1
+    assert nr.size - 1 == m == nn
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=m)
+    y0 = rng.normal(size=m)
+    ref = ol.csr_spmv(nr, nc, nv, x)
+    scale = np.abs(ref).max()
+    Nop = ca.NormalMatrixSPMV(ar, ac, av, an, d)
+    assert Nop.m() == Nop.n() == m
+    xd = torch.tensor(x, device=hip_device)
+    y = torch.zeros_like(xd)
+    Nop.multiply(xd, y)
+    yt = torch.zeros_like(xd)
+    Nop.multiply(xd, yt, transpose=True)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=0, atol=1e-10 * scale)
+    np.testing.assert_allclose(yt.cpu().numpy(), ref, rtol=0, atol=1e-10 * scale)
+    delta = 0.25
+    Nd = ca.NormalMatrixSPMV(ar, ac, av, an, d, delta=delta, ix=np.arange(m))
+    yd = torch.tensor(y0, device=hip_device)
+    Nd.multiply(xd, yd, alpha=2.0, beta=-0.5)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(yd.cpu().numpy(), 2.0 * (ref + delta * x) - 0.5 * y0, rtol=0,
+                               atol=1e-10 * scale)
+
+
 def test_sqmr_reference_semantics_vs_oracle(hip_device):
     import torch
     k, p = _kkt(2000, 800, 16, ca.ORDER_NATURAL)

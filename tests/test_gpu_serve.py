@@ -114,3 +114,36 @@ def test_budget_admission(hip_device):
         s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
     assert e.value.status == 6
     s.close()
+
+
+def test_pattern_solve_schedule(hip_device):
+    """A pattern served with block-Jacobi solves gives the single-system
+    result under the same schedule, bit for bit; the schedule is fixed once a
+    request for the pattern was submitted."""
+    import torch
+    (k, p), = _patterns([3000])
+    vals, rhs = _request(k, p, 11)
+    s = ca.Server(slots=2, prec="f32")
+    pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+    s.set_pattern_solve(pid, "jacobi", 2)
+    x_srv, res, _, _ = s.wait(s.submit(pid, vals, rhs))
+    with pytest.raises(ca.CulipError) as e:
+        s.set_pattern_solve(pid, "levels")
+    assert e.value.status == 4
+    s.close()
+    assert res["true_rel_residual"] <= 1e-8
+    g = ca.BlockiLDLt(p.row_ptr, p.col, vals, p.piv_starts, p.block_starts, "f32")
+    g.compute(0, 1.0, 1e-4)
+    g.set_solve_algorithm("jacobi", 2)
+    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, vals)
+    q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
+    x = torch.zeros(rhs.size, dtype=torch.float64, device=hip_device)
+    q.solve(torch.tensor(rhs, device=hip_device), x)
+    assert np.array_equal(x.cpu().numpy(), x_srv)
+    assert q.result["iterations"] == res["iterations"]
+    # the server-wide default applies to every pattern
+    s = ca.Server(slots=1, prec="f32", solve_algorithm="jacobi", jacobi_sweeps=2)
+    pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+    x2, _, _, _ = s.wait(s.submit(pid, vals, rhs))
+    s.close()
+    assert np.array_equal(x2, x_srv)

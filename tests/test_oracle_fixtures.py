@@ -105,3 +105,17 @@ def test_solve_D_against_dense_blocks():
             i += 1
     b = rng.normal(size=n)
     np.testing.assert_allclose(ol.solve_D(tri, is_piv, b), np.linalg.solve(D, b), rtol=1e-12)
+
+
+def test_normal_fixture_is_a_dinv_at(golden):
+    """spmv_N.mtx is A diag(d)^-1 A' for spmv_A / spmv_d (the operator of
+    NormalMatrixSPMV, spmv.impl.cuh:214-240): scipy, to the fixture's digits."""
+    ar, ac, av, an = _csr(f"{golden}/spmv_A.mtx")
+    d = ca.read_dense_vector(f"{golden}/spmv_d.mtx")
+    nr, nc, nv, nn = _csr(f"{golden}/spmv_N.mtx")
+    A = sp.csr_matrix((av, ac, ar), shape=(ar.size - 1, an))
+    N = sp.csr_matrix((nv, nc, nr), shape=(nr.size - 1, nn))
+    M = A @ sp.diags(1.0 / d) @ A.T
+    assert abs(M - N).max() <= 1e-12 * abs(N).max()
+    # and not A diag(d) A'
+    assert abs(A @ sp.diags(d) @ A.T - N).max() > 1.0
