@@ -51,6 +51,8 @@ def parse():
                     help="offered requests/s (0: 85%% of the warm-up throughput)")
     ap.add_argument("--max-block", type=int, default=32)
     ap.add_argument("--systems", type=int, default=512)
+    ap.add_argument("--serve-jacobi-sweeps", type=int, default=2,
+                    help="serve: block-Jacobi solves with this many sweeps (0: level solves)")
     ap.add_argument("--max-it", type=int, default=0,
                     help="SQMR iteration budget per request (0: 3000 for serve)")
     ap.add_argument("--order", default="color", choices=["color", "natural"])
@@ -147,7 +149,9 @@ def bench_serve(a):
     rng = np.random.default_rng(a.seed)
     sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
     order = ca.ORDER_BLOCK_COLOR
-    srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000)
+    sweeps = a.serve_jacobi_sweeps
+    srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000,
+                    solve_algorithm="jacobi" if sweeps > 0 else "levels", jacobi_sweeps=sweeps)
     nsets = 2
 
     def make(i):
@@ -217,6 +221,7 @@ def bench_serve(a):
                    "m": "0.4n", "max_block": a.max_block, "slots": a.slots,
                    "requests": len(res), "offered_rate_per_s": rate,
                    "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "max_it": a.max_it or 3000,
+                   "solve_schedule": f"jacobi{sweeps}" if sweeps > 0 else "levels",
                    "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "queue_p50_ms": float(np.percentile(que, 50)),

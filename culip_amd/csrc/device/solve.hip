@@ -25,32 +25,33 @@ namespace {
 
 // columns of U_ii^-1 by back substitution, in double; X[r][c] (32 x 32 LDS)
 template<typename VT>
-__device__ void diag_inverse(const VT* U, int h, double* X, double* Us)
+__device__ void diag_inverse(const VT* U, int h, double* X)
 {
-    const int lane = lane_id();
-    // the tile to LDS first (coalesced), then each lane c forms column c of
-    // U^-1 bottom-up from its own earlier results; exact-zero terms of the
-    // sums are skipped (they add nothing)
-    for(int e = lane; e < h * h; e += 64) {
-        const int r = e / h, k = e - r * h;
-        Us[32 * r + k] = (double) U[e];
+    const int c = lane_id();
+    // a tile with no entry right of its diagonal (most pivot blocks of a
+    // sparse pattern) has the diagonal inverse: the back substitution below
+    // would only add exact zeros to it
+    bool offd = false;
+    if(c < h)
+        for(int k = c + 1; k < h; ++k) offd = offd || (U[c * h + k] != (VT) 0);
+    if(wave_ballot(offd) == 0) {
+        for(int e = c; e < 1024; e += 64) {
+            const int r = e >> 5, q = e & 31;
+            X[e] = (r == q && r < h) ? 1.0 / (double) U[r * h + r] : 0.0;
+        }
+        wave_sync();
+        return;
     }
-    wave_sync();
-    const int c = lane;
     for(int r = 31; r >= 0; --r) {
         if(c < 32) {
             double v = 0.0;
             if(c < h && r < h && r <= c) {
-                const double d = Us[32 * r + r];
                 if(r == c) {
-                    v = 1.0 / d;
+                    v = 1.0 / (double) U[r * h + r];
                 } else {
                     double s = 0.0;
-                    for(int k = r + 1; k <= c; ++k) {
-                        const double u = Us[32 * r + k];
-                        if(u != 0.0) s = fma(u, X[32 * k + c], s);
-                    }
-                    v = -s / d;
+                    for(int k = r + 1; k <= c; ++k) s = fma((double) U[r * h + k], X[32 * k + c], s);
+                    v = -s / (double) U[r * h + r];
                 }
             }
             X[32 * r + c] = v;
@@ -95,14 +96,13 @@ template<typename VT, bool FILL>
 __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
 {
     __shared__ double X[1024];
-    __shared__ double Us[1024];
     const int lane = lane_id();
     const int i = blockIdx.x;
     const int fr = t.bs[i];
     const int h = t.bs[i + 1] - fr;
     // pivot block inverse rows (ui) and columns (ti)
     const BlockMeta dm = t.bout[t.csr_off[i]];
-    diag_inverse(val_out + dm.val_ptr, h, X, Us);
+    diag_inverse(val_out + dm.val_ptr, h, X);
     if(lane < h) {
         int cr = 0, cc = 0;
         int32_t wr = 0, wc = 0;
