@@ -252,10 +252,21 @@ __device__ void step_1x1(VT* T, double* tri, int p, int len)
     const VT ru = u * d;
     wave_sync();
     if(half == 0) T[32 * p + j] = u;
-    for(int i0 = p + 1; i0 < len; i0 += 2) {
-        const int i = i0 + half;
-        const VT rui = wave_bcast(ru, i & 31);
-        if(i < len && j < len) T[32 * i + j] = fma(-rui, u, T[32 * i + j]);
+    // trailing rows i > p, two per pass: only rows with a nonzero multiplier
+    // ru_i change, and only if the pivot row has a nonzero u_j right of the
+    // diagonal; the other updates would add exact zeros
+    const uint64_t ucols = wave_ballot(half == 0 && j > p && j < len && u != (VT) 0);
+    uint64_t rows = ucols ? wave_ballot(half == 0 && j > p && j < len && ru != (VT) 0) : 0ull;
+    while(rows) {
+        const int ia = __ffsll((long long) rows) - 1;
+        rows &= rows - 1;
+        const int ib = rows ? __ffsll((long long) rows) - 1 : -1;
+        if(rows) rows &= rows - 1;
+        const VT ra = wave_bcast(ru, ia);
+        const VT rb = wave_bcast(ru, ib < 0 ? ia : ib);
+        const int i = half ? ib : ia;
+        const VT rui = half ? rb : ra;
+        if(i >= 0 && j < len) T[32 * i + j] = fma(-rui, u, T[32 * i + j]);
     }
     wave_sync();
     if(mod && lane == 0) T[32 * p + p] = 1;
@@ -294,11 +305,22 @@ __device__ void step_2x2(VT* T, double* tri, int p, int len)
         T[32 * p + j] = (j < p) ? (VT) 0 : u1;
         T[32 * (p + 1) + j] = (j < p) ? (VT) 0 : u2;
     }
-    for(int i0 = p + 2; i0 < len; i0 += 2) {
-        const int i = i0 + half;
-        const double r1i = wave_bcast(r1, i & 31);
-        const double r2i = wave_bcast(r2, i & 31);
-        if(i < len)
+    // trailing rows i > p + 1 as in step_1x1: rows with a nonzero (r1_i,
+    // r2_i), when the pivot rows have a nonzero (u1_j, u2_j) beyond them
+    const uint64_t ucols = wave_ballot(half == 0 && j > p + 1 && j < len &&
+        (u1 != (VT) 0 || u2 != (VT) 0));
+    uint64_t rows = ucols ? wave_ballot(half == 0 && j > p + 1 && j < len &&
+        (r1 != 0.0 || r2 != 0.0)) : 0ull;
+    while(rows) {
+        const int ia = __ffsll((long long) rows) - 1;
+        rows &= rows - 1;
+        const int ib = rows ? __ffsll((long long) rows) - 1 : -1;
+        if(rows) rows &= rows - 1;
+        const double r1a = wave_bcast(r1, ia), r2a = wave_bcast(r2, ia);
+        const double r1b = wave_bcast(r1, ib < 0 ? ia : ib), r2b = wave_bcast(r2, ib < 0 ? ia : ib);
+        const int i = half ? ib : ia;
+        const double r1i = half ? r1b : r1a, r2i = half ? r2b : r2a;
+        if(i >= 0 && j < len)
             T[32 * i + j] = (VT) ((double) T[32 * i + j] -
                 fma(r1i, (double) u1, r2i * (double) u2));
     }
@@ -494,6 +516,10 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
     const int cb0 = a.csc_off[i];
     const int nd = a.csc_off[i + 1] - 1 - cb0;
     const int j = lane & 31, cq = lane >> 5;
+    // metadata of the first 64 dependency blocks, in flight during the tile
+    // load; later chunks are fetched while the previous one is walked
+    Dep nxt{};
+    if(lane < nd) nxt = load_dep(a.bout, a.bs, a.csc_idx[cb0 + lane]);
     // the pivot tile in registers (rows cq + 2t of column j)
     VT* T0 = s.X[0];
     tile_load(T0, m, a.bs, a.ix_in, a.val_in);
@@ -517,14 +543,15 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
     const bool owns_diag = ((j & 1) == cq) && j < len;
     const int tdiag = (j - cq) >> 1;
     for(int c0 = 0; c0 < nd; c0 += 64) {
-        Dep mine{};
+        const Dep mine = nxt;
         const int cnt = min(64, nd - c0);
-        if(lane < cnt) mine = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + lane]);
         int uc = -1;
         VT uv = (VT) 0;
         const bool simple = lane < cnt && column_update(mine, len, a.ix_out, a.val_out, a.tri,
             uc, uv);
         const uint64_t smask = wave_ballot(simple);
+        nxt = Dep{};
+        if(c0 + 64 + lane < nd) nxt = load_dep(a.bout, a.bs, a.csc_idx[cb0 + c0 + 64 + lane]);
         VT dg = diag_get(T, owns_diag, tdiag);
         int d = 0;
         while(d < cnt) {
@@ -1054,6 +1081,55 @@ __device__ __forceinline__ void to_wave_form(const FactorArgs<VT>& a, const Leve
     a.wave_list[atomicAdd(a.wave_cnt + lr.level, 1)] = q;
 }
 
+// metadata of the two factored blocks of one Schur pair, as far as the
+// one-lane form needs it; ok = both are sparse with at most TINY_E entries
+struct TinyMeta {
+    bool ok;
+    int nnz_i, cp_i, vp_i, nnz_j, cp_j, vp_j;
+};
+
+__device__ __forceinline__ TinyMeta tiny_meta(const BlockMeta* bout, int4 pp, bool valid)
+{
+    TinyMeta t{};
+    t.ok = true;
+    if(!valid) return t;
+    const BlockMeta mi = bout[pp.x], mj = bout[pp.y];
+    t.ok = mi.format == FMT_SPARSE && mi.nnz <= TINY_E && mj.format == FMT_SPARSE &&
+        mj.nnz <= TINY_E;
+    t.nnz_i = mi.nnz;
+    t.cp_i = mi.col_ptr;
+    t.vp_i = mi.val_ptr;
+    t.nnz_j = mj.nnz;
+    t.cp_j = mj.col_ptr;
+    t.vp_j = mj.val_ptr;
+    return t;
+}
+
+// the stored entries of both blocks (nothing for a pair outside the form)
+template<typename VT>
+struct TinyRaw {
+    int ixi[TINY_E], ixj[TINY_E];
+    VT vi[TINY_E], vj[TINY_E];
+};
+
+template<typename VT>
+__device__ __forceinline__ void tiny_raw(const int32_t* ix_out, const VT* val_out,
+    const TinyMeta& m, TinyRaw<VT>& r)
+{
+#pragma unroll
+    for(int e = 0; e < TINY_E; ++e) {
+        r.ixi[e] = 0; r.vi[e] = 0; r.ixj[e] = 0; r.vj[e] = 0;
+        if(m.ok && e < m.nnz_i) {
+            r.ixi[e] = ix_out[(int64_t) m.cp_i + e];
+            r.vi[e] = val_out[(int64_t) m.vp_i + e];
+        }
+        if(m.ok && e < m.nnz_j) {
+            r.ixj[e] = ix_out[(int64_t) m.cp_j + e];
+            r.vj[e] = val_out[(int64_t) m.vp_j + e];
+        }
+    }
+}
+
 template<typename VT>
 __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, LevelRange lr)
 {
@@ -1089,39 +1165,37 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         V(n) = a.val_in[(int64_t) t1.w + e];
         ++n;
     }
-    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order. Loads run two
-    // pairs ahead: the pair record of pair pr + 2 and the metadata of pair
-    // pr + 1 are in flight while pair pr is computed.
-    int4 pp_b = (np > 0) ? pairs[0] : make_int4(0, 0, 0, 0);
-    int4 pp_a = (np > 1) ? pairs[1] : make_int4(0, 0, 0, 0);
-    BlockMeta mi_b = a.bout[pp_b.x], mj_b = a.bout[pp_b.y];
+    // A_ij -= U_ki' D_k U_kj over the Schur pairs, in order. Loads run
+    // ahead of the arithmetic: while pair pr is computed, the entries of
+    // pair pr + 1, the metadata of pair pr + 2 and the pair record of pair
+    // pr + 3 are in flight.
+    auto rec = [&](int k) { return k < np ? pairs[k] : make_int4(0, 0, 0, 0); };
+    int4 pc = rec(0), p1 = rec(1), p2 = rec(2);
+    TinyMeta mc = tiny_meta(a.bout, pc, np > 0), m1 = tiny_meta(a.bout, p1, np > 1);
+    TinyRaw<VT> rc;
+    tiny_raw(a.ix_out, a.val_out, mc, rc);
     for(int pr = 0; pr < np; ++pr) {
-        const int4 pp = pp_b;
-        const BlockMeta mi = mi_b;
-        const BlockMeta mj = mj_b;
-        if(pr + 1 < np) {
-            pp_b = pp_a;
-            mi_b = a.bout[pp_b.x];
-            mj_b = a.bout[pp_b.y];
-            if(pr + 2 < np) pp_a = pairs[pr + 2];
-        }
-        if(mi.format != FMT_SPARSE || mi.nnz > TINY_E || mj.format != FMT_SPARSE ||
-            mj.nnz > TINY_E) {
+        if(!mc.ok) {
             to_wave_form(a, lr, q);
             return;
         }
-        const int rb = pp.z;
+        // next entries, metadata and record
+        TinyRaw<VT> rn;
+        tiny_raw(a.ix_out, a.val_out, m1, rn);
+        const TinyMeta m2 = tiny_meta(a.bout, p2, pr + 2 < np);
+        const int4 p3 = rec(pr + 3);
+        const int rb = pc.z;
         // U_ki: rows l of block k, columns c (rows of block i), width hgt
         int al[TINY_E], ac[TINY_E];
         VT av[TINY_E];
 #pragma unroll
         for(int e = 0; e < TINY_E; ++e) {
             al[e] = -4; ac[e] = -1; av[e] = 0;
-            if(e < mi.nnz) {
-                const int ix = a.ix_out[(int64_t) mi.col_ptr + e];
+            if(e < mc.nnz_i) {
+                const int ix = rc.ixi[e];
                 al[e] = ix / hgt;
                 ac[e] = ix - al[e] * hgt;
-                av[e] = a.val_out[(int64_t) mi.val_ptr + e];
+                av[e] = rc.vi[e];
             }
         }
         // U_kj: rows l, columns j, width wid; nonzero entries by (column, row)
@@ -1130,9 +1204,9 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
 #pragma unroll
         for(int e = 0; e < TINY_E; ++e) {
             rl[e] = 0; rj[e] = 64; rv[e] = 0;
-            if(e < mj.nnz) {
-                const int ix = a.ix_out[(int64_t) mj.col_ptr + e];
-                const VT v = a.val_out[(int64_t) mj.val_ptr + e];
+            if(e < mc.nnz_j) {
+                const int ix = rc.ixj[e];
+                const VT v = rc.vj[e];
                 if(v != (VT) 0) {
                     rl[e] = ix / wid;
                     rj[e] = ix - rl[e] * wid;
@@ -1140,6 +1214,12 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
                 }
             }
         }
+        rc = rn;
+        mc = m1;
+        m1 = m2;
+        pc = p1;
+        p1 = p2;
+        p2 = p3;
         int key[TINY_E], rank[TINY_E];
 #pragma unroll
         for(int e = 0; e < TINY_E; ++e) key[e] = rj[e] * 32 + rl[e];

@@ -221,7 +221,8 @@ __device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
 
 template<int K0>
 __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk, double* pk,
-    const double* pkm1, const double* pkm2, double* x, const double* scal, double* part)
+    const double* pkm1, const double* pkm2, double* x, const double* scal, double* part,
+    const int32_t* flags)
 {
     __shared__ double sh[16];
     double S[NSCAL];
@@ -229,6 +230,10 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
     iteration_scalars<K0>(S, scal, part, G, sh);
     const double ib = 1.0 / S[S_BETA + (K0 + 1) % 3];
     const double nbrr = -S[S_BRR], neps = -S[S_EPS], ir = 1.0 / S[S_RHO], tau = S[S_TAU];
+    // after a Lanczos breakdown (flagged by the previous iteration) the
+    // recurrence divides by a vanishing delta: x keeps the value it had when
+    // the breakdown was detected
+    const bool frozen = flags[0] != 0;
     double ss = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
         w1[g] = w1[g] * ib;
@@ -237,8 +242,8 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
         p = fma(neps, pkm2[g], p);
         p = p * ir;
         pk[g] = p;
-        const double xv = fma(tau, p, x[g]);
-        x[g] = xv;
+        const double xv = frozen ? x[g] : fma(tau, p, x[g]);
+        if(!frozen) x[g] = xv;
         ss = fma(xv, xv, ss);
     }
     ss = block_sum(ss, sh);
@@ -247,12 +252,12 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
 
 __global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
     double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
-    double* part)
+    double* part, const int32_t* flags)
 {
     switch(k % 3) {
-    case 0: pupdate<0>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
-    case 1: pupdate<1>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
-    default: pupdate<2>(n, w1, vk, pk, pkm1, pkm2, x, scal, part); break;
+    case 0: pupdate<0>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    case 1: pupdate<1>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    default: pupdate<2>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
     }
 }
 
@@ -508,7 +513,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
         if(ops.left) ops.left(w.av[nxt], st);
         k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
         k_pupdate<<<G, TPB, 0, st>>>(n, it, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
-            w.scal, w.part);
+            w.scal, w.part, w.flags);
         k_scalars<<<1, TPB, 0, st>>>(it, G, w.scal, w.part, w.flags, opt.tol,
             period > 0 ? 1e-14 : -1.0);
         cur = nxt;

@@ -99,3 +99,61 @@ def ldu_problem(golden, prefix):
             is_piv[i + 1] = 0
             i += 2
     return (ar, ac, av), (lr, lc, lv), tri, is_piv, p, s, n
+
+
+def _dense_csr(A):
+    rp, ci, cv = [0], [], []
+    for i in range(A.shape[0]):
+        nz = np.nonzero(A[i])[0]
+        ci.extend(nz)
+        cv.extend(A[i, nz])
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(cv, dtype=np.float64)
+
+
+def edge_problem(name):
+    """Small edge cases: (csr, piv_starts, block_starts, dense A).
+    chain-b1: tridiagonal with one-row blocks, so the level count equals the
+      block count (the reference's one-past-end case at
+      block_ildlt.impl.cuh:775, 789-791);
+    one-1x1 / one-2x2: a single row / a single 2x2 pivot;
+    diagonal: D = diag(A), U = I;
+    dense-32: one full 32 x 32 block with 2x2 pivots;
+    zero-pivots: zero diagonal entries that hit the tiny-pivot rules."""
+    rng = np.random.default_rng(sum(map(ord, name)))
+    if name == "chain-b1":
+        n = 40
+        A = np.diag(rng.uniform(2, 4, n)) + np.diag(np.full(n - 1, -1.0), 1) + np.diag(np.full(n - 1, -1.0), -1)
+        piv, bs = np.arange(n), np.arange(n)
+    elif name == "one-1x1":
+        A = np.array([[3.0]])
+        piv, bs = np.array([0]), np.array([0])
+    elif name == "one-2x2":
+        A = np.array([[1e-3, 2.0], [2.0, 1e-3]])
+        piv, bs = np.array([0]), np.array([0])
+    elif name == "diagonal":
+        n = 50
+        A = np.diag(rng.uniform(-3, 3, n) + 0.5)
+        piv, bs = np.arange(n), np.arange(0, n, 8)
+    elif name == "dense-32":
+        n = 32
+        Q = rng.normal(size=(n, n))
+        A = Q + Q.T + np.diag(np.linspace(-8, 8, n))
+        piv = np.array([0, 2, 3, 5, 6] + list(range(8, 32)))
+        bs = np.array([0])
+    elif name == "zero-pivots":
+        n = 48
+        A = np.zeros((n, n))
+        for i in range(n - 1):
+            if rng.random() < 0.3:
+                A[i, i + 1] = A[i + 1, i] = rng.normal()
+        d = rng.normal(size=n) * 3
+        d[::5] = 0.0
+        A[np.arange(n), np.arange(n)] = d
+        piv, bs = np.arange(n), np.arange(0, n, 12)
+    else:
+        raise KeyError(name)
+    return _dense_csr(A), np.asarray(piv, np.int32), np.asarray(bs, np.int32), A
+
+
+EDGE_CASES = ["chain-b1", "one-1x1", "one-2x2", "diagonal", "dense-32", "zero-pivots"]

@@ -11,7 +11,7 @@ import scipy.sparse as sp
 
 import culip_amd as ca
 import oracle_lib as ol
-from conftest import sym_problem
+from conftest import EDGE_CASES, edge_problem, sym_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -392,3 +392,38 @@ def test_export_factorized(hip_device, tmp_path, case):
     if case == "single_block":
         A = sp.csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n)).toarray()
         np.testing.assert_allclose(Ue.T @ D @ Ue, A, rtol=0, atol=1e-9 * np.abs(A).max())
+
+
+@pytest.mark.parametrize("case", EDGE_CASES)
+def test_edge_cases(hip_device, case):
+    """Single rows, a single 2x2 pivot, one-row blocks with as many levels
+    as blocks, a diagonal matrix, one full 32 x 32 block and zero pivots:
+    the FP64 factor equals the oracle's bit for bit, every solve and every
+    solve schedule agrees with the oracle's solves, and SQMR with the
+    true-residual gate solves the system."""
+    import torch
+    csr, piv, bs, A = edge_problem(case)
+    g, o = _pair(csr, piv, bs, "d", thr=1e-4)
+    _factor_equal(g, o)
+    n = A.shape[0]
+    rng = np.random.default_rng(n)
+    b = rng.normal(size=n)
+    db = torch.tensor(b, device=hip_device)
+    for algo, sw in (("levels", 0), ("flagged", 0), ("jacobi", 64)):
+        g.set_solve_algorithm(algo, sw)
+        for side in ("left", "middle", "right"):
+            x = torch.zeros_like(db)
+            getattr(g, f"solve_{side}")(db, x)
+            torch.cuda.synchronize()
+            ref = o.solve(side, b)
+            err = np.linalg.norm(x.cpu().numpy() - ref) / max(np.linalg.norm(ref), 1e-300)
+            assert err < 1e-12, (algo, side, err)
+    g.set_solve_algorithm("levels", 0)
+    if case == "zero-pivots":
+        return      # tiny-pivot replacements make M a poor preconditioner of A
+    Aop = ca.CSRMatrixSPMV(*csr)
+    q = ca.SQMR(Aop, 1e-8, 10 * n + 20, g, true_residual_period=5, max_restarts=10)
+    x = torch.zeros_like(db)
+    q.solve(db, x)
+    xs = x.cpu().numpy()
+    assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) <= 1e-8, q.result

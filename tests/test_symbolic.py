@@ -9,7 +9,7 @@ import pytest
 
 import culip_amd as ca
 import oracle_lib as ol
-from conftest import sym_problem
+from conftest import EDGE_CASES, edge_problem, sym_problem
 
 NAMES = ["block_starts", "coarse_csr_off", "coarse_csr_col", "coarse_csc_off", "coarse_csc_idx",
          "a_block_nnz", "lvl_off", "lvl_rows", "row_level", "lvlb_off", "lvlb_idx",
@@ -38,7 +38,7 @@ CASES = [
     ("kkt-uneven-blocks", lambda: _kkt_case(1003, 397, 13, ca.ORDER_BLOCK_COLOR, seed=3), 0, 1.5),
     ("sym-2x2-fill2", lambda: sym_problem(n_blocks=14, bmax=32, density=0.08, seed=9)[:3], 2, 1.3),
     ("sym-dense", lambda: sym_problem(n_blocks=6, bmax=16, density=0.9, seed=2)[:3], 0, 1.0),
-]
+] + [(e, (lambda e=e: edge_problem(e)[:3]), 0, 1.0) for e in EDGE_CASES]
 
 
 @pytest.mark.parametrize("name,make,fl,ff", CASES, ids=[c[0] for c in CASES])
@@ -181,3 +181,18 @@ def test_format_threshold_depends_on_T():
         g = ca.BlockiLDLt(*args, "f32" if t == "f" else "f64")
         g.symbolic(0, 1.0)
         assert np.array_equal(g.get_int("blocks"), o.get("blocks"))
+
+
+def test_bildlt_rejects_empty_and_straddling_inputs():
+    """No pivot starts (an empty matrix) and a 2x2 pivot straddling a block
+    boundary (block starts must be pivot starts, quirk Q7 of the reference)
+    are refused with INVALID."""
+    csr, piv, bs, _ = edge_problem("one-2x2")
+    with pytest.raises(ca.CulipError) as e:
+        ca.BlockiLDLt(*csr, np.zeros(0, np.int32), bs)
+    assert e.value.status == 1
+    csr, piv, bs, _ = edge_problem("dense-32")
+    g = ca.BlockiLDLt(*csr, piv, np.array([0, 1], np.int32))   # row 1 is inside a 2x2 pivot
+    with pytest.raises(ca.CulipError) as e:
+        g.symbolic(0, 1.0)
+    assert e.value.status == 1
