@@ -21,7 +21,7 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
            "SQMR", "ImportLDUPreconditioner", "NormalMatrixSPMV", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
-           "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR"]
+           "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR", "permute_scale"]
 
 
 def _i32(a):
@@ -472,3 +472,25 @@ def read_dense_vector(path):
     v = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
     lib().culip_host_free(p)
     return v
+
+
+def permute_scale(row_ptr, col, val, row_perm, col_perm=None, perm_is_old_to_new=False,
+                  row_scale=None, col_scale=None, scale_before_match=False):
+    """T_matrix_permute_scale (libs/la/sparse_la.impl.cuh:600-657): with new ->
+    old vectors B(i, j) = A(p_r[i], p_c[j]) / scale; returns (row_ptr, col, val)."""
+    s, keep = csr_struct(row_ptr, col, val)
+    pr = _i32(row_perm)
+    pc = _i32(col_perm if col_perm is not None else row_perm)
+    rs = _f64(row_scale) if row_scale is not None else None
+    cs = _f64(col_scale) if col_scale is not None else None
+    h = C.c_void_p()
+    check(lib().culip_csr_permute_scale(C.byref(s), _ptr(pr, C.c_int32), _ptr(pc, C.c_int32),
+                                        1 if perm_is_old_to_new else 0,
+                                        _ptr(rs, C.c_double) if rs is not None else None,
+                                        _ptr(cs, C.c_double) if cs is not None else None,
+                                        1 if scale_before_match else 0, C.byref(h)))
+    c = _lib.Csr()
+    check(lib().culip_host_matrix_view(h, C.byref(c)))
+    out = _copy_csr(c)
+    lib().culip_host_matrix_destroy(h)
+    return out

@@ -79,6 +79,10 @@ _SIGS = [
     ("culip_mm_read_csr", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("culip_mm_read_vector", C.c_int, [C.c_char_p, C.POINTER(C.c_int64),
                                        C.POINTER(C.POINTER(C.c_double))]),
+    ("culip_csr_permute_scale", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.c_int32,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                                          C.POINTER(C.c_void_p)]),
     ("culip_host_matrix_view", C.c_int, [C.c_void_p, C.POINTER(Csr)]),
     ("culip_host_matrix_destroy", None, [C.c_void_p]),
     ("culip_host_free", None, [C.c_void_p]),

@@ -97,8 +97,10 @@ void BildltPattern::upload_static()
                 t[8] = (int32_t) s.upd_off[b];
                 t[9] = (int32_t) (s.upd_off[b + 1] - s.upd_off[b]);
                 // one-lane kernel: sparse input of at most 8 entries, at
-                // most 64 Schur pairs, 1x1 pivots in the block row
-                t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= 64 && all1) ? 1 : 0;
+                // most 16 Schur pairs (a lane walks its pairs one after the
+                // other; longer lists are faster in the wave form, which
+                // fetches them 64 at a time), 1x1 pivots in the block row
+                t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= 16 && all1) ? 1 : 0;
             }
         }
         upload(d_off_task, task);

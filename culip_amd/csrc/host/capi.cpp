@@ -248,11 +248,10 @@ culip_status culip_mm_read_vector(const char* path, int64_t* len, double** vals)
         int64_t m, n;
         { std::istringstream ss(lines.at(0)); ss >> m >> n; }
         const int64_t k = std::max(m, n);
+        CULIP_REQUIRE(k >= 0 && (size_t) k + 1 <= lines.size(), "mm: truncated vector");
         double* v = (double*) std::malloc(sizeof(double) * (size_t) std::max<int64_t>(1, k));
-        for(int64_t q = 0; q < k; ++q) {
-            CULIP_REQUIRE((size_t) q + 1 < lines.size(), "mm: truncated vector");
-            v[q] = std::strtod(lines[q + 1].c_str(), nullptr);
-        }
+        if(!v) throw std::bad_alloc();
+        for(int64_t q = 0; q < k; ++q) v[q] = std::strtod(lines[q + 1].c_str(), nullptr);
         *len = k;
         *vals = v;
     });
@@ -267,6 +266,57 @@ culip_status culip_host_matrix_view(const culip_host_matrix* a, culip_csr* out)
 }
 
 void culip_host_matrix_destroy(culip_host_matrix* a) { delete a; }
+
+culip_status culip_csr_permute_scale(const culip_csr* A, const int32_t* row_perm,
+    const int32_t* col_perm, int32_t perm_is_old_to_new, const double* row_scale,
+    const double* col_scale, int32_t scale_before_match, culip_host_matrix** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out && row_perm && col_perm, "permute: null argument");
+        HostCsr a = to_host(A);
+        CULIP_REQUIRE(a.m == a.n, "permute: matrix must be square");
+        const int32_t n = a.m;
+        // old -> new maps (k_inv_perm inverts a new -> old vector)
+        auto to_new = [&](const int32_t* p) {
+            std::vector<int32_t> q(n), seen(n, 0);
+            for(int32_t i = 0; i < n; ++i) {
+                CULIP_REQUIRE(p[i] >= 0 && p[i] < n && !seen[p[i]], "permute: not a permutation");
+                seen[p[i]] = 1;
+            }
+            for(int32_t i = 0; i < n; ++i) {
+                if(perm_is_old_to_new) q[i] = p[i];
+                else q[p[i]] = i;
+            }
+            return q;
+        };
+        const std::vector<int32_t> pr = to_new(row_perm), pc = to_new(col_perm);
+        struct E { int32_t r, c; double v; };
+        std::vector<E> es;
+        es.reserve((size_t) a.nnz());
+        for(int32_t r = 0; r < n; ++r)
+            for(int32_t q = a.row_ptr[r]; q < a.row_ptr[r + 1]; ++q) {
+                const int32_t c = a.col[q], nr = pr[r], nc = pc[c];
+                const double sr = row_scale ? row_scale[scale_before_match ? r : nr] : 1.0;
+                const double sc = col_scale ? col_scale[scale_before_match ? c : nc] : 1.0;
+                es.push_back({nr, nc, a.val[q] / (sr * sc)});
+            }
+        std::stable_sort(es.begin(), es.end(),
+            [](const E& x, const E& y) { return x.r < y.r || (x.r == y.r && x.c < y.c); });
+        auto* h = new culip_host_matrix;
+        HostCsr& b = h->a;
+        b.m = b.n = n;
+        b.row_ptr.assign(n + 1, 0);
+        b.col.resize(es.size());
+        b.val.resize(es.size());
+        for(size_t q = 0; q < es.size(); ++q) {
+            ++b.row_ptr[es[q].r + 1];
+            b.col[q] = es[q].c;
+            b.val[q] = es[q].v;
+        }
+        for(int32_t i = 0; i < n; ++i) b.row_ptr[i + 1] += b.row_ptr[i];
+        *out = h;
+    });
+}
 void culip_host_free(void* p) { std::free(p); }
 
 // ---- BlockiLDLt -------------------------------------------------------------

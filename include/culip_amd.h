@@ -109,6 +109,17 @@ culip_status culip_host_matrix_view(const culip_host_matrix * a, culip_csr * out
 void culip_host_matrix_destroy(culip_host_matrix * a);
 void culip_host_free(void * p);
 
+/* T_matrix_permute_scale (libs/la/sparse_la.impl.cuh:600-657), as the
+ * reference driver's host_permute uses it (block_ildlt_invariants.h:29-69):
+ * entry (r, c) of A moves to (P_r(r), P_c(c)) and is divided by
+ * s_r[r] s_c[c] (scale_before_match) or s_r[P_r(r)] s_c[P_c(c)]; with
+ * perm_is_old_to_new = 0 the vectors are new -> old, so B(i, j) =
+ * A(p_r[i], p_c[j]) / scale. Scales may be NULL (1); rows of the result are
+ * sorted by column. Host arrays in, a host matrix out. */
+culip_status culip_csr_permute_scale(const culip_csr * A, const int32_t * row_perm,
+    const int32_t * col_perm, int32_t perm_is_old_to_new, const double * row_scale,
+    const double * col_scale, int32_t scale_before_match, culip_host_matrix ** out);
+
 /* ------------------------------------------------------------------------ */
 /* BlockiLDLt (libs/la/preconditioner/block_ildlt.cuh:188-311).            */
 

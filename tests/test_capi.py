@@ -172,3 +172,36 @@ def test_imported_factor_checks_inputs_before_the_device():
     with pytest.raises(ca.CulipError) as e:
         ca.ImportLDUPreconditioner(L)
     assert e.value.status == 8
+
+
+def test_permute_scale_matches_the_reference_convention():
+    """culip_csr_permute_scale with a new -> old vector gives B(i, j) =
+    A(p[i], p[j]) (host_permute with perm_is_old_to_new = false,
+    block_ildlt_invariants.h:29-69), columns sorted; an old -> new vector
+    is its inverse; scales divide entries by s_r s_c of the new (or, before
+    matching, the old) indices. Checked against scipy."""
+    rng = np.random.default_rng(2)
+    n = 30
+    A = sp.random(n, n, density=0.2, random_state=3, format="csr")
+    A = (A + A.T).tocsr()
+    A.sort_indices()
+    p = rng.permutation(n).astype(np.int32)
+    rp, ci, cv = ca.permute_scale(A.indptr, A.indices, A.data, p)
+    B = sp.csr_matrix((cv, ci, rp), shape=(n, n)).toarray()
+    np.testing.assert_array_equal(B, A.toarray()[np.ix_(p, p)])
+    assert all((np.diff(ci[rp[i]:rp[i + 1]]) > 0).all() for i in range(n))
+    inv = np.argsort(p).astype(np.int32)
+    rp2, ci2, cv2 = ca.permute_scale(A.indptr, A.indices, A.data, inv, perm_is_old_to_new=True)
+    np.testing.assert_array_equal(rp2, rp)
+    np.testing.assert_array_equal(ci2, ci)
+    np.testing.assert_array_equal(cv2, cv)
+    s = rng.uniform(0.5, 2.0, n)
+    rp3, ci3, cv3 = ca.permute_scale(A.indptr, A.indices, A.data, p, row_scale=s, col_scale=s)
+    B3 = sp.csr_matrix((cv3, ci3, rp3), shape=(n, n)).toarray()
+    np.testing.assert_allclose(B3, A.toarray()[np.ix_(p, p)] / np.outer(s, s), rtol=1e-15)
+    rp4, ci4, cv4 = ca.permute_scale(A.indptr, A.indices, A.data, p, row_scale=s, col_scale=s,
+                                     scale_before_match=True)
+    B4 = sp.csr_matrix((cv4, ci4, rp4), shape=(n, n)).toarray()
+    np.testing.assert_allclose(B4, (A.toarray() / np.outer(s, s))[np.ix_(p, p)], rtol=1e-15)
+    with pytest.raises(ca.CulipError):
+        ca.permute_scale(A.indptr, A.indices, A.data, np.zeros(n, np.int32))
