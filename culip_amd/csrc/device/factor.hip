@@ -1213,12 +1213,40 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
                 }
             }
         }
+        const bool one_one = mc.nnz_i == 1 && mc.nnz_j == 1;
         rc = rn;
         mc = m1;
         m1 = m2;
         pc = p1;
         p1 = p2;
         p2 = p3;
+        if(one_one) {
+            // one entry on each side (nearly every pair at level-0 fill):
+            // the general form below reduces to one product for the single
+            // element (column of U_ki's entry, column of U_kj's entry), with
+            // the same operations
+            if(rj[0] < 64) {
+                const int l = rl[0];
+                const double* td = a.tri + 3 * (int64_t) (rb + l);
+                const VT sup = (VT) td[0], dia = (VT) td[1], sub = (VT) td[2];
+                const VT left = (al[0] == l - 1) ? av[0] : (VT) 0;
+                const VT mid = (al[0] == l) ? av[0] : (VT) 0;
+                const VT right = (al[0] == l + 1) ? av[0] : (VT) 0;
+                const VT lt = fma(sub, right, fma(dia, mid, sup * left));
+                const VT acc = fma(rv[0], lt, (VT) 0);
+                const int idx = 32 * ac[0] + rj[0];
+                int k = find(idx);
+                if(k < 0) {
+                    if(acc == (VT) 0) continue;
+                    if(n == TINY_T) { to_wave_form(a, lr, q); return; }
+                    k = n++;
+                    I(k) = (int16_t) idx;
+                    V(k) = 0;
+                }
+                V(k) -= acc;
+            }
+            continue;
+        }
         int key[TINY_E], rank[TINY_E];
 #pragma unroll
         for(int e = 0; e < TINY_E; ++e) key[e] = rj[e] * 32 + rl[e];
@@ -1379,34 +1407,18 @@ __global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
     VT acc = a.rn[fr + lane];
     if(nblk > 0) {
         const VT* part = a.rn_part + (int64_t) (lr.npiv + a.rfo[i]) * 32 + lane;
+        // sixteen loads in flight per pass; the sum stays in block order
         int t = 0;
-        for(; t + 4 <= nblk; t += 4) {
-            const VT v0 = part[32 * (t + 0)], v1 = part[32 * (t + 1)];
-            const VT v2 = part[32 * (t + 2)], v3 = part[32 * (t + 3)];
-            acc += v0;
-            acc += v1;
-            acc += v2;
-            acc += v3;
+        for(; t + 16 <= nblk; t += 16) {
+            VT v[16];
+#pragma unroll
+            for(int u = 0; u < 16; ++u) v[u] = part[32 * (t + u)];
+#pragma unroll
+            for(int u = 0; u < 16; ++u) acc += v[u];
         }
         for(; t < nblk; ++t) acc += part[32 * t];
     }
     a.rn[fr + lane] = acc;
-}
-
-// entries of a sparse position that survive |v| >= tau * sqrtf(norm)
-// (kernels :1919-1946)
-template<typename VT>
-__device__ int drop_survivors(const FactorArgs<VT>& a, int q, int fr)
-{
-    const int cnt = a.if_cnt[q];
-    const VT* vals = a.if_val + (int64_t) q * 1024;
-    const uint16_t* ixs = a.if_idx + (int64_t) q * 1024;
-    int keep = 0;
-    for(int e = 0; e < cnt; ++e) {
-        const VT nrm = drop_norm(a.rn[fr + (ixs[e] >> 5)]);
-        keep += !(fabs(vals[e]) < a.threshold * nrm);
-    }
-    return keep;
 }
 
 // Leftover pool, granted extra space and output offsets of a level's blocks
@@ -1471,17 +1483,60 @@ __global__ __launch_bounds__(SC_TPB) void k_level_maps(FactorArgs<VT> a, LevelRa
 {
     __shared__ PoolMap sh[SC_NW];
     const int c = blockIdx.x;
+    // entries of a sparse position that survive |v| >= tau * sqrtf(norm)
+    // (kernels :1919-1946); the thread's positions are read in three rounds
+    // of independent loads
+    // (capacity / first row / entry count, then each block's first entry,
+    // then that entry's row norm) instead of one dependent chain per
+    // position; further entries (rare at level-0 fill) are read in order
+    int capk[SC_IPT], frk[SC_IPT], cntk[SC_IPT];
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        capk[k] = CAP_NONE;
+        frk[k] = 0;
+        cntk[k] = 0;
+        if(q < lr.size) {
+            capk[k] = a.lq_cap[lr.lo + q];
+            frk[k] = a.lq_fr[lr.lo + q];
+            cntk[k] = a.if_cnt[q];
+        }
+    }
+    int ix0[SC_IPT];
+    VT v0[SC_IPT];
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        ix0[k] = 0;
+        v0[k] = (VT) 0;
+        if(capk[k] >= 0 && cntk[k] > 0) {
+            ix0[k] = a.if_idx[(int64_t) q * 1024];
+            v0[k] = a.if_val[(int64_t) q * 1024];
+        }
+    }
+    VT n0[SC_IPT];
+#pragma unroll
+    for(int k = 0; k < SC_IPT; ++k) {
+        n0[k] = (VT) 0;
+        if(capk[k] >= 0 && cntk[k] > 0) n0[k] = a.rn[frk[k] + (ix0[k] >> 5)];
+    }
     PoolMap m{0, 0};
 #pragma unroll
     for(int k = 0; k < SC_IPT; ++k) {
         const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
-        if(q < lr.size) {
-            const int cap = a.lq_cap[lr.lo + q];
-            if(cap >= 0) {
-                const int cnt = drop_survivors(a, q, a.lq_fr[lr.lo + q]);
-                a.drop_cnt[q] = cnt;
-                m = MapOp()(m, PoolMap{(int64_t) cap - cnt, 0});
+        if(capk[k] >= 0) {
+            int keep = 0;
+            if(cntk[k] > 0) {
+                keep = !(fabs(v0[k]) < a.threshold * drop_norm(n0[k]));
+                const VT* vals = a.if_val + (int64_t) q * 1024;
+                const uint16_t* ixs = a.if_idx + (int64_t) q * 1024;
+                for(int e = 1; e < cntk[k]; ++e) {
+                    const VT nrm = drop_norm(a.rn[frk[k] + (ixs[e] >> 5)]);
+                    keep += !(fabs(vals[e]) < a.threshold * nrm);
+                }
             }
+            a.drop_cnt[q] = keep;
+            m = MapOp()(m, PoolMap{(int64_t) capk[k] - keep, 0});
         }
     }
     PoolMap tot;

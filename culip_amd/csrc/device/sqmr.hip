@@ -549,8 +549,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
                 // drift: the recurrence's own residual estimate has fallen a
                 // thousandfold since the true residual last halved
                 const bool drifted = stall > 0 && gam < 1e-3 * gam_at_best;
-                const bool far_enough = ops.restart_rel > 0 && nr <= ops.restart_rel * nbn;
-                if(w.host_flags[0] || stall >= STALL_PERIODS || drifted || far_enough) {
+                if(w.host_flags[0] || stall >= STALL_PERIODS || drifted) {
                     ++it;
                     break;
                 }

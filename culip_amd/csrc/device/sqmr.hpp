@@ -15,11 +15,6 @@ struct SqmrOps {
     std::function<void(double* x, void* stream)> right;                   // x = Pr \ x (NULL: I)
     const double* tri = nullptr;      // middle D (NULL: identity)
     const int32_t* is_piv = nullptr;
-    // true-residual mode: a run hands over to a refinement restart once its
-    // true residual has dropped by this factor (0: only on breakdown or
-    // stagnation). Set for a low-precision preconditioner, whose rounding
-    // limits how far one recurrence can go.
-    double restart_rel = 0.0;
 };
 
 // Device work space of one solve (allocated by the caller of sqmr_solve).
