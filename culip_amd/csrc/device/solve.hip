@@ -328,41 +328,61 @@ constexpr int TRSV_PF = 4;   // 64-entry chunks whose loads are issued together
 // one level: one wave per block row; y = x_i - sum(off-diag) ; x_i = Uinv y.
 // All entry loads of up to TRSV_PF chunks are issued before the x gathers;
 // the chunks' segmented sums are then applied to y in chunk order.
+// One block row of U x = b: x_i = U_ii^-1 (b_i - sum_j U_ij x_j) with FP64
+// sums, the off-diagonal products summed in entry order by segmented scans
+// over 64-entry chunks, then the rows of U_ii^-1. Every load that does not
+// depend on another block row's result is issued before the first wait (b_i,
+// the first 64 entries of U_ii^-1, the first off-diagonal chunk and, once its
+// columns are in, its gathers of x), so a level costs two dependent memory
+// round trips.
 template<typename VT>
-__global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32_t* o_col,
-    const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
-    const uint8_t* i_lr, double* x)
+__device__ __forceinline__ void block_row_solve(int4 t0, int4 t1, bool offdiag,
+    const int32_t* o_col, const VT* o_val, const uint8_t* o_lr, const int32_t* i_col,
+    const VT* i_val, const uint8_t* i_lr, const double* b, const double* xsrc, double* xout,
+    double* y, double* res)
 {
-    __shared__ double y[33];     // FP64 accumulation for both factor precisions
-    __shared__ double res[33];
     const int lane = lane_id();
-    const int4 t0 = task[2 * blockIdx.x];
-    const int4 t1 = task[2 * blockIdx.x + 1];
-    const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
-    const double xo = (lane < h) ? x[fr + lane] : 0.0;
-    int icol = fr;
-    VT ival = 0;
-    int isg = 64;
-    if(ib + lane < ie) {
-        icol = i_col[ib + lane];
-        ival = i_val[ib + lane];
-        isg = i_lr[ib + lane];
-    }
-    if(lane < h) { y[lane] = xo; res[lane] = 0; }
-    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
-        int c[TRSV_PF], sg[TRSV_PF];
-        VT v[TRSV_PF];
+    const int fr = t0.x, h = t0.y, ob = t0.z, oe = offdiag ? t0.w : t0.z, ib = t1.x, ie = t1.y;
+    // loads run from clamped (in-range) indices without branches, and the
+    // lanes past the end are masked afterwards, so the compiler issues them
+    // all before the first wait
+    const double bo_raw = b[fr + min(lane, h - 1)];
+    const double bo = (lane < h) ? bo_raw : 0.0;
+    // the first 64 entries of U_ii^-1 (every block row has at least its
+    // diagonal there), selected only where they are used
+    const int ei = ib + min(lane, max(ie - ib - 1, 0));
+    const int ic0 = i_col[ei];
+    const VT iv0 = i_val[ei];
+    const int il0 = i_lr[ei];
+    int c[TRSV_PF], sg[TRSV_PF];
+    VT v[TRSV_PF];
+    double xv[TRSV_PF];
+    auto load = [&](int e0) {
+        int cr[TRSV_PF], sr[TRSV_PF];
+        VT vr[TRSV_PF];
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
-            const int e = e0 + 64 * k + lane;
-            c[k] = 0;
-            v[k] = 0;
-            sg[k] = 64;
-            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
+            const int e = min(e0 + 64 * k + lane, oe - 1);
+            cr[k] = o_col[e];
+            vr[k] = o_val[e];
+            sr[k] = o_lr[e];
         }
-        double xv[TRSV_PF];
+        double xr[TRSV_PF];
 #pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? x[c[k]] : 0.0;
+        for(int k = 0; k < TRSV_PF; ++k) xr[k] = xsrc[cr[k]];
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            const bool in = e0 + 64 * k + lane < oe;
+            c[k] = in ? cr[k] : 0;
+            v[k] = in ? vr[k] : (VT) 0;
+            sg[k] = in ? sr[k] : 64;
+            xv[k] = in ? xr[k] : 0.0;
+        }
+    };
+    if(ob < oe) load(ob);
+    if(lane < h) { y[lane] = bo; res[lane] = 0; }
+    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
+        if(e0 != ob) load(e0);
         wave_sync();
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
@@ -374,6 +394,10 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
         }
     }
     wave_sync();
+    const bool in0 = ib + lane < ie;
+    int icol = in0 ? ic0 : fr;
+    VT ival = in0 ? iv0 : (VT) 0;
+    int isg = in0 ? il0 : 64;
     for(int e0 = ib; e0 < ie; e0 += 64) {
         if(e0 != ib) {
             const int e = e0 + lane;
@@ -386,7 +410,18 @@ __global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32
         if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
         wave_sync();
     }
-    if(lane < h) x[fr + lane] = (double) res[lane];
+    if(lane < h) xout[fr + lane] = (double) res[lane];
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_trsv_level(const int4* task, const int32_t* o_col,
+    const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
+    const uint8_t* i_lr, double* x)
+{
+    __shared__ double y[33];     // FP64 accumulation for both factor precisions
+    __shared__ double res[33];
+    block_row_solve<VT>(task[2 * blockIdx.x], task[2 * blockIdx.x + 1], true, o_col, o_val,
+        o_lr, i_col, i_val, i_lr, x, x, x, y, res);
 }
 
 // One block-Jacobi sweep over all block rows (solve_U_jacobi /
@@ -402,58 +437,8 @@ __global__ __launch_bounds__(64) void k_jacobi_sweep(const int4* task, const int
 {
     __shared__ double y[33];
     __shared__ double res[33];
-    const int lane = lane_id();
-    const int4 t0 = task[2 * blockIdx.x];
-    const int4 t1 = task[2 * blockIdx.x + 1];
-    const int fr = t0.x, h = t0.y, ob = t0.z, oe = xin ? t0.w : t0.z, ib = t1.x, ie = t1.y;
-    const double bo = (lane < h) ? b[fr + lane] : 0.0;
-    int icol = fr;
-    VT ival = 0;
-    int isg = 64;
-    if(ib + lane < ie) {
-        icol = i_col[ib + lane];
-        ival = i_val[ib + lane];
-        isg = i_lr[ib + lane];
-    }
-    if(lane < h) { y[lane] = bo; res[lane] = 0; }
-    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
-        int c[TRSV_PF], sg[TRSV_PF];
-        VT v[TRSV_PF];
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) {
-            const int e = e0 + 64 * k + lane;
-            c[k] = 0;
-            v[k] = 0;
-            sg[k] = 64;
-            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
-        }
-        double xv[TRSV_PF];
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) xv[k] = (sg[k] < 64) ? xin[c[k]] : 0.0;
-        wave_sync();
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) {
-            if(e0 + 64 * k >= oe) break;
-            double p = seg_scan((double) v[k] * xv[k], sg[k]);
-            const int nsg = __shfl_down(sg[k], 1, 64);
-            if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
-            wave_sync();
-        }
-    }
-    wave_sync();
-    for(int e0 = ib; e0 < ie; e0 += 64) {
-        if(e0 != ib) {
-            const int e = e0 + lane;
-            isg = 64;
-            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
-        }
-        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-        p = seg_scan(p, isg);
-        const int nsg = __shfl_down(isg, 1, 64);
-        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
-        wave_sync();
-    }
-    if(lane < h) xout[fr + lane] = res[lane];
+    block_row_solve<VT>(task[2 * blockIdx.x], task[2 * blockIdx.x + 1], xin != nullptr, o_col,
+        o_val, o_lr, i_col, i_val, i_lr, b, xin, xout, y, res);
 }
 
 // Whole triangular solve in one launch: workgroup w takes the block row at

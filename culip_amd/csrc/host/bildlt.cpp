@@ -289,7 +289,8 @@ void Bildlt::factor_impl(hipStream_t st)
     HIP_CHECK(hipStreamSynchronize(st));
     for(int q = 0; q < 4; ++q) {
         plan_tot[q] = htot[q];
-        const size_t cnt = (size_t) std::max(1, htot[q]);
+        // 64 entries of slack: the solve kernels read clamped indices
+        const size_t cnt = (size_t) std::max(1, htot[q]) + 64;
         d_col[q].ensure(sizeof(int32_t) * cnt);
         d_val[q].ensure(sizeof(VT) * cnt);
         d_lr[q].ensure(cnt);
@@ -446,7 +447,7 @@ int64_t Bildlt::numeric_bytes_estimate(bool with_layout) const
             const int64_t h = s.block_starts[i + 1] - s.block_starts[i];
             inv += h * h;
         }
-        b += (2 * s.val_space + 2 * inv) * (4 + v + 1);
+        b += (2 * s.val_space + 2 * inv + 4 * 64) * (4 + v + 1);
     }
     if(solve_algo == CULIP_TRSV_JACOBI) b += 3 * 8 * n;   // block-Jacobi vectors
     return b;
