@@ -1057,10 +1057,11 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
 // One lane per off-diagonal block for small blocks (at level-0 fill of the
 // KKT workloads: 1-2 entries each; a wave then works on 64 blocks instead of
 // one). Static eligibility (task bit, set by the host): sparse input with at
-// most TINY_E entries, at most 16 Schur pairs, 1x1 pivots in the block row. At run time every dependency
-// must be sparse with at most TINY_E entries and the tile must stay within
-// TINY_T nonzeros; otherwise the position goes to the wave form before
-// anything is written. The tile is a list of (row * 32 + col, value) in LDS,
+// most TINY_E entries, at most 8 Schur pairs (16 at a level with 8192 or
+// more off-diagonal blocks), 1x1 pivots in the block row.
+// At run time every dependency must be sparse with at most TINY_E entries
+// and the tile must stay within TINY_T nonzeros; otherwise the position goes
+// to the wave form before anything is written. The tile is a list of (row * 32 + col, value) in LDS,
 // one column of the list arrays per lane; every value is formed by the same
 // operations in the same order as in the wave form (terms that are exact
 // zeros in the wave form are skipped).

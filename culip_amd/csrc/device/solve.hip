@@ -68,13 +68,14 @@ template<typename VT>
 struct PlanStage {
     int32_t fmt[64], nnz[64], cp[64], vp[64], off[64], ext[64];
     uint8_t small[64];
-    int16_t ix[64 * PLAN_E];
+    uint8_t rr[64 * PLAN_E], cc[64 * PLAN_E];   // local row and column of each entry
     VT v[64 * PLAN_E];
 };
 
-// off / ext: first fine index and size of the block's other dimension
+// off / ext: first fine index and size of the block's other dimension; w:
+// the block's width (the stride of its entry indices)
 template<typename VT>
-__device__ void plan_stage(PlanStage<VT>& sg, int k, const BlockMeta& m, int off, int ext,
+__device__ void plan_stage(PlanStage<VT>& sg, int k, const BlockMeta& m, int off, int ext, int w,
     const int32_t* ix_out, const VT* val_out)
 {
     sg.fmt[k] = m.format;
@@ -87,7 +88,10 @@ __device__ void plan_stage(PlanStage<VT>& sg, int k, const BlockMeta& m, int off
     sg.small[k] = sm ? 1 : 0;
     if(sm)
         for(int e = 0; e < m.nnz; ++e) {
-            sg.ix[PLAN_E * k + e] = (int16_t) ix_out[(int64_t) m.col_ptr + e];
+            const int q = ix_out[(int64_t) m.col_ptr + e];
+            const int r = q / w;
+            sg.rr[PLAN_E * k + e] = (uint8_t) r;
+            sg.cc[PLAN_E * k + e] = (uint8_t) (q - r * w);
             sg.v[PLAN_E * k + e] = val_out[(int64_t) m.val_ptr + e];
         }
 }
@@ -133,18 +137,16 @@ __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out
         wave_sync();
         if(lane < nblk) {
             const BlockMeta m = t.bout[c0 + lane];
-            plan_stage(sg, lane, m, t.bs[m.block_col], t.bs[m.block_col + 1] - t.bs[m.block_col],
-                t.ix_out, val_out);
+            const int wb = t.bs[m.block_col + 1] - t.bs[m.block_col];
+            plan_stage(sg, lane, m, t.bs[m.block_col], wb, wb, t.ix_out, val_out);
         }
         wave_sync();
         for(int k = 0; k < nblk; ++k) {
             const int fc = sg.off[k], w = sg.ext[k];
             if(sg.small[k]) {
                 for(int e = 0; e < sg.nnz[k]; ++e) {
-                    const int q = sg.ix[PLAN_E * k + e];
-                    const int r = q / w;
-                    if(r == lane) {
-                        if(FILL) { p.uo_col[base + cnt] = fc + (q - r * w); p.uo_val[base + cnt] = sg.v[PLAN_E * k + e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
+                    if(sg.rr[PLAN_E * k + e] == lane) {
+                        if(FILL) { p.uo_col[base + cnt] = fc + sg.cc[PLAN_E * k + e]; p.uo_val[base + cnt] = sg.v[PLAN_E * k + e]; p.uo_lr[base + cnt] = (uint8_t) lane; }
                         ++cnt;
                     }
                 }
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out
         wave_sync();
         if(lane < nblk) {
             const BlockMeta m = t.bout[t.csc_idx[c0 + lane]];
-            plan_stage(sg, lane, m, t.bs[m.block_row], t.bs[m.block_row + 1] - t.bs[m.block_row],
+            plan_stage(sg, lane, m, t.bs[m.block_row], t.bs[m.block_row + 1] - t.bs[m.block_row], w,
                 t.ix_out, val_out);
         }
         wave_sync();
@@ -201,10 +203,8 @@ __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out
             const int fr = sg.off[k], h = sg.ext[k];
             if(sg.small[k]) {
                 for(int e = 0; e < sg.nnz[k]; ++e) {
-                    const int qq = sg.ix[PLAN_E * k + e];
-                    const int r = qq / w;
-                    if(qq - r * w == lane) {
-                        if(FILL) { p.to_col[base + cnt] = fr + r; p.to_val[base + cnt] = sg.v[PLAN_E * k + e]; p.to_lr[base + cnt] = (uint8_t) lane; }
+                    if(sg.cc[PLAN_E * k + e] == lane) {
+                        if(FILL) { p.to_col[base + cnt] = fr + sg.rr[PLAN_E * k + e]; p.to_val[base + cnt] = sg.v[PLAN_E * k + e]; p.to_lr[base + cnt] = (uint8_t) lane; }
                         ++cnt;
                     }
                 }

@@ -8,6 +8,17 @@
 
 namespace culip {
 
+// Schur pairs up to which an off-diagonal block takes the one-lane factor
+// kernel. At a level with few off-diagonal blocks the one-lane kernel's time
+// is set by its longest list, and a block with more than TINY_PAIRS pairs is
+// faster in the wave form, which fetches its pairs 64 at a time. At a level
+// with TINY_WIDE_LEVEL or more the one-lane kernel is bound by throughput and
+// the wave form by the number of waves the chip holds, so lists up to
+// TINY_PAIRS_WIDE stay on the one-lane kernel.
+constexpr int32_t TINY_PAIRS = 8;
+constexpr int32_t TINY_PAIRS_WIDE = 16;
+constexpr int32_t TINY_WIDE_LEVEL = 8192;
+
 void hip_check(hipError_t e, const char* what)
 {
     if(e != hipSuccess) fail(CULIP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -79,6 +90,7 @@ void BildltPattern::upload_static()
         for(int32_t l = 0; l < s.num_levels; ++l) {
             const int32_t lo = s.lvlb_off[l], hi = s.lvlb_off[l + 1];
             const int32_t npiv = s.lvl_off[l + 1] - s.lvl_off[l];
+            const int32_t lim = (hi - lo - npiv >= TINY_WIDE_LEVEL) ? TINY_PAIRS_WIDE : TINY_PAIRS;
             for(int32_t p = lo + npiv; p < hi; ++p) {
                 const int32_t b = s.lvlb_idx[p];
                 const BlockMeta& m = s.blocks[b];
@@ -96,11 +108,9 @@ void BildltPattern::upload_static()
                 t[4] = m.format; t[5] = m.nnz; t[6] = m.col_ptr; t[7] = m.val_ptr;
                 t[8] = (int32_t) s.upd_off[b];
                 t[9] = (int32_t) (s.upd_off[b + 1] - s.upd_off[b]);
-                // one-lane kernel: sparse input of at most 8 entries, at
-                // most 16 Schur pairs (a lane walks its pairs one after the
-                // other; longer lists are faster in the wave form, which
-                // fetches them 64 at a time), 1x1 pivots in the block row
-                t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= 16 && all1) ? 1 : 0;
+                // one-lane kernel: sparse input of at most 8 entries, a
+                // short Schur pair list, 1x1 pivots in the block row
+                t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= lim && all1) ? 1 : 0;
             }
         }
         upload(d_off_task, task);
