@@ -62,8 +62,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP32-factor and block-Jacobi sub-runs")
-    ap.add_argument("--solve-algorithm", default="levels",
-                    choices=["levels", "flagged", "jacobi"],
+    ap.add_argument("--solve-algorithm", default="single",
+                    choices=["single", "levels", "jacobi"],
                     help="triangular-solve schedule (set_solve_algorithm)")
     ap.add_argument("--jacobi-sweeps", type=int, default=0)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -79,6 +79,7 @@ PROBE_COPY_DOUBLES = 1 << 25   # 256 MiB read + 256 MiB written
 # x1.000); kept in profiles/kkt400k_hbm_traffic.md. Configurations without
 # such a run report null.
 MEASURED_TRAFFIC = {
+    ("kkt-400k", "f64", "color", "single"): {"k_trsv_chain": 137.0e6, "k_csr_spmv": 269.6e6},
     ("kkt-400k", "f64", "color", "levels"): {"k_trsv_level": 322.1e6, "k_csr_spmv": 269.8e6},
 }
 
@@ -459,9 +460,9 @@ def main():
         first = int(ps[1]) * (sv + 4) + 2 * N * 8 + (N + 1) * 4
         bytes_trsv = first + (a.jacobi_sweeps - 1) * (bytes_trsv + N * 8)
         label = f"k_jacobi_sweep (one U solve, {a.jacobi_sweeps} sweeps)"
-    elif a.solve_algorithm == "flagged":
+    elif a.solve_algorithm == "single":
         launches = 1
-        label = "k_trsv_flagged (one U solve)"
+        label = "k_trsv_chain (one U solve)"
     rl_trsv = {"bound": "hbm", "kernel": label,
                "achieved": bytes_trsv / ms_trsv / 1e6, "peak": peak, "unit": "GB/s",
                "frac": bytes_trsv / ms_trsv / 1e6 / peak, "traffic": None,
@@ -469,7 +470,7 @@ def main():
                "launches": launches}
     tr = MEASURED_TRAFFIC.get((a.workload, a.prec, a.order, a.solve_algorithm))
     if tr is not None:
-        rl_trsv["traffic"] = tr["k_trsv_level"]
+        rl_trsv["traffic"] = tr[label.split()[0]]
         rl_spmv["traffic"] = tr["k_csr_spmv"]
         for r in (rl_trsv, rl_spmv):
             r["traffic_source"] = "profiles/kkt400k_hbm_traffic.md"
@@ -504,7 +505,7 @@ def main():
     other = jac = None
     if a.prec == "f64" and not a.no_variants:
         other = variant("f32", a.solve_algorithm, a.jacobi_sweeps)
-        if a.solve_algorithm == "levels":
+        if a.solve_algorithm in ("single", "levels"):
             log("variant: jacobi solves, 2 sweeps")
             jac = variant("f64", "jacobi", 2)
 

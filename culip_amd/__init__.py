@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT,
-                   ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_FLAGGED, TRSV_JACOBI,
+                   ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_SINGLE, TRSV_JACOBI,
                    check, lib)
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
@@ -37,8 +37,8 @@ def _ptr(a, ct):
 
 
 def _solve_algorithm(a):
-    """'levels' | 'flagged' | 'jacobi' (or the CULIP_TRSV_* value)."""
-    return {"levels": TRSV_LEVELS, "flagged": TRSV_FLAGGED, "jacobi": TRSV_JACOBI}.get(a, a)
+    """'single' | 'levels' | 'jacobi' (or the CULIP_TRSV_* value)."""
+    return {"levels": TRSV_LEVELS, "single": TRSV_SINGLE, "jacobi": TRSV_JACOBI}.get(a, a)
 
 
 def csr_struct(row_ptr, col, val, n=None):
@@ -177,9 +177,9 @@ class BlockiLDLt:
                                               threshold, _stream(stream)))
 
     def set_solve_algorithm(self, algorithm, num_jacobi_sweeps=0):
-        """set_solve_algorithm: 'levels' (one launch per level), 'flagged'
-        (one launch per solve) or 'jacobi' (num_jacobi_sweeps block-Jacobi
-        sweeps)."""
+        """set_solve_algorithm: 'single' (default: one launch per solve),
+        'levels' (one launch per level) or 'jacobi' (num_jacobi_sweeps
+        block-Jacobi sweeps)."""
         check(lib().culip_bildlt_set_solve_algorithm(self._h, _solve_algorithm(algorithm),
                                                      num_jacobi_sweeps))
 

@@ -16,7 +16,7 @@ STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOMEM", 4: "STATE", 5: "NUMERIC",
 PREC_F64, PREC_F32 = 0, 1
 SOLVE_LEFT, SOLVE_MIDDLE, SOLVE_RIGHT = 0, 1, 2
 ORDER_NATURAL, ORDER_BLOCK_COLOR = 0, 1
-TRSV_LEVELS, TRSV_FLAGGED, TRSV_JACOBI = 0, 1, 2
+TRSV_LEVELS, TRSV_SINGLE, TRSV_JACOBI = 0, 1, 2
 
 
 class CulipError(RuntimeError):

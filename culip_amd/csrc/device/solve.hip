@@ -296,20 +296,6 @@ __global__ __launch_bounds__(PS_TPB) void k_chunk_scan(int32_t* a, int32_t n, co
     }
 }
 
-// segmented inclusive sum over lanes; segments are runs of equal sg
-template<typename T>
-__device__ __forceinline__ T seg_scan(T v, int sg)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for(int o = 1; o < 64; o <<= 1) {
-        const T u = __shfl_up(v, o, 64);
-        const int su = __shfl_up(sg, o, 64);
-        if(lane >= o && su == sg) v += u;
-    }
-    return v;
-}
-
 // task descriptors of the block rows in level order, two int4 each:
 // {fr, h, off-diagonal begin, end}, {inverse begin, end, 0, 0}
 __global__ void k_plan_tasks(int32_t nb, const int32_t* lvl_rows, const int32_t* bs,
@@ -387,8 +373,8 @@ __device__ __forceinline__ void block_row_solve(int4 t0, int4 t1, bool offdiag,
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
             if(e0 + 64 * k >= oe) break;
-            double p = seg_scan((double) v[k] * xv[k], sg[k]);
-            const int nsg = __shfl_down(sg[k], 1, 64);
+            double p = seg_scan_runs((double) v[k] * xv[k], sg[k]);
+            const int nsg = next_lane(sg[k], -1);
             if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
             wave_sync();
         }
@@ -405,8 +391,8 @@ __device__ __forceinline__ void block_row_solve(int4 t0, int4 t1, bool offdiag,
             if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
         }
         double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-        p = seg_scan(p, isg);
-        const int nsg = __shfl_down(isg, 1, 64);
+        p = seg_scan_runs(p, isg);
+        const int nsg = next_lane(isg, -1);
         if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
         wave_sync();
     }
@@ -441,108 +427,202 @@ __global__ __launch_bounds__(64) void k_jacobi_sweep(const int4* task, const int
         o_val, o_lr, i_col, i_val, i_lr, b, xin, xout, y, res);
 }
 
-// Whole triangular solve in one launch: workgroup w takes the block row at
-// position w of the level order (position nb - 1 - w for the U solve), so
-// every block row it depends on has a smaller workgroup index. Before it
-// reads x of another block row it waits for that row's flag to carry this
-// solve's epoch; after writing its own rows it publishes its flag with
-// release ordering at agent scope (the L2 of each XCD is separate).
-// Workgroups are dispatched in index order, so the lowest unfinished one
-// always has its inputs ready and the solve drains. Every wait is bounded:
-// past SPIN_LIMIT polls the wave records an error in *err and goes on.
+// Whole triangular solve in one launch of G resident waves:
+// wave g takes the block rows at processing indices g, g + G, g + 2G, ... of
+// the level order (reverse level order for the U solve), so every block row
+// it depends on has a smaller index. The lowest unfinished index always has
+// its inputs published and every wave is resident, so the solve drains.
+// Results are published value by value: before the solve every entry of pub
+// holds CHAIN_EMPTY, a signalling NaN that no arithmetic produces, and a
+// block row writes its x values there with 8-byte agent-coherent stores (the
+// L2 of each XCD is separate). A reader loads the values it needs with
+// agent-coherent loads and takes each one once it differs from CHAIN_EMPTY,
+// reloading the missing ones until all are there; no flag and no
+// store-completion wait sits between a producer and its readers. While a
+// block row waits and computes, the wave already loads the task and first
+// entries of its next one. Each wave also presets its rows of next_pub, the
+// buffer of the next single-launch solve. Every wait is bounded: past
+// SPIN_LIMIT polls the wave records an error in *err and goes on.
 constexpr int SPIN_LIMIT = 1 << 22;
+constexpr unsigned long long CHAIN_EMPTY = 0x7ff4c0de5eed1234ull;
 
-__device__ __forceinline__ void wait_flag(const int32_t* f, int32_t epoch, int32_t* err)
+__device__ __forceinline__ double pub_load(const double* p)
 {
-    int spins = 0;
-    while(__hip_atomic_load(const_cast<int32_t*>(f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-        epoch) {
-        __builtin_amdgcn_s_sleep(2);
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool published(double v)
+{
+    return (unsigned long long) __double_as_longlong(v) != CHAIN_EMPTY;
+}
+
+// waits until every lane's value xv[k] (column c[k]) is published; rdy[k]
+// on entry: xv[k] already usable
+__device__ void wait_published(const double* pub, const int (&c)[TRSV_PF], double (&xv)[TRSV_PF],
+    bool (&rdy)[TRSV_PF], int32_t* err, int& spins)
+{
+    bool all = true;
+#pragma unroll
+    for(int k = 0; k < TRSV_PF; ++k) all = all && rdy[k];
+    while(wave_ballot(!all)) {
+        __builtin_amdgcn_s_sleep(1);
         if(++spins > SPIN_LIMIT) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
+            return;
+        }
+        all = true;
+#pragma unroll
+        for(int k = 0; k < TRSV_PF; ++k) {
+            if(!rdy[k]) {
+                xv[k] = pub_load(pub + c[k]);
+                rdy[k] = published(xv[k]);
+            }
+            all = all && rdy[k];
         }
     }
 }
 
+// what a block row loads before it can wait: its first TRSV_PF chunks of
+// off-diagonal entries (clamped in range), its own right-hand side and the
+// first 64 entries of U_ii^-1
 template<typename VT>
-__global__ __launch_bounds__(64) void k_trsv_flagged(const int4* task, int32_t nb, int32_t reverse,
+struct ChainHead {
+    int c[TRSV_PF], sg[TRSV_PF];
+    VT v[TRSV_PF];
+    double bo;
+    int ic, il;
+    VT iv;
+};
+
+template<typename VT>
+__device__ __forceinline__ void chain_fetch(int4 t0, int4 t1, const int32_t* o_col, const VT* o_val,
+    const uint8_t* o_lr, const int32_t* i_col, const VT* i_val, const uint8_t* i_lr,
+    const double* x, ChainHead<VT>& f)
+{
+    const int lane = lane_id();
+    const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
+#pragma unroll
+    for(int k = 0; k < TRSV_PF; ++k) {
+        const int e = max(min(ob + 64 * k + lane, oe - 1), 0);
+        f.c[k] = o_col[e];
+        f.v[k] = o_val[e];
+        f.sg[k] = o_lr[e];
+    }
+    // this block row's own right-hand side: no other wave writes x
+    f.bo = x[fr + min(lane, h - 1)];
+    const int ei = ib + min(lane, max(ie - ib - 1, 0));
+    f.ic = i_col[ei];
+    f.iv = i_val[ei];
+    f.il = i_lr[ei];
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb, int32_t reverse,
     const int32_t* o_col, const VT* o_val, const uint8_t* o_lr, const int32_t* i_col,
-    const VT* i_val, const uint8_t* i_lr, const int32_t* row_block, int32_t* flag, int32_t epoch,
-    int32_t* err, double* x)
+    const VT* i_val, const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x)
 {
     __shared__ double y[33];     // FP64 accumulation for both factor precisions
     __shared__ double res[33];
     const int lane = lane_id();
-    const int pos = reverse ? (nb - 1 - (int) blockIdx.x) : (int) blockIdx.x;
-    const int4 t0 = task[2 * pos];
-    const int4 t1 = task[2 * pos + 1];
-    const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
-    const double xo = (lane < h) ? x[fr + lane] : 0.0;
-    int icol = fr;
-    VT ival = 0;
-    int isg = 64;
-    if(ib + lane < ie) {
-        icol = i_col[ib + lane];
-        ival = i_val[ib + lane];
-        isg = i_lr[ib + lane];
-    }
-    if(lane < h) { y[lane] = xo; res[lane] = 0; }
-    for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
-        int c[TRSV_PF], sg[TRSV_PF];
-        VT v[TRSV_PF];
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) {
-            const int e = e0 + 64 * k + lane;
-            c[k] = 0;
-            v[k] = 0;
-            sg[k] = 64;
-            if(e < oe) { c[k] = o_col[e]; v[k] = o_val[e]; sg[k] = o_lr[e]; }
-        }
-        int dep[TRSV_PF];
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) dep[k] = (sg[k] < 64) ? row_block[c[k]] : -1;
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k)
-            if(dep[k] >= 0) wait_flag(flag + dep[k], epoch, err);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        // x of other block rows: agent-coherent loads (no cache-wide
-        // invalidation per block row)
-        double xv[TRSV_PF];
-#pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k)
-            xv[k] = (sg[k] < 64) ?
-                __hip_atomic_load(x + c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0.0;
+    const int G = gridDim.x;
+    const double empty = __longlong_as_double((long long) CHAIN_EMPTY);
+    auto task_at = [&](int w, int4& a, int4& b) {
+        const int q = min(w, nb - 1);
+        const int pos = reverse ? (nb - 1 - q) : q;
+        a = task[2 * pos];
+        b = task[2 * pos + 1];
+    };
+    int spins = 0;
+    int w = blockIdx.x;
+    int4 t0, t1;
+    task_at(w, t0, t1);
+    ChainHead<VT> cur;
+    chain_fetch<VT>(t0, t1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, cur);
+    for(;;) {
+        int4 n0, n1;
+        task_at(w + G, n0, n1);
+        ChainHead<VT> nxt;
+        bool fetched = false;
+        const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
+        if(lane < h) next_pub[fr + lane] = empty;
         wave_sync();
+        if(lane < h) { y[lane] = cur.bo; res[lane] = 0; }
+        for(int e0 = ob; e0 < oe; e0 += 64 * TRSV_PF) {
+            int c[TRSV_PF], sg[TRSV_PF];
+            VT v[TRSV_PF];
+            if(e0 == ob) {
 #pragma unroll
-        for(int k = 0; k < TRSV_PF; ++k) {
-            if(e0 + 64 * k >= oe) break;
-            double p = seg_scan((double) v[k] * xv[k], sg[k]);
-            const int nsg = __shfl_down(sg[k], 1, 64);
-            if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
+                for(int k = 0; k < TRSV_PF; ++k) { c[k] = cur.c[k]; v[k] = cur.v[k]; sg[k] = cur.sg[k]; }
+            } else {
+#pragma unroll
+                for(int k = 0; k < TRSV_PF; ++k) {
+                    const int e = min(e0 + 64 * k + lane, oe - 1);
+                    c[k] = o_col[e];
+                    v[k] = o_val[e];
+                    sg[k] = o_lr[e];
+                }
+            }
+            double xv[TRSV_PF];
+            bool rdy[TRSV_PF];
+#pragma unroll
+            for(int k = 0; k < TRSV_PF; ++k) xv[k] = pub_load(pub + c[k]);
+#pragma unroll
+            for(int k = 0; k < TRSV_PF; ++k) {
+                const bool in = e0 + 64 * k + lane < oe;
+                v[k] = in ? v[k] : (VT) 0;
+                sg[k] = in ? sg[k] : 64;
+                rdy[k] = !in || published(xv[k]);
+            }
+            wait_published(pub, c, xv, rdy, err, spins);
+            if(!fetched) {
+                chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, nxt);
+                fetched = true;
+            }
+            wave_sync();
+#pragma unroll
+            for(int k = 0; k < TRSV_PF; ++k) {
+                if(e0 + 64 * k >= oe) break;
+                double p = seg_scan_runs((double) v[k] * (sg[k] < 64 ? xv[k] : 0.0), sg[k]);
+                const int nsg = next_lane(sg[k], -1);
+                if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
+                wave_sync();
+            }
+        }
+        if(!fetched) chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, nxt);
+        wave_sync();
+        const bool in0 = ib + lane < ie;
+        int icol = in0 ? cur.ic : fr;
+        VT ival = in0 ? cur.iv : (VT) 0;
+        int isg = in0 ? cur.il : 64;
+        for(int e0 = ib; e0 < ie; e0 += 64) {
+            if(e0 != ib) {
+                const int e = e0 + lane;
+                isg = 64;
+                if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+            }
+            double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
+            p = seg_scan_runs(p, isg);
+            const int nsg = next_lane(isg, -1);
+            if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
             wave_sync();
         }
-    }
-    wave_sync();
-    for(int e0 = ib; e0 < ie; e0 += 64) {
-        if(e0 != ib) {
-            const int e = e0 + lane;
-            isg = 64;
-            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+        if(lane < h) {
+            const double r = res[lane];
+            __hip_atomic_store(pub + fr + lane, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x[fr + lane] = r;
         }
-        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-        p = seg_scan(p, isg);
-        const int nsg = __shfl_down(isg, 1, 64);
-        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
-        wave_sync();
+        w += G;
+        if(w >= nb) break;
+        t0 = n0;
+        t1 = n1;
+        cur = nxt;
     }
-    // agent-coherent stores of this block row, completed before the flag
-    if(lane < h)
-        __hip_atomic_store(x + fr + lane, (double) res[lane], __ATOMIC_RELAXED,
-            __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if(lane == 0)
-        __hip_atomic_store(flag + row_block[fr], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_bits(double* p, int64_t n, unsigned long long bits)
+{
+    const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) p[i] = __longlong_as_double((long long) bits);
 }
 
 __global__ void k_solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b,
@@ -639,22 +719,71 @@ template void solve_plan_count<double>(const SolveTopo&, const double*, double*,
 template void solve_plan_count<float>(const SolveTopo&, const float*, float*, SolvePlan<float>&, void*);
 template void solve_plan_fill<double>(const SolveTopo&, const double*, const double*, SolvePlan<double>&, void*);
 template void solve_plan_fill<float>(const SolveTopo&, const float*, const float*, SolvePlan<float>&, void*);
+// Waves of k_trsv_chain per CU. The solve needs all of its waves resident;
+// with 4 of the 32 wave slots a CU holds for it, eight such solves fit on
+// the GPU at the same time, and other kernels end on their own.
+constexpr int CHAIN_WAVES_PER_CU = 4;
+
+// grid of k_trsv_chain: CHAIN_WAVES_PER_CU waves per CU, fewer when the
+// kernel's occupancy is lower (CU count and occupancy queried once per device)
 template<typename VT>
-void trsv_upper_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
-    void* stream)
+int chain_grid(int32_t nb)
 {
-    if(t.nb <= 0) return;
-    k_trsv_flagged<VT><<<t.nb, 64, 0, (hipStream_t) stream>>>(p.task_u, t.nb, 1, p.uo_col, p.uo_val,
-        p.uo_lr, p.ui_col, p.ui_val, p.ui_lr, t.row_block, t.flag_u, epoch, t.err, x);
+    static int cus[64] = {0}, occ[64] = {0};
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if(cus[dev] == 0) {
+        hipDeviceProp_t pr;
+        int o = 0;
+        if(hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
+        if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_trsv_chain<VT>, 64, 0) != hipSuccess)
+            return 0;
+        occ[dev] = o;
+        cus[dev] = pr.multiProcessorCount;
+    }
+    const int g = cus[dev] * min(CHAIN_WAVES_PER_CU, occ[dev]);
+    return min(g, (int) nb);
 }
 
 template<typename VT>
-void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+hipError_t trsv_chain(const int4* task, int32_t nb, int32_t reverse, const int32_t* o_col,
+    const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
+    const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x, hipStream_t st)
+{
+    const int g = chain_grid<VT>(nb);
+    if(g <= 0) return hipErrorInvalidConfiguration;
+    k_trsv_chain<VT><<<g, 64, 0, st>>>(task, nb, reverse, o_col, o_val, o_lr, i_col, i_val, i_lr,
+        pub, next_pub, err, x);
+    return hipGetLastError();
+}
+
+template<typename VT>
+int trsv_upper_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
     void* stream)
 {
-    if(t.nb <= 0) return;
-    k_trsv_flagged<VT><<<t.nb, 64, 0, (hipStream_t) stream>>>(p.task_t, t.nb, 0, p.to_col, p.to_val,
-        p.to_lr, p.ti_col, p.ti_val, p.ti_lr, t.row_block, t.flag_t, epoch, t.err, x);
+    if(t.nb <= 0) return 0;
+    double* pub = t.chain_pub + (parity ? t.n : 0);
+    double* next = t.chain_pub + (parity ? 0 : t.n);
+    return (int) trsv_chain<VT>(p.task_u, t.nb, 1, p.uo_col, p.uo_val, p.uo_lr, p.ui_col, p.ui_val,
+        p.ui_lr, pub, next, t.err, x, (hipStream_t) stream);
+}
+
+template<typename VT>
+int trsv_upper_t_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
+    void* stream)
+{
+    if(t.nb <= 0) return 0;
+    double* pub = t.chain_pub + (parity ? t.n : 0);
+    double* next = t.chain_pub + (parity ? 0 : t.n);
+    return (int) trsv_chain<VT>(p.task_t, t.nb, 0, p.to_col, p.to_val, p.to_lr, p.ti_col, p.ti_val,
+        p.ti_lr, pub, next, t.err, x, (hipStream_t) stream);
+}
+
+void chain_reset(double* pub, int64_t count, void* stream)
+{
+    if(count <= 0) return;
+    k_fill_bits<<<(unsigned) ((count + 255) / 256), 256, 0, (hipStream_t) stream>>>(pub, count,
+        CHAIN_EMPTY);
 }
 
 template<typename VT>
@@ -680,10 +809,10 @@ void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, d
 
 template void jacobi_sweeps<double>(const SolveTopo&, const SolvePlan<double>&, bool, double*, int32_t, double*, double*, double*, void*);
 template void jacobi_sweeps<float>(const SolveTopo&, const SolvePlan<float>&, bool, double*, int32_t, double*, double*, double*, void*);
-template void trsv_upper_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
-template void trsv_upper_flagged<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
-template void trsv_upper_t_flagged<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
-template void trsv_upper_t_flagged<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
+template int trsv_upper_single<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
+template int trsv_upper_single<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
+template int trsv_upper_t_single<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
+template int trsv_upper_t_single<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
 template void trsv_upper<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
 template void trsv_upper<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
 template void trsv_upper_t<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);

@@ -38,9 +38,7 @@ struct SolveTopo {
     const double* tri;
     const int32_t* is_piv;
     // single-launch solves
-    const int32_t* row_block;   // fine row -> block row
-    int32_t* flag_u;            // per block row: epoch of its last U solve
-    int32_t* flag_t;            // per block row: epoch of its last U' solve
+    double* chain_pub;          // 2n published values, two halves used in turn
     int32_t* err;               // set when a dependency wait gave up
 };
 
@@ -64,15 +62,19 @@ template<typename VT>
 void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
 template<typename VT>
 void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
-// The same solves as one launch each, block rows ordered by level and
-// synchronised through per-block-row flags; epoch must differ from the
-// previous call's on the same flags.
+// The same solves as one launch each (at most 8 at a time on one GPU),
+// block rows ordered by level; readers wait for each value they need in
+// chain_pub. A call
+// publishes into half `parity` of chain_pub and presets the other half for
+// the next call, which must pass the other parity; chain_reset presets both
+// halves once. Returns the launch's hipError_t.
 template<typename VT>
-void trsv_upper_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+int trsv_upper_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
     void* stream);
 template<typename VT>
-void trsv_upper_t_flagged(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t epoch,
+int trsv_upper_t_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
     void* stream);
+void chain_reset(double* pub, int64_t count, void* stream);
 // x <- approximately U \ b (U' \ b when transpose) by `sweeps` block-Jacobi
 // sweeps from 0; b_tmp holds b (n doubles), x_a / x_b are scratch vectors
 template<typename VT>

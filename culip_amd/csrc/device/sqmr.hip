@@ -62,12 +62,19 @@ __device__ double block_sum(double v, double* sh)
     return t;
 }
 
-// every workgroup sums the partials part[q * stride + off], q < G, the same way
-__device__ double reduce_part(const double* part, int G, int stride, int off, double* sh)
+// Partial sums: slot s of workgroup q at part[s * G + q] (G workgroups), so
+// that a slot's partials are contiguous. Every workgroup sums the partials
+// of a slot the same way.
+__device__ double reduce_part(const double* part, int G, int slot, double* sh)
 {
     double v = 0.0;
-    for(int q = threadIdx.x; q < G; q += blockDim.x) v += part[q * stride + off];
+    for(int q = threadIdx.x; q < G; q += blockDim.x) v += part[slot * G + q];
     return block_sum(v, sh);
+}
+
+__device__ __forceinline__ double& part_at(double* part, int slot)
+{
+    return part[slot * (int) gridDim.x + (int) blockIdx.x];
 }
 
 struct DP { double g1, g2; bool w1, w2; };
@@ -154,7 +161,7 @@ __global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, cons
     }
     num = block_sum(num, sh);
     den = block_sum(den, sh);
-    if(threadIdx.x == 0) { part[8 * blockIdx.x + 0] = num; part[8 * blockIdx.x + 1] = den; }
+    if(threadIdx.x == 0) { part_at(part, 0) = num; part_at(part, 1) = den; }
 }
 
 __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* Av,
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* A
 {
     __shared__ double sh[16];
     const int G = gridDim.x;
-    const double alpha = reduce_part(part, G, 8, 0, sh) / reduce_part(part, G, 8, 1, sh);
+    const double alpha = reduce_part(part, G, 0, sh) / reduce_part(part, G, 1, sh);
     const double beta = scal[S_BETA + k % 3];
     double ss = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
@@ -171,7 +178,7 @@ __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* A
         ss = fma(t, t, ss);
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part[8 * blockIdx.x + 2] = ss;
+    if(threadIdx.x == 0) part_at(part, 2) = ss;
 }
 
 __global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, double* v1,
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, dou
 {
     __shared__ double sh[16];
     const int G = gridDim.x;
-    const double delta = sqrt(reduce_part(part, G, 8, 2, sh));
+    const double delta = sqrt(reduce_part(part, G, 2, sh));
     const double t = 1.0 / delta;
     const double we = scal[S_ETA] * delta;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
     }
     num = block_sum(num, sh);
     den = block_sum(den, sh);
-    if(threadIdx.x == 0) { part[8 * blockIdx.x + 3] = num; part[8 * blockIdx.x + 4] = den; }
+    if(threadIdx.x == 0) { part_at(part, 3) = num; part_at(part, 4) = den; }
 }
 
 template<int K0>
@@ -213,9 +220,9 @@ __device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
 {
 #pragma unroll
     for(int q = 0; q < NSCAL; ++q) S[q] = scal[q];
-    const double alpha = reduce_part(part, G, 8, 0, sh) / reduce_part(part, G, 8, 1, sh);
-    const double delta = sqrt(reduce_part(part, G, 8, 2, sh));
-    const double beta = reduce_part(part, G, 8, 3, sh) / reduce_part(part, G, 8, 4, sh);
+    const double alpha = reduce_part(part, G, 0, sh) / reduce_part(part, G, 1, sh);
+    const double delta = sqrt(reduce_part(part, G, 2, sh));
+    const double beta = reduce_part(part, G, 3, sh) / reduce_part(part, G, 4, sh);
     scalar_step<K0>(S, alpha, delta, beta);
 }
 
@@ -247,7 +254,7 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
         ss = fma(xv, xv, ss);
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part[8 * blockIdx.x + 5] = ss;
+    if(threadIdx.x == 0) part_at(part, 5) = ss;
 }
 
 __global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
@@ -272,7 +279,7 @@ __device__ __forceinline__ void scalars(int k, int G, double* scal, const double
     __shared__ double sh[16];
     double S[NSCAL];
     iteration_scalars<K0>(S, scal, part, G, sh);
-    const double nx = sqrt(reduce_part(part, G, 8, 5, sh));
+    const double nx = sqrt(reduce_part(part, G, 5, sh));
     if(threadIdx.x == 0) {
         constexpr int k1 = (K0 + 1) % 3;
         S[S_NORMX] = nx;
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(TPB) void k_sumsq(int32_t n, const double* a, doubl
     double ss = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) ss = fma(a[g], a[g], ss);
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part[8 * blockIdx.x + slot] = ss;
+    if(threadIdx.x == 0) part_at(part, slot) = ss;
 }
 
 __global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, double* v0, double* z,
@@ -311,7 +318,7 @@ __global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, dou
 {
     __shared__ double sh[16];
     const int G = gridDim.x;
-    const double t = 1.0 / sqrt(reduce_part(part, G, 8, 6, sh));
+    const double t = 1.0 / sqrt(reduce_part(part, G, 6, sh));
     double ss = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
         const double r1 = tv[g] * t;
@@ -322,7 +329,7 @@ __global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, dou
         if(d.w2) { z[g + 1] = d.g2; ss = fma(d.g2, d.g2, ss); }
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part[8 * blockIdx.x + 7] = ss;
+    if(threadIdx.x == 0) part_at(part, 7) = ss;
 }
 
 __global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, double* w0, double* ev,
@@ -330,8 +337,8 @@ __global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, doub
 {
     __shared__ double sh[16];
     const int G = gridDim.x;
-    const double norm_pb = sqrt(reduce_part(part, G, 8, 6, sh));
-    const double eta = 1.0 / sqrt(reduce_part(part, G, 8, 7, sh));
+    const double norm_pb = sqrt(reduce_part(part, G, 6, sh));
+    const double eta = 1.0 / sqrt(reduce_part(part, G, 7, sh));
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
         w0[g] = z[g] * eta;
         ev[g] = z[g];
@@ -393,14 +400,14 @@ __global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, 
     }
     sa = block_sum(sa, sh);
     sb = block_sum(sb, sh);
-    if(threadIdx.x == 0) { part[8 * blockIdx.x + 0] = sa; part[8 * blockIdx.x + 1] = sb; }
+    if(threadIdx.x == 0) { part_at(part, 0) = sa; part_at(part, 1) = sb; }
 }
 
 __global__ __launch_bounds__(TPB) void k_norm_final(int G, const double* part, double* out)
 {
     __shared__ double sh[16];
-    const double a = reduce_part(part, G, 8, 0, sh);
-    const double b = reduce_part(part, G, 8, 1, sh);
+    const double a = reduce_part(part, G, 0, sh);
+    const double b = reduce_part(part, G, 1, sh);
     if(threadIdx.x == 0) { out[0] = sqrt(a); out[1] = sqrt(b); }
 }
 

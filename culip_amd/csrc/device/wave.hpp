@@ -48,6 +48,53 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
+// DPP lane moves (GFX9 data-parallel primitives, one VALU op per 32 bits):
+// lane i receives src of the lane selected by CTRL within the rows enabled in
+// ROW_MASK; lanes without a source, or in disabled rows, receive old.
+// CTRL: 0x111 + k - 1 = row_shr:k (from lane i - k of the same 16-lane row),
+// 0x142 = row_bcast:15 (lane 15 of the row before), 0x143 = row_bcast:31
+// (lane 31), 0x130 = wave_shl:1 (from lane i + 1).
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_move(int old, int src)
+{
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, ROW_MASK, 0xf, false);
+}
+
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double old, double src)
+{
+    const int hi = dpp_move<CTRL, ROW_MASK>(__double2hiint(old), __double2hiint(src));
+    const int lo = dpp_move<CTRL, ROW_MASK>(__double2loint(old), __double2loint(src));
+    return __hiloint2double(hi, lo);
+}
+
+template<int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_scan_step(double& v, int sg)
+{
+    const double u = dpp_move<CTRL, ROW_MASK>(0.0, v);
+    const int su = dpp_move<CTRL, ROW_MASK>(-1, sg);
+    v = (su == sg) ? v + u : v;
+}
+
+// Segmented inclusive sum over the 64 lanes of a wave, all lanes active:
+// lane i returns the sum of v over the lanes j <= i with sg_j == sg_i. sg
+// must be non-decreasing over the lanes (segments are runs) and >= 0. The
+// sums run in a fixed tree: within 16-lane rows by distances 1, 2, 4, 8,
+// then across rows through lanes 15 and 31.
+__device__ __forceinline__ double seg_scan_runs(double v, int sg)
+{
+    seg_scan_step<0x111, 0xf>(v, sg);
+    seg_scan_step<0x112, 0xf>(v, sg);
+    seg_scan_step<0x114, 0xf>(v, sg);
+    seg_scan_step<0x118, 0xf>(v, sg);
+    seg_scan_step<0x142, 0xa>(v, sg);
+    seg_scan_step<0x143, 0xc>(v, sg);
+    return v;
+}
+
+// v of lane i + 1 (fill on lane 63), all lanes active
+__device__ __forceinline__ int next_lane(int v, int fill) { return dpp_move<0x130, 0xf>(fill, v); }
+
 // Ordered scans over the threads of a workgroup for an associative (not
 // necessarily commutative) operation on a value type V with members that
 // __shfl can move. Op: V op(const V& earlier, const V& later), id: identity.

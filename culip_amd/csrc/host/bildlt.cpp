@@ -40,7 +40,7 @@ int64_t BildltPattern::device_bytes() const
     int64_t b = 0;
     for(const DevBuf* d : {&d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off,
             &d_upd_ki, &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in,
-            &d_value_slot, &d_rfo, &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4, &d_row_block})
+            &d_value_slot, &d_rfo, &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4})
         b += (int64_t) d->bytes;
     return b;
 }
@@ -125,7 +125,6 @@ void BildltPattern::upload_static()
         }
         upload(d_upd4, u4);
     }
-    upload(d_row_block, s.row_block);
     uploaded = true;
 }
 
@@ -164,11 +163,12 @@ void Bildlt::upload_pattern()
     for(int q = 0; q < 2; ++q) d_task[q].ensure(sizeof(int32_t) * 8 * (size_t) s.nb);
     d_totals.ensure(sizeof(int32_t) * 4);
     d_scan_tmp.ensure(sizeof(int32_t) * scan_tmp_ints(n));
-    d_flag_u.ensure(sizeof(int32_t) * std::max(1, s.nb));
-    d_flag_t.ensure(sizeof(int32_t) * std::max(1, s.nb));
+    d_chain_pub.ensure(sizeof(double) * 2 * (size_t) std::max(1, n));
     d_trsv_err.ensure(sizeof(int32_t));
-    HIP_CHECK(hipMemset(d_flag_u.p, 0, d_flag_u.bytes));
-    HIP_CHECK(hipMemset(d_flag_t.p, 0, d_flag_t.bytes));
+    chain_reset(d_chain_pub.as<double>(), 2 * (int64_t) n, nullptr);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    chain_parity = 0;
     HIP_CHECK(hipMemset(d_trsv_err.p, 0, d_trsv_err.bytes));
     uploaded = true;
 }
@@ -261,9 +261,7 @@ SolveTopo Bildlt::topo() const
     t.ix_out = d_ix_out.as<int32_t>();
     t.tri = d_tri.as<double>();
     t.is_piv = d_is_piv.as<int32_t>();
-    t.row_block = d_row_block.as<int32_t>();
-    t.flag_u = d_flag_u.as<int32_t>();
-    t.flag_t = d_flag_t.as<int32_t>();
+    t.chain_pub = d_chain_pub.as<double>();
     t.err = d_trsv_err.as<int32_t>();
     return t;
 }
@@ -334,7 +332,7 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
 
 void Bildlt::set_solve(culip_solve_algorithm a, int32_t sweeps)
 {
-    if(a != CULIP_TRSV_LEVELS && a != CULIP_TRSV_FLAGGED && a != CULIP_TRSV_JACOBI)
+    if(a != CULIP_TRSV_LEVELS && a != CULIP_TRSV_SINGLE && a != CULIP_TRSV_JACOBI)
         fail(CULIP_ERR_INVALID, "bildlt: unknown solve algorithm");
     if(a == CULIP_TRSV_JACOBI && sweeps < 1)
         fail(CULIP_ERR_INVALID, "bildlt: block-Jacobi solves need at least one sweep");
@@ -364,10 +362,12 @@ void Bildlt::trsv_u(double* x, hipStream_t st) const
         jacobi(x, false, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_FLAGGED) {
-        const int32_t e = ++epoch_u;
-        if(f32()) trsv_upper_flagged<float>(topo(), plan<float>(), x, e, st);
-        else trsv_upper_flagged<double>(topo(), plan<double>(), x, e, st);
+    if(solve_algo == CULIP_TRSV_SINGLE) {
+        const int32_t par = chain_parity;
+        chain_parity ^= 1;
+        const int rc = f32() ? trsv_upper_single<float>(topo(), plan<float>(), x, par, st)
+                             : trsv_upper_single<double>(topo(), plan<double>(), x, par, st);
+        hip_check((hipError_t) rc, "single-launch U solve");
         return;
     }
     if(f32()) trsv_upper<float>(topo(), plan<float>(), x, st);
@@ -380,10 +380,12 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
         jacobi(x, true, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_FLAGGED) {
-        const int32_t e = ++epoch_t;
-        if(f32()) trsv_upper_t_flagged<float>(topo(), plan<float>(), x, e, st);
-        else trsv_upper_t_flagged<double>(topo(), plan<double>(), x, e, st);
+    if(solve_algo == CULIP_TRSV_SINGLE) {
+        const int32_t par = chain_parity;
+        chain_parity ^= 1;
+        const int rc = f32() ? trsv_upper_t_single<float>(topo(), plan<float>(), x, par, st)
+                             : trsv_upper_t_single<double>(topo(), plan<double>(), x, par, st);
+        hip_check((hipError_t) rc, "single-launch U' solve");
         return;
     }
     if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
@@ -424,7 +426,7 @@ int64_t Bildlt::numeric_bytes() const
             &d_carry, &d_totals, &d_task[0], &d_task[1], &d_keep, &d_granted, &d_chunk,
             &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_wave_list, &d_wave_cnt,
             &d_store_list, &d_store_cnt,
-            &d_flag_u, &d_flag_t,
+            &d_chain_pub,
             &d_trsv_err, &d_jac[0], &d_jac[1], &d_jac[2]})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
@@ -447,7 +449,7 @@ int64_t Bildlt::numeric_bytes_estimate(bool with_layout) const
     b += r16(4 * s.ix_space) + 24 * n + v * n;
     b += v * 1024 * mlb + 2048 * mlb + 4 * mlb + v * 32 * mlb + 4 * mlb + mlb;
     b += 4 * nb + 32 + 4 * 4 * (n + 1) + 2 * 32 * nb + 16 + 4 * ((int64_t) scan_tmp_ints(s.n));
-    b += 8 * nb + 4;
+    b += 16 * std::max<int64_t>(1, n) + 4;   // single-launch solves' published values, error
     b += 2 * r16(4 * mlb) + 2 * r16(4 * std::max(1, s.num_levels));   // wave / store lists
     if(with_layout) {
         // U and U' rows: each stored entry once per layout, plus the pivot

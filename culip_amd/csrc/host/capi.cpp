@@ -401,7 +401,7 @@ culip_status culip_bildlt_set_solve_algorithm(culip_bildlt* h, culip_solve_algor
 {
     return guarded([&] {
         CULIP_REQUIRE(h, "bildlt: null handle");
-        CULIP_REQUIRE(algorithm == CULIP_TRSV_LEVELS || algorithm == CULIP_TRSV_FLAGGED ||
+        CULIP_REQUIRE(algorithm == CULIP_TRSV_LEVELS || algorithm == CULIP_TRSV_SINGLE ||
                 algorithm == CULIP_TRSV_JACOBI,
             "bildlt: unknown solve algorithm");
         CULIP_REQUIRE(algorithm != CULIP_TRSV_JACOBI || num_jacobi_sweeps >= 1,

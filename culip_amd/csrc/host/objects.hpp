@@ -78,7 +78,7 @@ struct BildltPattern {
     culip_prec prec = CULIP_PREC_F64;
     DevBuf d_bs, d_is_piv, d_bin, d_csr_off, d_csc_off, d_csc_idx, d_upd_off, d_upd_ki, d_upd_kj,
         d_lvlb_idx, d_block_in_lvl, d_lvl_rows, d_ix_in, d_value_slot, d_rfo;
-    DevBuf d_lq_cap, d_lq_fr, d_off_task, d_upd4, d_row_block;
+    DevBuf d_lq_cap, d_lq_fr, d_off_task, d_upd4;
     void upload_static();
     int64_t device_bytes() const;
 };
@@ -93,7 +93,7 @@ struct Bildlt final : Precond {
     bool& have_sym;
     DevBuf &d_bs, &d_is_piv, &d_bin, &d_csr_off, &d_csc_off, &d_csc_idx, &d_upd_off, &d_upd_ki,
         &d_upd_kj, &d_lvlb_idx, &d_block_in_lvl, &d_lvl_rows, &d_ix_in, &d_value_slot, &d_rfo;
-    DevBuf &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4, &d_row_block;
+    DevBuf &d_lq_cap, &d_lq_fr, &d_off_task, &d_upd4;
 
     culip_prec prec = CULIP_PREC_F64;
     bool uploaded = false;   // this instance's numeric buffers exist
@@ -110,11 +110,12 @@ struct Bildlt final : Precond {
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
     DevBuf d_totals, d_scan_tmp;
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
-    // single-launch triangular solves: flags, wait error, epochs
-    DevBuf d_flag_u, d_flag_t, d_trsv_err;
-    culip_solve_algorithm solve_algo = CULIP_TRSV_LEVELS;
+    // single-launch triangular solves: published values, wait error, and
+    // the half of d_chain_pub the next such solve publishes into
+    DevBuf d_chain_pub, d_trsv_err;
+    culip_solve_algorithm solve_algo = CULIP_TRSV_SINGLE;
     int32_t jacobi_sweeps = 0;
-    mutable int32_t epoch_u = 0, epoch_t = 0;
+    mutable int32_t chain_parity = 0;
     DevBuf d_jac[3];          // block-Jacobi right-hand side and iterates
 
     // host copies for inspection
@@ -130,7 +131,7 @@ struct Bildlt final : Precond {
           d_lvlb_idx(p->d_lvlb_idx), d_block_in_lvl(p->d_block_in_lvl), d_lvl_rows(p->d_lvl_rows),
           d_ix_in(p->d_ix_in), d_value_slot(p->d_value_slot), d_rfo(p->d_rfo),
           d_lq_cap(p->d_lq_cap), d_lq_fr(p->d_lq_fr), d_off_task(p->d_off_task),
-          d_upd4(p->d_upd4), d_row_block(p->d_row_block), prec(p->prec)
+          d_upd4(p->d_upd4), prec(p->prec)
     {
     }
     Bildlt(const Bildlt&) = delete;

@@ -6,15 +6,20 @@
 
 namespace culip {
 
+// single-launch solves run at most 8 at a time on one GPU
+constexpr int32_t SINGLE_MAX_SLOTS = 8;
+
 Server::Server(const culip_server_opts& o) : o_(o)
 {
     CULIP_REQUIRE(o.slots >= 1 && o.slots <= 64, "server: slots must be in [1, 64]");
     CULIP_REQUIRE(o.prec == CULIP_PREC_F64 || o.prec == CULIP_PREC_F32, "server: unknown precision");
     CULIP_REQUIRE(o.fill_level >= 0 && o.fill_factor >= 1.0, "server: bad fill parameters");
     CULIP_REQUIRE(o.sqmr.max_it >= 0 && o.sqmr.tol >= 0, "server: bad SQMR options");
-    CULIP_REQUIRE(o.solve_algorithm == CULIP_TRSV_LEVELS || o.solve_algorithm == CULIP_TRSV_FLAGGED ||
+    CULIP_REQUIRE(o.solve_algorithm == CULIP_TRSV_LEVELS || o.solve_algorithm == CULIP_TRSV_SINGLE ||
             (o.solve_algorithm == CULIP_TRSV_JACOBI && o.jacobi_sweeps >= 1),
         "server: bad solve algorithm");
+    CULIP_REQUIRE(o.solve_algorithm != CULIP_TRSV_SINGLE || o.slots <= SINGLE_MAX_SLOTS,
+        "server: single-launch solves allow at most 8 slots");
     HIP_CHECK(hipGetDevice(&device_));
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
@@ -103,9 +108,11 @@ void Server::estimate(ServePattern& P) const
 
 void Server::set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t sweeps)
 {
-    CULIP_REQUIRE(a == CULIP_TRSV_LEVELS || a == CULIP_TRSV_FLAGGED ||
+    CULIP_REQUIRE(a == CULIP_TRSV_LEVELS || a == CULIP_TRSV_SINGLE ||
             (a == CULIP_TRSV_JACOBI && sweeps >= 1),
         "server: bad solve algorithm");
+    CULIP_REQUIRE(a != CULIP_TRSV_SINGLE || o_.slots <= SINGLE_MAX_SLOTS,
+        "server: single-launch solves allow at most 8 slots");
     std::lock_guard<std::mutex> g(mu_);
     CULIP_REQUIRE(pattern >= 0 && pattern < (int32_t) patterns_.size(), "server: unknown pattern");
     ServePattern& P = *patterns_[pattern];

@@ -162,11 +162,13 @@ culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
 int32_t culip_bildlt_n(const culip_bildlt * h);
 
 /* set_solve_algorithm(algorithm, num_jacobi_sweeps) (block_ildlt.cuh:
- * 226-228): how the block triangular solves run. LEVELS (default,
- * SOLVE_BLOCK): one launch per level. FLAGGED: one launch per solve, block
- * rows in level order, each waiting on per-row flags of the rows it reads
- * (slower on gfx950 at the kkt-400k size). LEVELS and FLAGGED compute the
- * same values in the same order. JACOBI (SOLVE_JACOBI,
+ * 226-228): how the block triangular solves run. SINGLE (default,
+ * SOLVE_BLOCK): one launch per solve; resident waves take the block rows in
+ * level order and each block row waits for the individual values it reads.
+ * At most 8 single-launch solves may run on one GPU at the same time (each
+ * holds an eighth of the GPU's wave slots until it ends). LEVELS: one launch
+ * per level. SINGLE and LEVELS compute the same values in the same order.
+ * JACOBI (SOLVE_JACOBI,
  * block_ildlt.impl.cuh:2454-2493): num_jacobi_sweeps >= 1 block-Jacobi
  * sweeps from x = 0, x <- U_ii^-1 (b - U_off x), one launch each — an
  * approximate solve that is a fixed linear operator (exact once the sweeps
@@ -174,7 +176,7 @@ int32_t culip_bildlt_n(const culip_bildlt * h);
  * one stream. */
 typedef enum {
     CULIP_TRSV_LEVELS = 0,
-    CULIP_TRSV_FLAGGED = 1,
+    CULIP_TRSV_SINGLE = 1,
     CULIP_TRSV_JACOBI = 2
 } culip_solve_algorithm;
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
@@ -326,7 +328,8 @@ culip_status culip_sqmr_create_ldu(const culip_spmv * A, const culip_ldu * prec,
  * overlap on the device. Per request: BlockiLDLt refactor + SQMR.       */
 
 typedef struct {
-    int32_t slots;              /* concurrent solves, 1..64 */
+    int32_t slots;              /* concurrent solves, 1..64 (1..8 with
+                                   single-launch triangular solves) */
     int64_t hbm_budget;         /* device bytes for patterns and per-slot
                                    instances; 0: 90% of free memory */
     culip_prec prec;            /* factor precision (SQMR is FP64) */
@@ -349,7 +352,8 @@ culip_status culip_server_add_pattern(culip_server * s, const culip_csr * K,
     int32_t num_pivs, const int32_t * piv_starts, int32_t num_blocks,
     const int32_t * block_starts, int32_t * pattern);
 /* The triangular-solve schedule of one pattern (default: the server's);
- * only before the pattern's first submit (CULIP_ERR_STATE after). */
+ * only before the pattern's first submit (CULIP_ERR_STATE after);
+ * CULIP_TRSV_SINGLE needs slots <= 8. */
 culip_status culip_server_set_pattern_solve(culip_server * s, int32_t pattern,
     culip_solve_algorithm algorithm, int32_t jacobi_sweeps);
 /* Queues a solve: vals = nnz values of K in the pattern's entry order,

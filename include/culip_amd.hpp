@@ -92,7 +92,7 @@ public:
     void set_solve_algorithm(SOLVE_ALGORITHM algorithm, int32_t num_jacobi_sweeps)
     {
         check(culip_bildlt_set_solve_algorithm(h_,
-            algorithm == SOLVE_JACOBI ? CULIP_TRSV_JACOBI : CULIP_TRSV_LEVELS,
+            algorithm == SOLVE_JACOBI ? CULIP_TRSV_JACOBI : CULIP_TRSV_SINGLE,
             num_jacobi_sweeps));
     }
 

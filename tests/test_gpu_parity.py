@@ -337,9 +337,10 @@ def test_cpp_driver_from_files(hip_device, tmp_path):
 
 
 @pytest.mark.parametrize("t", ["d", "f"])
-def test_flagged_solves_equal_level_solves(hip_device, t):
-    """The single-launch (flag-synchronised) triangular solves give the same
-    bits as the per-level launches, on repeated calls (epochs advance)."""
+def test_single_launch_solves_equal_level_solves(hip_device, t):
+    """The single-launch triangular solves give the same bits as the
+    per-level launches, on repeated calls (the published-value buffers
+    alternate and are preset by the call before)."""
     import torch
     k, p = _kkt(4000, 1600, 16, ca.ORDER_BLOCK_COLOR)
     g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts,
@@ -347,7 +348,7 @@ def test_flagged_solves_equal_level_solves(hip_device, t):
     g.compute(0, 1.0, 1e-4)
     b = torch.tensor(np.random.default_rng(9).normal(size=p.row_ptr.size - 1), device=hip_device)
     outs = {}
-    for algo in ("levels", "flagged"):
+    for algo in ("levels", "single"):
         g.set_solve_algorithm(algo)
         for side in ("left", "right"):
             for _ in range(3):
@@ -465,7 +466,7 @@ def test_edge_cases(hip_device, case):
     rng = np.random.default_rng(n)
     b = rng.normal(size=n)
     db = torch.tensor(b, device=hip_device)
-    for algo, sw in (("levels", 0), ("flagged", 0), ("jacobi", 64)):
+    for algo, sw in (("levels", 0), ("single", 0), ("jacobi", 64)):
         g.set_solve_algorithm(algo, sw)
         for side in ("left", "middle", "right"):
             x = torch.zeros_like(db)
