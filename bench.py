@@ -151,8 +151,9 @@ def bench_serve(a):
     sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
     order = ca.ORDER_BLOCK_COLOR
     sweeps = a.serve_jacobi_sweeps
+    exact = "single" if a.slots <= 8 else "levels"   # single launches: at most 8 at a time
     srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000,
-                    solve_algorithm="jacobi" if sweeps > 0 else "levels", jacobi_sweeps=sweeps)
+                    solve_algorithm="jacobi" if sweeps > 0 else exact, jacobi_sweeps=sweeps)
     nsets = 2
 
     def make(i):
@@ -222,7 +223,7 @@ def bench_serve(a):
                    "m": "0.4n", "max_block": a.max_block, "slots": a.slots,
                    "requests": len(res), "offered_rate_per_s": rate,
                    "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "max_it": a.max_it or 3000,
-                   "solve_schedule": f"jacobi{sweeps}" if sweeps > 0 else "levels",
+                   "solve_schedule": f"jacobi{sweeps}" if sweeps > 0 else exact,
                    "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "queue_p50_ms": float(np.percentile(que, 50)),
@@ -253,12 +254,15 @@ def bench_batch(a):
         return next(i for i, (lo, hi) in enumerate(buckets) if lo <= n < hi)
 
     # autotuner: per bucket, the block size and triangular-solve schedule
-    # (level launches, or block-Jacobi with 2 or 3 sweeps) with the shortest
+    # (single launch when at most 8 slots run at once, level launches, or
+    # block-Jacobi with 2 or 3 sweeps) with the shortest
     # refactor + solve to the true-residual tolerance on the bucket's first
     # system (FP32 factor)
     import torch
     dev = torch.device("cuda:0")
     scheds = [("levels", 0), ("jacobi", 2), ("jacobi", 3)]
+    if a.slots <= 8:
+        scheds.insert(0, ("single", 0))
     tuned, tune_ms = {}, {}
     for bi in range(len(buckets)):
         idx = [i for i in range(a.systems) if bucket(sizes[i]) == bi]
@@ -289,7 +293,7 @@ def bench_batch(a):
                 ms = (time.perf_counter() - t0) / 3 * 1e3
                 if q.result["true_rel_residual"] > 1e-8:
                     ms = float("inf")
-                name = algo if algo == "levels" else f"{algo}{sw}"
+                name = algo if algo != "jacobi" else f"{algo}{sw}"
                 tune_ms[f"{buckets[bi][0]}-{buckets[bi][1] - 1}:B{B}:{name}"] = ms
                 if best is None or ms < best[0]:
                     best = (ms, B, algo, sw)
@@ -305,7 +309,8 @@ def bench_batch(a):
     log(f"{len(systems)} systems generated")
 
     def solve_all(slots, prec, tuned_schedule):
-        srv = ca.Server(slots=slots, prec=prec)
+        srv = ca.Server(slots=slots, prec=prec,
+                        solve_algorithm="single" if slots <= 8 else "levels")
         with cf.ThreadPoolExecutor(16) as ex:
             pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,
                                                          s[0].piv_starts, s[0].block_starts),
@@ -325,7 +330,7 @@ def bench_batch(a):
         return out, el
 
     res, el = solve_all(a.slots, a.prec, True)
-    # the plain path: FP64 factor, level-scheduled solves (the reference
+    # the plain path: FP64 factor, exact single-launch solves (the reference
     # driver's SOLVE_BLOCK), one system at a time
     base, el_base = solve_all(1, "f64", False)
     trr = max(r[1]["true_rel_residual"] for r in res)
@@ -342,7 +347,7 @@ def bench_batch(a):
                                            for b in tuned},
                    "solve_schedule_by_bucket": {
                        f"{buckets[b][0]}-{buckets[b][1] - 1}":
-                           tuned[b][1] if tuned[b][1] == "levels" else f"{tuned[b][1]}{tuned[b][2]}"
+                           tuned[b][1] if tuned[b][1] != "jacobi" else f"{tuned[b][1]}{tuned[b][2]}"
                        for b in tuned},
                    "tol": 1e-8, "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),

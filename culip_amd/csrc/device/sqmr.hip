@@ -148,16 +148,79 @@ __device__ __forceinline__ void scalar_step(double* S, double alpha, double delt
     S[S_TAU] = c * S[S_GAMMA];
 }
 
+// The vector kernels give each thread VR elements per round, g0 + r S
+// (S = G * TPB), and issue every load of a round before the first use, so a
+// round costs one memory round trip; per thread the elements are still
+// visited in increasing g, so every partial sum keeps its order.
+constexpr int VR = 4;
+
+// what the D solve of row g reads (k_solve_D's inputs), loaded up front
+struct DRow {
+    double p11, p12, p22n;
+    int this_piv, next_piv;
+};
+
+__device__ __forceinline__ DRow dload(int64_t g, int32_t n, const double* tri, const int32_t* ip)
+{
+    DRow d;
+    const int64_t g1 = min(g + 1, (int64_t) n - 1);
+    d.this_piv = ip[g];
+    const int nxt = ip[g1];
+    d.next_piv = (g < n - 1) ? nxt : 1;
+    d.p11 = tri[3 * g + 1];
+    d.p12 = tri[3 * g + 2];
+    d.p22n = tri[3 * g1 + 1];
+    return d;
+}
+
+// dsolve on loaded inputs (the same operations)
+__device__ __forceinline__ DP dsolve_loaded(const DRow& d, double r1, double r2)
+{
+    DP o;
+    const double p22 = !d.next_piv ? d.p22n : 1.0;
+    const double gr = sqrt(fma(d.p11, d.p11, d.p12 * d.p12));
+    const double c = d.p11 / gr, s = d.p12 / gr;
+    double g1 = fma(c, r1, s * r2);
+    double g2 = fma(-s, r1, c * r2);
+    const double t12 = fma(c, d.p12, s * p22);
+    const double t22 = fma(-s, d.p12, c * p22);
+    g2 = g2 / t22;
+    g1 = fma(-t12, g2, g1) / gr;
+    o.g1 = g1; o.g2 = g2; o.w1 = d.this_piv != 0; o.w2 = !d.next_piv;
+    return o;
+}
+
 __global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, const double* v,
     const double* z, const double* tri, const int32_t* ip, double* part)
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
-        const DP d = dsolve(g, n, tri, ip, Av[g], (g < n - 1) ? Av[g + 1] : 0.0);
-        if(d.w1) num = fma(v[g], d.g1, num);
-        if(d.w2) num = fma(v[g + 1], d.g2, num);
-        den = fma(v[g], z[g], den);
+    const int64_t S = (int64_t) gridDim.x * TPB;
+    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+        double av[VR], av1[VR], vv[VR], vv1[VR], zz[VR];
+        DRow d[VR];
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = min(g0 + r * S, (int64_t) n - 1);
+            const int64_t g1 = min(g + 1, (int64_t) n - 1);
+            av[r] = Av[g];
+            av1[r] = Av[g1];
+            vv[r] = v[g];
+            vv1[r] = v[g1];
+            zz[r] = z[g];
+            if(tri) d[r] = dload(g, n, tri, ip);
+        }
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = g0 + r * S;
+            if(g >= n) break;
+            DP o;
+            if(tri) o = dsolve_loaded(d[r], av[r], (g < n - 1) ? av1[r] : 0.0);
+            else { o.g1 = av[r]; o.g2 = 0; o.w1 = true; o.w2 = false; }
+            if(o.w1) num = fma(vv[r], o.g1, num);
+            if(o.w2) num = fma(vv1[r], o.g2, num);
+            den = fma(vv[r], zz[r], den);
+        }
     }
     num = block_sum(num, sh);
     den = block_sum(den, sh);
@@ -172,10 +235,24 @@ __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* A
     const double alpha = reduce_part(part, G, 0, sh) / reduce_part(part, G, 1, sh);
     const double beta = scal[S_BETA + k % 3];
     double ss = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
-        const double t = fma(-beta, vkm1[g], fma(-alpha, vk[g], Av[g]));
-        tv[g] = t;
-        ss = fma(t, t, ss);
+    const int64_t S = (int64_t) G * TPB;
+    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+        double a[VR], b[VR], c[VR];
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = min(g0 + r * S, (int64_t) n - 1);
+            a[r] = Av[g];
+            b[r] = vk[g];
+            c[r] = vkm1[g];
+        }
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = g0 + r * S;
+            if(g >= n) break;
+            const double t = fma(-beta, c[r], fma(-alpha, b[r], a[r]));
+            tv[g] = t;
+            ss = fma(t, t, ss);
+        }
     }
     ss = block_sum(ss, sh);
     if(threadIdx.x == 0) part_at(part, 2) = ss;
@@ -190,13 +267,30 @@ __global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, dou
     const double delta = sqrt(reduce_part(part, G, 2, sh));
     const double t = 1.0 / delta;
     const double we = scal[S_ETA] * delta;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
-        const double r1 = tv[g] * t;
-        const double r2 = (g < n - 1) ? tv[g + 1] * t : 0.0;
-        v1[g] = r1;
-        const DP d = dsolve(g, n, tri, ip, r1, r2);
-        if(d.w1) { z[g] = d.g1; w1[g] = d.g1 * we; ev[g] = d.g1; }
-        if(d.w2) { z[g + 1] = d.g2; w1[g + 1] = d.g2 * we; ev[g + 1] = d.g2; }
+    const int64_t S = (int64_t) G * TPB;
+    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+        double a[VR], a1[VR];
+        DRow d[VR];
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = min(g0 + r * S, (int64_t) n - 1);
+            a[r] = tv[g];
+            a1[r] = tv[min(g + 1, (int64_t) n - 1)];
+            if(tri) d[r] = dload(g, n, tri, ip);
+        }
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = g0 + r * S;
+            if(g >= n) break;
+            const double r1 = a[r] * t;
+            const double r2 = (g < n - 1) ? a1[r] * t : 0.0;
+            v1[g] = r1;
+            DP o;
+            if(tri) o = dsolve_loaded(d[r], r1, r2);
+            else { o.g1 = r1; o.g2 = 0; o.w1 = true; o.w2 = false; }
+            if(o.w1) { z[g] = o.g1; w1[g] = o.g1 * we; ev[g] = o.g1; }
+            if(o.w2) { z[g + 1] = o.g2; w1[g + 1] = o.g2 * we; ev[g + 1] = o.g2; }
+        }
     }
 }
 
@@ -205,9 +299,22 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
-        num = fma(wk[g], Av[g], num);
-        den = fma(wk[g], vk[g], den);
+    const int64_t S = (int64_t) gridDim.x * TPB;
+    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+        double a[VR], b[VR], c[VR];
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = min(g0 + r * S, (int64_t) n - 1);
+            a[r] = wk[g];
+            b[r] = Av[g];
+            c[r] = vk[g];
+        }
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            if(g0 + r * S >= n) break;
+            num = fma(a[r], b[r], num);
+            den = fma(a[r], c[r], den);
+        }
     }
     num = block_sum(num, sh);
     den = block_sum(den, sh);
@@ -242,16 +349,32 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
     // the breakdown was detected
     const bool frozen = flags[0] != 0;
     double ss = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
-        w1[g] = w1[g] * ib;
-        double p = vk[g];
-        p = fma(nbrr, pkm1[g], p);
-        p = fma(neps, pkm2[g], p);
-        p = p * ir;
-        pk[g] = p;
-        const double xv = frozen ? x[g] : fma(tau, p, x[g]);
-        if(!frozen) x[g] = xv;
-        ss = fma(xv, xv, ss);
+    const int64_t SG = (int64_t) G * TPB;
+    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += SG * VR) {
+        double ww[VR], a[VR], b[VR], c[VR], xx[VR];
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = min(g0 + r * SG, (int64_t) n - 1);
+            ww[r] = w1[g];
+            a[r] = vk[g];
+            b[r] = pkm1[g];
+            c[r] = pkm2[g];
+            xx[r] = x[g];
+        }
+#pragma unroll
+        for(int r = 0; r < VR; ++r) {
+            const int64_t g = g0 + r * SG;
+            if(g >= n) break;
+            w1[g] = ww[r] * ib;
+            double p = a[r];
+            p = fma(nbrr, b[r], p);
+            p = fma(neps, c[r], p);
+            p = p * ir;
+            pk[g] = p;
+            const double xv = frozen ? xx[r] : fma(tau, p, xx[r]);
+            if(!frozen) x[g] = xv;
+            ss = fma(xv, xv, ss);
+        }
     }
     ss = block_sum(ss, sh);
     if(threadIdx.x == 0) part_at(part, 5) = ss;
