@@ -309,6 +309,40 @@ __global__ void k_plan_tasks(int32_t nb, const int32_t* lvl_rows, const int32_t*
     task[2 * t + 1] = make_int4(i_ptr[fr], i_ptr[fr + h], 0, 0);
 }
 
+// res = U_ii^-1 y for one block row (y, res: LDS, res zeroed; all lanes
+// active), from the
+// row-ordered entries ib..ie of U_ii^-1; ic0 / iv0 / il0: this lane's entry
+// of the first 64. Returns res[lane] (lanes < h). A diagonal U_ii^-1 (one
+// entry per row, on the diagonal) is applied lane by lane; the
+// general form sums each row's products by segmented scans. Both give
+// 0 + (products in row order), the same bits.
+template<typename VT>
+__device__ __forceinline__ double apply_inverse(int fr, int h, int ib, int ie, int ic0, VT iv0,
+    int il0, const int32_t* i_col, const VT* i_val, const uint8_t* i_lr, const double* y,
+    double* res)
+{
+    const int lane = lane_id();
+    if(ie - ib == h && wave_ballot(lane < h && (il0 != lane || ic0 != fr + lane)) == 0)
+        return (lane < h) ? 0.0 + (double) iv0 * y[lane] : 0.0;
+    const bool in0 = ib + lane < ie;
+    int icol = in0 ? ic0 : fr;
+    VT ival = in0 ? iv0 : (VT) 0;
+    int isg = in0 ? il0 : 64;
+    for(int e0 = ib; e0 < ie; e0 += 64) {
+        if(e0 != ib) {
+            const int e = e0 + lane;
+            isg = 64;
+            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
+        }
+        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
+        p = seg_scan_runs(p, isg);
+        const int nsg = next_lane(isg, -1);
+        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
+        wave_sync();
+    }
+    return (lane < h) ? res[lane] : 0.0;
+}
+
 constexpr int TRSV_PF = 4;   // 64-entry chunks whose loads are issued together
 
 // one level: one wave per block row; y = x_i - sum(off-diag) ; x_i = Uinv y.
@@ -380,23 +414,9 @@ __device__ __forceinline__ void block_row_solve(int4 t0, int4 t1, bool offdiag,
         }
     }
     wave_sync();
-    const bool in0 = ib + lane < ie;
-    int icol = in0 ? ic0 : fr;
-    VT ival = in0 ? iv0 : (VT) 0;
-    int isg = in0 ? il0 : 64;
-    for(int e0 = ib; e0 < ie; e0 += 64) {
-        if(e0 != ib) {
-            const int e = e0 + lane;
-            isg = 64;
-            if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
-        }
-        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-        p = seg_scan_runs(p, isg);
-        const int nsg = next_lane(isg, -1);
-        if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
-        wave_sync();
-    }
-    if(lane < h) xout[fr + lane] = (double) res[lane];
+    // every lane takes part (the general form scans across the wave)
+    const double r = apply_inverse<VT>(fr, h, ib, ie, ic0, iv0, il0, i_col, i_val, i_lr, y, res);
+    if(lane < h) xout[fr + lane] = r;
 }
 
 template<typename VT>
@@ -590,24 +610,9 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
         }
         if(!fetched) chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, nxt);
         wave_sync();
-        const bool in0 = ib + lane < ie;
-        int icol = in0 ? cur.ic : fr;
-        VT ival = in0 ? cur.iv : (VT) 0;
-        int isg = in0 ? cur.il : 64;
-        for(int e0 = ib; e0 < ie; e0 += 64) {
-            if(e0 != ib) {
-                const int e = e0 + lane;
-                isg = 64;
-                if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
-            }
-            double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-            p = seg_scan_runs(p, isg);
-            const int nsg = next_lane(isg, -1);
-            if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
-            wave_sync();
-        }
+        const double r = apply_inverse<VT>(fr, h, ib, ie, cur.ic, cur.iv, cur.il, i_col, i_val, i_lr, y,
+            res);
         if(lane < h) {
-            const double r = res[lane];
             __hip_atomic_store(pub + fr + lane, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             x[fr + lane] = r;
         }
