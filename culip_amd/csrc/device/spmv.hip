@@ -13,6 +13,7 @@ namespace culip {
 namespace {
 
 constexpr int LPR = 8;   // lanes per row
+constexpr int SPMV_U = 3; // entries per lane and round of a CSR row
 
 __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __restrict__ rp,
     const int32_t* __restrict__ ci, const double* __restrict__ cv, const double* __restrict__ x,
@@ -25,7 +26,23 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __re
     for(int64_t row = row0; row < m; row += stride) {
         double acc = 0.0;
         const int32_t s = rp[row], e = rp[row + 1];
-        for(int32_t q = s + sub; q < e; q += LPR) acc = fma(cv[q], x[ci[q]], acc);
+        // the lane's entries q0, q0 + LPR, q0 + 2 LPR are loaded together
+        // (clamped in range), then summed in that order
+        for(int32_t q0 = s + sub; q0 < e; q0 += LPR * SPMV_U) {
+            int32_t c[SPMV_U];
+            double v[SPMV_U], xv[SPMV_U];
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u) {
+                const int32_t q = min(q0 + u * LPR, e - 1);
+                c[u] = ci[q];
+                v[u] = cv[q];
+            }
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u)
+                if(q0 + u * LPR < e) acc = fma(v[u], xv[u], acc);
+        }
 #pragma unroll
         for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
         if(sub == 0) {
