@@ -619,6 +619,8 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
         for(int c = lane + 1; c < len; ++c)
             if(T0[32 * lane + c] != (VT) 0) um |= 1u << c;
         a.piv_umask[(int64_t) q * 32 + lane] = um;
+    } else if(lane < 32) {
+        a.piv_umask[(int64_t) q * 32 + lane] = 0u;   // all 32 words defined
     }
     VT* out = a.if_val + (int64_t) q * 1024;
     for(int e = lane; e < len * len; e += 64) {
@@ -1147,6 +1149,19 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
     const int pq = t0.w;
     const int np = t2.y;
     const int4* pairs = (const int4*) a.upd4 + t2.x;
+    // the pivot block's 32 row masks of entries right of the diagonal,
+    // loaded together now and needed only after the Schur updates: a pivot
+    // block without any (nearly all at level-0 fill) leaves the forward
+    // substitution below with nothing to do
+    uint32_t um_any = 0;
+    {
+        const uint4* um4 = (const uint4*) (a.piv_umask + (int64_t) pq * 32);
+        uint4 w[8];
+#pragma unroll
+        for(int u = 0; u < 8; ++u) w[u] = um4[u];
+#pragma unroll
+        for(int u = 0; u < 8; ++u) um_any |= w[u].x | w[u].y | w[u].z | w[u].w;
+    }
     VT* tv = &s.v[0][tid];
     int16_t* ti = &s.ix[0][tid];
     auto V = [&](int k) -> VT& { return tv[k * TINY_TPB]; };
@@ -1323,7 +1338,7 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
     }
     // U_ii' forward substitution over the rows in ascending order
     const VT* piv = a.if_val + (int64_t) pq * 1024;
-    for(int ii = 0; ii < hgt; ++ii) {
+    for(int ii = 0; ii < hgt && um_any != 0; ++ii) {
         bool any = false;
         for(int k = 0; k < n; ++k) any = any || ((I(k) >> 5) == ii && V(k) != (VT) 0);
         if(!any) continue;
@@ -1349,9 +1364,17 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
             }
         }
     }
-    // D solve (1x1 pivots): the Givens-form solve reduces to a division
+    // D solve (1x1 pivots): the Givens-form solve reduces to a division;
+    // the divisors of eight entries are loaded together
     const double* D = a.tri + 3 * (int64_t) fr;
-    for(int k = 0; k < n; ++k) V(k) = (VT) ((double) V(k) / D[3 * (I(k) >> 5) + 1]);
+    for(int k0 = 0; k0 < n; k0 += 8) {
+        double dv[8];
+#pragma unroll
+        for(int u = 0; u < 8; ++u) dv[u] = D[3 * (I(min(k0 + u, n - 1)) >> 5) + 1];
+#pragma unroll
+        for(int u = 0; u < 8; ++u)
+            if(k0 + u < n) V(k0 + u) = (VT) ((double) V(k0 + u) / dv[u]);
+    }
     // tile order (row-major)
     for(int k = 1; k < n; ++k) {
         const int16_t ik = I(k);
