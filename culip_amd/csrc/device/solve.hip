@@ -96,6 +96,36 @@ __device__ void plan_stage(PlanStage<VT>& sg, int k, const BlockMeta& m, int off
         }
 }
 
+// true when every staged block of the round is small with at most one
+// entry (nearly every off-diagonal block at level-0 fill): the round is then
+// placed in parallel instead of walked block by block
+template<typename VT>
+__device__ __forceinline__ bool plan_one_entry_round(const PlanStage<VT>& sg, int nblk)
+{
+    const int lane = lane_id();
+    const bool simple = lane >= nblk || (sg.small[lane] && sg.nnz[lane] <= 1);
+    return wave_ballot(!simple) == 0;
+}
+
+// For a one-entry round: key = the local row (by_row) or column of lane k's
+// entry (64: none), rank = entries of the same key on lower lanes (earlier
+// blocks), and on lane r < ext the round's number of entries with key r.
+template<typename VT>
+__device__ __forceinline__ void plan_rank(const PlanStage<VT>& sg, int nblk, int ext, bool by_row,
+    int& key, int& rank, int& add)
+{
+    const int lane = lane_id();
+    const bool has = lane < nblk && sg.nnz[lane] == 1;
+    key = has ? (by_row ? sg.rr[PLAN_E * lane] : sg.cc[PLAN_E * lane]) : 64;
+    rank = 0;
+    add = 0;
+    for(int r = 0; r < ext; ++r) {
+        const uint64_t m = wave_ballot(key == r);
+        if(key == r) rank = lanes_below(m);
+        if(lane == r) add = __popcll(m);
+    }
+}
+
 template<typename VT, bool FILL>
 __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
 {
@@ -141,6 +171,22 @@ __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out
             plan_stage(sg, lane, m, t.bs[m.block_col], wb, wb, t.ix_out, val_out);
         }
         wave_sync();
+        if(plan_one_entry_round(sg, nblk)) {
+            // every staged block holds at most one entry: lane k places block
+            // k's entry after the entries of its row from earlier blocks
+            int key, rank, add;
+            plan_rank(sg, nblk, h, true, key, rank, add);
+            const int prev = __shfl(cnt, key & 63, 64);
+            const int32_t rbase = __shfl(base, key & 63, 64);
+            if(FILL && key < 64) {
+                const int32_t pos = rbase + prev + rank;
+                p.uo_col[pos] = sg.off[lane] + sg.cc[PLAN_E * lane];
+                p.uo_val[pos] = sg.v[PLAN_E * lane];
+                p.uo_lr[pos] = (uint8_t) key;
+            }
+            cnt += add;
+            continue;
+        }
         for(int k = 0; k < nblk; ++k) {
             const int fc = sg.off[k], w = sg.ext[k];
             if(sg.small[k]) {
@@ -199,6 +245,21 @@ __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out
                 t.ix_out, val_out);
         }
         wave_sync();
+        if(plan_one_entry_round(sg, nblk)) {
+            // as in k_plan_rows, by column
+            int key, rank, add;
+            plan_rank(sg, nblk, w, false, key, rank, add);
+            const int prev = __shfl(cnt, key & 63, 64);
+            const int32_t cbase = __shfl(base, key & 63, 64);
+            if(FILL && key < 64) {
+                const int32_t pos = cbase + prev + rank;
+                p.to_col[pos] = sg.off[lane] + sg.rr[PLAN_E * lane];
+                p.to_val[pos] = sg.v[PLAN_E * lane];
+                p.to_lr[pos] = (uint8_t) key;
+            }
+            cnt += add;
+            continue;
+        }
         for(int k = 0; k < nblk; ++k) {
             const int fr = sg.off[k], h = sg.ext[k];
             if(sg.small[k]) {
