@@ -448,7 +448,9 @@ def main():
     bytes_spmv = Z * 12 + (N + 1) * 4 + 2 * N * 8
     sv = 8 if a.prec == "f64" else 4
     ps = g.get_int("plan_sizes")
-    ms_trsv = event_time(lambda: g.solve_right(db, y), 10)
+    # in place, as SQMR applies it: the launches of one U solve and nothing else
+    y.copy_(db)
+    ms_trsv = event_time(lambda: g.solve_right(y, y), 10)
     bytes_trsv = (int(ps[0]) + int(ps[1])) * (sv + 4) + 2 * N * 8 + 2 * (N + 1) * 4
     its = res["iterations"]
     per_it_spmv = ms_spmv

@@ -154,8 +154,8 @@ culip_status culip_bildlt_refactor(culip_bildlt * h, const double * vals,
 
 /* solve_left / solve_middle / solve_right (block_ildlt.impl.cuh:183-246):
  * left x = U' \ (P' b), middle x = D \ b, right x = P (U \ b); transpose
- * swaps left and right. b, x: device doubles of length n; may alias only
- * for CULIP_SOLVE_MIDDLE. */
+ * swaps left and right. b, x: device doubles of length n; x may be b (the
+ * solve then runs in place, as SQMR applies it). */
 culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
     int32_t transpose, const double * b, double * x, void * stream);
 

@@ -361,6 +361,25 @@ def test_single_launch_solves_equal_level_solves(hip_device, t):
             assert np.array_equal(x, xs[0]), side
 
 
+def test_in_place_solves_equal_out_of_place(hip_device):
+    """solve_left / solve_right with x = b (in place, as SQMR applies them)
+    give the same bits as with separate arrays, for every schedule."""
+    import torch
+    k, p = _kkt(3000, 1200, 16, ca.ORDER_BLOCK_COLOR)
+    g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, "f64")
+    g.compute(0, 1.0, 1e-4)
+    b = torch.tensor(np.random.default_rng(5).normal(size=p.row_ptr.size - 1), device=hip_device)
+    for algo, sw in (("single", 0), ("levels", 0), ("jacobi", 2)):
+        g.set_solve_algorithm(algo, sw)
+        for side in ("left", "right"):
+            x = torch.zeros_like(b)
+            getattr(g, f"solve_{side}")(b, x)
+            y = b.clone()
+            getattr(g, f"solve_{side}")(y, y)
+            torch.cuda.synchronize()
+            assert np.array_equal(x.cpu().numpy(), y.cpu().numpy()), (algo, side)
+
+
 def test_jacobi_solves(hip_device):
     """Block-Jacobi triangular solves (SOLVE_JACOBI): with as many sweeps as
     levels they reproduce the level-scheduled solve bit for bit; with two
