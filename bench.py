@@ -151,7 +151,7 @@ def bench_serve(a):
     sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
     order = ca.ORDER_BLOCK_COLOR
     sweeps = a.serve_jacobi_sweeps
-    exact = "single" if a.slots <= 8 else "levels"   # single launches: at most 8 at a time
+    exact = "single" if a.slots <= 4 else "levels"   # single launches: at most 4 at a time
     srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000,
                     solve_algorithm="jacobi" if sweeps > 0 else exact, jacobi_sweeps=sweeps)
     nsets = 2
@@ -254,14 +254,14 @@ def bench_batch(a):
         return next(i for i, (lo, hi) in enumerate(buckets) if lo <= n < hi)
 
     # autotuner: per bucket, the block size and triangular-solve schedule
-    # (single launch when at most 8 slots run at once, level launches, or
+    # (single launch when at most 4 slots run at once, level launches, or
     # block-Jacobi with 2 or 3 sweeps) with the shortest
     # refactor + solve to the true-residual tolerance on the bucket's first
     # system (FP32 factor)
     import torch
     dev = torch.device("cuda:0")
     scheds = [("levels", 0), ("jacobi", 2), ("jacobi", 3)]
-    if a.slots <= 8:
+    if a.slots <= 4:
         scheds.insert(0, ("single", 0))
     tuned, tune_ms = {}, {}
     for bi in range(len(buckets)):
@@ -310,7 +310,7 @@ def bench_batch(a):
 
     def solve_all(slots, prec, tuned_schedule):
         srv = ca.Server(slots=slots, prec=prec,
-                        solve_algorithm="single" if slots <= 8 else "levels")
+                        solve_algorithm="single" if slots <= 4 else "levels")
         with cf.ThreadPoolExecutor(16) as ex:
             pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,
                                                          s[0].piv_starts, s[0].block_starts),

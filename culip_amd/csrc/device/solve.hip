@@ -575,10 +575,77 @@ struct ChainHead {
     VT iv;
 };
 
+// The right-hand side of one block row as K src (rows fr .. fr + h - 1 of a
+// CSR matrix), formed exactly as k_csr_spmv forms those rows: eight lanes
+// per row, lane sub summing entries s + sub, s + sub + 8, ... by fma in that
+// order, then the xor tree over the eight lanes. Four rounds of eight rows
+// issue their loads together. Returns row fr + lane's value (lanes < h).
+struct CsrRhs {
+    const int32_t* rp;
+    const int32_t* ci;
+    const double* cv;
+    const double* src;
+};
+
+constexpr int RHS_U = 3;   // entries per lane and row whose loads go together
+
+__device__ double block_rhs(const CsrRhs& k, int fr, int h)
+{
+    const int lane = lane_id();
+    const int sub = lane & 7, g = lane >> 3;
+    int s[4], e[4];
+#pragma unroll
+    for(int r = 0; r < 4; ++r) {
+        const int row = fr + min(8 * r + g, h - 1);
+        s[r] = k.rp[row];
+        e[r] = k.rp[row + 1];
+    }
+    double acc[4];
+#pragma unroll
+    for(int r = 0; r < 4; ++r) acc[r] = 0.0;
+    for(int q0 = sub;; q0 += 8 * RHS_U) {
+        int c[4][RHS_U];
+        double v[4][RHS_U], xv[4][RHS_U];
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+#pragma unroll
+            for(int u = 0; u < RHS_U; ++u) {
+                const int q = max(min(s[r] + q0 + 8 * u, e[r] - 1), 0);
+                const int cq = k.ci[q];
+                c[r][u] = (s[r] + q0 + 8 * u < e[r]) ? cq : 0;
+                v[r][u] = k.cv[q];
+            }
+#pragma unroll
+        for(int r = 0; r < 4; ++r)
+#pragma unroll
+            for(int u = 0; u < RHS_U; ++u) xv[r][u] = k.src[c[r][u]];
+        bool more = false;
+#pragma unroll
+        for(int r = 0; r < 4; ++r) {
+#pragma unroll
+            for(int u = 0; u < RHS_U; ++u)
+                if(s[r] + q0 + 8 * u < e[r]) acc[r] = fma(v[r][u], xv[r][u], acc[r]);
+            more = more || s[r] + q0 + 8 * RHS_U < e[r];
+        }
+        if(wave_ballot(more) == 0) break;
+    }
+    double mine = 0.0;
+#pragma unroll
+    for(int r = 0; r < 4; ++r) {
+        double a = acc[r];
+#pragma unroll
+        for(int o = 4; o > 0; o >>= 1) a += __shfl_xor(a, o, 8);
+        const double row_val = 1.0 * a;          // y = alpha (A x), alpha = 1
+        const double got = __shfl(row_val, 8 * (lane & 7), 64);
+        if((lane >> 3) == r) mine = got;
+    }
+    return mine;
+}
+
 template<typename VT>
 __device__ __forceinline__ void chain_fetch(int4 t0, int4 t1, const int32_t* o_col, const VT* o_val,
     const uint8_t* o_lr, const int32_t* i_col, const VT* i_val, const uint8_t* i_lr,
-    const double* x, ChainHead<VT>& f)
+    const double* x, bool rhs_fused, ChainHead<VT>& f)
 {
     const int lane = lane_id();
     const int fr = t0.x, h = t0.y, ob = t0.z, oe = t0.w, ib = t1.x, ie = t1.y;
@@ -589,23 +656,26 @@ __device__ __forceinline__ void chain_fetch(int4 t0, int4 t1, const int32_t* o_c
         f.v[k] = o_val[e];
         f.sg[k] = o_lr[e];
     }
-    // this block row's own right-hand side: no other wave writes x
-    f.bo = x[fr + min(lane, h - 1)];
+    // this block row's own right-hand side: no other wave writes x (with a
+    // fused product it is formed after the current block row instead)
+    f.bo = rhs_fused ? 0.0 : x[fr + min(lane, h - 1)];
     const int ei = ib + min(lane, max(ie - ib - 1, 0));
     f.ic = i_col[ei];
     f.iv = i_val[ei];
     f.il = i_lr[ei];
 }
 
-template<typename VT>
+template<typename VT, bool FUSED>
 __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb, int32_t reverse,
     const int32_t* o_col, const VT* o_val, const uint8_t* o_lr, const int32_t* i_col,
-    const VT* i_val, const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x)
+    const VT* i_val, const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x,
+    CsrRhs rhs)
 {
     __shared__ double y[33];     // FP64 accumulation for both factor precisions
     __shared__ double res[33];
     const int lane = lane_id();
     const int G = gridDim.x;
+    constexpr bool fused = FUSED;
     const double empty = __longlong_as_double((long long) CHAIN_EMPTY);
     auto task_at = [&](int w, int4& a, int4& b) {
         const int q = min(w, nb - 1);
@@ -618,7 +688,8 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
     int4 t0, t1;
     task_at(w, t0, t1);
     ChainHead<VT> cur;
-    chain_fetch<VT>(t0, t1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, cur);
+    chain_fetch<VT>(t0, t1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, fused, cur);
+    if(fused) cur.bo = block_rhs(rhs, t0.x, t0.y);
     for(;;) {
         int4 n0, n1;
         task_at(w + G, n0, n1);
@@ -656,7 +727,7 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
             }
             wait_published(pub, c, xv, rdy, err, spins);
             if(!fetched) {
-                chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, nxt);
+                chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, fused, nxt);
                 fetched = true;
             }
             wave_sync();
@@ -669,7 +740,7 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
                 wave_sync();
             }
         }
-        if(!fetched) chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, nxt);
+        if(!fetched) chain_fetch<VT>(n0, n1, o_col, o_val, o_lr, i_col, i_val, i_lr, x, fused, nxt);
         wave_sync();
         const double r = apply_inverse<VT>(fr, h, ib, ie, cur.ic, cur.iv, cur.il, i_col, i_val, i_lr, y,
             res);
@@ -679,6 +750,9 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
         }
         w += G;
         if(w >= nb) break;
+        // the next block row's K src: its loads overlap the wait for its
+        // inputs, which rarely have all arrived by now
+        if(fused) nxt.bo = block_rhs(rhs, n0.x, n0.y);
         t0 = n0;
         t1 = n1;
         cur = nxt;
@@ -785,13 +859,15 @@ template void solve_plan_count<double>(const SolveTopo&, const double*, double*,
 template void solve_plan_count<float>(const SolveTopo&, const float*, float*, SolvePlan<float>&, void*);
 template void solve_plan_fill<double>(const SolveTopo&, const double*, const double*, SolvePlan<double>&, void*);
 template void solve_plan_fill<float>(const SolveTopo&, const float*, const float*, SolvePlan<float>&, void*);
-// Waves of k_trsv_chain per CU. The solve needs all of its waves resident;
-// with 4 of the 32 wave slots a CU holds for it, eight such solves fit on
-// the GPU at the same time, and other kernels end on their own.
+// Waves of k_trsv_chain per CU. A solve needs all of its waves resident.
+// The kernel's registers let a CU hold 16 of its waves, so with 4 per CU
+// TRSV_SINGLE_MAX_CONCURRENT = 4 such solves fit on the GPU at the same
+// time; other kernels end on their own.
 constexpr int CHAIN_WAVES_PER_CU = 4;
 
 // grid of k_trsv_chain: CHAIN_WAVES_PER_CU waves per CU, fewer when the
-// kernel's occupancy is lower (CU count and occupancy queried once per device)
+// kernel's occupancy would not hold TRSV_SINGLE_MAX_CONCURRENT solves (CU
+// count and occupancy queried once per device)
 template<typename VT>
 int chain_grid(int32_t nb)
 {
@@ -802,24 +878,30 @@ int chain_grid(int32_t nb)
         hipDeviceProp_t pr;
         int o = 0;
         if(hipGetDeviceProperties(&pr, dev) != hipSuccess) return 0;
-        if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_trsv_chain<VT>, 64, 0) != hipSuccess)
+        if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_trsv_chain<VT, true>, 64, 0) != hipSuccess)
             return 0;
         occ[dev] = o;
         cus[dev] = pr.multiProcessorCount;
     }
-    const int g = cus[dev] * min(CHAIN_WAVES_PER_CU, occ[dev]);
+    const int per = max(1, min(CHAIN_WAVES_PER_CU, occ[dev] / TRSV_SINGLE_MAX_CONCURRENT));
+    const int g = cus[dev] * per;
     return min(g, (int) nb);
 }
 
 template<typename VT>
 hipError_t trsv_chain(const int4* task, int32_t nb, int32_t reverse, const int32_t* o_col,
     const VT* o_val, const uint8_t* o_lr, const int32_t* i_col, const VT* i_val,
-    const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x, hipStream_t st)
+    const uint8_t* i_lr, double* pub, double* next_pub, int32_t* err, double* x, CsrRhs rhs,
+    hipStream_t st)
 {
     const int g = chain_grid<VT>(nb);
     if(g <= 0) return hipErrorInvalidConfiguration;
-    k_trsv_chain<VT><<<g, 64, 0, st>>>(task, nb, reverse, o_col, o_val, o_lr, i_col, i_val, i_lr,
-        pub, next_pub, err, x);
+    if(rhs.rp)
+        k_trsv_chain<VT, true><<<g, 64, 0, st>>>(task, nb, reverse, o_col, o_val, o_lr, i_col, i_val,
+            i_lr, pub, next_pub, err, x, rhs);
+    else
+        k_trsv_chain<VT, false><<<g, 64, 0, st>>>(task, nb, reverse, o_col, o_val, o_lr, i_col,
+            i_val, i_lr, pub, next_pub, err, x, rhs);
     return hipGetLastError();
 }
 
@@ -831,18 +913,28 @@ int trsv_upper_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int
     double* pub = t.chain_pub + (parity ? t.n : 0);
     double* next = t.chain_pub + (parity ? 0 : t.n);
     return (int) trsv_chain<VT>(p.task_u, t.nb, 1, p.uo_col, p.uo_val, p.uo_lr, p.ui_col, p.ui_val,
-        p.ui_lr, pub, next, t.err, x, (hipStream_t) stream);
+        p.ui_lr, pub, next, t.err, x, CsrRhs{nullptr, nullptr, nullptr, nullptr},
+        (hipStream_t) stream);
 }
 
 template<typename VT>
 int trsv_upper_t_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
     void* stream)
 {
+    return trsv_upper_t_single_product<VT>(t, p, nullptr, nullptr, x, parity, stream);
+}
+
+template<typename VT>
+int trsv_upper_t_single_product(const SolveTopo& t, const SolvePlan<VT>& p, const DevCsr* K,
+    const double* src, double* x, int32_t parity, void* stream)
+{
     if(t.nb <= 0) return 0;
     double* pub = t.chain_pub + (parity ? t.n : 0);
     double* next = t.chain_pub + (parity ? 0 : t.n);
+    const CsrRhs rhs = K ? CsrRhs{K->row_ptr, K->col, K->val, src}
+                         : CsrRhs{nullptr, nullptr, nullptr, nullptr};
     return (int) trsv_chain<VT>(p.task_t, t.nb, 0, p.to_col, p.to_val, p.to_lr, p.ti_col, p.ti_val,
-        p.ti_lr, pub, next, t.err, x, (hipStream_t) stream);
+        p.ti_lr, pub, next, t.err, x, rhs, (hipStream_t) stream);
 }
 
 void chain_reset(double* pub, int64_t count, void* stream)
@@ -879,6 +971,8 @@ template int trsv_upper_single<double>(const SolveTopo&, const SolvePlan<double>
 template int trsv_upper_single<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
 template int trsv_upper_t_single<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
 template int trsv_upper_t_single<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
+template int trsv_upper_t_single_product<double>(const SolveTopo&, const SolvePlan<double>&, const DevCsr*, const double*, double*, int32_t, void*);
+template int trsv_upper_t_single_product<float>(const SolveTopo&, const SolvePlan<float>&, const DevCsr*, const double*, double*, int32_t, void*);
 template void trsv_upper<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
 template void trsv_upper<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
 template void trsv_upper_t<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);

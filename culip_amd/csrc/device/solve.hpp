@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "factor.hpp"
+#include "spmv.hpp"
 
 namespace culip {
 
@@ -62,7 +63,11 @@ template<typename VT>
 void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
 template<typename VT>
 void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);
-// The same solves as one launch each (at most 8 at a time on one GPU),
+// single-launch solves that may run on one GPU at the same time
+constexpr int TRSV_SINGLE_MAX_CONCURRENT = 4;
+
+// The same solves as one launch each (at most TRSV_SINGLE_MAX_CONCURRENT at
+// a time on one GPU),
 // block rows ordered by level; readers wait for each value they need in
 // chain_pub. A call
 // publishes into half `parity` of chain_pub and presets the other half for
@@ -75,6 +80,12 @@ template<typename VT>
 int trsv_upper_t_single(const SolveTopo& t, const SolvePlan<VT>& p, double* x, int32_t parity,
     void* stream);
 void chain_reset(double* pub, int64_t count, void* stream);
+// x = U' \ (K src) as one single-launch solve: every block row forms its
+// rows of K src itself (in k_csr_spmv's order) before it waits for its
+// inputs; K: the CSR matrix on the device, n x n
+template<typename VT>
+int trsv_upper_t_single_product(const SolveTopo& t, const SolvePlan<VT>& p, const DevCsr* K,
+    const double* src, double* x, int32_t parity, void* stream);
 // x <- approximately U \ b (U' \ b when transpose) by `sweeps` block-Jacobi
 // sweeps from 0; b_tmp holds b (n doubles), x_a / x_b are scratch vectors
 template<typename VT>

@@ -616,10 +616,15 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     k_sumsq<<<G, TPB, 0, st>>>(n, w.tv, w.part, 6);
     k_init_v<<<G, TPB, 0, st>>>(n, w.tv, w.v[0], w.z, ops.tri, ops.is_piv, w.part);
     k_init_w<<<G, TPB, 0, st>>>(n, w.z, w.w[0], w.ev, w.scal, w.part, w.flags);
+    // Av = Pl \ (A x), folded into one pass where the preconditioner can
+    auto apply_left_op = [&](const double* xin, double* yout) {
+        if(ops.spmv_left && ops.spmv_left(xin, yout, st)) return;
+        ops.spmv(xin, yout, st);
+        if(ops.left) ops.left(yout, st);
+    };
     int cur = 0;
     if(ops.right) ops.right(w.ev, st);
-    ops.spmv(w.ev, w.av[cur], st);
-    if(ops.left) ops.left(w.av[cur], st);
+    apply_left_op(w.ev, w.av[cur]);
 
     hipEvent_t ev_copy;
     HIPCHK(hipEventCreateWithFlags(&ev_copy, hipEventDisableTiming));
@@ -639,8 +644,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
             w.scal, w.part);
         const int nxt = cur ^ 1;
         if(ops.right) ops.right(w.ev, st);
-        ops.spmv(w.ev, w.av[nxt], st);
-        if(ops.left) ops.left(w.av[nxt], st);
+        apply_left_op(w.ev, w.av[nxt]);
         k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
         k_pupdate<<<G, TPB, 0, st>>>(n, it, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
             w.scal, w.part, w.flags);

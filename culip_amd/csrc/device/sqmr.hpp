@@ -13,6 +13,9 @@ struct SqmrOps {
     std::function<void(const double* x, double* y, void* stream)> spmv;   // y = A x
     std::function<void(double* x, void* stream)> left;                    // x = Pl \ x (NULL: I)
     std::function<void(double* x, void* stream)> right;                   // x = Pr \ x (NULL: I)
+    // y = Pl \ (A x) in one pass, the same values as spmv then left; returns
+    // false when not available (NULL: never)
+    std::function<bool(const double* x, double* y, void* stream)> spmv_left;
     const double* tri = nullptr;      // middle D (NULL: identity)
     const int32_t* is_piv = nullptr;
 };

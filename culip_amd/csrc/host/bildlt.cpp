@@ -392,6 +392,18 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
     else trsv_upper_t<double>(topo(), plan<double>(), x, st);
 }
 
+bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStream_t st) const
+{
+    // the single-launch U' solve forms each block row's K src itself
+    if(solve_algo != CULIP_TRSV_SINGLE || K.m != n() || K.n != n()) return false;
+    const int32_t par = chain_parity;
+    chain_parity ^= 1;
+    const int rc = f32() ? trsv_upper_t_single_product<float>(topo(), plan<float>(), &K, src, x, par, st)
+                         : trsv_upper_t_single_product<double>(topo(), plan<double>(), &K, src, x, par, st);
+    hip_check((hipError_t) rc, "single-launch U' solve with the operator product");
+    return true;
+}
+
 void Bildlt::check_solve_error(hipStream_t st) const
 {
     if(!uploaded) return;
@@ -661,6 +673,10 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         const Precond* P = prec;
         ops.left = [P](double* xx, void* s) { P->left(xx, (hipStream_t) s); };
         ops.right = [P](double* xx, void* s) { P->right(xx, (hipStream_t) s); };
+        if(!op->kkt && !op->normal)
+            ops.spmv_left = [P, op](const double* xx, double* yy, void* s) {
+                return P->left_product(op->A, xx, yy, (hipStream_t) s);
+            };
         ops.tri = P->tri();
         ops.is_piv = P->is_piv();
     }

@@ -64,6 +64,13 @@ struct Precond {
     virtual const int32_t* is_piv() const = 0;
     // after a solve: report errors the device recorded (may synchronise st)
     virtual void after_solve(hipStream_t) const {}
+    // x = Pl \ (K src) in one pass where the preconditioner can fold the
+    // product into its left solve, with the same result as K src followed by
+    // left(); false: not done (the caller multiplies, then calls left)
+    virtual bool left_product(const DevCsr&, const double*, double*, hipStream_t) const
+    {
+        return false;
+    }
 };
 
 // Per sparsity pattern: the matrix as given, its symbolic analysis and the
@@ -175,6 +182,7 @@ struct Bildlt final : Precond {
     bool ready() const override { return factored; }
     void left(double* x, hipStream_t st) const override { trsv_ut(x, st); }
     void right(double* x, hipStream_t st) const override { trsv_u(x, st); }
+    bool left_product(const DevCsr& K, const double* src, double* x, hipStream_t st) const override;
     const double* tri() const override { return d_tri.as<double>(); }
     const int32_t* is_piv() const override { return d_is_piv.as<int32_t>(); }
     void after_solve(hipStream_t st) const override { check_solve_error(st); }

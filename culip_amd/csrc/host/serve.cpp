@@ -6,8 +6,8 @@
 
 namespace culip {
 
-// single-launch solves run at most 8 at a time on one GPU
-constexpr int32_t SINGLE_MAX_SLOTS = 8;
+// single-launch solves run at most TRSV_SINGLE_MAX_CONCURRENT at a time
+constexpr int32_t SINGLE_MAX_SLOTS = TRSV_SINGLE_MAX_CONCURRENT;
 
 Server::Server(const culip_server_opts& o) : o_(o)
 {
@@ -19,7 +19,7 @@ Server::Server(const culip_server_opts& o) : o_(o)
             (o.solve_algorithm == CULIP_TRSV_JACOBI && o.jacobi_sweeps >= 1),
         "server: bad solve algorithm");
     CULIP_REQUIRE(o.solve_algorithm != CULIP_TRSV_SINGLE || o.slots <= SINGLE_MAX_SLOTS,
-        "server: single-launch solves allow at most 8 slots");
+        "server: single-launch solves allow at most 4 slots");
     HIP_CHECK(hipGetDevice(&device_));
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
@@ -112,7 +112,7 @@ void Server::set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t
             (a == CULIP_TRSV_JACOBI && sweeps >= 1),
         "server: bad solve algorithm");
     CULIP_REQUIRE(a != CULIP_TRSV_SINGLE || o_.slots <= SINGLE_MAX_SLOTS,
-        "server: single-launch solves allow at most 8 slots");
+        "server: single-launch solves allow at most 4 slots");
     std::lock_guard<std::mutex> g(mu_);
     CULIP_REQUIRE(pattern >= 0 && pattern < (int32_t) patterns_.size(), "server: unknown pattern");
     ServePattern& P = *patterns_[pattern];

@@ -165,8 +165,8 @@ int32_t culip_bildlt_n(const culip_bildlt * h);
  * 226-228): how the block triangular solves run. SINGLE (default,
  * SOLVE_BLOCK): one launch per solve; resident waves take the block rows in
  * level order and each block row waits for the individual values it reads.
- * At most 8 single-launch solves may run on one GPU at the same time (each
- * holds an eighth of the GPU's wave slots until it ends). LEVELS: one launch
+ * At most 4 single-launch solves may run on one GPU at the same time (each
+ * holds a quarter of the wave slots the kernel can use until it ends). LEVELS: one launch
  * per level. SINGLE and LEVELS compute the same values in the same order.
  * JACOBI (SOLVE_JACOBI,
  * block_ildlt.impl.cuh:2454-2493): num_jacobi_sweeps >= 1 block-Jacobi
@@ -328,7 +328,7 @@ culip_status culip_sqmr_create_ldu(const culip_spmv * A, const culip_ldu * prec,
  * overlap on the device. Per request: BlockiLDLt refactor + SQMR.       */
 
 typedef struct {
-    int32_t slots;              /* concurrent solves, 1..64 (1..8 with
+    int32_t slots;              /* concurrent solves, 1..64 (1..4 with
                                    single-launch triangular solves) */
     int64_t hbm_budget;         /* device bytes for patterns and per-slot
                                    instances; 0: 90% of free memory */
@@ -353,7 +353,7 @@ culip_status culip_server_add_pattern(culip_server * s, const culip_csr * K,
     const int32_t * block_starts, int32_t * pattern);
 /* The triangular-solve schedule of one pattern (default: the server's);
  * only before the pattern's first submit (CULIP_ERR_STATE after);
- * CULIP_TRSV_SINGLE needs slots <= 8. */
+ * CULIP_TRSV_SINGLE needs slots <= 4. */
 culip_status culip_server_set_pattern_solve(culip_server * s, int32_t pattern,
     culip_solve_algorithm algorithm, int32_t jacobi_sweeps);
 /* Queues a solve: vals = nnz values of K in the pattern's entry order,

@@ -361,6 +361,33 @@ def test_single_launch_solves_equal_level_solves(hip_device, t):
             assert np.array_equal(x, xs[0]), side
 
 
+@pytest.mark.parametrize("t", ["d", "f"])
+def test_sqmr_single_launch_equals_levels(hip_device, t):
+    """SQMR over single-launch solves, where the left solve forms K ev itself
+    block row by block row, gives the same iterations, residuals and x, bit
+    for bit, as over level launches with a separate SpMV."""
+    import torch
+    k, p = _kkt(4000, 1600, 16, ca.ORDER_BLOCK_COLOR)
+    g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts,
+                      "f32" if t == "f" else "f64")
+    g.compute(0, 1.0, 1e-4)
+    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    b = torch.tensor(k.rhs[p.perm], device=hip_device)
+    out = {}
+    for algo in ("single", "levels"):
+        g.set_solve_algorithm(algo)
+        q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
+        x = torch.zeros_like(b)
+        q.solve(b, x)
+        torch.cuda.synchronize()
+        out[algo] = (x.cpu().numpy(), dict(q.result))
+    xs, rs = out["single"]
+    xl, rl = out["levels"]
+    assert rs["iterations"] == rl["iterations"] and rs["restarts"] == rl["restarts"]
+    assert rs["true_rel_residual"] == rl["true_rel_residual"] <= 1e-8
+    assert np.array_equal(xs, xl)
+
+
 def test_in_place_solves_equal_out_of_place(hip_device):
     """solve_left / solve_right with x = b (in place, as SQMR applies them)
     give the same bits as with separate arrays, for every schedule."""

@@ -151,30 +151,34 @@ def test_pattern_solve_schedule(hip_device):
 
 def test_single_launch_solves_limit_slots(hip_device):
     """A single-launch solve keeps its waves resident until it ends, so a
-    server refuses that schedule with more than 8 slots, as its default and
-    per pattern; with 8 slots it serves it and matches the one-system path
+    server refuses that schedule with more than 4 slots, as its default and
+    per pattern; with 4 slots it serves it and matches the one-system path
     bit for bit."""
     import torch
     with pytest.raises(ca.CulipError) as e:
-        ca.Server(slots=9, prec="f32", solve_algorithm="single")
+        ca.Server(slots=5, prec="f32", solve_algorithm="single")
     assert e.value.status == 1
     (k, p), = _patterns([3000])
-    s = ca.Server(slots=9, prec="f32", solve_algorithm="levels")
+    s = ca.Server(slots=5, prec="f32", solve_algorithm="levels")
     pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
     with pytest.raises(ca.CulipError) as e:
         s.set_pattern_solve(pid, "single")
     assert e.value.status == 1
     s.close()
-    vals, rhs = _request(k, p, 12)
-    s = ca.Server(slots=8, prec="f32", solve_algorithm="single")
+    # four requests at once on the four slots: four single launches resident
+    # together, each equal to the one-system path
+    reqs = [_request(k, p, 12 + r) for r in range(4)]
+    s = ca.Server(slots=4, prec="f32", solve_algorithm="single")
     pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
-    x_srv, res, _, _ = s.wait(s.submit(pid, vals, rhs))
+    ids = [s.submit(pid, v, r) for v, r in reqs]
+    outs = [s.wait(i) for i in ids]
     s.close()
-    assert res["true_rel_residual"] <= 1e-8
-    g = ca.BlockiLDLt(p.row_ptr, p.col, vals, p.piv_starts, p.block_starts, "f32")
-    g.compute(0, 1.0, 1e-4)
-    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, vals)
-    q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
-    x = torch.zeros(rhs.size, dtype=torch.float64, device=hip_device)
-    q.solve(torch.tensor(rhs, device=hip_device), x)
-    assert np.array_equal(x.cpu().numpy(), x_srv)
+    for (vals, rhs), (x_srv, res, _, _) in zip(reqs, outs):
+        assert res["true_rel_residual"] <= 1e-8
+        g = ca.BlockiLDLt(p.row_ptr, p.col, vals, p.piv_starts, p.block_starts, "f32")
+        g.compute(0, 1.0, 1e-4)
+        A = ca.CSRMatrixSPMV(p.row_ptr, p.col, vals)
+        q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
+        x = torch.zeros(rhs.size, dtype=torch.float64, device=hip_device)
+        q.solve(torch.tensor(rhs, device=hip_device), x)
+        assert np.array_equal(x.cpu().numpy(), x_srv)
