@@ -72,6 +72,44 @@ __device__ double reduce_part(const double* part, int G, int slot, double* sh)
     return block_sum(v, sh);
 }
 
+// reduce_part for K slots at once: the loads of all slots are issued
+// together, and every slot is summed exactly as reduce_part sums it (per
+// thread in q order, wave sums, then the wave totals in order). sh holds at
+// least K * (waves + 1) doubles.
+template<int K>
+__device__ void reduce_parts(const double* part, int G, const int (&slot)[K], double (&out)[K],
+    double* sh)
+{
+    const int tid = threadIdx.x, nw = (int) (blockDim.x >> 6);
+    double v[K];
+#pragma unroll
+    for(int k = 0; k < K; ++k) v[k] = 0.0;
+    for(int q = tid; q < G; q += blockDim.x) {
+        double x[K];
+#pragma unroll
+        for(int k = 0; k < K; ++k) x[k] = part[slot[k] * G + q];
+#pragma unroll
+        for(int k = 0; k < K; ++k) v[k] += x[k];
+    }
+#pragma unroll
+    for(int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if((tid & 63) == 0) {
+#pragma unroll
+        for(int k = 0; k < K; ++k) sh[k * nw + (tid >> 6)] = v[k];
+    }
+    __syncthreads();
+    if(tid < K) {
+        double t = 0.0;
+        for(int q = 0; q < nw; ++q) t += sh[tid * nw + q];
+        sh[K * nw + tid] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for(int k = 0; k < K; ++k) out[k] = sh[K * nw + k];
+    __syncthreads();
+}
+
 __device__ __forceinline__ double& part_at(double* part, int slot)
 {
     return part[slot * (int) gridDim.x + (int) blockIdx.x];
@@ -232,7 +270,9 @@ __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* A
 {
     __shared__ double sh[16];
     const int G = gridDim.x;
-    const double alpha = reduce_part(part, G, 0, sh) / reduce_part(part, G, 1, sh);
+    double r[2];
+    reduce_parts<2>(part, G, {0, 1}, r, sh);
+    const double alpha = r[0] / r[1];
     const double beta = scal[S_BETA + k % 3];
     double ss = 0.0;
     const int64_t S = (int64_t) G * TPB;
@@ -323,13 +363,13 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
 
 template<int K0>
 __device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
-    const double* part, int G, double* sh)
+    const double (&r)[5])
 {
 #pragma unroll
     for(int q = 0; q < NSCAL; ++q) S[q] = scal[q];
-    const double alpha = reduce_part(part, G, 0, sh) / reduce_part(part, G, 1, sh);
-    const double delta = sqrt(reduce_part(part, G, 2, sh));
-    const double beta = reduce_part(part, G, 3, sh) / reduce_part(part, G, 4, sh);
+    const double alpha = r[0] / r[1];
+    const double delta = sqrt(r[2]);
+    const double beta = r[3] / r[4];
     scalar_step<K0>(S, alpha, delta, beta);
 }
 
@@ -338,10 +378,12 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
     const double* pkm1, const double* pkm2, double* x, const double* scal, double* part,
     const int32_t* flags)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[64];
     double S[NSCAL];
     const int G = gridDim.x;
-    iteration_scalars<K0>(S, scal, part, G, sh);
+    double r[5];
+    reduce_parts<5>(part, G, {0, 1, 2, 3, 4}, r, sh);
+    iteration_scalars<K0>(S, scal, r);
     const double ib = 1.0 / S[S_BETA + (K0 + 1) % 3];
     const double nbrr = -S[S_BRR], neps = -S[S_EPS], ir = 1.0 / S[S_RHO], tau = S[S_TAU];
     // after a Lanczos breakdown (flagged by the previous iteration) the
@@ -399,10 +441,13 @@ template<int K0>
 __device__ __forceinline__ void scalars(int k, int G, double* scal, const double* part,
     int32_t* flags, double tol, double brel)
 {
-    __shared__ double sh[16];
+    __shared__ double sh[64];
     double S[NSCAL];
-    iteration_scalars<K0>(S, scal, part, G, sh);
-    const double nx = sqrt(reduce_part(part, G, 5, sh));
+    double r6[6];
+    reduce_parts<6>(part, G, {0, 1, 2, 3, 4, 5}, r6, sh);
+    const double r[5] = {r6[0], r6[1], r6[2], r6[3], r6[4]};
+    iteration_scalars<K0>(S, scal, r);
+    const double nx = sqrt(r6[5]);
     if(threadIdx.x == 0) {
         constexpr int k1 = (K0 + 1) % 3;
         S[S_NORMX] = nx;
