@@ -110,6 +110,26 @@ __device__ void reduce_parts(const double* part, int G, const int (&slot)[K], do
     __syncthreads();
 }
 
+// block_sum of two values at once (each summed exactly as block_sum sums it)
+__device__ void block_sum2(double& a, double& b, double* sh)
+{
+    const int tid = threadIdx.x, nw = (int) (blockDim.x >> 6);
+    a = wave_sum(a);
+    b = wave_sum(b);
+    __syncthreads();
+    if((tid & 63) == 0) { sh[tid >> 6] = a; sh[nw + (tid >> 6)] = b; }
+    __syncthreads();
+    if(tid < 2) {
+        double t = 0.0;
+        for(int q = 0; q < nw; ++q) t += sh[tid * nw + q];
+        sh[2 * nw + tid] = t;
+    }
+    __syncthreads();
+    a = sh[2 * nw];
+    b = sh[2 * nw + 1];
+    __syncthreads();
+}
+
 __device__ __forceinline__ double& part_at(double* part, int slot)
 {
     return part[slot * (int) gridDim.x + (int) blockIdx.x];
@@ -260,8 +280,7 @@ __global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, cons
             den = fma(vv[r], zz[r], den);
         }
     }
-    num = block_sum(num, sh);
-    den = block_sum(den, sh);
+    block_sum2(num, den, sh);
     if(threadIdx.x == 0) { part_at(part, 0) = num; part_at(part, 1) = den; }
 }
 
@@ -356,8 +375,7 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
             den = fma(a[r], c[r], den);
         }
     }
-    num = block_sum(num, sh);
-    den = block_sum(den, sh);
+    block_sum2(num, den, sh);
     if(threadIdx.x == 0) { part_at(part, 3) = num; part_at(part, 4) = den; }
 }
 
