@@ -79,7 +79,7 @@ PROBE_COPY_DOUBLES = 1 << 25   # 256 MiB read + 256 MiB written
 # x1.000); kept in profiles/kkt400k_hbm_traffic.md. Configurations without
 # such a run report null.
 MEASURED_TRAFFIC = {
-    ("kkt-400k", "f64", "color", "single"): {"k_trsv_chain": 137.0e6, "k_csr_spmv": 269.6e6},
+    ("kkt-400k", "f64", "color", "single"): {"k_trsv_chain": 140.7e6, "k_csr_spmv": 233.1e6},
     ("kkt-400k", "f64", "color", "levels"): {"k_trsv_level": 322.1e6, "k_csr_spmv": 269.8e6},
 }
 
