@@ -759,6 +759,24 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
     }
 }
 
+// The first block-Jacobi sweep with b = K src: each block row forms its rows
+// of K src as block_rhs does (the values k_csr_spmv gives), keeps them in b
+// for the later sweeps and solves from x_0 = 0 as k_jacobi_sweep does.
+template<typename VT>
+__global__ __launch_bounds__(64) void k_jacobi_first_product(const int4* task,
+    const int32_t* o_col, const VT* o_val, const uint8_t* o_lr, const int32_t* i_col,
+    const VT* i_val, const uint8_t* i_lr, CsrRhs rhs, double* b, double* xout)
+{
+    __shared__ double y[33];
+    __shared__ double res[33];
+    const int4 t0 = task[2 * blockIdx.x];
+    const double bo = block_rhs(rhs, t0.x, t0.y);
+    // each lane reads back only the row it wrote (lanes >= h are masked)
+    if(lane_id() < t0.y) b[t0.x + lane_id()] = bo;
+    block_row_solve<VT>(t0, task[2 * blockIdx.x + 1], false, o_col, o_val, o_lr, i_col, i_val,
+        i_lr, b, nullptr, xout, y, res);
+}
+
 __global__ void k_fill_bits(double* p, int64_t n, unsigned long long bits)
 {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -948,6 +966,14 @@ template<typename VT>
 void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, double* x,
     int32_t sweeps, double* b_tmp, double* x_a, double* x_b, void* stream)
 {
+    jacobi_sweeps_product<VT>(t, p, transpose, nullptr, nullptr, x, sweeps, b_tmp, x_a, x_b, stream);
+}
+
+template<typename VT>
+void jacobi_sweeps_product(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose,
+    const DevCsr* K, const double* src, double* x, int32_t sweeps, double* b_tmp, double* x_a,
+    double* x_b, void* stream)
+{
     if(t.nb <= 0 || sweeps <= 0) return;
     hipStream_t st = (hipStream_t) stream;
     const int4* task = transpose ? p.task_t : p.task_u;
@@ -960,13 +986,19 @@ void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, d
     const double* in = nullptr;
     for(int32_t s = 0; s < sweeps; ++s) {
         double* out = (s == sweeps - 1) ? x : ((s & 1) ? x_b : x_a);
-        k_jacobi_sweep<VT><<<t.nb, 64, 0, st>>>(task, oc, ov, ol, ic, iv, il, b_tmp, in, out);
+        if(s == 0 && K)
+            k_jacobi_first_product<VT><<<t.nb, 64, 0, st>>>(task, oc, ov, ol, ic, iv, il,
+                CsrRhs{K->row_ptr, K->col, K->val, src}, b_tmp, out);
+        else
+            k_jacobi_sweep<VT><<<t.nb, 64, 0, st>>>(task, oc, ov, ol, ic, iv, il, b_tmp, in, out);
         in = out;
     }
 }
 
 template void jacobi_sweeps<double>(const SolveTopo&, const SolvePlan<double>&, bool, double*, int32_t, double*, double*, double*, void*);
 template void jacobi_sweeps<float>(const SolveTopo&, const SolvePlan<float>&, bool, double*, int32_t, double*, double*, double*, void*);
+template void jacobi_sweeps_product<double>(const SolveTopo&, const SolvePlan<double>&, bool, const DevCsr*, const double*, double*, int32_t, double*, double*, double*, void*);
+template void jacobi_sweeps_product<float>(const SolveTopo&, const SolvePlan<float>&, bool, const DevCsr*, const double*, double*, int32_t, double*, double*, double*, void*);
 template int trsv_upper_single<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);
 template int trsv_upper_single<float>(const SolveTopo&, const SolvePlan<float>&, double*, int32_t, void*);
 template int trsv_upper_t_single<double>(const SolveTopo&, const SolvePlan<double>&, double*, int32_t, void*);

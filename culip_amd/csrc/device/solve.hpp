@@ -91,6 +91,11 @@ int trsv_upper_t_single_product(const SolveTopo& t, const SolvePlan<VT>& p, cons
 template<typename VT>
 void jacobi_sweeps(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose, double* x,
     int32_t sweeps, double* b_tmp, double* x_a, double* x_b, void* stream);
+// the same sweeps on b = K src, formed by the first sweep (into b_tmp)
+template<typename VT>
+void jacobi_sweeps_product(const SolveTopo& t, const SolvePlan<VT>& p, bool transpose,
+    const DevCsr* K, const double* src, double* x, int32_t sweeps, double* b_tmp, double* x_a,
+    double* x_b, void* stream);
 void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
     void* stream);
 

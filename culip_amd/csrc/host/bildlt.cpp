@@ -394,8 +394,23 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
 
 bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStream_t st) const
 {
+    if(K.m != n() || K.n != n()) return false;
+    if(solve_algo == CULIP_TRSV_JACOBI) {
+        // the first U' sweep forms each block row's K src itself
+        const size_t bytes = sizeof(double) * (size_t) std::max(1, n());
+        DevBuf* jb = const_cast<DevBuf*>(d_jac);
+        for(int q = 0; q < 3; ++q) jb[q].ensure(bytes);
+        if(f32())
+            jacobi_sweeps_product<float>(topo(), plan<float>(), true, &K, src, x, jacobi_sweeps,
+                jb[0].as<double>(), jb[1].as<double>(), jb[2].as<double>(), st);
+        else
+            jacobi_sweeps_product<double>(topo(), plan<double>(), true, &K, src, x, jacobi_sweeps,
+                jb[0].as<double>(), jb[1].as<double>(), jb[2].as<double>(), st);
+        HIP_CHECK(hipGetLastError());
+        return true;
+    }
     // the single-launch U' solve forms each block row's K src itself
-    if(solve_algo != CULIP_TRSV_SINGLE || K.m != n() || K.n != n()) return false;
+    if(solve_algo != CULIP_TRSV_SINGLE) return false;
     const int32_t par = chain_parity;
     chain_parity ^= 1;
     const int rc = f32() ? trsv_upper_t_single_product<float>(topo(), plan<float>(), &K, src, x, par, st)
