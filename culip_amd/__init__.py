@@ -356,9 +356,10 @@ class SQMR:
     """SQMR<double>(spmv, tol, max_it, prec); solve(b, x) -> (residual, iterations)."""
 
     def __init__(self, spmv, tol=1e-8, max_it=1000, prec=None, true_residual_period=0,
-                 max_restarts=0):
+                 max_restarts=0, separate_product=False):
         self._spmv, self._prec = spmv, prec
-        o = _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts)
+        o = _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts,
+                          1 if separate_product else 0)
         h = C.c_void_p()
         if isinstance(prec, ImportLDUPreconditioner):
             check(lib().culip_sqmr_create_ldu(spmv._h, prec._h, C.byref(o), C.byref(h)))

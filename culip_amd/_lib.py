@@ -41,7 +41,8 @@ class KktParams(C.Structure):
 
 class SqmrOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_it", C.c_int32),
-                ("true_residual_period", C.c_int32), ("max_restarts", C.c_int32)]
+                ("true_residual_period", C.c_int32), ("max_restarts", C.c_int32),
+                ("separate_product", C.c_int32)]
 
 
 class SqmrResult(C.Structure):

@@ -688,7 +688,7 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         const Precond* P = prec;
         ops.left = [P](double* xx, void* s) { P->left(xx, (hipStream_t) s); };
         ops.right = [P](double* xx, void* s) { P->right(xx, (hipStream_t) s); };
-        if(!op->kkt && !op->normal)
+        if(!op->kkt && !op->normal && opts.separate_product == 0)
             ops.spmv_left = [P, op](const double* xx, double* yy, void* s) {
                 return P->left_product(op->A, xx, yy, (hipStream_t) s);
             };

@@ -254,6 +254,11 @@ typedef struct {
      * drifted. */
     int32_t true_residual_period;
     int32_t max_restarts;
+    /* 0 (default): with a CSR operator, the left preconditioner forms
+     * A x itself where it can (single-launch and block-Jacobi BILDLt solves),
+     * with the same values; 1: apply A, then the left preconditioner, as two
+     * steps */
+    int32_t separate_product;
 } culip_sqmr_opts;
 
 typedef struct {

@@ -250,7 +250,7 @@ public:
         int32_t max_restarts = 0)
         : stream_(stream)
     {
-        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts};
+        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts, 0};
         check(culip_sqmr_create(spmv->handle(), prec ? prec->handle() : nullptr, &o, &s_));
     }
     template<typename T>
@@ -259,7 +259,7 @@ public:
         int32_t max_restarts = 0)
         : stream_(stream)
     {
-        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts};
+        culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts, 0};
         if(prec) check(culip_sqmr_create_ldu(spmv->handle(), prec->handle(), &o, &s_));
         else check(culip_sqmr_create(spmv->handle(), nullptr, &o, &s_));
     }

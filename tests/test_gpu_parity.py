@@ -388,6 +388,31 @@ def test_sqmr_single_launch_equals_levels(hip_device, t):
     assert np.array_equal(xs, xl)
 
 
+@pytest.mark.parametrize("algo,sw", [("single", 0), ("jacobi", 2)])
+def test_folded_product_equals_separate(hip_device, algo, sw):
+    """The left solve that forms K ev itself (single launch; first block-Jacobi
+    sweep) gives SQMR the same bits as K ev by the SpMV kernel followed by
+    the left solve."""
+    import torch
+    k, p = _kkt(4000, 1600, 16, ca.ORDER_BLOCK_COLOR)
+    g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, "f64")
+    g.compute(0, 1.0, 1e-4)
+    g.set_solve_algorithm(algo, sw)
+    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    b = torch.tensor(k.rhs[p.perm], device=hip_device)
+    out = []
+    for separate in (False, True):
+        q = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10,
+                    separate_product=separate)
+        x = torch.zeros_like(b)
+        q.solve(b, x)
+        torch.cuda.synchronize()
+        out.append((x.cpu().numpy(), dict(q.result)))
+    (xf, rf), (xs, rs) = out
+    assert rf == rs
+    assert np.array_equal(xf, xs)
+
+
 def test_in_place_solves_equal_out_of_place(hip_device):
     """solve_left / solve_right with x = b (in place, as SQMR applies them)
     give the same bits as with separate arrays, for every schedule."""
