@@ -9,8 +9,11 @@ Python mirror of the reference's C++ interfaces (dthuerck/culip):
 over the C-ABI of libculip_amd.so. Device vectors are torch CUDA (HIP)
 float64 tensors; torch only provides device memory and streams.
 """
+import atexit
 import ctypes as C
+import itertools
 import os
+import weakref
 
 import numpy as np
 
@@ -77,7 +80,47 @@ def _stream(stream):
     return C.c_void_p(int(stream))
 
 
-class KKT:
+# Live library objects, by creation order. At interpreter exit they are
+# released from an atexit hook, while the module, torch and the HIP runtime
+# are all still intact (servers first, then solvers, preconditioners,
+# operators and host objects; newer before older within a kind); after that
+# every __del__ does nothing. A __del__ that ran later, during module
+# teardown, could find the library's globals already cleared.
+_live = weakref.WeakValueDictionary()
+_serial = itertools.count()
+_closing = False
+
+
+class _Handle:
+    """An object that owns a library handle in self._h; subclasses free it
+    in _release()."""
+
+    def _track(self):
+        _live[next(_serial)] = self
+
+    def __del__(self):
+        if _closing is not False:
+            return
+        self._release()
+
+
+def _release_all():
+    global _closing
+    objs = [o for _, o in sorted(_live.items(), key=lambda kv: -kv[0])]
+    for kind in (Server, SQMR, BlockiLDLt, ImportLDUPreconditioner, _Spmv, Prep, KKT):
+        for o in objs:
+            if isinstance(o, kind):
+                try:
+                    o._release()
+                except Exception:   # exit goes on; the process frees the rest
+                    pass
+    _closing = True
+
+
+atexit.register(_release_all)
+
+
+class KKT(_Handle):
     """Synthetic KKT system [[H, A'], [A, -delta I]] (host)."""
 
     def __init__(self, n, m, h_offdiag_per_row=4, a_nnz_per_col=3, h_diag=(1.0, 10.0),
@@ -88,6 +131,7 @@ class KKT:
         h = C.c_void_p()
         check(lib().culip_kkt_generate(C.byref(p), C.byref(h)))
         self._h = h
+        self._track()
         self.n, self.m = n, m
         c = _lib.Csr()
         rhs = C.POINTER(C.c_double)()
@@ -101,13 +145,13 @@ class KKT:
         check(lib().culip_kkt_barrier_values(self._h, w_lo, w_hi, seed, _ptr(out, C.c_double)))
         return out
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_kkt_destroy(self._h)
             self._h = None
 
 
-class Prep:
+class Prep(_Handle):
     """Ordering, contiguous blocking and static 1x1 pivot starts."""
 
     def __init__(self, row_ptr, col, val, n_primal, max_block=32, ordering=ORDER_NATURAL):
@@ -115,6 +159,7 @@ class Prep:
         h = C.c_void_p()
         check(lib().culip_prep_create(C.byref(s), n_primal, max_block, ordering, C.byref(h)))
         self._h = h
+        self._track()
         c = _lib.Csr()
         perm = C.POINTER(C.c_int32)()
         nbk = C.c_int32()
@@ -135,13 +180,13 @@ class Prep:
                                               _ptr(out, C.c_double)))
         return out
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_prep_destroy(self._h)
             self._h = None
 
 
-class BlockiLDLt:
+class BlockiLDLt(_Handle):
     """BlockiLDLt<T, false, false> (static pivots); prec 'f64' or 'f32' = T."""
 
     def __init__(self, row_ptr, col, val, piv_starts, block_starts, prec="f64"):
@@ -152,6 +197,7 @@ class BlockiLDLt:
         check(lib().culip_bildlt_create(C.byref(s), len(ps), _ptr(ps, C.c_int32), len(bs),
                                         _ptr(bs, C.c_int32), pr, C.byref(h)))
         self._h = h
+        self._track()
         self.N = len(row_ptr) - 1
         self.prec = "f32" if pr == PREC_F32 else "f64"
 
@@ -218,13 +264,17 @@ class BlockiLDLt:
     def device_bytes(self):
         return lib().culip_bildlt_device_bytes(self._h)
 
-    def __del__(self):
+    def _inject_solve_error(self):
+        """Testing hook: records a timed-out single-launch solve from the host."""
+        check(lib().culip_bildlt_inject_solve_error(self._h))
+
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_bildlt_destroy(self._h)
             self._h = None
 
 
-class _Spmv:
+class _Spmv(_Handle):
     def multiply(self, x, y, alpha=1.0, beta=0.0, transpose=False, stream=None):
         check(lib().culip_spmv_multiply(self._h, _dptr(x), _dptr(y), alpha, beta,
                                         1 if transpose else 0, _stream(stream)))
@@ -235,7 +285,7 @@ class _Spmv:
     def n(self):
         return lib().culip_spmv_n(self._h)
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_spmv_destroy(self._h)
             self._h = None
@@ -247,6 +297,7 @@ class CSRMatrixSPMV(_Spmv):
         h = C.c_void_p()
         check(lib().culip_spmv_create_csr(C.byref(s), C.byref(h)))
         self._h = h
+        self._track()
 
     def update_values(self, vals, stream=None):
         v = _f64(vals)
@@ -266,6 +317,7 @@ class AugmentedMatrixSPMV(_Spmv):
                                           _ptr(ixa, C.c_int32) if len(ixa) else None,
                                           C.byref(h)))
         self._h = h
+        self._track()
 
 
 class NormalMatrixSPMV(_Spmv):
@@ -281,9 +333,10 @@ class NormalMatrixSPMV(_Spmv):
                                              _ptr(ixa, C.c_int32) if len(ixa) else None,
                                              C.byref(h)))
         self._h = h
+        self._track()
 
 
-class ImportLDUPreconditioner:
+class ImportLDUPreconditioner(_Handle):
     """ImportLDUPreconditioner<T> (import_ldu_preconditioner.cuh:36-125):
     M = inv(S_r) P_r' L D R P_c' inv(S_c) with R = L' (U None) or U.
 
@@ -304,6 +357,7 @@ class ImportLDUPreconditioner:
                                      _ptr(p, C.c_int32) if p is not None else None,
                                      1 if exact_solve else 0, pr, C.byref(h)))
         self._h = h
+        self._track()
         self.N = len(L[0]) - 1
 
     def n(self):
@@ -346,13 +400,13 @@ class ImportLDUPreconditioner:
     def solve_right(self, b, x, transpose=False, stream=None):
         self._solve(SOLVE_RIGHT, b, x, transpose, stream)
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_ldu_destroy(self._h)
             self._h = None
 
 
-class SQMR:
+class SQMR(_Handle):
     """SQMR<double>(spmv, tol, max_it, prec); solve(b, x) -> (residual, iterations)."""
 
     def __init__(self, spmv, tol=1e-8, max_it=1000, prec=None, true_residual_period=0,
@@ -367,6 +421,7 @@ class SQMR:
             check(lib().culip_sqmr_create(spmv._h, prec._h if prec is not None else None,
                                           C.byref(o), C.byref(h)))
         self._h = h
+        self._track()
         self.result = None
 
     def solve(self, b, x, stream=None):
@@ -377,13 +432,13 @@ class SQMR:
                        "true_rel_residual": r.true_rel_residual}
         return r.residual, r.iterations
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None):
             lib().culip_sqmr_destroy(self._h)
             self._h = None
 
 
-class Server:
+class Server(_Handle):
     """Request loop for KKT solves over resident sparsity patterns.
 
     add_pattern(K, piv_starts, block_starts) -> pattern id (symbolic analysis
@@ -401,6 +456,7 @@ class Server:
         h = C.c_void_p()
         check(lib().culip_server_create(C.byref(o), C.byref(h)))
         self._h = h
+        self._track()
         self._n = {}
         self._req_n = {}
 
@@ -450,7 +506,7 @@ class Server:
             lib().culip_server_destroy(self._h)
             self._h = None
 
-    def __del__(self):
+    def _release(self):
         self.close()
 
 

@@ -1,0 +1,146 @@
+"""Batches of independent KKT systems of uneven size: the block-size and
+solve-schedule autotuner, and the batched solve over a request server.
+
+The autotuner sorts systems into size buckets and, per bucket, times every
+candidate (maximum block size, triangular-solve schedule) on the bucket's
+first system: numeric refactor plus SQMR to the FP64 true-residual
+tolerance, best of a few repetitions after a warm-up. A candidate that
+misses the tolerance is never picked. The batch then runs on a Server with
+each system's pattern registered under its bucket's block size and
+schedule, so that the systems' solves overlap on the device.
+"""
+import time
+
+import numpy as np
+
+from . import (BlockiLDLt, CSRMatrixSPMV, KKT, ORDER_BLOCK_COLOR, Prep, SQMR, Server)
+
+BUCKETS = ((1000, 10000), (10000, 20000), (20000, 30000), (30000, 40001))
+BLOCK_SIZES = (8, 16, 32)
+
+
+def bucket_of(n, buckets=BUCKETS):
+    for i, (lo, hi) in enumerate(buckets):
+        if lo <= n < hi:
+            return i
+    raise ValueError(f"system size {n} outside every bucket")
+
+
+def schedules(slots):
+    """Candidate triangular-solve schedules: level launches and block-Jacobi
+    with 2 or 3 sweeps, and the single launch (which the library runs as
+    level launches whenever four others are in flight)."""
+    return [("single", 0), ("levels", 0), ("jacobi", 2), ("jacobi", 3)]
+
+
+def kkt_system(n, seed, B, order=ORDER_BLOCK_COLOR, m=None):
+    """The batch generator: n primal, m = 0.4 n dual, 4 H off-diagonals
+    per row, 3 A entries per column; returns (KKT, Prep)."""
+    k = KKT(n, int(0.4 * n) if m is None else m, 4, 3, seed=seed)
+    p = Prep(k.row_ptr, k.col, k.val, n, B, order)
+    return k, p
+
+
+def _time_candidate(p, rhs, prec, B, algo, sweeps, reps, tol, dev):
+    import torch
+    g = BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, prec)
+    g.compute(0, 1.0, 1e-4)
+    A = CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    b = torch.tensor(rhs, device=dev)
+    dv = torch.tensor(p.val, device=dev)
+    g.set_solve_algorithm(algo, sweeps)
+    q = SQMR(A, tol, 1000, g, true_residual_period=20, max_restarts=10)
+    x = torch.zeros_like(b)
+    g.refactor(dv, 1e-4)
+    q.solve(b, x)
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g.refactor(dv, 1e-4)
+        q.solve(b, x)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    if q.result["true_rel_residual"] > tol:
+        return float("inf")
+    return best * 1e3
+
+
+def autotune(sizes, make_system, prec="f32", slots=16, block_sizes=BLOCK_SIZES, reps=3,
+             tol=1e-8, buckets=BUCKETS, log=None):
+    """Per size bucket present in `sizes`: (block size, schedule, sweeps)
+    with the shortest refactor + solve of the bucket's first system.
+    make_system(i, B) -> (KKT, Prep) of system i with maximum block B.
+    Returns (choice by bucket index, all timings in ms by key)."""
+    import torch
+    dev = torch.device("cuda:0")
+    tuned, timings = {}, {}
+    for bi, (lo, hi) in enumerate(buckets):
+        idx = [i for i, n in enumerate(sizes) if lo <= n < hi]
+        if not idx:
+            continue
+        i = idx[0]
+        best = None
+        for B in block_sizes:
+            k, p = make_system(i, B)
+            rhs = k.rhs[p.perm]
+            for algo, sw in schedules(slots):
+                ms = _time_candidate(p, rhs, prec, B, algo, sw, reps, tol, dev)
+                name = algo if algo != "jacobi" else f"{algo}{sw}"
+                timings[f"{lo}-{hi - 1}:B{B}:{name}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, B, algo, sw)
+        if best[0] == float("inf"):
+            raise RuntimeError(f"autotune: no candidate meets the tolerance in bucket {lo}-{hi - 1}")
+        tuned[bi] = best[1:]
+        if log:
+            log(f"autotune bucket {lo}-{hi - 1}: block size {best[1]}, {best[2]} {best[3]}")
+    return tuned, timings
+
+
+def solve_batch(systems, schedule_of, slots, prec, tol=1e-8, max_it=1000, passes=2, log=None):
+    """Solves every system (prep, rhs) on a Server with `slots` streams.
+    schedule_of(i) -> (algorithm, sweeps) or None for the server default
+    (single launch). Registration (symbolic analysis and upload) is timed
+    apart; each pass submits every system at once and waits for all; the
+    first pass also creates the slots' numeric instances.
+    Returns (results of the last pass, seconds per pass, registration
+    seconds)."""
+    import concurrent.futures as cf
+    srv = Server(slots=slots, prec=prec, tol=tol, max_it=max_it, solve_algorithm="single")
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(16) as ex:
+        pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,
+                                                     s[0].piv_starts, s[0].block_starts),
+                           systems))
+    t_reg = time.perf_counter() - t0
+    for i, pid in enumerate(pids):
+        sch = schedule_of(i)
+        if sch is not None:
+            srv.set_pattern_solve(pid, sch[0], sch[1])
+    secs = []
+    out = None
+    for rep in range(passes):
+        t0 = time.perf_counter()
+        ids = [srv.submit(pid, s[0].val, s[1]) for pid, s in zip(pids, systems)]
+        out = [srv.wait(r) for r in ids]
+        secs.append(time.perf_counter() - t0)
+        if log:
+            log(f"batch pass {rep} ({slots} slots, {prec}): {len(out) / secs[-1]:.1f} solves/s")
+    srv.close()
+    return out, secs, t_reg
+
+
+def single_system_solve(p, rhs, prec, algo, sweeps, tol=1e-8, max_it=1000):
+    """The one-system path with the server's SQMR options: x (host) and the
+    result dict."""
+    import torch
+    g = BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, prec)
+    g.compute(0, 1.0, 1e-4)
+    g.set_solve_algorithm(algo, sweeps)
+    A = CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    q = SQMR(A, tol, max_it, g, true_residual_period=20, max_restarts=10)
+    b = torch.tensor(np.ascontiguousarray(rhs), device="cuda:0")
+    x = torch.zeros_like(b)
+    q.solve(b, x)
+    return x.cpu().numpy(), dict(q.result)

@@ -548,7 +548,8 @@ __device__ void wait_published(const double* pub, const int (&c)[TRSV_PF], doubl
     while(wave_ballot(!all)) {
         __builtin_amdgcn_s_sleep(1);
         if(++spins > SPIN_LIMIT) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // err is pinned host memory: the host reads it without a copy
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return;
         }
         all = true;

@@ -164,12 +164,12 @@ void Bildlt::upload_pattern()
     d_totals.ensure(sizeof(int32_t) * 4);
     d_scan_tmp.ensure(sizeof(int32_t) * scan_tmp_ints(n));
     d_chain_pub.ensure(sizeof(double) * 2 * (size_t) std::max(1, n));
-    d_trsv_err.ensure(sizeof(int32_t));
+    h_trsv_err.ensure();
     chain_reset(d_chain_pub.as<double>(), 2 * (int64_t) n, nullptr);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipDeviceSynchronize());
     chain_parity = 0;
-    HIP_CHECK(hipMemset(d_trsv_err.p, 0, d_trsv_err.bytes));
+    (void) h_trsv_err.take();
     uploaded = true;
 }
 
@@ -262,7 +262,7 @@ SolveTopo Bildlt::topo() const
     t.tri = d_tri.as<double>();
     t.is_piv = d_is_piv.as<int32_t>();
     t.chain_pub = d_chain_pub.as<double>();
-    t.err = d_trsv_err.as<int32_t>();
+    t.err = h_trsv_err.p;
     return t;
 }
 
@@ -312,6 +312,7 @@ void Bildlt::factor_impl(hipStream_t st)
 void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t st)
 {
     if(!have_sym) fail(CULIP_ERR_STATE, "factor: symbolic phase has not run");
+    take_async_error();
     if(!uploaded) upload_pattern();
     threshold = thr;
     const size_t bytes = sizeof(double) * (size_t) A.nnz();
@@ -362,14 +363,15 @@ void Bildlt::trsv_u(double* x, hipStream_t st) const
         jacobi(x, false, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE) {
-        const int32_t par = chain_parity;
-        chain_parity ^= 1;
-        const int rc = f32() ? trsv_upper_single<float>(topo(), plan<float>(), x, par, st)
-                             : trsv_upper_single<double>(topo(), plan<double>(), x, par, st);
-        hip_check((hipError_t) rc, "single-launch U solve");
+    if(solve_algo == CULIP_TRSV_SINGLE &&
+        chain_admitted(st, [&] {
+            const int32_t par = chain_parity;
+            chain_parity ^= 1;
+            return (hipError_t) (f32() ? trsv_upper_single<float>(topo(), plan<float>(), x, par, st)
+                                       : trsv_upper_single<double>(topo(), plan<double>(), x, par, st));
+        }))
         return;
-    }
+    // level launches (also when a single launch was not admitted: same bits)
     if(f32()) trsv_upper<float>(topo(), plan<float>(), x, st);
     else trsv_upper<double>(topo(), plan<double>(), x, st);
 }
@@ -380,14 +382,14 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
         jacobi(x, true, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE) {
-        const int32_t par = chain_parity;
-        chain_parity ^= 1;
-        const int rc = f32() ? trsv_upper_t_single<float>(topo(), plan<float>(), x, par, st)
-                             : trsv_upper_t_single<double>(topo(), plan<double>(), x, par, st);
-        hip_check((hipError_t) rc, "single-launch U' solve");
+    if(solve_algo == CULIP_TRSV_SINGLE &&
+        chain_admitted(st, [&] {
+            const int32_t par = chain_parity;
+            chain_parity ^= 1;
+            return (hipError_t) (f32() ? trsv_upper_t_single<float>(topo(), plan<float>(), x, par, st)
+                                       : trsv_upper_t_single<double>(topo(), plan<double>(), x, par, st));
+        }))
         return;
-    }
     if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
     else trsv_upper_t<double>(topo(), plan<double>(), x, st);
 }
@@ -409,29 +411,37 @@ bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStre
         HIP_CHECK(hipGetLastError());
         return true;
     }
-    // the single-launch U' solve forms each block row's K src itself
+    // the single-launch U' solve forms each block row's K src itself; when
+    // it is not admitted the caller multiplies and calls left() (same bits)
     if(solve_algo != CULIP_TRSV_SINGLE) return false;
-    const int32_t par = chain_parity;
-    chain_parity ^= 1;
-    const int rc = f32() ? trsv_upper_t_single_product<float>(topo(), plan<float>(), &K, src, x, par, st)
-                         : trsv_upper_t_single_product<double>(topo(), plan<double>(), &K, src, x, par, st);
-    hip_check((hipError_t) rc, "single-launch U' solve with the operator product");
-    return true;
+    return chain_admitted(st, [&] {
+        const int32_t par = chain_parity;
+        chain_parity ^= 1;
+        return (hipError_t) (f32()
+                ? trsv_upper_t_single_product<float>(topo(), plan<float>(), &K, src, x, par, st)
+                : trsv_upper_t_single_product<double>(topo(), plan<double>(), &K, src, x, par, st));
+    });
 }
 
 void Bildlt::check_solve_error(hipStream_t st) const
 {
     if(!uploaded) return;
-    int32_t e = 0;
-    HIP_CHECK(hipMemcpyAsync(&e, d_trsv_err.p, sizeof(e), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    if(e) fail(CULIP_ERR_HIP, "triangular solve: a dependency wait timed out");
+    take_async_error();
+}
+
+void Bildlt::take_async_error() const
+{
+    if(h_trsv_err.take())
+        fail(CULIP_ERR_HIP, "triangular solve: a dependency wait of a single-launch solve timed "
+                            "out (its x holds NaN)");
 }
 
 void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
     hipStream_t st) const
 {
     if(!factored) fail(CULIP_ERR_STATE, "solve: preconditioner has not been computed");
+    take_async_error();
     const size_t bytes = sizeof(double) * (size_t) n();
     if(side == CULIP_SOLVE_MIDDLE) {
         solve_D(n(), d_tri.as<double>(), d_is_piv.as<int32_t>(), b, x, st);
@@ -453,8 +463,7 @@ int64_t Bildlt::numeric_bytes() const
             &d_carry, &d_totals, &d_task[0], &d_task[1], &d_keep, &d_granted, &d_chunk,
             &d_scan_tmp, &d_piv_umask, &d_defer_list, &d_defer_cnt, &d_wave_list, &d_wave_cnt,
             &d_store_list, &d_store_cnt,
-            &d_chain_pub,
-            &d_trsv_err, &d_jac[0], &d_jac[1], &d_jac[2]})
+            &d_chain_pub, &d_jac[0], &d_jac[1], &d_jac[2]})
         b += (int64_t) d->bytes;
     for(int q = 0; q < 4; ++q)
         b += (int64_t) (d_ptr[q].bytes + d_col[q].bytes + d_val[q].bytes + d_lr[q].bytes);

@@ -500,6 +500,15 @@ culip_status culip_bildlt_get_real(culip_bildlt* h, const char* name, const doub
     });
 }
 
+culip_status culip_bildlt_inject_solve_error(culip_bildlt* h)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null argument");
+        if(!h->b.uploaded || !h->b.h_trsv_err.p) fail(CULIP_ERR_STATE, "bildlt: not uploaded");
+        __atomic_store_n(h->b.h_trsv_err.p, 1, __ATOMIC_RELEASE);
+    });
+}
+
 void culip_bildlt_destroy(culip_bildlt* h) { delete h; }
 
 // ---- SPMV -----------------------------------------------------------------

@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 
 #include "common.hpp"
 #include "symbolic.hpp"
@@ -43,6 +44,47 @@ struct DevBuf {
     }
     template<typename T> T* as() const { return (T*) p; }
 };
+
+// One int32 of pinned, host-coherent memory that kernels can write and the
+// host can read without a copy (error flags).
+struct PinnedFlag {
+    int32_t* p = nullptr;
+    PinnedFlag() = default;
+    PinnedFlag(const PinnedFlag&) = delete;
+    PinnedFlag& operator=(const PinnedFlag&) = delete;
+    ~PinnedFlag()
+    {
+        if(p) (void) hipHostFree(p);
+    }
+    void ensure()
+    {
+        if(p) return;
+        HIP_CHECK(hipHostMalloc((void**) &p, sizeof(int32_t), hipHostMallocCoherent));
+        *p = 0;
+    }
+    // the flag's value, cleared (0 when never allocated)
+    int32_t take()
+    {
+        if(!p) return 0;
+        const int32_t v = __atomic_exchange_n(p, 0, __ATOMIC_ACQ_REL);
+        return v;
+    }
+};
+
+// Runs `launch` (a single-launch triangular solve on st) if the process-wide
+// admission of such solves allows it now; returns false, without calling
+// launch, when TRSV_SINGLE_MAX_CONCURRENT other streams of the device still
+// have one in flight or st is being captured. chain_guard.hpp explains the
+// rule.
+bool chain_admitted_launch(hipStream_t st, hipError_t (*launch)(void*), void* ctx);
+
+template<typename F>
+bool chain_admitted(hipStream_t st, F&& f)
+{
+    using Fn = std::remove_reference_t<F>;
+    return chain_admitted_launch(
+        st, [](void* c) -> hipError_t { return (*static_cast<Fn*>(c))(); }, (void*) &f);
+}
 
 template<typename T>
 void upload(DevBuf& d, const std::vector<T>& h)
@@ -117,9 +159,11 @@ struct Bildlt final : Precond {
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
     DevBuf d_totals, d_scan_tmp;
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
-    // single-launch triangular solves: published values, wait error, and
-    // the half of d_chain_pub the next such solve publishes into
-    DevBuf d_chain_pub, d_trsv_err;
+    // single-launch triangular solves: published values, and the half of
+    // d_chain_pub the next such solve publishes into; h_trsv_err is set by
+    // a solve whose dependency wait gave up (pinned: read without a copy)
+    DevBuf d_chain_pub;
+    mutable PinnedFlag h_trsv_err;
     culip_solve_algorithm solve_algo = CULIP_TRSV_SINGLE;
     int32_t jacobi_sweeps = 0;
     mutable int32_t chain_parity = 0;
@@ -164,8 +208,12 @@ struct Bildlt final : Precond {
     void jacobi(double* x, bool transpose, hipStream_t st) const;
     // set_solve_algorithm; allocates the block-Jacobi vectors when needed
     void set_solve(culip_solve_algorithm a, int32_t sweeps);
-    // fails if a single-launch solve gave up waiting (synchronises st)
+    // fails if a single-launch solve gave up waiting (synchronises st);
+    // the flag is cleared, so the failure is reported once
     void check_solve_error(hipStream_t st) const;
+    // the same without synchronising: reports what solves that already
+    // completed recorded (entry of every solve, factor and SQMR call)
+    void take_async_error() const;
     SolveTopo topo() const;
     int64_t device_bytes() const;
     void download_factor();

@@ -6,9 +6,6 @@
 
 namespace culip {
 
-// single-launch solves run at most TRSV_SINGLE_MAX_CONCURRENT at a time
-constexpr int32_t SINGLE_MAX_SLOTS = TRSV_SINGLE_MAX_CONCURRENT;
-
 Server::Server(const culip_server_opts& o) : o_(o)
 {
     CULIP_REQUIRE(o.slots >= 1 && o.slots <= 64, "server: slots must be in [1, 64]");
@@ -18,8 +15,6 @@ Server::Server(const culip_server_opts& o) : o_(o)
     CULIP_REQUIRE(o.solve_algorithm == CULIP_TRSV_LEVELS || o.solve_algorithm == CULIP_TRSV_SINGLE ||
             (o.solve_algorithm == CULIP_TRSV_JACOBI && o.jacobi_sweeps >= 1),
         "server: bad solve algorithm");
-    CULIP_REQUIRE(o.solve_algorithm != CULIP_TRSV_SINGLE || o.slots <= SINGLE_MAX_SLOTS,
-        "server: single-launch solves allow at most 4 slots");
     HIP_CHECK(hipGetDevice(&device_));
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
@@ -111,8 +106,6 @@ void Server::set_pattern_solve(int32_t pattern, culip_solve_algorithm a, int32_t
     CULIP_REQUIRE(a == CULIP_TRSV_LEVELS || a == CULIP_TRSV_SINGLE ||
             (a == CULIP_TRSV_JACOBI && sweeps >= 1),
         "server: bad solve algorithm");
-    CULIP_REQUIRE(a != CULIP_TRSV_SINGLE || o_.slots <= SINGLE_MAX_SLOTS,
-        "server: single-launch solves allow at most 4 slots");
     std::lock_guard<std::mutex> g(mu_);
     CULIP_REQUIRE(pattern >= 0 && pattern < (int32_t) patterns_.size(), "server: unknown pattern");
     ServePattern& P = *patterns_[pattern];
@@ -322,6 +315,21 @@ void Server::worker(Slot* s)
             if(!lk.owns_lock()) lk.lock();
             r->status = CULIP_ERR_INVALID;
             r->err = e.what();
+        }
+        if(in && r->status != CULIP_OK) {
+            // a failed request may leave its instance in any state: the
+            // next request of the pattern builds a fresh one
+            auto it = s->cache.find(r->pattern);
+            if(it != s->cache.end() && it->second.get() == in) {
+                std::unique_ptr<ServeInstance> dead = std::move(it->second);
+                s->cache.erase(it);
+                used_ -= dead->bytes;
+                --instances_;
+                lk.unlock();
+                dead.reset();
+                lk.lock();
+            }
+            in = nullptr;
         }
         if(in) in->busy = false;
         r->t_end = now();

@@ -165,8 +165,13 @@ int32_t culip_bildlt_n(const culip_bildlt * h);
  * 226-228): how the block triangular solves run. SINGLE (default,
  * SOLVE_BLOCK): one launch per solve; resident waves take the block rows in
  * level order and each block row waits for the individual values it reads.
- * At most 4 single-launch solves may run on one GPU at the same time (each
- * holds a quarter of the wave slots the kernel can use until it ends). LEVELS: one launch
+ * Such a solve holds a quarter of the wave slots the kernel can use until
+ * it ends, so the library admits one only while fewer than 4 other streams
+ * of the device have one in flight; otherwise (and on a stream under graph
+ * capture) that solve runs as LEVELS, with the same bits. Solves that other
+ * processes run on the same GPU are not counted: a single-launch solve that
+ * then gives up waiting leaves NaN in x and the next solve, refactor or
+ * SQMR call on the object returns CULIP_ERR_HIP (once). LEVELS: one launch
  * per level. SINGLE and LEVELS compute the same values in the same order.
  * JACOBI (SOLVE_JACOBI,
  * block_ildlt.impl.cuh:2454-2493): num_jacobi_sweeps >= 1 block-Jacobi
@@ -209,6 +214,10 @@ culip_status culip_bildlt_get_real(culip_bildlt * h, const char * name,
     const double ** ptr, int64_t * len);
 /* Device bytes held by this object. */
 int64_t culip_bildlt_device_bytes(const culip_bildlt * h);
+/* Testing hook, no counterpart in the reference: records, from the host, a
+ * single-launch solve whose dependency wait gave up, as the kernel would
+ * (after upload; CULIP_ERR_STATE before). */
+culip_status culip_bildlt_inject_solve_error(culip_bildlt * h);
 void culip_bildlt_destroy(culip_bildlt * h);
 
 /* ------------------------------------------------------------------------ */
@@ -357,8 +366,9 @@ culip_status culip_server_add_pattern(culip_server * s, const culip_csr * K,
     int32_t num_pivs, const int32_t * piv_starts, int32_t num_blocks,
     const int32_t * block_starts, int32_t * pattern);
 /* The triangular-solve schedule of one pattern (default: the server's);
- * only before the pattern's first submit (CULIP_ERR_STATE after);
- * CULIP_TRSV_SINGLE needs slots <= 4. */
+ * only before the pattern's first submit (CULIP_ERR_STATE after). With
+ * CULIP_TRSV_SINGLE and more than 4 slots, solves beyond 4 at a time run as
+ * level launches (same bits). */
 culip_status culip_server_set_pattern_solve(culip_server * s, int32_t pattern,
     culip_solve_algorithm algorithm, int32_t jacobi_sweeps);
 /* Queues a solve: vals = nnz values of K in the pattern's entry order,

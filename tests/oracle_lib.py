@@ -168,3 +168,27 @@ def solve_D(tri, is_piv, b):
     tr, ip, bb = _d(tri), _i(is_piv), _d(b).copy()
     lib().or_solve_D(len(bb), _pd(tr), _pi(ip), _pd(bb))
     return bb
+
+
+def sqmr_bildlt_refined(row_ptr, col, val, prec, b, tol=1e-8, max_it=1000, max_rounds=20):
+    """The reference SQMR (its quasi-residual stop, sqmr.impl.cuh:89-107)
+    followed by rounds of iterative refinement, x += SQMR(b - K x), until
+    the FP64 true relative residual ||b - K x|| / ||b|| meets tol: the
+    stopping rule of the device path's true-residual gate. Returns
+    (x, total iterations, refinement rounds, true relative residual)."""
+    b = _d(b)
+    x = np.zeros_like(b)
+    nb = np.linalg.norm(b)
+    its, rounds = 0, 0
+    rel = 1.0
+    for rounds in range(max_rounds + 1):
+        r = b - csr_spmv(row_ptr, col, val, x)
+        rel = np.linalg.norm(r) / nb if nb > 0 else 0.0
+        if rel <= tol or rounds == max_rounds:
+            break
+        # the round's own stop is relative to ||r|| and to ||K|| ||x||, so it
+        # aims at 1% of the remaining target tol ||b|| / ||r||
+        dx, res = sqmr_bildlt(row_ptr, col, val, prec, r, min(1.0, 0.01 * tol / rel), max_it)
+        its += res.iterations
+        x += dx
+    return x, its, rounds, rel

@@ -149,24 +149,12 @@ def test_pattern_solve_schedule(hip_device):
     assert np.array_equal(x2, x_srv)
 
 
-def test_single_launch_solves_limit_slots(hip_device):
-    """A single-launch solve keeps its waves resident until it ends, so a
-    server refuses that schedule with more than 4 slots, as its default and
-    per pattern; with 4 slots it serves it and matches the one-system path
-    bit for bit."""
+def test_single_launch_solves_four_slots(hip_device):
+    """Four requests at once on four slots: four single launches resident
+    together, each equal to the one-system path bit for bit (more slots:
+    test_gpu_concurrency)."""
     import torch
-    with pytest.raises(ca.CulipError) as e:
-        ca.Server(slots=5, prec="f32", solve_algorithm="single")
-    assert e.value.status == 1
     (k, p), = _patterns([3000])
-    s = ca.Server(slots=5, prec="f32", solve_algorithm="levels")
-    pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
-    with pytest.raises(ca.CulipError) as e:
-        s.set_pattern_solve(pid, "single")
-    assert e.value.status == 1
-    s.close()
-    # four requests at once on the four slots: four single launches resident
-    # together, each equal to the one-system path
     reqs = [_request(k, p, 12 + r) for r in range(4)]
     s = ca.Server(slots=4, prec="f32", solve_algorithm="single")
     pid = s.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
