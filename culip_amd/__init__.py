@@ -240,6 +240,14 @@ class BlockiLDLt(_Handle):
         check(lib().culip_bildlt_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),
                                        _stream(stream)))
 
+    def solve_noalloc(self, side, b, x, transpose=False, stream=None):
+        """solve_<side> without allocating (culip_bildlt_solve_noalloc):
+        exact solves as level launches, block-Jacobi work in the library's
+        static scratch; side 'left' | 'middle' | 'right'."""
+        sd = {"left": SOLVE_LEFT, "middle": SOLVE_MIDDLE, "right": SOLVE_RIGHT}[side]
+        check(lib().culip_bildlt_solve_noalloc(self._h, sd, 1 if transpose else 0, _dptr(b),
+                                               _dptr(x), _stream(stream)))
+
     def solve_left(self, b, x, transpose=False, stream=None):
         self._solve(SOLVE_LEFT, b, x, transpose, stream)
 
@@ -390,6 +398,14 @@ class ImportLDUPreconditioner(_Handle):
     def _solve(self, side, b, x, transpose, stream):
         check(lib().culip_ldu_solve(self._h, side, 1 if transpose else 0, _dptr(b), _dptr(x),
                                     _stream(stream)))
+
+    def solve_noalloc(self, side, b, x, transpose=False, stream=None):
+        """solve_<side> without allocating (culip_bildlt_solve_noalloc):
+        exact solves as level launches, block-Jacobi work in the library's
+        static scratch; side 'left' | 'middle' | 'right'."""
+        sd = {"left": SOLVE_LEFT, "middle": SOLVE_MIDDLE, "right": SOLVE_RIGHT}[side]
+        check(lib().culip_bildlt_solve_noalloc(self._h, sd, 1 if transpose else 0, _dptr(b),
+                                               _dptr(x), _stream(stream)))
 
     def solve_left(self, b, x, transpose=False, stream=None):
         self._solve(SOLVE_LEFT, b, x, transpose, stream)

@@ -106,6 +106,8 @@ _SIGS = [
                                         C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64)]),
     ("culip_bildlt_device_bytes", C.c_int64, [C.c_void_p]),
     ("culip_bildlt_inject_solve_error", C.c_int, [C.c_void_p]),
+    ("culip_bildlt_solve_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
     ("culip_bildlt_destroy", None, [C.c_void_p]),
     ("culip_spmv_create_csr", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_void_p)]),
     ("culip_spmv_create_kkt", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_double), C.c_double,

@@ -455,6 +455,46 @@ void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
     else trsv_ut(x, st);
 }
 
+void Bildlt::solve_noalloc(culip_side side, bool transpose, const double* b, double* x,
+    hipStream_t st) const
+{
+    if(n() > NOALLOC_MAX_ROWS)
+        fail(CULIP_ERR_INVALID, "solve_noalloc: " + std::to_string(n()) + " rows exceed the "
+            "static scratch of " + std::to_string(NOALLOC_MAX_ROWS) + " rows");
+    if(!factored) fail(CULIP_ERR_STATE, "solve: preconditioner has not been computed");
+    // the scratch array's address on this device (looked up on the first
+    // call per device)
+    double* s0 = noalloc_scratch();
+    if(!s0) fail(CULIP_ERR_HIP, "solve_noalloc: the scratch array's device address is unknown");
+    take_async_error();
+    if(side == CULIP_SOLVE_MIDDLE) {
+        solve_D(n(), d_tri.as<double>(), d_is_piv.as<int32_t>(), b, x, st);
+        return;
+    }
+    const bool right = (side == CULIP_SOLVE_RIGHT) != transpose;
+    const size_t bytes = sizeof(double) * (size_t) n();
+    if(b != x) HIP_CHECK(hipMemcpyAsync(x, b, bytes, hipMemcpyDeviceToDevice, st));
+    if(solve_algo != CULIP_TRSV_JACOBI) {
+        // the exact solve as level launches: no published-value buffer
+        if(right) {
+            if(f32()) trsv_upper<float>(topo(), plan<float>(), x, st);
+            else trsv_upper<double>(topo(), plan<double>(), x, st);
+        } else {
+            if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
+            else trsv_upper_t<double>(topo(), plan<double>(), x, st);
+        }
+        return;
+    }
+    double* s1 = s0 + NOALLOC_MAX_ROWS;
+    double* s2 = s1 + NOALLOC_MAX_ROWS;
+    HIP_CHECK(hipMemcpyAsync(s0, x, bytes, hipMemcpyDeviceToDevice, st));
+    if(f32())
+        ::culip::jacobi_sweeps<float>(topo(), plan<float>(), !right, x, jacobi_sweeps, s0, s1, s2, st);
+    else
+        ::culip::jacobi_sweeps<double>(topo(), plan<double>(), !right, x, jacobi_sweeps, s0, s1, s2,
+            st);
+}
+
 int64_t Bildlt::numeric_bytes() const
 {
     int64_t b = 0;

@@ -384,6 +384,18 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
     });
 }
 
+culip_status culip_bildlt_solve_noalloc(const culip_bildlt* h, culip_side side,
+    int32_t transpose, const double* b, double* x, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && b && x, "bildlt: null argument");
+        CULIP_REQUIRE(side == CULIP_SOLVE_LEFT || side == CULIP_SOLVE_MIDDLE || side == CULIP_SOLVE_RIGHT,
+            "bildlt: unknown side");
+        h->b.solve_noalloc(side, transpose != 0, b, x, (hipStream_t) stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
 int32_t culip_bildlt_n(const culip_bildlt* h) { return h ? h->b.n() : -1; }
 
 culip_status culip_bildlt_export_factorized(culip_bildlt* h, const char* folder,

@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "symbolic.hpp"
 #include "../device/factor.hpp"
+#include "../device/noalloc.hpp"
 #include "../device/solve.hpp"
 #include "../device/spmv.hpp"
 #include "../device/sqmr.hpp"
@@ -203,6 +204,11 @@ struct Bildlt final : Precond {
     int64_t numeric_bytes_estimate(bool with_layout) const;
     void factor(const double* vals, bool vals_on_device, double thr, hipStream_t st);
     void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
+    // the same without allocating: exact solves as level launches,
+    // block-Jacobi work vectors in the static scratch array (n must not
+    // exceed NOALLOC_MAX_ROWS)
+    void solve_noalloc(culip_side side, bool transpose, const double* b, double* x,
+        hipStream_t st) const;
     void trsv_u(double* x, hipStream_t st) const;
     void trsv_ut(double* x, hipStream_t st) const;
     void jacobi(double* x, bool transpose, hipStream_t st) const;

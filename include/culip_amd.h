@@ -159,6 +159,20 @@ culip_status culip_bildlt_refactor(culip_bildlt * h, const double * vals,
 culip_status culip_bildlt_solve(const culip_bildlt * h, culip_side side,
     int32_t transpose, const double * b, double * x, void * stream);
 
+/* Allocation-free variant of culip_bildlt_solve. It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). Same arguments and
+ * results: exact solves (SINGLE or LEVELS) run as level launches, which
+ * need no scratch and give the same bits; block-Jacobi solves keep b and
+ * their two iterates in a static device array of 3 x 1,048,576 doubles
+ * (24 MiB) that every GPU the library runs on holds until the process ends,
+ * instead of the object's lazily allocated vectors. Systems above
+ * 1,048,576 rows are refused with CULIP_ERR_INVALID. Every call on one GPU
+ * shares that GPU's array and it keeps the last call's contents: calls
+ * that overlap on one GPU (other streams, other threads) race. */
+culip_status culip_bildlt_solve_noalloc(const culip_bildlt * h, culip_side side,
+    int32_t transpose, const double * b, double * x, void * stream);
+
 int32_t culip_bildlt_n(const culip_bildlt * h);
 
 /* set_solve_algorithm(algorithm, num_jacobi_sweeps) (block_ildlt.cuh:
