@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--serve-jacobi-sweeps", type=int, default=2,
                     help="serve: block-Jacobi solves with this many sweeps (0: level solves)")
+    ap.add_argument("--equilibrate-sweeps", type=int, default=8,
+                    help="serve: device equilibration sweeps per request (0: off)")
     ap.add_argument("--max-it", type=int, default=0,
                     help="SQMR iteration budget per request (0: 3000 for serve)")
     ap.add_argument("--order", default="color", choices=["color", "natural"])
@@ -255,9 +257,10 @@ def bench_serve(a):
     sizes = rng.integers(a.n_lo, a.n_hi + 1, size=a.patterns)
     order = ca.ORDER_BLOCK_COLOR
     sweeps = a.serve_jacobi_sweeps
-    exact = "single" if a.slots <= 4 else "levels"   # single launches: at most 4 at a time
+    exact = "single"   # beyond four at a time the library runs level launches
     srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000,
-                    solve_algorithm="jacobi" if sweeps > 0 else exact, jacobi_sweeps=sweeps)
+                    solve_algorithm="jacobi" if sweeps > 0 else exact, jacobi_sweeps=sweeps,
+                    equilibrate_sweeps=a.equilibrate_sweeps)
     nsets = 2
 
     def make(i):
@@ -332,7 +335,7 @@ def bench_serve(a):
                    "requests": len(res), "offered_rate_per_s": rate,
                    "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "max_it": a.max_it or 3000,
                    "solve_schedule": f"jacobi{sweeps}" if sweeps > 0 else exact,
-                   "parallelism": "none"},
+                   "equilibrate_sweeps": a.equilibrate_sweeps, "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "queue_p50_ms": float(np.percentile(que, 50)),
         "service_p50_ms": float(np.percentile(lat - que, 50)),

@@ -24,7 +24,8 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
 
 __all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
            "SQMR", "ImportLDUPreconditioner", "NormalMatrixSPMV", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
-           "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR", "permute_scale"]
+           "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR", "permute_scale",
+           "weighted_matching", "structural_rank", "btf_rank_bound"]
 
 
 def _i32(a):
@@ -307,6 +308,11 @@ class CSRMatrixSPMV(_Spmv):
         self._h = h
         self._track()
 
+    def equilibrate(self, sweeps, d, stream=None):
+        """Values <- D A D by `sweeps` equilibration sweeps; d (device
+        float64 tensor, n) receives diag(D)."""
+        check(lib().culip_spmv_equilibrate(self._h, sweeps, _dptr(d), _stream(stream)))
+
     def update_values(self, vals, stream=None):
         v = _f64(vals)
         check(lib().culip_spmv_update_values(self._h, v.ctypes.data_as(C.c_void_p), 0,
@@ -440,6 +446,12 @@ class SQMR(_Handle):
         self._track()
         self.result = None
 
+    def set_residual_scale(self, s):
+        """Measure true residuals as ||(b - A x) ./ s|| / ||b ./ s|| (s: device
+        tensor kept alive here; None resets)."""
+        self._res_scale = s
+        check(lib().culip_sqmr_set_residual_scale(self._h, _dptr(s)))
+
     def solve(self, b, x, stream=None):
         r = _lib.SqmrResult()
         check(lib().culip_sqmr_solve(self._h, _dptr(b), _dptr(x), C.byref(r), _stream(stream)))
@@ -464,11 +476,13 @@ class Server(_Handle):
 
     def __init__(self, slots=4, prec="f32", hbm_budget=0, fill_level=0, fill_factor=1.0,
                  threshold=1e-4, tol=1e-8, max_it=1000, true_residual_period=20,
-                 max_restarts=10, solve_algorithm="levels", jacobi_sweeps=0):
+                 max_restarts=10, solve_algorithm="levels", jacobi_sweeps=0,
+                 equilibrate_sweeps=0):
         pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
         o = _lib.ServerOpts(slots, int(hbm_budget), pr, fill_level, fill_factor, threshold,
                             _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts),
-                            _solve_algorithm(solve_algorithm), jacobi_sweeps)
+                            _solve_algorithm(solve_algorithm), jacobi_sweeps,
+                            equilibrate_sweeps)
         h = C.c_void_p()
         check(lib().culip_server_create(C.byref(o), C.byref(h)))
         self._h = h
@@ -524,6 +538,51 @@ class Server(_Handle):
 
     def _release(self):
         self.close()
+
+
+def weighted_matching(row_ptr, col, val, n):
+    """Maximum-product matching of a square matrix with its scaling
+    (culip_weighted_matching): dict of row_match, u, v, row_scale,
+    col_scale, sym_scale, objective, feasible."""
+    s, keep = csr_struct(row_ptr, col, val, n)
+    m = s.m
+    rm = np.empty(m, np.int32)
+    out = {k: np.empty(m) for k in ("u", "v", "row_scale", "col_scale", "sym_scale")}
+    obj = C.c_double()
+    feas = C.c_int32()
+    check(lib().culip_weighted_matching(C.byref(s), _ptr(rm, C.c_int32),
+                                        *[_ptr(out[k], C.c_double) for k in
+                                          ("u", "v", "row_scale", "col_scale", "sym_scale")],
+                                        C.byref(obj), C.byref(feas)))
+    out.update(row_match=rm, objective=obj.value, feasible=bool(feas.value))
+    return out
+
+
+def _pattern_struct(row_ptr, col, m, n):
+    rp, ci = _i32(row_ptr), _i32(col)
+    s = _lib.Csr(m, n, len(ci), _ptr(rp, C.c_int32), _ptr(ci, C.c_int32), None)
+    return s, (rp, ci)
+
+
+def structural_rank(row_ptr, col, m, n):
+    """Structural rank of an m x n pattern and a maximum matching
+    (culip_structural_rank): (rank, row_match with -1 for unmatched rows)."""
+    s, keep = _pattern_struct(row_ptr, col, m, n)
+    r = C.c_int32()
+    rm = np.empty(m, np.int32)
+    check(lib().culip_structural_rank(C.byref(s), C.byref(r), _ptr(rm, C.c_int32)))
+    return r.value, rm
+
+
+def btf_rank_bound(row_ptr, col, m, n):
+    """Rank lower bound by greedy triangular peeling (culip_btf_rank_bound):
+    (bound, 1-based row order, 1-based column order)."""
+    s, keep = _pattern_struct(row_ptr, col, m, n)
+    b = C.c_int32()
+    ro, co = np.empty(m, np.int32), np.empty(n, np.int32)
+    check(lib().culip_btf_rank_bound(C.byref(s), C.byref(b), _ptr(ro, C.c_int32),
+                                     _ptr(co, C.c_int32)))
+    return b.value, ro, co
 
 
 def read_matrix_csr(path):

@@ -55,7 +55,8 @@ class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("hbm_budget", C.c_int64), ("prec", C.c_int),
                 ("fill_level", C.c_int32), ("fill_factor", C.c_double),
                 ("threshold", C.c_double), ("sqmr", SqmrOpts),
-                ("solve_algorithm", C.c_int), ("jacobi_sweeps", C.c_int32)]
+                ("solve_algorithm", C.c_int), ("jacobi_sweeps", C.c_int32),
+                ("equilibrate_sweeps", C.c_int32)]
 
 
 _lib = None
@@ -152,6 +153,17 @@ _SIGS = [
                                     C.POINTER(C.c_int32)]),
     ("culip_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("culip_server_destroy", None, [C.c_void_p]),
+    ("culip_spmv_equilibrate", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    ("culip_sqmr_set_residual_scale", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("culip_weighted_matching", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int32)]),
+    ("culip_structural_rank", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]),
+    ("culip_btf_rank_bound", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

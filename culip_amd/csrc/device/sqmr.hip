@@ -575,14 +575,21 @@ __global__ void k_axpy1(int32_t n, const double* a, double* b)
     if(g < n) b[g] += a[g];
 }
 
+// sums of squares of a and b, each element divided by s[g] when s is set
+// (the norms of the unscaled system when a and b belong to D A D)
 __global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, const double* b,
-    double* part)
+    const double* s, double* part)
 {
     __shared__ double sh[16];
     double sa = 0.0, sb = 0.0;
     for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
-        sa = fma(a[g], a[g], sa);
-        sb = fma(b[g], b[g], sb);
+        double av = a[g], bv = b[g];
+        if(s) {
+            av /= s[g];
+            bv /= s[g];
+        }
+        sa = fma(av, av, sa);
+        sb = fma(bv, bv, sb);
     }
     sa = block_sum(sa, sh);
     sb = block_sum(sb, sh);
@@ -644,7 +651,7 @@ static void residual_norms(const SqmrOps& ops, SqmrWork& w, const double* b, con
     const int32_t n = ops.n;
     ops.spmv(x, w.rt, st);
     k_residual<<<blocks_for(n), 256, 0, st>>>(n, b, w.rt);
-    k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, w.rt, b, w.part);
+    k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, w.rt, b, ops.res_scale, w.part);
     k_norm_final<<<1, TPB, 0, st>>>(w.grid, w.part, w.scal + 40);
     HIPCHK(hipMemcpyAsync(w.host_scal + 40, w.scal + 40, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -792,7 +799,7 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
     double gate_abs = -1.0;
     if(opt.true_residual_period > 0) {
         double nbn;
-        k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, b, b, w.part);
+        k_norm2_pair<<<w.grid, TPB, 0, st>>>(n, b, b, ops.res_scale, w.part);
         k_norm_final<<<1, TPB, 0, st>>>(w.grid, w.part, w.scal + 40);
         HIPCHK(hipMemcpyAsync(w.host_scal + 40, w.scal + 40, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

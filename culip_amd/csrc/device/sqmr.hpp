@@ -18,6 +18,9 @@ struct SqmrOps {
     std::function<bool(const double* x, double* y, void* stream)> spmv_left;
     const double* tri = nullptr;      // middle D (NULL: identity)
     const int32_t* is_piv = nullptr;
+    // true residuals measured as ||(b - A x) ./ s|| / ||b ./ s|| (NULL: s = 1):
+    // with A = D K D, b = D b0 and s = diag(D) that is K's own residual
+    const double* res_scale = nullptr;
 };
 
 // Device work space of one solve (allocated by the caller of sqmr_solve).

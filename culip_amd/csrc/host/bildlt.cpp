@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstdio>
+
+#include "../device/scale.hpp"
 #include <cstring>
 
 namespace culip {
@@ -318,8 +320,10 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
     const size_t bytes = sizeof(double) * (size_t) A.nnz();
     if(bytes) {
         if(vals == nullptr) vals = A.val.data(), on_device = false;
-        HIP_CHECK(hipMemcpyAsync(d_vals.p, vals, bytes,
-            on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+        // values already in the factor's own buffer need no copy
+        if(vals != d_vals.p)
+            HIP_CHECK(hipMemcpyAsync(d_vals.p, vals, bytes,
+                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     }
     if(f32()) factor_impl<float>(st);
     else factor_impl<double>(st);
@@ -691,6 +695,17 @@ void Spmv::multiply(const double* x, double* y, double alpha, double beta, bool 
     else csr_spmv(A, x, y, alpha, beta, st);
 }
 
+void Spmv::equilibrate(int32_t sweeps, double* d, hipStream_t st)
+{
+    if(kkt || normal) fail(CULIP_ERR_UNSUPPORTED, "equilibrate: only CSR operators");
+    if(have_at) fail(CULIP_ERR_UNSUPPORTED, "equilibrate: the operator must be symmetric");
+    if(A.m != A.n) fail(CULIP_ERR_INVALID, "equilibrate: the operator must be square");
+    if(sweeps < 1) fail(CULIP_ERR_INVALID, "equilibrate: at least one sweep");
+    b_eq.ensure(sizeof(double) * (size_t) std::max(1, A.m));
+    equilibrate_csr(A, sweeps, d, b_eq.as<double>(), st);
+    HIP_CHECK(hipGetLastError());
+}
+
 void upload_csr(DevCsr& d, DevBuf& br, DevBuf& bc, DevBuf& bv, const HostCsr& h)
 {
     upload(br, h.row_ptr);
@@ -744,6 +759,7 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         ops.tri = P->tri();
         ops.is_piv = P->is_piv();
     }
+    ops.res_scale = res_scale;
     sqmr_solve(ops, w, b, x, opts, res, st);
     if(prec) prec->after_solve(st);
 }

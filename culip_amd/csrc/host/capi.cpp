@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "kkt.hpp"
+#include "matching.hpp"
 #include "serve.hpp"
 
 struct culip_server { std::unique_ptr<culip::Server> s; };
@@ -318,6 +319,62 @@ culip_status culip_csr_permute_scale(const culip_csr* A, const int32_t* row_perm
     });
 }
 void culip_host_free(void* p) { std::free(p); }
+
+culip_status culip_weighted_matching(const culip_csr* A, int32_t* row_match, double* u, double* v,
+    double* row_scale, double* col_scale, double* sym_scale, double* objective, int32_t* feasible)
+{
+    return guarded([&] {
+        const HostCsr a = to_host(A);
+        const MatchingResult r = weighted_matching(a);
+        const size_t n = (size_t) a.m;
+        if(row_match) std::copy(r.row_match.begin(), r.row_match.end(), row_match);
+        if(u) std::copy(r.u.begin(), r.u.end(), u);
+        if(v) std::copy(r.v.begin(), r.v.end(), v);
+        if(row_scale) std::copy(r.row_scale.begin(), r.row_scale.end(), row_scale);
+        if(col_scale) std::copy(r.col_scale.begin(), r.col_scale.end(), col_scale);
+        if(sym_scale) {
+            const std::vector<double> d = symmetric_matching_scaling(r);
+            std::copy(d.begin(), d.begin() + n, sym_scale);
+        }
+        if(objective) *objective = r.objective;
+        if(feasible) *feasible = r.feasible ? 1 : 0;
+    });
+}
+
+culip_status culip_structural_rank(const culip_csr* A, int32_t* rank, int32_t* row_match)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(A && A->row_ptr && rank, "structural_rank: null argument");
+        CULIP_REQUIRE(A->nnz == 0 || A->col, "structural_rank: null col");
+        std::vector<int32_t> rm;
+        *rank = structural_rank(A->m, A->n, A->row_ptr, A->col, row_match ? &rm : nullptr);
+        if(row_match) std::copy(rm.begin(), rm.end(), row_match);
+    });
+}
+
+culip_status culip_btf_rank_bound(const culip_csr* A, int32_t* bound, int32_t* row_order,
+    int32_t* col_order)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(A && A->row_ptr && bound, "btf_rank_bound: null argument");
+        CULIP_REQUIRE(A->nnz == 0 || A->col, "btf_rank_bound: null col");
+        const int32_t m = A->m, n = A->n;
+        // the pattern by columns (A' as CSR)
+        std::vector<int32_t> cp(n + 1, 0), cr((size_t) A->nnz);
+        for(int64_t q = 0; q < A->nnz; ++q) {
+            CULIP_REQUIRE(A->col[q] >= 0 && A->col[q] < n, "btf_rank_bound: column out of range");
+            ++cp[A->col[q] + 1];
+        }
+        for(int32_t j = 0; j < n; ++j) cp[j + 1] += cp[j];
+        std::vector<int32_t> fill(cp.begin(), cp.end() - 1);
+        for(int32_t i = 0; i < m; ++i)
+            for(int32_t q = A->row_ptr[i]; q < A->row_ptr[i + 1]; ++q) cr[fill[A->col[q]]++] = i;
+        std::vector<int32_t> ro(m), co(n);
+        *bound = btf_rank_bound(m, n, cp.data(), cr.data(), A->row_ptr, A->col, ro.data(), co.data());
+        if(row_order) std::copy(ro.begin(), ro.end(), row_order);
+        if(col_order) std::copy(co.begin(), co.end(), col_order);
+    });
+}
 
 // ---- BlockiLDLt -------------------------------------------------------------
 
@@ -653,6 +710,15 @@ culip_status culip_spmv_multiply(const culip_spmv* op, const double* x, double* 
     });
 }
 
+culip_status culip_spmv_equilibrate(culip_spmv* op, int32_t sweeps, double* d, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(op && d, "spmv: null argument");
+        device_present();
+        op->s.equilibrate(sweeps, d, (hipStream_t) stream);
+    });
+}
+
 int32_t culip_spmv_m(const culip_spmv* op) { return op ? op->s.m : -1; }
 int32_t culip_spmv_n(const culip_spmv* op) { return op ? op->s.n : -1; }
 void culip_spmv_destroy(culip_spmv* op) { delete op; }
@@ -682,6 +748,14 @@ culip_status culip_sqmr_solve(culip_sqmr* s, const double* b, double* x, culip_s
     return guarded([&] {
         CULIP_REQUIRE(s && b && x && res, "sqmr: null argument");
         s->q.run(b, x, *res, (hipStream_t) stream);
+    });
+}
+
+culip_status culip_sqmr_set_residual_scale(culip_sqmr* s, const double* scale)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(s, "sqmr: null argument");
+        s->q.res_scale = scale;
     });
 }
 

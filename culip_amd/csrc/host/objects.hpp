@@ -299,8 +299,11 @@ struct Spmv {
     DevBuf b_row, b_col, b_val, bt_row, bt_col, bt_val, b_d, b_dix, b_arow, b_acol, b_aval,
         b_atrow, b_atcol, b_atval;
     int32_t m = 0, n = 0;
+    DevBuf b_eq;            // equilibration work vector
     void multiply(const double* x, double* y, double alpha, double beta, bool transpose,
         hipStream_t st) const;
+    // CSR operators only: values <- D A D, d <- diag(D) (device)
+    void equilibrate(int32_t sweeps, double* d, hipStream_t st);
 };
 
 // device copy of a host CSR matrix (row_ptr, col, val buffers)
@@ -310,6 +313,9 @@ struct Sqmr {
     const Spmv* A = nullptr;
     const Precond* prec = nullptr;
     culip_sqmr_opts opts{};
+    // true residuals measured on the unscaled system (device, borrowed;
+    // NULL: as given)
+    const double* res_scale = nullptr;
     DevBuf work;
     int32_t* host_flags = nullptr;
     double* host_scal = nullptr;

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 
+#include "../device/scale.hpp"
+
 namespace culip {
 
 Server::Server(const culip_server_opts& o) : o_(o)
@@ -12,6 +14,8 @@ Server::Server(const culip_server_opts& o) : o_(o)
     CULIP_REQUIRE(o.prec == CULIP_PREC_F64 || o.prec == CULIP_PREC_F32, "server: unknown precision");
     CULIP_REQUIRE(o.fill_level >= 0 && o.fill_factor >= 1.0, "server: bad fill parameters");
     CULIP_REQUIRE(o.sqmr.max_it >= 0 && o.sqmr.tol >= 0, "server: bad SQMR options");
+    CULIP_REQUIRE(o.equilibrate_sweeps >= 0 && o.equilibrate_sweeps <= 64,
+        "server: equilibrate_sweeps must be in [0, 64]");
     CULIP_REQUIRE(o.solve_algorithm == CULIP_TRSV_LEVELS || o.solve_algorithm == CULIP_TRSV_SINGLE ||
             (o.solve_algorithm == CULIP_TRSV_JACOBI && o.jacobi_sweeps >= 1),
         "server: bad solve algorithm");
@@ -96,7 +100,8 @@ void Server::estimate(ServePattern& P) const
 {
     Bildlt tmp(P.pat);
     tmp.solve_algo = P.algo;
-    const int64_t vec = (int64_t) sqmr_work_bytes(P.pat->A.m) + 2 * 8 * (int64_t) P.pat->A.m + 512;
+    const int64_t nvec = o_.equilibrate_sweeps > 0 ? 4 : 2;   // rhs, x (+ scale, work)
+    const int64_t vec = (int64_t) sqmr_work_bytes(P.pat->A.m) + nvec * 8 * (int64_t) P.pat->A.m + 512;
     P.est_max = tmp.numeric_bytes_estimate(true) + vec;
     P.est_min = tmp.numeric_bytes_estimate(false) + vec;
 }
@@ -250,6 +255,10 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
         in->q.init(&in->op, in->f.get(), o_.sqmr);
         in->rhs.ensure(sizeof(double) * (size_t) std::max(1, P.N));
         in->x.ensure(sizeof(double) * (size_t) std::max(1, P.N));
+        if(o_.equilibrate_sweeps > 0) {
+            in->scale.ensure(sizeof(double) * (size_t) std::max(1, P.N));
+            in->scale_tmp.ensure(sizeof(double) * (size_t) std::max(1, P.N));
+        }
     } catch(...) {
         lk.lock();
         used_ -= need;
@@ -271,10 +280,26 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
 void Server::run(ServeInstance* in, ServeRequest* r, hipStream_t st)
 {
     Bildlt& f = *in->f;
-    f.factor(r->vals.data(), false, o_.threshold, st);
     const size_t vb = sizeof(double) * r->rhs.size();
-    HIP_CHECK(hipMemcpyAsync(in->rhs.p, r->rhs.data(), vb, hipMemcpyHostToDevice, st));
-    in->q.run(in->rhs.as<double>(), in->x.as<double>(), r->res, st);
+    const int32_t N = (int32_t) r->rhs.size();
+    if(o_.equilibrate_sweeps > 0) {
+        // K's values into the factor's value buffer, scaled there to D K D
+        // (the instance's SpMV reads the same buffer); b -> D b
+        HIP_CHECK(hipMemcpyAsync(f.d_vals.p, r->vals.data(), sizeof(double) * r->vals.size(),
+            hipMemcpyHostToDevice, st));
+        equilibrate_csr(in->op.A, o_.equilibrate_sweeps, in->scale.as<double>(),
+            in->scale_tmp.as<double>(), st);
+        f.factor(f.d_vals.as<double>(), true, o_.threshold, st);
+        HIP_CHECK(hipMemcpyAsync(in->rhs.p, r->rhs.data(), vb, hipMemcpyHostToDevice, st));
+        scale_vector(N, in->scale.as<double>(), in->rhs.as<double>(), in->rhs.as<double>(), st);
+        in->q.res_scale = in->scale.as<double>();
+        in->q.run(in->rhs.as<double>(), in->x.as<double>(), r->res, st);
+        scale_vector(N, in->scale.as<double>(), in->x.as<double>(), in->x.as<double>(), st);
+    } else {
+        f.factor(r->vals.data(), false, o_.threshold, st);
+        HIP_CHECK(hipMemcpyAsync(in->rhs.p, r->rhs.data(), vb, hipMemcpyHostToDevice, st));
+        in->q.run(in->rhs.as<double>(), in->x.as<double>(), r->res, st);
+    }
     r->x.resize(r->rhs.size());
     HIP_CHECK(hipMemcpyAsync(r->x.data(), in->x.p, vb, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
@@ -301,7 +326,7 @@ void Server::worker(Slot* s)
             // the solve layout is sized by the first factorization: account
             // the instance's actual bytes
             const int64_t actual = in->f->numeric_bytes() + (int64_t) (in->q.work.bytes +
-                in->rhs.bytes + in->x.bytes);
+                in->rhs.bytes + in->x.bytes + in->scale.bytes + in->scale_tmp.bytes);
             used_ += actual - in->bytes;
             in->bytes = actual;
             peak_ = std::max(peak_, used_);

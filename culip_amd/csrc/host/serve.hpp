@@ -48,6 +48,7 @@ struct ServeInstance {
     Spmv op;
     Sqmr q;
     DevBuf rhs, x;
+    DevBuf scale, scale_tmp;  // equilibration: diag(D) and sweep work
     int64_t bytes = 0;
     uint64_t last_use = 0;
     bool busy = false;        // a request is running on it

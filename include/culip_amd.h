@@ -116,6 +116,31 @@ void culip_host_free(void * p);
  * perm_is_old_to_new = 0 the vectors are new -> old, so B(i, j) =
  * A(p_r[i], p_c[j]) / scale. Scales may be NULL (1); rows of the result are
  * sorted by column. Host arrays in, a host matrix out. */
+/* Maximum-product matching of a square A (WeightedBipartiteMatching::match
+ * and get_scaling, libs/algorithms/matching.cuh:69-125): costs
+ * c_ij = log2(max_k |a_kj|) - log2 |a_ij| (kept on a grid of 2^-20), a
+ * perfect matching of least total cost with its dual variables u, v
+ * (u_i + v_j <= c_ij, equality on the matching). row_scale r = 2^u and
+ * col_scale c = 2^v / max_k |a_kj| bound |r_i a_ij c_j| by 1, with 1 on
+ * the matching; sym_scale d = sqrt(r c) bounds |d_i a_ij d_j| by 1 for a
+ * symmetric A. Every output may be NULL (n entries each). *feasible = 0
+ * when A has no perfect matching (the outputs then hold a partial one).
+ * Host only. */
+culip_status culip_weighted_matching(const culip_csr * A, int32_t * row_match, double * u,
+    double * v, double * row_scale, double * col_scale, double * sym_scale,
+    double * objective, int32_t * feasible);
+/* Structural rank of an m x n pattern (sprank, libs/algorithms/sprank.cuh:
+ * 15-36) and a maximum matching: row_match[i] = column of row i or -1
+ * (m entries, may be NULL). Host only. */
+culip_status culip_structural_rank(const culip_csr * A, int32_t * rank, int32_t * row_match);
+/* Lower bound on the rank of an m x n pattern by greedy peeling into a
+ * triangular form (BTF::permute, libs/algorithms/permute_btf.h:28-42):
+ * row_order (m) and col_order (n) receive 1-based orders with the peeled
+ * upper-triangular part first, as the reference writes them (may be NULL).
+ * Host only. */
+culip_status culip_btf_rank_bound(const culip_csr * A, int32_t * bound, int32_t * row_order,
+    int32_t * col_order);
+
 culip_status culip_csr_permute_scale(const culip_csr * A, const int32_t * row_perm,
     const int32_t * col_perm, int32_t perm_is_old_to_new, const double * row_scale,
     const double * col_scale, int32_t scale_before_match, culip_host_matrix ** out);
@@ -260,6 +285,14 @@ culip_status culip_spmv_update_values(culip_spmv * op, const double * vals,
 /* multiply (spmv.cuh:24-31): y = alpha * op(A) x + beta * y. */
 culip_status culip_spmv_multiply(const culip_spmv * op, const double * x,
     double * y, double alpha, double beta, int32_t transpose, void * stream);
+/* Symmetric equilibration of a CSR operator's values on the device (no
+ * counterpart in the reference; input preparation for barrier-conditioned
+ * KKT systems): sweeps >= 1 sweeps from d = 1, each setting, for all rows
+ * at once, d_i <- d_i / sqrt(max_j |(d_i a_ij) d_j|) (Ruiz's iteration);
+ * then the operator's values become D A D. d: device, n doubles, receives
+ * diag(D). Operators from culip_spmv_create_csr with a symmetric matrix. */
+culip_status culip_spmv_equilibrate(culip_spmv * op, int32_t sweeps, double * d,
+    void * stream);
 int32_t culip_spmv_m(const culip_spmv * op);
 int32_t culip_spmv_n(const culip_spmv * op);
 void culip_spmv_destroy(culip_spmv * op);
@@ -304,6 +337,12 @@ culip_status culip_sqmr_create(const culip_spmv * A, const culip_bildlt * prec,
  * `stream` before returning. */
 culip_status culip_sqmr_solve(culip_sqmr * s, const double * b, double * x,
     culip_sqmr_result * res, void * stream);
+/* The true-residual gate and the reported true residual of later solves
+ * measure ||(b - A x) ./ s|| / ||b ./ s|| (no counterpart in the
+ * reference): with A = D K D, b = D b0 and s = diag(D) that is the residual
+ * of K for x0 = D x. s: device, n doubles, borrowed until reset; NULL
+ * resets to s = 1. */
+culip_status culip_sqmr_set_residual_scale(culip_sqmr * s, const double * scale);
 void culip_sqmr_destroy(culip_sqmr * s);
 
 /* ------------------------------------------------------------------------ */
@@ -356,8 +395,7 @@ culip_status culip_sqmr_create_ldu(const culip_spmv * A, const culip_ldu * prec,
  * overlap on the device. Per request: BlockiLDLt refactor + SQMR.       */
 
 typedef struct {
-    int32_t slots;              /* concurrent solves, 1..64 (1..4 with
-                                   single-launch triangular solves) */
+    int32_t slots;              /* concurrent solves, 1..64 */
     int64_t hbm_budget;         /* device bytes for patterns and per-slot
                                    instances; 0: 90% of free memory */
     culip_prec prec;            /* factor precision (SQMR is FP64) */
@@ -369,6 +407,10 @@ typedef struct {
      * (culip_server_set_pattern_solve); as culip_bildlt_set_solve_algorithm */
     culip_solve_algorithm solve_algorithm;
     int32_t jacobi_sweeps;
+    /* > 0: every request's K is equilibrated on the device with this many
+     * sweeps (culip_spmv_equilibrate) and solved as D K D y = D b with the
+     * true residual measured on K, x = D y; 0: K as given */
+    int32_t equilibrate_sweeps;
 } culip_server_opts;
 
 typedef struct culip_server culip_server;
