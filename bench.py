@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--serve-jacobi-sweeps", type=int, default=2,
                     help="serve: block-Jacobi solves with this many sweeps (0: level solves)")
+    ap.add_argument("--graphs", type=int, default=-1,
+                    help="replay refactors and SQMR segments from HIP graphs (default: on for "
+                         "serve, off for batch, where the small systems measured slower)")
     ap.add_argument("--equilibrate-sweeps", type=int, default=8,
                     help="serve: device equilibration sweeps per request (0: off)")
     ap.add_argument("--max-it", type=int, default=0,
@@ -260,7 +263,7 @@ def bench_serve(a):
     exact = "single"   # beyond four at a time the library runs level launches
     srv = ca.Server(slots=a.slots, prec=a.prec, max_it=a.max_it or 3000,
                     solve_algorithm="jacobi" if sweeps > 0 else exact, jacobi_sweeps=sweeps,
-                    equilibrate_sweeps=a.equilibrate_sweeps)
+                    equilibrate_sweeps=a.equilibrate_sweeps, graphs=bool(a.graphs))
     nsets = 2
 
     def make(i):
@@ -335,7 +338,8 @@ def bench_serve(a):
                    "requests": len(res), "offered_rate_per_s": rate,
                    "barrier_weights": [1e-4, 1e4], "tol": 1e-8, "max_it": a.max_it or 3000,
                    "solve_schedule": f"jacobi{sweeps}" if sweeps > 0 else exact,
-                   "equilibrate_sweeps": a.equilibrate_sweeps, "parallelism": "none"},
+                   "equilibrate_sweeps": a.equilibrate_sweeps, "graphs": bool(a.graphs),
+                   "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "queue_p50_ms": float(np.percentile(que, 50)),
         "service_p50_ms": float(np.percentile(lat - que, 50)),
@@ -380,7 +384,7 @@ def bench_batch(a):
     t_gen = time.perf_counter() - t0
     log(f"{len(systems)} systems generated in {t_gen:.1f}s")
     res, secs, t_reg = cb.solve_batch(systems, lambda i: tuned[cb.bucket_of(sizes[i])][1:],
-                                      a.slots, a.prec, log=log)
+                                      a.slots, a.prec, log=log, graphs=bool(a.graphs))
     el = secs[-1]
     # the plain path: FP64 factor, exact single-launch solves (the reference
     # driver's SOLVE_BLOCK), one system at a time
@@ -403,7 +407,7 @@ def bench_batch(a):
                        f"{cb.BUCKETS[b][0]}-{cb.BUCKETS[b][1] - 1}":
                            tuned[b][1] if tuned[b][1] != "jacobi" else f"{tuned[b][1]}{tuned[b][2]}"
                        for b in tuned},
-                   "tol": 1e-8, "parallelism": "none"},
+                   "tol": 1e-8, "graphs": bool(a.graphs), "parallelism": "none"},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_true_rel_residual": trr,
         "converged_fraction": float(np.mean(conv)),
@@ -439,6 +443,8 @@ def main():
     if a.workload in ("serve", "batch"):
         if a.prec is None:
             a.prec = "f32"
+        if a.graphs < 0:
+            a.graphs = 1 if a.workload == "serve" else 0
         out = bench_serve(a) if a.workload == "serve" else bench_batch(a)
         print(json.dumps(out))
         return

@@ -273,6 +273,10 @@ class BlockiLDLt(_Handle):
     def device_bytes(self):
         return lib().culip_bildlt_device_bytes(self._h)
 
+    def set_graphs(self, enable=True):
+        """Refactors on explicit streams replay a captured HIP graph."""
+        check(lib().culip_bildlt_set_graphs(self._h, 1 if enable else 0))
+
     def _inject_solve_error(self):
         """Testing hook: records a timed-out single-launch solve from the host."""
         check(lib().culip_bildlt_inject_solve_error(self._h))
@@ -320,16 +324,23 @@ class CSRMatrixSPMV(_Spmv):
 
 
 class AugmentedMatrixSPMV(_Spmv):
-    """K = [diag(d) A'; A delta*I(ix)] (spmv.cuh:116-149), dimension n + m."""
+    """K = [diag(d) A'; A delta*I(ix)] (spmv.cuh:116-149), dimension n + m;
+    with H = (row_ptr, col, val) of an n x n symmetric CSR matrix in place
+    of d, K = [H A'; A delta*I(ix)] (culip_spmv_create_kkt_csr)."""
 
-    def __init__(self, a_row_ptr, a_col, a_val, a_ncols, d, delta=0.0, ix=None):
+    def __init__(self, a_row_ptr, a_col, a_val, a_ncols, d=None, delta=0.0, ix=None, H=None):
         s, self._keep = csr_struct(a_row_ptr, a_col, a_val, a_ncols)
-        dd = _f64(d)
         h = C.c_void_p()
         ixa = _i32(ix if ix is not None else [])
-        check(lib().culip_spmv_create_kkt(C.byref(s), _ptr(dd, C.c_double), delta, len(ixa),
-                                          _ptr(ixa, C.c_int32) if len(ixa) else None,
-                                          C.byref(h)))
+        ixp = _ptr(ixa, C.c_int32) if len(ixa) else None
+        if H is not None:
+            hs, self._hkeep = csr_struct(*H)
+            check(lib().culip_spmv_create_kkt_csr(C.byref(hs), C.byref(s), delta, len(ixa), ixp,
+                                                  C.byref(h)))
+        else:
+            dd = _f64(d)
+            check(lib().culip_spmv_create_kkt(C.byref(s), _ptr(dd, C.c_double), delta,
+                                              len(ixa), ixp, C.byref(h)))
         self._h = h
         self._track()
 
@@ -477,12 +488,12 @@ class Server(_Handle):
     def __init__(self, slots=4, prec="f32", hbm_budget=0, fill_level=0, fill_factor=1.0,
                  threshold=1e-4, tol=1e-8, max_it=1000, true_residual_period=20,
                  max_restarts=10, solve_algorithm="levels", jacobi_sweeps=0,
-                 equilibrate_sweeps=0):
+                 equilibrate_sweeps=0, graphs=False):
         pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
         o = _lib.ServerOpts(slots, int(hbm_budget), pr, fill_level, fill_factor, threshold,
                             _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts),
                             _solve_algorithm(solve_algorithm), jacobi_sweeps,
-                            equilibrate_sweeps)
+                            equilibrate_sweeps, 1 if graphs else 0)
         h = C.c_void_p()
         check(lib().culip_server_create(C.byref(o), C.byref(h)))
         self._h = h

@@ -56,7 +56,7 @@ class ServerOpts(C.Structure):
                 ("fill_level", C.c_int32), ("fill_factor", C.c_double),
                 ("threshold", C.c_double), ("sqmr", SqmrOpts),
                 ("solve_algorithm", C.c_int), ("jacobi_sweeps", C.c_int32),
-                ("equilibrate_sweeps", C.c_int32)]
+                ("equilibrate_sweeps", C.c_int32), ("graphs", C.c_int32)]
 
 
 _lib = None
@@ -107,6 +107,7 @@ _SIGS = [
                                         C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64)]),
     ("culip_bildlt_device_bytes", C.c_int64, [C.c_void_p]),
     ("culip_bildlt_inject_solve_error", C.c_int, [C.c_void_p]),
+    ("culip_bildlt_set_graphs", C.c_int, [C.c_void_p, C.c_int32]),
     ("culip_bildlt_solve_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p,
                                              C.c_void_p, C.c_void_p]),
     ("culip_bildlt_destroy", None, [C.c_void_p]),
@@ -153,6 +154,9 @@ _SIGS = [
                                     C.POINTER(C.c_int32)]),
     ("culip_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("culip_server_destroy", None, [C.c_void_p]),
+    ("culip_spmv_create_kkt_csr", C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.c_double,
+                                            C.c_int32, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_void_p)]),
     ("culip_spmv_equilibrate", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     ("culip_sqmr_set_residual_scale", C.c_int, [C.c_void_p, C.c_void_p]),
     ("culip_weighted_matching", C.c_int, [C.POINTER(Csr), C.POINTER(C.c_int32),

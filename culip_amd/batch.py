@@ -98,16 +98,19 @@ def autotune(sizes, make_system, prec="f32", slots=16, block_sizes=BLOCK_SIZES, 
     return tuned, timings
 
 
-def solve_batch(systems, schedule_of, slots, prec, tol=1e-8, max_it=1000, passes=2, log=None):
+def solve_batch(systems, schedule_of, slots, prec, tol=1e-8, max_it=1000, passes=2, log=None,
+                graphs=False):
     """Solves every system (prep, rhs) on a Server with `slots` streams.
     schedule_of(i) -> (algorithm, sweeps) or None for the server default
     (single launch). Registration (symbolic analysis and upload) is timed
     apart; each pass submits every system at once and waits for all; the
-    first pass also creates the slots' numeric instances.
+    first pass also creates the slots' numeric instances. graphs: replay
+    refactors and SQMR segments from captured HIP graphs.
     Returns (results of the last pass, seconds per pass, registration
     seconds)."""
     import concurrent.futures as cf
-    srv = Server(slots=slots, prec=prec, tol=tol, max_it=max_it, solve_algorithm="single")
+    srv = Server(slots=slots, prec=prec, tol=tol, max_it=max_it, solve_algorithm="single",
+                 graphs=graphs)
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(16) as ex:
         pids = list(ex.map(lambda s: srv.add_pattern(s[0].row_ptr, s[0].col, s[0].val,

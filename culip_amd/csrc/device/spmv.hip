@@ -52,23 +52,27 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __re
     }
 }
 
-// KKT: b1 = d .* x1 + A' x2 ; b2 = delta * x2(ix) + A x1 ; y = alpha b + beta y
+// KKT: b1 = H x1 + A' x2 (H = diag(d), or a CSR matrix hp/hi/hv) ;
+// b2 = delta * x2(ix) + A x1 ; y = alpha b + beta y. A row of H and the
+// row of A' are summed by the same eight lanes, H's entries first.
 __global__ __launch_bounds__(256) void k_kkt_primal(int32_t n, const double* d,
-    const int32_t* atp, const int32_t* ati, const double* atv, const double* x, double* y,
-    double alpha, double beta)
+    const int32_t* hp, const int32_t* hi, const double* hv, const int32_t* atp,
+    const int32_t* ati, const double* atv, const double* x, double* y, double alpha, double beta)
 {
     const int64_t tid = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
     const int sub = threadIdx.x & (LPR - 1);
     const int64_t row = tid / LPR;
     double acc = 0.0;
     if(row < n) {
+        if(hp)
+            for(int32_t q = hp[row] + sub; q < hp[row + 1]; q += LPR) acc = fma(hv[q], x[hi[q]], acc);
         const double* x2 = x + n;
         for(int32_t q = atp[row] + sub; q < atp[row + 1]; q += LPR) acc = fma(atv[q], x2[ati[q]], acc);
     }
 #pragma unroll
     for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
     if(row < n && sub == 0) {
-        const double b = fma(d[row], x[row], acc);
+        const double b = hp ? acc : fma(d[row], x[row], acc);
         y[row] = (beta == 0.0) ? alpha * b : fma(beta, y[row], alpha * b);
     }
 }
@@ -156,7 +160,7 @@ void kkt_spmv(const DevKkt& K, const double* x, double* y, double alpha, double 
     hipStream_t st = (hipStream_t) stream;
     if(K.n > 0)
         k_kkt_primal<<<(unsigned) (((int64_t) K.n * LPR + 255) / 256), 256, 0, st>>>(K.n, K.d,
-            K.At.row_ptr, K.At.col, K.At.val, x, y, alpha, beta);
+            K.H.row_ptr, K.H.col, K.H.val, K.At.row_ptr, K.At.col, K.At.val, x, y, alpha, beta);
     if(K.m > 0)
         k_kkt_dual<<<(unsigned) (((int64_t) K.m * LPR + 255) / 256), 256, 0, st>>>(K.n, K.m,
             K.dix, K.A.row_ptr, K.A.col, K.A.val, x, y, alpha, beta);

@@ -13,11 +13,12 @@ struct DevCsr {
     double* val = nullptr;
 };
 
-// K = [diag(d) A'; A diag(dix)] with dix = delta on the rows in ix, else 0.
+// K = [H A'; A diag(dix)] with H = diag(d) or a CSR matrix (H.row_ptr set)
+// and dix = delta on the rows in ix, else 0.
 struct DevKkt {
     int32_t n = 0, m = 0;
-    DevCsr A, At;
-    double* d = nullptr;    // n
+    DevCsr A, At, H;
+    double* d = nullptr;    // n (diagonal H)
     double* dix = nullptr;  // m
 };
 

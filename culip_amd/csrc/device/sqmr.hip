@@ -671,7 +671,7 @@ double true_rel_residual(const SqmrOps& ops, SqmrWork& w, const double* b, const
 // x = Pr \ (D \ y) in xout. check_every > 0 adds the true-residual gate.
 static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* xout,
     const culip_sqmr_opts& opt, culip_sqmr_result& res, hipStream_t st, int32_t& total_it,
-    bool& gate_met, double gate_abs)
+    bool& gate_met, double gate_abs, SqmrGraphs* graphs)
 {
     const int32_t n = ops.n;
     const int G = w.grid;
@@ -706,22 +706,64 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     constexpr int STALL_PERIODS = 25;
     double best_nr = -1.0, gam_at_best = 0.0;
     int stall = 0;
-    while(it < opt.max_it) {
-        const int k0 = it % 3, k1 = (it + 1) % 3, km1 = (it + 2) % 3, km2 = (it + 1) % 3;
+    // the kernels of iteration `i`: they depend on i only through i mod 3
+    // (the 3-cyclic vectors) and the parity of `cur` (the two A ev vectors),
+    // so iterations i and i + 6 launch the same work
+    auto launch_iteration = [&](int i) {
+        const int k0 = i % 3, k1 = (i + 1) % 3, km1 = (i + 2) % 3, km2 = (i + 1) % 3;
         k_alpha<<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
-        k_vnext<<<G, TPB, 0, st>>>(n, it, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal, w.part);
+        k_vnext<<<G, TPB, 0, st>>>(n, i, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal, w.part);
         k_vscale<<<G, TPB, 0, st>>>(n, w.tv, w.v[k1], w.z, w.w[k1], w.ev, ops.tri, ops.is_piv,
             w.scal, w.part);
         const int nxt = cur ^ 1;
         if(ops.right) ops.right(w.ev, st);
         apply_left_op(w.ev, w.av[nxt]);
         k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
-        k_pupdate<<<G, TPB, 0, st>>>(n, it, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
+        k_pupdate<<<G, TPB, 0, st>>>(n, i, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
             w.scal, w.part, w.flags);
-        k_scalars<<<1, TPB, 0, st>>>(it, G, w.scal, w.part, w.flags, opt.tol,
+        k_scalars<<<1, TPB, 0, st>>>(i, G, w.scal, w.part, w.flags, opt.tol,
             period > 0 ? 1e-14 : -1.0);
         cur = nxt;
         HIPCHK(hipGetLastError());
+    };
+    // with the true-residual gate, the `period` iterations between two
+    // checks can replay a captured graph (one per phase it mod 6; the phase
+    // fixes every buffer the kernels use)
+    hipStreamCaptureStatus cst = hipStreamCaptureStatusNone;
+    const bool segments = graphs && period > 0 && st != nullptr &&
+        hipStreamIsCapturing(st, &cst) == hipSuccess && cst == hipStreamCaptureStatusNone;
+    if(segments && graphs->period != period) {
+        graphs->release();
+        graphs->period = period;
+    }
+    while(it < opt.max_it) {
+        if(segments && it % period == 0 && it + period <= opt.max_it) {
+            const int phase = it % 6;
+            hipGraphExec_t& ex = graphs->seg[phase];
+            const int cur0 = cur;
+            if(!ex) {
+                HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                try {
+                    for(int j = 0; j < period; ++j) launch_iteration(it + j);
+                } catch(...) {
+                    // leave the stream usable: end the broken capture
+                    hipGraph_t bad = nullptr;
+                    (void) hipStreamEndCapture(st, &bad);
+                    if(bad) (void) hipGraphDestroy(bad);
+                    throw;
+                }
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamEndCapture(st, &g));
+                const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void) hipGraphDestroy(g);
+                HIPCHK(e);
+            }
+            HIPCHK(hipGraphLaunch(ex, st));
+            cur = cur0 ^ (period & 1);
+            it += period - 1;
+        } else {
+            launch_iteration(it);
+        }
 
         if(period > 0) {
             // true-residual gate: the recurrence runs on past the reference's
@@ -739,9 +781,14 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
                 double nr, nbn;
                 residual_norms(ops, w, b, xout, st, nr, nbn);
                 if(nr <= gate_abs) { gate_met = true; ++it; break; }
-                HIPCHK(hipMemcpy(w.host_flags, w.flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
-                double gam = 0;
-                HIPCHK(hipMemcpy(&gam, w.scal + S_GAMMA, sizeof(double), hipMemcpyDeviceToHost));
+                // on the solve's own stream (copies on the null stream could
+                // disturb graph captures of other threads)
+                HIPCHK(hipMemcpyAsync(w.host_flags, w.flags, 2 * sizeof(int32_t),
+                    hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(w.host_scal + S_GAMMA, w.scal + S_GAMMA, sizeof(double),
+                    hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                double gam = w.host_scal[S_GAMMA];
                 gam = fabs(gam);
                 if(best_nr < 0 || nr < 0.5 * best_nr) {
                     best_nr = nr;
@@ -787,8 +834,16 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     }
 }
 
+void SqmrGraphs::release()
+{
+    for(hipGraphExec_t& e : seg) {
+        if(e) (void) hipGraphExecDestroy(e);
+        e = nullptr;
+    }
+}
+
 void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
-    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream)
+    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream, SqmrGraphs* graphs)
 {
     hipStream_t st = (hipStream_t) stream;
     const int32_t n = ops.n;
@@ -806,7 +861,7 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
         nbn = w.host_scal[40];
         gate_abs = opt.tol * nbn;
     }
-    sqmr_run(ops, w, b, x, opt, res, st, total_it, gate, gate_abs);
+    sqmr_run(ops, w, b, x, opt, res, st, total_it, gate, gate_abs, graphs);
     double nr, nbn;
     residual_norms(ops, w, b, x, st, nr, nbn);
     double rr = nbn > 0 ? nr / nbn : nr;
@@ -828,7 +883,7 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
         o2.max_it = opt.max_it - total_it;
         culip_sqmr_result r2{};
         bool g2 = false;
-        sqmr_run(ops, w, w.rhs, x, o2, r2, st, total_it, g2, gate_abs);
+        sqmr_run(ops, w, w.rhs, x, o2, r2, st, total_it, g2, gate_abs, graphs);
         k_axpy1<<<nb, 256, 0, st>>>(n, w.xacc, x);
         residual_norms(ops, w, b, x, st, nr, nbn);
         const double rr_new = nbn > 0 ? nr / nbn : nr;

@@ -1,6 +1,8 @@
 // SQMR (libs/la/iterative_solvers/sqmr.impl.cuh:109-355) on the device.
 #pragma once
 
+#include <hip/hip_runtime_api.h>
+
 #include <cstdint>
 #include <functional>
 
@@ -49,9 +51,26 @@ size_t sqmr_work_bytes(int32_t n);
 void sqmr_work_bind(SqmrWork& w, int32_t n, char* dev_base, int32_t* host_flags,
     double* host_scal);
 
+// Captured graphs of the iterations between two true-residual checks, one
+// per phase (iteration mod 6). Valid while the operator, the
+// preconditioner's buffers and solve schedule, and the work space stay the
+// same.
+struct SqmrGraphs {
+    int32_t period = 0;
+    hipGraphExec_t seg[6] = {};
+    SqmrGraphs() = default;
+    SqmrGraphs(const SqmrGraphs&) = delete;
+    SqmrGraphs& operator=(const SqmrGraphs&) = delete;
+    ~SqmrGraphs() { release(); }
+    void release();
+};
+
 // Solves A x = b. b, x device vectors. Throws Error on HIP failures.
+// graphs (may be null): replay the iterations between true-residual checks
+// from captured graphs (explicit streams only).
 void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
-    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream);
+    const culip_sqmr_opts& opt, culip_sqmr_result& res, void* stream,
+    SqmrGraphs* graphs = nullptr);
 
 // ||b - A x|| / ||b|| in FP64 (synchronises the stream).
 double true_rel_residual(const SqmrOps& ops, SqmrWork& w, const double* b, const double* x,

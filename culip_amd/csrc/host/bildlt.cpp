@@ -28,6 +28,7 @@ void hip_check(hipError_t e, const char* what)
 
 void Bildlt::symbolic(int32_t fill_level, double fill_factor)
 {
+    release_graphs();
     symbolic_analyze(sym, A, piv_starts, block_starts, fill_level, fill_factor, f32());
     have_sym = true;
     pat->uploaded = false;
@@ -130,8 +131,9 @@ void BildltPattern::upload_static()
     uploaded = true;
 }
 
-void Bildlt::upload_pattern()
+void Bildlt::upload_pattern(hipStream_t st)
 {
+    release_graphs();
     if(!pat->uploaded) pat->upload_static();
     const Symbolic& s = sym;
     const int32_t n = s.n, K = s.coarse_nnz;
@@ -164,12 +166,33 @@ void Bildlt::upload_pattern()
     for(int q = 0; q < 4; ++q) d_ptr[q].ensure(sizeof(int32_t) * (size_t) (n + 1));
     for(int q = 0; q < 2; ++q) d_task[q].ensure(sizeof(int32_t) * 8 * (size_t) s.nb);
     d_totals.ensure(sizeof(int32_t) * 4);
+    // solve layout at its upper bounds, so that a refactor needs no host
+    // round trip to size it: the off-diagonal rows hold at most every
+    // stored entry (leftover borrowing moves capacity, never adds to it),
+    // the pivot-inverse rows at most h^2 entries per block row; 64 entries
+    // of slack for the solve kernels' clamped reads
+    {
+        int64_t inv = 0;
+        for(int32_t i = 0; i < s.nb; ++i) {
+            const int64_t h = s.block_starts[i + 1] - s.block_starts[i];
+            inv += h * h;
+        }
+        const int64_t cap[4] = {s.val_space, inv, s.val_space, inv};
+        for(int q = 0; q < 4; ++q) {
+            const size_t cnt = (size_t) std::max<int64_t>(1, cap[q]) + 64;
+            d_col[q].ensure(sizeof(int32_t) * cnt);
+            d_val[q].ensure(vt() * cnt);
+            d_lr[q].ensure(cnt);
+        }
+    }
     d_scan_tmp.ensure(sizeof(int32_t) * scan_tmp_ints(n));
     d_chain_pub.ensure(sizeof(double) * 2 * (size_t) std::max(1, n));
     h_trsv_err.ensure();
-    chain_reset(d_chain_pub.as<double>(), 2 * (int64_t) n, nullptr);
+    // the presets on the caller's stream: no device-wide synchronisation,
+    // which would break graph captures other threads run meanwhile
+    chain_reset(d_chain_pub.as<double>(), 2 * (int64_t) n, st);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipStreamSynchronize(st));
     chain_parity = 0;
     (void) h_trsv_err.take();
     uploaded = true;
@@ -294,28 +317,53 @@ void Bildlt::factor_impl(hipStream_t st)
     int32_t* tot = d_totals.as<int32_t>();
     for(int q = 0; q < 4; ++q)
         scan_counts(d_ptr[q].as<int32_t>(), s.n, tot + q, d_scan_tmp.as<int32_t>(), st);
-    int32_t htot[4];
-    HIP_CHECK(hipMemcpyAsync(htot, tot, sizeof(htot), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipStreamSynchronize(st));
-    for(int q = 0; q < 4; ++q) {
-        plan_tot[q] = htot[q];
-        // 64 entries of slack: the solve kernels read clamped indices
-        const size_t cnt = (size_t) std::max(1, htot[q]) + 64;
-        d_col[q].ensure(sizeof(int32_t) * cnt);
-        d_val[q].ensure(sizeof(VT) * cnt);
-        d_lr[q].ensure(cnt);
-    }
-    p = plan<VT>();
+    // the layout buffers hold the upper bounds (upload_pattern)
     solve_plan_fill<VT>(t, a.val_out, nullptr, p, st);
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void Bildlt::release_graphs()
+{
+    if(factor_exec) (void) hipGraphExecDestroy(factor_exec);
+    factor_exec = nullptr;
+}
+
+bool Bildlt::factor_graph(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    // graphs are captured on (and replayed from) explicit streams only; a
+    // stream some caller is capturing gets the kernels themselves
+    if(!use_graphs || st == nullptr || hipStreamIsCapturing(st, &cs) != hipSuccess ||
+        cs != hipStreamCaptureStatusNone)
+        return false;
+    if(factor_exec && graph_threshold != threshold) release_graphs();
+    if(!factor_exec) {
+        HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            if(f32()) factor_impl<float>(st);
+            else factor_impl<double>(st);
+        } catch(...) {
+            hipGraph_t g = nullptr;
+            (void) hipStreamEndCapture(st, &g);
+            if(g) (void) hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamEndCapture(st, &g));
+        const hipError_t e = hipGraphInstantiate(&factor_exec, g, nullptr, nullptr, 0);
+        (void) hipGraphDestroy(g);
+        hip_check(e, "factor graph instantiation");
+        graph_threshold = threshold;
+    }
+    HIP_CHECK(hipGraphLaunch(factor_exec, st));
+    return true;
 }
 
 void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t st)
 {
     if(!have_sym) fail(CULIP_ERR_STATE, "factor: symbolic phase has not run");
     take_async_error();
-    if(!uploaded) upload_pattern();
+    if(!uploaded) upload_pattern(st);
     threshold = thr;
     const size_t bytes = sizeof(double) * (size_t) A.nnz();
     if(bytes) {
@@ -325,14 +373,28 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
             HIP_CHECK(hipMemcpyAsync(d_vals.p, vals, bytes,
                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     }
-    if(f32()) factor_impl<float>(st);
-    else factor_impl<double>(st);
+    if(!factor_graph(st)) {
+        if(f32()) factor_impl<float>(st);
+        else factor_impl<double>(st);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
     factored = true;
+    plan_known = false;
     host_int.erase("fblocks");
     host_int.erase("fix_store");
     host_real.erase("fval_store");
     host_real.erase("tri");
     host_real.erase("row_norms");
+}
+
+const int32_t* Bildlt::plan_sizes()
+{
+    if(!plan_known) {
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(plan_tot, d_totals.p, sizeof(plan_tot), hipMemcpyDeviceToHost));
+        plan_known = true;
+    }
+    return plan_tot;
 }
 
 void Bildlt::set_solve(culip_solve_algorithm a, int32_t sweeps)
@@ -367,7 +429,7 @@ void Bildlt::trsv_u(double* x, hipStream_t st) const
         jacobi(x, false, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE &&
+    if(solve_algo == CULIP_TRSV_SINGLE && !use_graphs &&
         chain_admitted(st, [&] {
             const int32_t par = chain_parity;
             chain_parity ^= 1;
@@ -386,7 +448,7 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
         jacobi(x, true, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE &&
+    if(solve_algo == CULIP_TRSV_SINGLE && !use_graphs &&
         chain_admitted(st, [&] {
             const int32_t par = chain_parity;
             chain_parity ^= 1;
@@ -417,7 +479,7 @@ bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStre
     }
     // the single-launch U' solve forms each block row's K src itself; when
     // it is not admitted the caller multiplies and calls left() (same bits)
-    if(solve_algo != CULIP_TRSV_SINGLE) return false;
+    if(solve_algo != CULIP_TRSV_SINGLE || use_graphs) return false;
     return chain_admitted(st, [&] {
         const int32_t par = chain_parity;
         chain_parity ^= 1;
@@ -760,7 +822,7 @@ void Sqmr::run(const double* b, double* x, culip_sqmr_result& res, hipStream_t s
         ops.is_piv = P->is_piv();
     }
     ops.res_scale = res_scale;
-    sqmr_solve(ops, w, b, x, opts, res, st);
+    sqmr_solve(ops, w, b, x, opts, res, st, graphs.get());
     if(prec) prec->after_solve(st);
 }
 

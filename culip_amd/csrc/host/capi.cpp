@@ -493,7 +493,8 @@ culip_status culip_bildlt_get_int(culip_bildlt* h, const char* name, const int32
         auto put = [&](const std::vector<int32_t>& v) { *ptr = v.data(); *len = (int64_t) v.size(); };
         if(nm == "plan_sizes") {
             CULIP_REQUIRE(b.factored, "bildlt: factor has not been computed");
-            b.host_int[nm] = std::vector<int32_t>(b.plan_tot, b.plan_tot + 4);
+            const int32_t* ps = b.plan_sizes();
+            b.host_int[nm] = std::vector<int32_t>(ps, ps + 4);
             put(b.host_int[nm]);
             return;
         }
@@ -566,6 +567,15 @@ culip_status culip_bildlt_get_real(culip_bildlt* h, const char* name, const doub
             return;
         }
         fail(CULIP_ERR_INVALID, "bildlt: unknown structure '" + nm + "'");
+    });
+}
+
+culip_status culip_bildlt_set_graphs(culip_bildlt* h, int32_t enable)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null argument");
+        h->b.use_graphs = enable != 0;
+        if(!h->b.use_graphs) h->b.release_graphs();
     });
 }
 
@@ -649,6 +659,37 @@ culip_status culip_spmv_create_kkt(const culip_csr* hA, const double* d, double 
             }
             upload(o.b_dix, dix);
             o.K.d = o.b_d.as<double>();
+            o.K.dix = o.b_dix.as<double>();
+            o.m = o.n = a.n + a.m;
+        } catch(...) { delete s; throw; }
+        *out = s;
+    });
+}
+
+culip_status culip_spmv_create_kkt_csr(const culip_csr* hH, const culip_csr* hA, double delta,
+    int32_t n_ix, const int32_t* ix, culip_spmv** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(out, "spmv: null argument");
+        device_present();
+        HostCsr h = to_host(hH);
+        HostCsr a = to_host(hA);
+        CULIP_REQUIRE(h.m == h.n && h.n == a.n, "spmv: H must be n x n with A m x n");
+        auto* s = new culip_spmv;
+        try {
+            Spmv& o = s->s;
+            o.kkt = true;
+            o.K.n = a.n;
+            o.K.m = a.m;
+            upload_csr(o.K.H, o.b_hrow, o.b_hcol, o.b_hval, h);
+            upload_csr(o.K.A, o.b_arow, o.b_acol, o.b_aval, a);
+            upload_csr(o.K.At, o.b_atrow, o.b_atcol, o.b_atval, transpose_csr(a));
+            std::vector<double> dix(a.m, 0.0);
+            for(int32_t q = 0; q < n_ix; ++q) {
+                CULIP_REQUIRE(ix && ix[q] >= 0 && ix[q] < a.m, "spmv: ix out of range");
+                dix[ix[q]] += delta;
+            }
+            upload(o.b_dix, dix);
             o.K.dix = o.b_dix.as<double>();
             o.m = o.n = a.n + a.m;
         } catch(...) { delete s; throw; }

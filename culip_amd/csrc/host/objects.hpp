@@ -160,6 +160,15 @@ struct Bildlt final : Precond {
     DevBuf d_ptr[4], d_col[4], d_val[4], d_lr[4], d_task[2];
     DevBuf d_totals, d_scan_tmp;
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
+    bool plan_known = false;              // plan_tot read back since the factor
+    // the numeric refactor captured as a HIP graph and replayed (explicit
+    // streams only); captured again when the threshold changes. With
+    // graphs the exact solves always run as level launches: captures must
+    // not depend on the single-launch admission (its event queries would
+    // touch streams that other threads are capturing)
+    bool use_graphs = false;
+    hipGraphExec_t factor_exec = nullptr;
+    double graph_threshold = 0.0;
     // single-launch triangular solves: published values, and the half of
     // d_chain_pub the next such solve publishes into; h_trsv_err is set by
     // a solve whose dependency wait gave up (pinned: read without a copy)
@@ -188,6 +197,7 @@ struct Bildlt final : Precond {
     }
     Bildlt(const Bildlt&) = delete;
     Bildlt& operator=(const Bildlt&) = delete;
+    ~Bildlt() override { release_graphs(); }
 
     int32_t n() const { return A.m; }
     bool f32() const { return prec == CULIP_PREC_F32; }
@@ -195,7 +205,7 @@ struct Bildlt final : Precond {
 
     void symbolic(int32_t fill_level, double fill_factor);
     // static structures of the pattern (once) and this instance's buffers
-    void upload_pattern();
+    void upload_pattern(hipStream_t st = nullptr);
     // device bytes of this instance's numeric buffers only
     int64_t numeric_bytes() const;
     // the same before allocation, from the symbolic analysis; the solve
@@ -230,6 +240,12 @@ struct Bildlt final : Precond {
     template<typename VT> FactorArgs<VT> factor_args() const;
     template<typename VT> SolvePlan<VT> plan() const;
     template<typename VT> void factor_impl(hipStream_t st);
+    // launches the refactor as a replay of its captured graph (capturing it
+    // first); false when graphs are off or st cannot be captured
+    bool factor_graph(hipStream_t st);
+    void release_graphs();
+    // entries of the four solve-layout rows (reads them back once)
+    const int32_t* plan_sizes();
 
     // M = U' D U: left U^-T, right U^-1
     int32_t size() const override { return n(); }
@@ -297,7 +313,7 @@ struct Spmv {
     bool have_at = false;
     DevKkt K;
     DevBuf b_row, b_col, b_val, bt_row, bt_col, bt_val, b_d, b_dix, b_arow, b_acol, b_aval,
-        b_atrow, b_atcol, b_atval;
+        b_atrow, b_atcol, b_atval, b_hrow, b_hcol, b_hval;
     int32_t m = 0, n = 0;
     DevBuf b_eq;            // equilibration work vector
     void multiply(const double* x, double* y, double alpha, double beta, bool transpose,
@@ -316,6 +332,9 @@ struct Sqmr {
     // true residuals measured on the unscaled system (device, borrowed;
     // NULL: as given)
     const double* res_scale = nullptr;
+    // replay the iterations between true-residual checks from captured
+    // graphs (set by owners that keep operator and preconditioner fixed)
+    std::unique_ptr<SqmrGraphs> graphs;
     DevBuf work;
     int32_t* host_flags = nullptr;
     double* host_scal = nullptr;

@@ -243,8 +243,9 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
     try {
         in = std::make_unique<ServeInstance>();
         in->f = std::make_unique<Bildlt>(P.pat);
-        in->f->upload_pattern();
+        in->f->upload_pattern(s->st);
         in->f->set_solve(P.algo, P.sweeps);
+        in->f->use_graphs = o_.graphs != 0;
         in->op.kkt = false;
         in->op.m = in->op.n = P.N;
         in->op.A.m = in->op.A.n = P.N;
@@ -253,6 +254,7 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
         in->op.A.col = P.k_col.as<int32_t>();
         in->op.A.val = in->f->d_vals.as<double>();   // the factor's copy of K's values
         in->q.init(&in->op, in->f.get(), o_.sqmr);
+        if(o_.graphs) in->q.graphs = std::make_unique<SqmrGraphs>();
         in->rhs.ensure(sizeof(double) * (size_t) std::max(1, P.N));
         in->x.ensure(sizeof(double) * (size_t) std::max(1, P.N));
         if(o_.equilibrate_sweeps > 0) {

@@ -253,6 +253,14 @@ culip_status culip_bildlt_get_real(culip_bildlt * h, const char * name,
     const double ** ptr, int64_t * len);
 /* Device bytes held by this object. */
 int64_t culip_bildlt_device_bytes(const culip_bildlt * h);
+/* Numeric refactors as HIP graph replays (no counterpart in the reference;
+ * the reference launches each level's kernels from a host loop): with
+ * enable != 0, a refactor on an explicit (non-NULL) stream that is not
+ * being captured replays a graph of the whole launch sequence, captured on
+ * the first such refactor and again after a threshold change or a new
+ * symbolic phase; same results as the launches themselves. Exact solves of
+ * an object with graphs run as level launches (same bits as SINGLE). */
+culip_status culip_bildlt_set_graphs(culip_bildlt * h, int32_t enable);
 /* Testing hook, no counterpart in the reference: records, from the host, a
  * single-launch solve whose dependency wait gave up, as the kernel would
  * (after upload; CULIP_ERR_STATE before). */
@@ -269,6 +277,13 @@ culip_status culip_spmv_create_csr(const culip_csr * hA, culip_spmv ** out);
  * matrix-free from A (m x n), d (n), delta and ix (n_ix rows of the dual
  * block; ix == NULL: none). Dimensions are n + m (the reference's m()/n()
  * return A->m, spmv.impl.cuh:319-337; fixed here). */
+/* AugmentedMatrixSPMV's operator with a general H (spmv.cuh:116-149 takes
+ * a diagonal d only): K = [H A'; A delta I(ix)] matrix-free from H (n x n
+ * CSR, symmetric), A (m x n), delta and ix, so an interior-point KKT
+ * system with a full Hessian needs no assembled K for SQMR. y_1 sums a
+ * row of H, then the row of A' (eight lanes, fixed order). */
+culip_status culip_spmv_create_kkt_csr(const culip_csr * hH, const culip_csr * hA,
+    double delta, int32_t n_ix, const int32_t * ix, culip_spmv ** out);
 culip_status culip_spmv_create_kkt(const culip_csr * hA, const double * d,
     double delta, int32_t n_ix, const int32_t * ix, culip_spmv ** out);
 /* NormalMatrixSPMV (spmv.cuh:74-107, spmv.impl.cuh:122-265): the m x m
@@ -411,6 +426,10 @@ typedef struct {
      * sweeps (culip_spmv_equilibrate) and solved as D K D y = D b with the
      * true residual measured on K, x = D y; 0: K as given */
     int32_t equilibrate_sweeps;
+    /* != 0: every instance replays its refactor and the SQMR iterations
+     * between true-residual checks from captured HIP graphs (same results,
+     * far fewer launches for small systems) */
+    int32_t graphs;
 } culip_server_opts;
 
 typedef struct culip_server culip_server;

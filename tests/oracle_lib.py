@@ -36,6 +36,8 @@ def lib():
             f.restype, f.argtypes = I, [P, I, D]
             f = getattr(h, f"or_bildlt_factor_{suf}")
             f.restype, f.argtypes = I, [P, D]
+            f = getattr(h, f"or_bildlt_set_pivoting_{suf}")
+            f.restype, f.argtypes = None, [P, I]
             for side in ("left", "middle", "right"):
                 f = getattr(h, f"or_bildlt_solve_{side}_{suf}")
                 f.restype, f.argtypes = None, [P, pd, pd]
@@ -94,6 +96,11 @@ class OracleBildlt:
 
     def factor(self, threshold):
         getattr(lib(), f"or_bildlt_factor_{self.t}")(self._h, threshold)
+
+    def set_pivoting(self, mode):
+        """'static' (default), 'bk' (Bunch-Kaufman) or 'rook'."""
+        m = {"static": 0, "bk": 1, "rook": 2}[mode]
+        getattr(lib(), f"or_bildlt_set_pivoting_{self.t}")(self._h, m)
 
     def get(self, name):
         p = C.c_void_p()
