@@ -273,6 +273,12 @@ class BlockiLDLt(_Handle):
     def device_bytes(self):
         return lib().culip_bildlt_device_bytes(self._h)
 
+    def set_pivoting(self, mode):
+        """'static' (default), 'bk' (Bunch-Kaufman) or 'rook': numeric
+        pivoting inside pivot blocks (BlockiLDLt<T, bk, rook>)."""
+        m = {"static": 0, "bk": 1, "rook": 2}[mode]
+        check(lib().culip_bildlt_set_pivoting(self._h, m))
+
     def set_graphs(self, enable=True):
         """Refactors on explicit streams replay a captured HIP graph."""
         check(lib().culip_bildlt_set_graphs(self._h, 1 if enable else 0))

@@ -45,6 +45,14 @@ __device__ __forceinline__ void tile_zero(VT* T)
     for(int t = 0; t < 16; ++t) T[64 * t + lane] = (VT) 0;
 }
 
+template<typename VT>
+__device__ __forceinline__ void tile_copy(VT* dst, const VT* src)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for(int t = 0; t < 16; ++t) dst[64 * t + lane] = src[64 * t + lane];
+}
+
 // d_load_block (kernels :22-81) for an input block: zero, then fill
 template<typename VT>
 __device__ void tile_load(VT* T, const BlockMeta& m, const int32_t* bs, const int32_t* ixs,
@@ -238,6 +246,30 @@ __device__ void product(VT* T, const VT* A, const VT* R, const uint32_t* mask, c
     }
 }
 
+// The Givens-form solve of d_left_solve_with_D_Ut (kernels :1533-1566) for
+// row l of a block of hgt rows, D pointing at row l's tridiagonal triple:
+// x1 for row l, x2 for row l + 1 when a 2x2 pivot starts at l (p2). In the
+// factor's type, as the reference computes it; a 1x1 pivot (d12 = 0) gives
+// x1 = rl / d11 exactly.
+template<typename VT>
+__device__ __forceinline__ void givens_d_solve(const double* D, int l, int hgt, VT rl, VT right,
+    bool p2, VT& x1, VT& x2)
+{
+    const VT d11 = (VT) D[1];
+    const VT d12 = (VT) D[2];
+    VT d22 = (l + 1 < hgt) ? (VT) D[4] : (VT) 0;
+    if(l == 31) d22 = d11;
+    const VT r = sqrt(fma(d11, d11, d12 * d12));
+    const VT c = d11 / r, sn = d12 / r;
+    x1 = fma(c, rl, sn * right);
+    x2 = fma(-sn, rl, c * right);
+    const VT t11 = fma(c, d11, sn * d12);
+    const VT t12 = fma(c, d12, sn * d22);
+    const VT t22 = fma(-sn, d12, c * d22);
+    x2 = p2 ? (x2 / t22) : (VT) 0;
+    x1 = fma(-t12, x2, x1) / t11;
+}
+
 // d_ldlt_1x1_step (kernels :538-585)
 template<typename VT>
 __device__ void step_1x1(VT* T, double* tri, int p, int len)
@@ -274,32 +306,33 @@ __device__ void step_1x1(VT* T, double* tri, int p, int len)
     wave_sync();
 }
 
-// d_ldlt_2x2_step (kernels :589-675); the 2x2 arithmetic runs in double
+// d_ldlt_2x2_step (kernels :589-675): Givens rotation of the 2x2 pivot,
+// in the factor's type as the reference computes it
 template<typename VT>
 __device__ void step_2x2(VT* T, double* tri, int p, int len)
 {
     const int lane = lane_id();
     const int j = lane & 31, half = lane >> 5;
     bool mod = false;
-    double d11 = (double) T[32 * p + p];
-    const double d12 = (double) T[32 * p + p + 1];
-    double d22 = (double) T[32 * (p + 1) + p + 1];
-    const double det = fma(d11, d22, -(d12 * d12));
-    if(fabs(det) < 1e-8) { d11 = 1e-6; d22 = 1e-6; mod = true; }
-    const double r = sqrt(fma(d11, d11, d12 * d12));
-    const double c = d11 / r, s = d12 / r;
-    const double t11 = fma(c, d11, s * d12);
-    const double t12 = fma(c, d12, s * d22);
-    const double t22 = fma(-s, d12, c * d22);
-    const double a1 = (j < p) ? 0.0 : (double) T[32 * p + j];
-    const double a2 = (j < p) ? 0.0 : ((j == p) ? d12 : (double) T[32 * (p + 1) + j]);
-    double x1 = fma(c, a1, s * a2);
-    double x2 = fma(-s, a1, c * a2);
+    VT d11 = T[32 * p + p];
+    const VT d12 = T[32 * p + p + 1];
+    VT d22 = T[32 * (p + 1) + p + 1];
+    const VT det = fma(d11, d22, -(d12 * d12));
+    if(fabs((double) det) < 1e-8) { d11 = (VT) 1e-6; d22 = (VT) 1e-6; mod = true; }
+    const VT r = sqrt(fma(d11, d11, d12 * d12));
+    const VT c = d11 / r, s = d12 / r;
+    const VT t11 = fma(c, d11, s * d12);
+    const VT t12 = fma(c, d12, s * d22);
+    const VT t22 = fma(-s, d12, c * d22);
+    const VT a1 = (j < p) ? (VT) 0 : T[32 * p + j];
+    const VT a2 = (j < p) ? (VT) 0 : ((j == p) ? d12 : T[32 * (p + 1) + j]);
+    VT x1 = fma(c, a1, s * a2);
+    VT x2 = fma(-s, a1, c * a2);
     x2 = x2 / t22;
     x1 = fma(-t12, x2, x1) / t11;
-    const VT u1 = (VT) x1, u2 = (VT) x2;
-    const double r1 = fma(x1, d11, x2 * d12);
-    const double r2 = fma(x1, d12, x2 * d22);
+    const VT u1 = x1, u2 = x2;
+    const VT r1 = fma(x1, d11, x2 * d12);
+    const VT r2 = fma(x1, d12, x2 * d22);
     wave_sync();
     if(half == 0) {
         T[32 * p + j] = (j < p) ? (VT) 0 : u1;
@@ -310,19 +343,17 @@ __device__ void step_2x2(VT* T, double* tri, int p, int len)
     const uint64_t ucols = wave_ballot(half == 0 && j > p + 1 && j < len &&
         (u1 != (VT) 0 || u2 != (VT) 0));
     uint64_t rows = ucols ? wave_ballot(half == 0 && j > p + 1 && j < len &&
-        (r1 != 0.0 || r2 != 0.0)) : 0ull;
+        (r1 != (VT) 0 || r2 != (VT) 0)) : 0ull;
     while(rows) {
         const int ia = __ffsll((long long) rows) - 1;
         rows &= rows - 1;
         const int ib = rows ? __ffsll((long long) rows) - 1 : -1;
         if(rows) rows &= rows - 1;
-        const double r1a = wave_bcast(r1, ia), r2a = wave_bcast(r2, ia);
-        const double r1b = wave_bcast(r1, ib < 0 ? ia : ib), r2b = wave_bcast(r2, ib < 0 ? ia : ib);
+        const VT r1a = wave_bcast(r1, ia), r2a = wave_bcast(r2, ia);
+        const VT r1b = wave_bcast(r1, ib < 0 ? ia : ib), r2b = wave_bcast(r2, ib < 0 ? ia : ib);
         const int i = half ? ib : ia;
-        const double r1i = half ? r1b : r1a, r2i = half ? r2b : r2a;
-        if(i >= 0 && j < len)
-            T[32 * i + j] = (VT) ((double) T[32 * i + j] -
-                fma(r1i, (double) u1, r2i * (double) u2));
+        const VT r1i = half ? r1b : r1a, r2i = half ? r2b : r2a;
+        if(i >= 0 && j < len) T[32 * i + j] = T[32 * i + j] - fma(r1i, u1, r2i * u2);
     }
     wave_sync();
     if(mod && lane == 0) {
@@ -501,6 +532,109 @@ struct alignas(16) OffLds {
 
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Numeric pivoting inside a pivot tile T (stride 32, the matrix in its upper
+// triangle), one wave. Lane l < 32 stands for row / column l, as a warp lane
+// of the reference does.
+
+// largest value over the wave and the lowest lane holding it among the
+// lanes with `in` set (the reference's shuffle reduction keeps the lower
+// lane on ties)
+template<typename VT>
+__device__ __forceinline__ int wave_argmax_first(VT v, bool in, VT& best)
+{
+    best = wave_max(in ? v : (VT) 0);
+    const uint64_t at = wave_ballot(in && v == best);
+    return at ? __ffsll((long long) at) - 1 : -1;
+}
+
+template<typename VT>
+__device__ __forceinline__ VT sym_abs(const VT* T, int a, int b)
+{
+    return (a < b) ? fabs(T[32 * a + b]) : fabs(T[32 * b + a]);
+}
+
+// d_select_pivot_bunch_kaufman (kernels :743-834); alpha = (1 + sqrtf(17))
+// / 8 evaluated in float as the reference writes it
+template<typename VT>
+__device__ void select_bk(const VT* T, int len, int p, int& order, int& p0, int& p1)
+{
+    const int l = lane_id();
+    const VT alpha = (VT) ((1.0f + sqrtf(17.0f)) / 8.0f);
+    order = 1;
+    p0 = p;
+    p1 = p + 1;
+    VT w1;
+    const VT v = (l > p && l < len) ? fabs(T[32 * p + min(l, 31)]) : (VT) 0;
+    const int r = wave_argmax_first(v, l > p && l < len, w1);
+    if(!(w1 > (VT) 0)) return;
+    const VT app = fabs(T[32 * p + p]);
+    if(app >= alpha * w1) return;
+    const bool in = l >= p && l < len && l != r;
+    const VT wr = wave_max(in ? sym_abs(T, min(l, 31), r) : (VT) 0);
+    if(app * wr >= alpha * w1 * w1) return;
+    if(fabs(T[32 * r + r]) >= alpha * wr) { p0 = r; return; }
+    order = 2;
+    p1 = r;
+}
+
+// d_select_pivot_rook (kernels :838-948)
+template<typename VT>
+__device__ void select_rook(const VT* T, int len, int p, int& order, int& p0, int& p1)
+{
+    const int l = lane_id();
+    const VT alpha = (VT) ((1.0f + sqrtf(17.0f)) / 8.0f);
+    order = 1;
+    p0 = p;
+    p1 = p + 1;
+    VT wi;
+    const VT v = (l > p && l < len) ? fabs(T[32 * p + min(l, 31)]) : (VT) 0;
+    int r = wave_argmax_first(v, l > p && l < len, wi);
+    if(!(wi > (VT) 0)) return;
+    int i = p;
+    if(fabs(T[32 * p + p]) >= alpha * wi) return;
+    for(;;) {
+        VT wr;
+        const bool in = l >= p && l < len && l != r;
+        int q = wave_argmax_first(in ? sym_abs(T, min(l, 31), r) : (VT) 0, in, wr);
+        // no nonzero candidate: the reference's reduction leaves lane 0's
+        // start index, max(0, p)
+        if(!(wr > (VT) 0)) q = p;
+        if(fabs(T[32 * r + r]) >= alpha * wr) { p0 = r; return; }
+        if(wi == wr) { order = 2; p0 = i; p1 = r; return; }
+        i = r;
+        r = q;
+        wi = wr;
+    }
+}
+
+// d_dense_pivot_op (kernels :679-739): symmetric interchange of rows and
+// columns a and b of the tile (earlier, factored rows included, their
+// entries in the two columns move too) and of perm[a], perm[b]
+template<typename VT>
+__device__ void dense_pivot_op(VT* T, int32_t* perm, int len, int pa, int pb)
+{
+    const int l = lane_id();
+    wave_sync();
+    if(l < len) {
+        int x, y;
+        if(l < pa) { x = 32 * l + pa; y = 32 * l + pb; }
+        else if(l == pa) { x = 32 * pa + pa; y = 32 * pb + pb; }
+        else if(l < pb) { x = 32 * pa + l; y = 32 * l + pb; }
+        else if(l == pb) { x = y = 32 * pa + pb; }
+        else { x = 32 * pa + l; y = 32 * pb + l; }
+        const VT tx = T[x], ty = T[y];
+        T[x] = ty;
+        T[y] = tx;
+    }
+    if(l == 0) {
+        const int32_t t = perm[pa];
+        perm[pa] = perm[pb];
+        perm[pb] = t;
+    }
+    wave_sync();
+}
+
 // k_factor_pivot_blocks (kernels :1146-1283)
 template<typename VT>
 __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
@@ -603,13 +737,34 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
 #pragma unroll
     for(int t = 0; t < 16; ++t) T0[32 * (cq + 2 * t) + j] = T[t];
     wave_sync();
-    // d_factor_pivot_factorize with static pivots (kernels :956-1083)
+    // d_factor_pivot_factorize (kernels :956-1083): static pivots, or a
+    // numeric choice per step with symmetric interchanges (the pivot starts
+    // and the block's rows in perm are rewritten as pivots are chosen)
     double* tri = a.tri + 3 * (int64_t) fr;
+    int32_t* fpiv = a.is_piv + fr;
+    int32_t* bperm = a.perm + fr;
     int p = 0;
     while(p < len) {
-        const int order = (p < len - 1 && !a.is_piv[fr + p + 1]) ? 2 : 1;
-        if(order == 1) step_1x1(T0, tri, p, len);
-        else step_2x2(T0, tri, p, len);
+        int order, p0 = p, p1 = p + 1;
+        if(a.pivoting == 1 && p < len - 1) select_bk(T0, len, p, order, p0, p1);
+        else if(a.pivoting == 2 && p < len - 1) select_rook(T0, len, p, order, p0, p1);
+        else order = (p < len - 1 && !fpiv[p + 1]) ? 2 : 1;
+        if(order == 1) {
+            if(a.pivoting && p0 != p) dense_pivot_op(T0, bperm, len, p, p0);
+            step_1x1(T0, tri, p, len);
+            if(a.pivoting && lane == 0) fpiv[p] = 1;
+        } else {
+            if(a.pivoting) {
+                if(p0 != p) dense_pivot_op(T0, bperm, len, p, p0);
+                dense_pivot_op(T0, bperm, len, p + 1, p1);
+            }
+            step_2x2(T0, tri, p, len);
+            if(a.pivoting && lane == 0) {
+                fpiv[p] = 1;
+                fpiv[p + 1] = 0;
+            }
+        }
+        wave_sync();
         p += order;
     }
     if(lane < len) {
@@ -702,6 +857,18 @@ __device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& l
         }
     }
 
+    // rows in the block row's pivoting order (kernels :1733-1751)
+    if(a.pivoting) {
+        tile_copy(s.A, s.T);
+        wave_sync();
+        for(int e = lane; e < hgt * 32; e += 64) {
+            const int r = e >> 5, c = e & 31;
+            s.T[e] = s.A[32 * (a.perm[fr + r] - fr) + c];
+        }
+        wave_sync();
+        tile_zero(s.A);
+        wave_sync();
+    }
     // pivot tile U_ii (row-major) into A; umask[r] = nonzero columns > r
     const VT* piv = a.if_val + (int64_t) pq * 1024;
     for(int e = lane; e < hgt * hgt; e += 64) {
@@ -745,24 +912,13 @@ __device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& l
             if(l < hgt) {
                 const bool p1 = ip[l] != 0;
                 const bool p2 = (l + 1 < hgt) && !ip[l + 1];
-                const double rl = (double) rhs[l];
-                const double right = (l < 31) ? (double) rhs[l + 1] : (double) rhs[31];
-                const double d11 = D[3 * l + 1];
-                const double d12 = D[3 * l + 2];
-                double d22 = (l + 1 < hgt) ? D[3 * (l + 1) + 1] : 0.0;
-                if(l == 31) d22 = d11;
-                const double r = sqrt(fma(d11, d11, d12 * d12));
-                const double c = d11 / r, sn = d12 / r;
-                double x1 = fma(c, rl, sn * right);
-                double x2 = fma(-sn, rl, c * right);
-                const double t11 = fma(c, d11, sn * d12);
-                const double t12 = fma(c, d12, sn * d22);
-                const double t22 = fma(-sn, d12, c * d22);
-                x2 = p2 ? (x2 / t22) : 0.0;
-                x1 = fma(-t12, x2, x1) / t11;
+                const VT rl = rhs[l];
+                const VT right = (l < 31) ? rhs[l + 1] : rhs[31];
+                VT x1, x2;
+                givens_d_solve(D + 3 * l, l, hgt, rl, right, p2, x1, x2);
                 if(j < wid) {
-                    if(p1) s.T[32 * l + j] = (VT) x1;
-                    if(p2) s.T[32 * (l + 1) + j] = (VT) x2;
+                    if(p1) s.T[32 * l + j] = x1;
+                    if(p2) s.T[32 * (l + 1) + j] = x2;
                 }
             }
         }
@@ -993,23 +1149,12 @@ __device__ void offdiag_small_one(const FactorArgs<VT>& a, const LevelRange& lr,
             for(int l = 0; l < hgt; ++l) {
                 const bool p1 = ip[l] != 0;
                 const bool p2 = (l + 1 < hgt) && !ip[l + 1];
-                const double rl = (double) col[32 * l];
-                const double right = (l < 31) ? (double) col[32 * (l + 1)] : rl;
-                const double d11 = D[3 * l + 1];
-                const double d12 = D[3 * l + 2];
-                double d22 = (l + 1 < hgt) ? D[3 * (l + 1) + 1] : 0.0;
-                if(l == 31) d22 = d11;
-                const double r = sqrt(fma(d11, d11, d12 * d12));
-                const double c = d11 / r, sn = d12 / r;
-                double x1 = fma(c, rl, sn * right);
-                double x2 = fma(-sn, rl, c * right);
-                const double t11 = fma(c, d11, sn * d12);
-                const double t12 = fma(c, d12, sn * d22);
-                const double t22 = fma(-sn, d12, c * d22);
-                x2 = p2 ? (x2 / t22) : 0.0;
-                x1 = fma(-t12, x2, x1) / t11;
-                if(p1) col[32 * l] = (VT) x1;
-                if(p2) col[32 * (l + 1)] = (VT) x2;
+                const VT rl = col[32 * l];
+                const VT right = (l < 31) ? col[32 * (l + 1)] : rl;
+                VT x1, x2;
+                givens_d_solve(D + 3 * l, l, hgt, rl, right, p2, x1, x2);
+                if(p1) col[32 * l] = x1;
+                if(p2) col[32 * (l + 1)] = x2;
             }
         }
     }
@@ -1143,6 +1288,11 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
     const int4 t0 = ((const int4*) a.off_task)[3 * p];
     const int4 t1 = ((const int4*) a.off_task)[3 * p + 1];
     const int4 t2 = ((const int4*) a.off_task)[3 * p + 2];
+    if(a.pivoting) {
+        // rows in pivoting order: every block takes the general form
+        a.defer_list[atomicAdd(a.defer_cnt + lr.level, 1)] = q;
+        return;
+    }
     if(!(t2.z & 1)) { to_wave_form(a, lr, q); return; }
     const int fr = t0.y;
     const int hgt = t0.z & 255, wid = (t0.z >> 8) & 255;
@@ -1829,6 +1979,44 @@ __global__ __launch_bounds__(256) void k_store_tiny(FactorArgs<VT> a, LevelRange
     }
 }
 
+// k_pivot_col_blocks (kernels :1352-1418) with d_col_pivot_block
+// (:1290-1348): one wave per block row of the level; the stored blocks above
+// its pivot block (earlier levels, final storage) take its pivoting order in
+// their columns: sparse entries get new column indices in place, dense rows
+// are gathered column by column.
+template<typename VT>
+__global__ __launch_bounds__(64) void k_pivot_cols(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ int inv[32];
+    const int lane = lane_id();
+    const int i = a.lvl_rows[lr.row_lo + blockIdx.x];
+    const int fr = a.bs[i];
+    const int w = a.bs[i + 1] - fr;
+    const int lp = (lane < w) ? a.perm[fr + lane] - fr : lane;
+    if(lane < w) inv[lp] = lane;
+    if(wave_ballot(lane < w && lp != lane) == 0) return;
+    wave_sync();
+    for(int ci = a.csc_off[i]; ci < a.csc_off[i + 1] - 1; ++ci) {
+        const BlockMeta m = a.bout[a.csc_idx[ci]];
+        if(m.format == FMT_SPARSE) {
+            int32_t* ix = a.ix_out + m.col_ptr;
+            for(int e = lane; e < m.nnz; e += 64) {
+                const int v = ix[e];
+                ix[e] = (v / w) * w + inv[v % w];
+            }
+        } else {
+            const int h = a.bs[m.block_row + 1] - a.bs[m.block_row];
+            VT* v = a.val_out + m.val_ptr;
+            for(int r = 0; r < h; ++r) {
+                const VT x = (lane < w) ? v[(int64_t) r * w + lp] : (VT) 0;
+                wave_sync();
+                if(lane < w) v[(int64_t) r * w + lane] = x;
+                wave_sync();
+            }
+        }
+    }
+}
+
 template<typename VT>
 __global__ void k_zero_carry(int32_t* carry)
 {
@@ -1850,6 +2038,23 @@ __global__ void k_zero(VT* a, int64_t n)
 }
 
 }  // namespace
+
+__global__ void k_reset_pivots(int32_t n, const int32_t* sp, int32_t* is_piv, int32_t* perm)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) {
+        is_piv[g] = sp[g];
+        perm[g] = g;
+    }
+}
+
+void factor_reset_pivots(int32_t n, const int32_t* static_is_piv, int32_t* is_piv, int32_t* perm,
+    void* stream)
+{
+    if(n > 0)
+        k_reset_pivots<<<(n + 255) / 256, 256, 0, (hipStream_t) stream>>>(n, static_is_piv, is_piv,
+            perm);
+}
 
 template<typename VT>
 void gather_values(const double* vals, const int64_t* slot, int64_t nnz, VT* val_in,
@@ -1885,6 +2090,7 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
         k_offdiag<VT><<<grid, 64, 0, st>>>(a, r);
         k_offdiag_general<VT><<<grid, 64, 0, st>>>(a, r);
     }
+    if(a.pivoting && r.npiv > 0) k_pivot_cols<VT><<<r.npiv, 64, 0, st>>>(a, r);
     if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
     const unsigned nch = (unsigned) ((r.size + SCAN_CHUNK - 1) / SCAN_CHUNK);
     k_level_maps<VT><<<nch, SC_TPB, 0, st>>>(a, r);

@@ -12,7 +12,13 @@ template<typename VT>
 struct FactorArgs {
     int32_t n = 0, nb = 0;
     const int32_t* bs = nullptr;          // block starts (nb + 1)
-    const int32_t* is_piv = nullptr;      // n
+    // n: pivot starts of this factor (a copy of the static ones; numeric
+    // pivoting rewrites it as it picks 1x1 and 2x2 pivots)
+    int32_t* is_piv = nullptr;
+    // n: new -> old rows (identity, interchanged inside pivot blocks by
+    // numeric pivoting); pivoting: 0 static, 1 Bunch-Kaufman, 2 rook
+    int32_t* perm = nullptr;
+    int32_t pivoting = 0;
     const BlockMeta* bin = nullptr;       // symbolic metadata (input stores)
     BlockMeta* bout = nullptr;            // factored metadata (output stores)
     const int32_t* csr_off = nullptr;     // coarse CSR offsets (block ids)
@@ -83,6 +89,9 @@ template<typename VT>
 void factor_begin(const FactorArgs<VT>& a, void* stream);
 template<typename VT>
 void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream);
+// start of a factor: is_piv <- the static pivot starts, perm <- identity
+void factor_reset_pivots(int32_t n, const int32_t* static_is_piv, int32_t* is_piv, int32_t* perm,
+    void* stream);
 // val_in[slot[q]] = vals[q] for slot[q] >= 0 (val_in zeroed first)
 template<typename VT>
 void gather_values(const double* vals, const int64_t* slot, int64_t nnz, VT* val_in,

@@ -868,6 +868,28 @@ void trsv_upper_t(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* s
     }
 }
 
+__global__ void k_perm_pull(int32_t n, const int32_t* perm, const double* in, double* out)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) out[g] = in[perm[g]];
+}
+
+__global__ void k_perm_push(int32_t n, const int32_t* perm, const double* in, double* out)
+{
+    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if(g < n) out[perm[g]] = in[g];
+}
+
+void permute_pull(int32_t n, const int32_t* perm, const double* in, double* out, void* stream)
+{
+    if(n > 0) k_perm_pull<<<(n + 255) / 256, 256, 0, (hipStream_t) stream>>>(n, perm, in, out);
+}
+
+void permute_push(int32_t n, const int32_t* perm, const double* in, double* out, void* stream)
+{
+    if(n > 0) k_perm_push<<<(n + 255) / 256, 256, 0, (hipStream_t) stream>>>(n, perm, in, out);
+}
+
 void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
     void* stream)
 {

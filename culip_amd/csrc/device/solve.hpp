@@ -98,5 +98,9 @@ void jacobi_sweeps_product(const SolveTopo& t, const SolvePlan<VT>& p, bool tran
     double* x_b, void* stream);
 void solve_D(int32_t n, const double* tri, const int32_t* is_piv, const double* b, double* x,
     void* stream);
+// k_permute_Pt: out[i] = in[perm[i]]; k_permute_P: out[perm[i]] = in[i]
+// (helper_kernels.impl.cuh:776-814); out must not be in
+void permute_pull(int32_t n, const int32_t* perm, const double* in, double* out, void* stream);
+void permute_push(int32_t n, const int32_t* perm, const double* in, double* out, void* stream);
 
 }  // namespace culip

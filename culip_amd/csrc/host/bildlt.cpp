@@ -186,6 +186,9 @@ void Bildlt::upload_pattern(hipStream_t st)
         }
     }
     d_scan_tmp.ensure(sizeof(int32_t) * scan_tmp_ints(n));
+    d_fis_piv.ensure(sizeof(int32_t) * (size_t) std::max(1, n));
+    d_perm.ensure(sizeof(int32_t) * (size_t) std::max(1, n));
+    d_perm_tmp.ensure(sizeof(double) * (size_t) std::max(1, n));
     d_chain_pub.ensure(sizeof(double) * 2 * (size_t) std::max(1, n));
     h_trsv_err.ensure();
     // the presets on the caller's stream: no device-wide synchronisation,
@@ -205,7 +208,9 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.n = sym.n;
     a.nb = sym.nb;
     a.bs = d_bs.as<int32_t>();
-    a.is_piv = d_is_piv.as<int32_t>();
+    a.is_piv = d_fis_piv.as<int32_t>();
+    a.perm = d_perm.as<int32_t>();
+    a.pivoting = (int32_t) pivoting;
     a.bin = d_bin.as<BlockMeta>();
     a.bout = d_bout.as<BlockMeta>();
     a.csr_off = d_csr_off.as<int32_t>();
@@ -285,7 +290,7 @@ SolveTopo Bildlt::topo() const
     t.bout = d_bout.as<BlockMeta>();
     t.ix_out = d_ix_out.as<int32_t>();
     t.tri = d_tri.as<double>();
-    t.is_piv = d_is_piv.as<int32_t>();
+    t.is_piv = d_fis_piv.as<int32_t>();
     t.chain_pub = d_chain_pub.as<double>();
     t.err = h_trsv_err.p;
     return t;
@@ -296,6 +301,7 @@ void Bildlt::factor_impl(hipStream_t st)
 {
     const Symbolic& s = sym;
     FactorArgs<VT> a = factor_args<VT>();
+    factor_reset_pivots(s.n, d_is_piv.as<int32_t>(), a.is_piv, a.perm, st);
     gather_values<VT>(d_vals.as<double>(), d_value_slot.as<int64_t>(), A.nnz(), d_val_in.as<VT>(),
         s.val_space, st);
     factor_begin<VT>(a, st);
@@ -381,6 +387,8 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
     factored = true;
     plan_known = false;
     host_int.erase("fblocks");
+    host_int.erase("perm");
+    host_int.erase("fis_piv");
     host_int.erase("fix_store");
     host_real.erase("fval_store");
     host_real.erase("tri");
@@ -423,7 +431,35 @@ void Bildlt::jacobi(double* x, bool transpose, hipStream_t st) const
             jb[0].as<double>(), jb[1].as<double>(), jb[2].as<double>(), st);
 }
 
+void Bildlt::perm_pull(double* x, hipStream_t st) const
+{
+    if(pivoting == CULIP_PIVOT_STATIC) return;
+    const size_t bytes = sizeof(double) * (size_t) n();
+    HIP_CHECK(hipMemcpyAsync(d_perm_tmp.p, x, bytes, hipMemcpyDeviceToDevice, st));
+    permute_pull(n(), d_perm.as<int32_t>(), d_perm_tmp.as<double>(), x, st);
+}
+
+void Bildlt::perm_push(double* x, hipStream_t st) const
+{
+    if(pivoting == CULIP_PIVOT_STATIC) return;
+    const size_t bytes = sizeof(double) * (size_t) n();
+    HIP_CHECK(hipMemcpyAsync(d_perm_tmp.p, x, bytes, hipMemcpyDeviceToDevice, st));
+    permute_push(n(), d_perm.as<int32_t>(), d_perm_tmp.as<double>(), x, st);
+}
+
 void Bildlt::trsv_u(double* x, hipStream_t st) const
+{
+    trsv_u_unpermuted(x, st);
+    perm_push(x, st);
+}
+
+void Bildlt::trsv_ut(double* x, hipStream_t st) const
+{
+    perm_pull(x, st);
+    trsv_ut_unpermuted(x, st);
+}
+
+void Bildlt::trsv_u_unpermuted(double* x, hipStream_t st) const
 {
     if(solve_algo == CULIP_TRSV_JACOBI) {
         jacobi(x, false, st);
@@ -442,7 +478,7 @@ void Bildlt::trsv_u(double* x, hipStream_t st) const
     else trsv_upper<double>(topo(), plan<double>(), x, st);
 }
 
-void Bildlt::trsv_ut(double* x, hipStream_t st) const
+void Bildlt::trsv_ut_unpermuted(double* x, hipStream_t st) const
 {
     if(solve_algo == CULIP_TRSV_JACOBI) {
         jacobi(x, true, st);
@@ -462,7 +498,8 @@ void Bildlt::trsv_ut(double* x, hipStream_t st) const
 
 bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStream_t st) const
 {
-    if(K.m != n() || K.n != n()) return false;
+    // the fused forms read K's rows in the factor's row order: not with P
+    if(K.m != n() || K.n != n() || pivoting != CULIP_PIVOT_STATIC) return false;
     if(solve_algo == CULIP_TRSV_JACOBI) {
         // the first U' sweep forms each block row's K src itself
         const size_t bytes = sizeof(double) * (size_t) std::max(1, n());
@@ -510,7 +547,7 @@ void Bildlt::solve(culip_side side, bool transpose, const double* b, double* x,
     take_async_error();
     const size_t bytes = sizeof(double) * (size_t) n();
     if(side == CULIP_SOLVE_MIDDLE) {
-        solve_D(n(), d_tri.as<double>(), d_is_piv.as<int32_t>(), b, x, st);
+        solve_D(n(), d_tri.as<double>(), d_fis_piv.as<int32_t>(), b, x, st);
         return;
     }
     // transpose swaps left and right (block_ildlt.impl.cuh:192-201, 236-245)
@@ -534,12 +571,18 @@ void Bildlt::solve_noalloc(culip_side side, bool transpose, const double* b, dou
     if(!s0) fail(CULIP_ERR_HIP, "solve_noalloc: the scratch array's device address is unknown");
     take_async_error();
     if(side == CULIP_SOLVE_MIDDLE) {
-        solve_D(n(), d_tri.as<double>(), d_is_piv.as<int32_t>(), b, x, st);
+        solve_D(n(), d_tri.as<double>(), d_fis_piv.as<int32_t>(), b, x, st);
         return;
     }
     const bool right = (side == CULIP_SOLVE_RIGHT) != transpose;
     const size_t bytes = sizeof(double) * (size_t) n();
     if(b != x) HIP_CHECK(hipMemcpyAsync(x, b, bytes, hipMemcpyDeviceToDevice, st));
+    const bool piv = pivoting != CULIP_PIVOT_STATIC;
+    // P' before a left solve (the scratch holds the copy it reads)
+    if(piv && !right) {
+        HIP_CHECK(hipMemcpyAsync(s0, x, bytes, hipMemcpyDeviceToDevice, st));
+        permute_pull(n(), d_perm.as<int32_t>(), s0, x, st);
+    }
     if(solve_algo != CULIP_TRSV_JACOBI) {
         // the exact solve as level launches: no published-value buffer
         if(right) {
@@ -549,16 +592,22 @@ void Bildlt::solve_noalloc(culip_side side, bool transpose, const double* b, dou
             if(f32()) trsv_upper_t<float>(topo(), plan<float>(), x, st);
             else trsv_upper_t<double>(topo(), plan<double>(), x, st);
         }
-        return;
+    } else {
+        double* s1 = s0 + NOALLOC_MAX_ROWS;
+        double* s2 = s1 + NOALLOC_MAX_ROWS;
+        HIP_CHECK(hipMemcpyAsync(s0, x, bytes, hipMemcpyDeviceToDevice, st));
+        if(f32())
+            ::culip::jacobi_sweeps<float>(topo(), plan<float>(), !right, x, jacobi_sweeps, s0, s1,
+                s2, st);
+        else
+            ::culip::jacobi_sweeps<double>(topo(), plan<double>(), !right, x, jacobi_sweeps, s0, s1,
+                s2, st);
     }
-    double* s1 = s0 + NOALLOC_MAX_ROWS;
-    double* s2 = s1 + NOALLOC_MAX_ROWS;
-    HIP_CHECK(hipMemcpyAsync(s0, x, bytes, hipMemcpyDeviceToDevice, st));
-    if(f32())
-        ::culip::jacobi_sweeps<float>(topo(), plan<float>(), !right, x, jacobi_sweeps, s0, s1, s2, st);
-    else
-        ::culip::jacobi_sweeps<double>(topo(), plan<double>(), !right, x, jacobi_sweeps, s0, s1, s2,
-            st);
+    // P after a right solve
+    if(piv && right) {
+        HIP_CHECK(hipMemcpyAsync(s0, x, bytes, hipMemcpyDeviceToDevice, st));
+        permute_push(n(), d_perm.as<int32_t>(), s0, x, st);
+    }
 }
 
 int64_t Bildlt::numeric_bytes() const
@@ -641,6 +690,13 @@ void Bildlt::download_factor()
     std::vector<double> tri(3 * (size_t) n());
     HIP_CHECK(hipMemcpy(tri.data(), d_tri.p, sizeof(double) * tri.size(), hipMemcpyDeviceToHost));
     host_real["tri"] = std::move(tri);
+    std::vector<int32_t> perm(n()), fpiv(n());
+    if(n()) {
+        HIP_CHECK(hipMemcpy(perm.data(), d_perm.p, sizeof(int32_t) * n(), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(fpiv.data(), d_fis_piv.p, sizeof(int32_t) * n(), hipMemcpyDeviceToHost));
+    }
+    host_int["perm"] = std::move(perm);
+    host_int["fis_piv"] = std::move(fpiv);
 }
 
 // export_factorized (block_ildlt.impl.cuh:1700-1916): U as fine CSR (the
@@ -737,10 +793,12 @@ double Bildlt::export_factorized(const std::string& folder)
         drp[i + 1] = (int32_t) dci.size();
     }
     write_mm(folder + "/out_D.mtx", N, drp, dci, dcv);
-    // P: the identity of static pivoting
+    // P: row i holds a one in column perm[i] (new -> old; the identity for
+    // static pivots), as the reference writes it (:1902-1910)
+    const std::vector<int32_t>& perm = host_int.at("perm");
     std::vector<int32_t> prp(N + 1), pci(N);
     for(int32_t i = 0; i <= N; ++i) prp[i] = i;
-    for(int32_t i = 0; i < N; ++i) pci[i] = i;
+    for(int32_t i = 0; i < N; ++i) pci[i] = perm[i];
     write_mm(folder + "/out_P.mtx", N, prp, pci, std::vector<double>(N, 1.0));
     return (2.0 * (double) nnz_u + (double) dci.size()) / (double) A.nnz();
 }

@@ -498,7 +498,8 @@ culip_status culip_bildlt_get_int(culip_bildlt* h, const char* name, const int32
             put(b.host_int[nm]);
             return;
         }
-        if(nm == "fblocks" || nm == "fix_store" || nm == "carry") {
+        if(nm == "fblocks" || nm == "fix_store" || nm == "carry" || nm == "perm" ||
+            nm == "fis_piv") {
             b.download_factor();
             put(b.host_int.at(nm));
             return;
@@ -567,6 +568,17 @@ culip_status culip_bildlt_get_real(culip_bildlt* h, const char* name, const doub
             return;
         }
         fail(CULIP_ERR_INVALID, "bildlt: unknown structure '" + nm + "'");
+    });
+}
+
+culip_status culip_bildlt_set_pivoting(culip_bildlt* h, culip_pivoting mode)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null argument");
+        CULIP_REQUIRE(mode == CULIP_PIVOT_STATIC || mode == CULIP_PIVOT_BUNCH_KAUFMAN ||
+                mode == CULIP_PIVOT_ROOK, "bildlt: unknown pivoting");
+        if(h->b.pivoting != mode) h->b.release_graphs();
+        h->b.pivoting = mode;
     });
 }
 

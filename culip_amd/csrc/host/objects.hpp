@@ -149,6 +149,11 @@ struct Bildlt final : Precond {
     bool uploaded = false;   // this instance's numeric buffers exist
     bool factored = false;
     double threshold = 0.0;
+    // BlockiLDLt<T, bk, rook>: numeric pivoting inside pivot blocks
+    culip_pivoting pivoting = CULIP_PIVOT_STATIC;
+    // this factor's pivot starts and row permutation (new -> old), and a
+    // work vector for applying P in the solves
+    DevBuf d_fis_piv, d_perm, d_perm_tmp;
 
     // numeric (per instance)
     DevBuf d_vals, d_val_in, d_bout, d_ix_out, d_val_out, d_tri, d_rn;
@@ -219,8 +224,11 @@ struct Bildlt final : Precond {
     // exceed NOALLOC_MAX_ROWS)
     void solve_noalloc(culip_side side, bool transpose, const double* b, double* x,
         hipStream_t st) const;
+    // x <- P (U \ x) and x <- U' \ (P' x); the _unpermuted forms leave P out
     void trsv_u(double* x, hipStream_t st) const;
     void trsv_ut(double* x, hipStream_t st) const;
+    void trsv_u_unpermuted(double* x, hipStream_t st) const;
+    void trsv_ut_unpermuted(double* x, hipStream_t st) const;
     void jacobi(double* x, bool transpose, hipStream_t st) const;
     // set_solve_algorithm; allocates the block-Jacobi vectors when needed
     void set_solve(culip_solve_algorithm a, int32_t sweeps);
@@ -254,8 +262,12 @@ struct Bildlt final : Precond {
     void right(double* x, hipStream_t st) const override { trsv_u(x, st); }
     bool left_product(const DevCsr& K, const double* src, double* x, hipStream_t st) const override;
     const double* tri() const override { return d_tri.as<double>(); }
-    const int32_t* is_piv() const override { return d_is_piv.as<int32_t>(); }
+    const int32_t* is_piv() const override { return d_fis_piv.as<int32_t>(); }
     void after_solve(hipStream_t st) const override { check_solve_error(st); }
+    // x <- x(perm) before a left solve, x(perm) <- x after a right solve
+    // (k_permute_Pt / k_permute_P); nothing for static pivots
+    void perm_pull(double* x, hipStream_t st) const;
+    void perm_push(double* x, hipStream_t st) const;
 };
 
 // ImportLDUPreconditioner (import_ldu_preconditioner.cuh:36-125): an

@@ -223,6 +223,18 @@ typedef enum {
     CULIP_TRSV_SINGLE = 1,
     CULIP_TRSV_JACOBI = 2
 } culip_solve_algorithm;
+/* BlockiLDLt<T, bk, rook> (block_ildlt_instances.cu:54-60): static pivots
+ * from piv_starts, or numeric Bunch-Kaufman / rook pivoting inside every
+ * pivot block (block_ildlt_kernels.impl.cuh:743-948) with the symmetric
+ * interchanges applied to the block row and to the columns above
+ * (:1352-1418); the solves then apply P (k_permute_P / k_permute_Pt). Set
+ * before compute / refactor. */
+typedef enum {
+    CULIP_PIVOT_STATIC = 0,
+    CULIP_PIVOT_BUNCH_KAUFMAN = 1,
+    CULIP_PIVOT_ROOK = 2
+} culip_pivoting;
+culip_status culip_bildlt_set_pivoting(culip_bildlt * h, culip_pivoting mode);
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
     culip_solve_algorithm algorithm, int32_t num_jacobi_sweeps);
 
@@ -244,7 +256,8 @@ culip_status culip_bildlt_export_factorized(culip_bildlt * h, const char * folde
  *   solve_off, solve_idx, blocks (9 int32 per block: id, format, nnz,
  *   max_nnz, got_extra, block_row, block_col, col_ptr, val_ptr), ix_store,
  *   upd_off, upd_ki, upd_kj, scalars, and after compute fblocks, fix_store,
- *   carry, plan_sizes.
+ *   carry, plan_sizes, perm (new -> old rows), fis_piv (pivot starts as
+ *   factored).
  * real names: val_store; after compute fval_store, tri (3 per row:
  *   D(i-1,i), D(i,i), D(i+1,i)), row_norms. */
 culip_status culip_bildlt_get_int(culip_bildlt * h, const char * name,

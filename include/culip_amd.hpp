@@ -40,11 +40,14 @@ inline void check(culip_status s)
 /* block_ildlt.cuh:24-30, same names and values. */
 enum SOLVE_ALGORITHM { SOLVE_STRIPE = 0, SOLVE_BLOCK = 1, SOLVE_SCALAR = 2, SOLVE_JACOBI = 3 };
 
-/* Preconditioner<T> with solve_left/middle/right (preconditioner.cuh). */
-template<typename T>
+/* Preconditioner<T> with solve_left/middle/right (preconditioner.cuh).
+ * BK / ROOK: BlockiLDLt<T, bk, rook> (block_ildlt_instances.cu:54-60),
+ * numeric Bunch-Kaufman or rook pivoting inside every pivot block. */
+template<typename T, bool BK = false, bool ROOK = false>
 class BlockiLDLt {
     static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
         "BlockiLDLt<T>: T is double or float");
+    static_assert(!(BK && ROOK), "BlockiLDLt<T, bk, rook>: one pivoting rule at most");
 
 public:
     /* block_ildlt.impl.cuh:44-79 */
@@ -54,6 +57,9 @@ public:
     {
         check(culip_bildlt_create(h_A, num_pivs, piv_starts, num_blocks, block_starts,
             std::is_same<T, float>::value ? CULIP_PREC_F32 : CULIP_PREC_F64, &h_));
+        if(BK || ROOK)
+            check(culip_bildlt_set_pivoting(h_,
+                BK ? CULIP_PIVOT_BUNCH_KAUFMAN : CULIP_PIVOT_ROOK));
     }
     ~BlockiLDLt() { culip_bildlt_destroy(h_); }
     BlockiLDLt(const BlockiLDLt&) = delete;
@@ -244,14 +250,20 @@ public:
  * preconditioner); true_residual_period = 0 keeps its stopping test. */
 class SQMR {
 public:
-    template<typename T>
+    template<typename T, bool BK, bool ROOK>
     SQMR(void* stream, const SPMV* spmv, double tol, int32_t max_it,
-        const BlockiLDLt<T>* prec = nullptr, int32_t true_residual_period = 0,
+        const BlockiLDLt<T, BK, ROOK>* prec, int32_t true_residual_period = 0,
         int32_t max_restarts = 0)
         : stream_(stream)
     {
         culip_sqmr_opts o{tol, max_it, true_residual_period, max_restarts, 0};
-        check(culip_sqmr_create(spmv->handle(), prec ? prec->handle() : nullptr, &o, &s_));
+        check(culip_sqmr_create(spmv->handle(), prec->handle(), &o, &s_));
+    }
+    SQMR(void* stream, const SPMV* spmv, double tol, int32_t max_it)
+        : stream_(stream)
+    {
+        culip_sqmr_opts o{tol, max_it, 0, 0, 0};
+        check(culip_sqmr_create(spmv->handle(), nullptr, &o, &s_));
     }
     template<typename T>
     SQMR(void* stream, const SPMV* spmv, double tol, int32_t max_it,

@@ -1,7 +1,7 @@
 // The reference driver's call sequence (apps/block_ildlt/block_ildlt_invariants.h:
 // 126-194: BlockiLDLt ctor -> compute -> solve_analysis -> b = A * 1 ->
 // CSRMatrixSPMV -> SQMR(1e-8, 1000) -> solve) written against culip_amd.hpp.
-// Usage: kkt_driver N_PRIMAL N_DUAL [f32]. Prints one JSON line; exit 0 when
+// Usage: kkt_driver N_PRIMAL N_DUAL [f32] [bk|rook]. Prints one JSON line; exit 0 when
 // the true relative residual is <= 1e-8. With --files the driver reads its
 // inputs as the reference driver does (run_files below).
 #include <hip/hip_runtime_api.h>
@@ -25,7 +25,7 @@ using namespace culip_amd;
         }                                                                              \
     } while(0)
 
-template<typename T>
+template<typename T, bool BK = false, bool ROOK = false>
 static int run(int32_t n, int32_t m)
 {
     culip_kkt_params p{};
@@ -53,7 +53,7 @@ static int run(int32_t n, int32_t m)
 
     int rc = 1;
     {
-        BlockiLDLt<T> blocky(nullptr, &Kp, np, ps, nb, bs);
+        BlockiLDLt<T, BK, ROOK> blocky(nullptr, &Kp, np, ps, nb, bs);
         blocky.compute(1, T(1.0), T(1e-4));
         blocky.set_solve_algorithm(SOLVE_BLOCK, 0);
         blocky.solve_analysis();
@@ -181,14 +181,21 @@ int main(int argc, char** argv)
         }
     }
     if(argc < 3) {
-        std::fprintf(stderr, "usage: %s N_PRIMAL N_DUAL [f32]\n"
+        std::fprintf(stderr, "usage: %s N_PRIMAL N_DUAL [f32] [bk|rook]\n"
                              "       %s --files A BLOCK_STARTS PERM IS_PIV FILL FF THRESH [f32]\n",
             argv[0], argv[0]);
         return 2;
     }
     const int32_t n = std::atoi(argv[1]), m = std::atoi(argv[2]);
-    const bool f32 = argc > 3 && std::strcmp(argv[3], "f32") == 0;
+    bool f32 = false, bk = false, rook = false;
+    for(int a = 3; a < argc; ++a) {
+        f32 = f32 || std::strcmp(argv[a], "f32") == 0;
+        bk = bk || std::strcmp(argv[a], "bk") == 0;
+        rook = rook || std::strcmp(argv[a], "rook") == 0;
+    }
     try {
+        if(bk) return f32 ? run<float, true, false>(n, m) : run<double, true, false>(n, m);
+        if(rook) return f32 ? run<float, false, true>(n, m) : run<double, false, true>(n, m);
         return f32 ? run<float>(n, m) : run<double>(n, m);
     } catch(const Error& e) {
         std::fprintf(stderr, "culip_amd error %d: %s\n", (int) e.status, e.what());

@@ -77,8 +77,8 @@ def _fp32_factor_close(g, o, rtol, min_match=0.999):
     oracle): same formats everywhere; on the blocks whose stored entry count
     and whose sparse positions are the same (the others differ by a
     threshold or top-k decision at FP32 rounding), values within rtol of the
-    block's largest |value| (the device computes 2x2 pivots and D in FP64, the oracle in
-    FP32). Returns (fraction of matching blocks, worst relative error)."""
+    block's largest |value|. Returns (fraction of matching blocks, worst
+    relative error)."""
     fb_g, fb_o = g.get_int("fblocks").reshape(-1, 9), o.get("fblocks").reshape(-1, 9)
     assert np.array_equal(fb_g[:, 1], fb_o[:, 1]), "formats differ"
     assert np.array_equal(fb_g[:, 5:7], fb_o[:, 5:7])
@@ -131,55 +131,20 @@ def test_factor_fp32_barrier_conditioned_equals_oracle(hip_device):
     _factor_equal(g, o)
 
 
-@pytest.mark.parametrize("seed", [2, 3, 6, 11])
-def test_factor_fp32_2x2_at_least_as_accurate_as_oracle(hip_device, seed):
-    """FP32 factor with 32-45 2x2 pivots on random indefinite matrices whose
-    FP32 factor is meaningful (diagonal scale 20, no dropping; on the
-    worse-conditioned draws of this generator both FP32 factors are off by
-    O(1) from the FP64 one). The device forms 2x2 determinants, inverses and
-    D in FP64 where BlockiLDLt<float> (the f oracle) stays in FP32, so the
-    two FP32 factors differ in rounding. Checked: on the blocks whose stored
-    entries agree in all three, the device's distance to the FP64 oracle
-    factor is at most twice the f oracle's (plus 1e-6) and below 1e-5 of
-    each block's largest value."""
+@pytest.mark.parametrize("seed,fill,ff,thr,shift", [
+    (1, 1, 2.0, 1e-3, 4.0), (2, 1, 2.0, 1e-3, 4.0), (3, 1, 2.0, 1e-3, 4.0),
+    (2, 0, 1.0, 0.0, 20.0), (6, 0, 1.0, 0.0, 20.0), (11, 0, 1.0, 0.0, 20.0)])
+def test_factor_fp32_2x2_equals_oracle(hip_device, seed, fill, ff, thr, shift):
+    """FP32 factor (BlockiLDLt<float>) with static 2x2 pivots on random
+    indefinite matrices, with level-1 fill and dropping and with no
+    dropping: the 2x2 steps and the Givens-form D solves of the off-diagonal
+    blocks run in FP32 as the reference computes them, so the factor equals
+    the f oracle bit for bit."""
     csr, piv, bs, _ = sym_problem(n_blocks=9, bmax=32, density=0.12, seed=seed,
-                                  diag_shift=20.0)
-    g, of = _pair(csr, piv, bs, "f", fill_level=0, ff=1.0, thr=0.0)
-    od = ol.OracleBildlt(*csr, piv, bs, "d")
-    od.symbolic(0, 1.0)
-    od.factor(0.0)
-    fbs = [x.get_int("fblocks") if x is g else x.get("fblocks") for x in (g, of, od)]
-    fbs = [f.reshape(-1, 9) for f in fbs]
-    fix = [g.get_int("fix_store"), of.get("fix_store"), od.get("fix_store")]
-    fv = [g.get_real("fval_store"), of.get("fval_store"), od.get("fval_store")]
-    bsa = np.asarray(of.get("block_starts"), np.int64)
-    h = bsa[fbs[1][:, 5] + 1] - bsa[fbs[1][:, 5]]
-    w = bsa[fbs[1][:, 6] + 1] - bsa[fbs[1][:, 6]]
-    err_dev = err_f = 0.0
-    common = 0
-    for b in range(fbs[0].shape[0]):
-        if not (fbs[0][b, 2] == fbs[1][b, 2] == fbs[2][b, 2]):
-            continue
-        dense = fbs[1][b, 1] == 1
-        cnt = int(h[b] * w[b]) if dense else int(fbs[1][b, 2])
-        if cnt == 0:
-            continue
-        if not dense:
-            pos = [fx[f[b, 7]:f[b, 7] + cnt] for fx, f in zip(fix, fbs)]
-            if not (np.array_equal(pos[0], pos[2]) and np.array_equal(pos[1], pos[2])):
-                continue
-        v = [x[f[b, 8]:f[b, 8] + cnt] for x, f in zip(fv, fbs)]
-        scale = np.abs(v[2]).max()
-        if scale == 0:
-            continue
-        common += 1
-        err_dev = max(err_dev, float(np.abs(v[0] - v[2]).max() / scale))
-        err_f = max(err_f, float(np.abs(v[1] - v[2]).max() / scale))
-    print(f"fp32 2x2 factor vs fp64: device {err_dev:.2e}, f oracle {err_f:.2e} "
-          f"over {common} blocks")
-    assert common >= 0.95 * fbs[0].shape[0]
-    assert err_dev <= 2.0 * err_f + 1e-6
-    assert err_dev < 1e-5
+                                  diag_shift=shift)
+    assert np.diff(np.append(piv, csr[0].size - 1)).max() == 2
+    g, o = _pair(csr, piv, bs, "f", fill_level=fill, ff=ff, thr=thr)
+    _factor_equal(g, o)
 
 
 @pytest.mark.parametrize("t", ["d", "f"])
@@ -364,14 +329,15 @@ def test_factor_bitwise_indefinite_barrier_fp64(hip_device):
     assert np.sum(np.abs(tri) == 1e-6) > 0, "no tiny pivot replaced"
 
 
-@pytest.mark.parametrize("prec", ["", "f32"])
-def test_cpp_driver(hip_device, prec):
+@pytest.mark.parametrize("prec,piv", [("", ""), ("f32", ""), ("", "bk"), ("f32", "rook")])
+def test_cpp_driver(hip_device, prec, piv):
     """The reference driver's call sequence written against the C++ classes
-    of include/culip_amd.hpp, as a separate process."""
+    of include/culip_amd.hpp, as a separate process; also with
+    BlockiLDLt<T, bk, rook>."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "kkt_driver")
-    args = [exe, "2000", "800"] + ([prec] if prec else [])
+    args = [exe, "2000", "800"] + [a for a in (prec, piv) if a]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, (r.stdout, r.stderr)
     import json
