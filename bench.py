@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--serve-jacobi-sweeps", type=int, default=2,
                     help="serve: block-Jacobi solves with this many sweeps (0: level solves)")
     ap.add_argument("--graphs", type=int, default=-1,
-                    help="replay refactors and SQMR segments from HIP graphs (default: on for "
-                         "serve, off for batch, where the small systems measured slower)")
+                    help="replay refactors (and, when serving, SQMR segments) from HIP graphs "
+                         "(default: on for serve and the single-system workloads, off for "
+                         "batch, where the small systems measured slower)")
     ap.add_argument("--equilibrate-sweeps", type=int, default=8,
                     help="serve: device equilibration sweeps per request (0: off)")
     ap.add_argument("--max-it", type=int, default=0,
@@ -450,8 +451,13 @@ def main():
         return
     if a.prec is None:
         a.prec = "f64"
+    if a.graphs < 0:
+        a.graphs = 1
     wl = WORKLOADS[a.workload]
     dev = torch.device("cuda:0")
+    # one explicit stream for every launch (refactor graphs replay on
+    # explicit streams only; the HIP events below record on it too)
+    torch.cuda.set_stream(torch.cuda.Stream())
     kkt = ca.KKT(wl["n"], wl["m"], wl["o"], wl["c"], h_diag=wl["h_diag"], h_diag_log=wl["log"],
                  delta=wl["delta"], seed=a.seed, barrier=wl.get("barrier"))
     order = ca.ORDER_BLOCK_COLOR if a.order == "color" else ca.ORDER_NATURAL
@@ -465,6 +471,7 @@ def main():
     t_sym = time.perf_counter() - t_sym0
     sc = g.get_int("scalars")
     g.set_solve_algorithm(a.solve_algorithm, a.jacobi_sweeps)
+    g.set_graphs(bool(a.graphs))
     A = ca.CSRMatrixSPMV(prep.row_ptr, prep.col, prep.val)
     solver = ca.SQMR(A, 1e-8, 1000, g, true_residual_period=20, max_restarts=10)
     dvals = torch.tensor(prep.val, device=dev)
@@ -554,6 +561,7 @@ def main():
                            prep.block_starts, prec)
         gv.symbolic(0, a.fill_factor)
         gv.set_solve_algorithm(algorithm, sweeps)
+        gv.set_graphs(bool(a.graphs))
         sv_ = ca.SQMR(A, 1e-8, 1000, gv, true_residual_period=20, max_restarts=10)
         xv = torch.zeros_like(x)
         for _ in range(a.warmup):
@@ -612,7 +620,8 @@ def main():
                    "max_block": wl["B"], "fill_level": 0, "fill_factor": a.fill_factor,
                    "drop_tol": a.threshold, "ordering": a.order, "levels": int(sc[6]),
                    "coarse_blocks": int(sc[5]), "tol": 1e-8, "parallelism": "none",
-                   "solve_algorithm": a.solve_algorithm, "jacobi_sweeps": a.jacobi_sweeps},
+                   "solve_algorithm": a.solve_algorithm, "jacobi_sweeps": a.jacobi_sweeps,
+                   "refactor_graphs": bool(a.graphs)},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "factor_ms_mean": float(np.mean(fac) * 1e3),

@@ -279,6 +279,11 @@ class BlockiLDLt(_Handle):
         m = {"static": 0, "bk": 1, "rook": 2}[mode]
         check(lib().culip_bildlt_set_pivoting(self._h, m))
 
+    def set_pivot_precision(self, p):
+        """FP32 factor: 'f64' (default) forms the 2x2 pivot steps and D
+        solves in FP64, 'factor' in FP32 as BlockiLDLt<float> does."""
+        check(lib().culip_bildlt_set_pivot_precision(self._h, {"f64": 0, "factor": 1}[p]))
+
     def set_graphs(self, enable=True):
         """Refactors on explicit streams replay a captured HIP graph."""
         check(lib().culip_bildlt_set_graphs(self._h, 1 if enable else 0))

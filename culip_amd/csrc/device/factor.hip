@@ -248,26 +248,38 @@ __device__ void product(VT* T, const VT* A, const VT* R, const uint32_t* mask, c
 
 // The Givens-form solve of d_left_solve_with_D_Ut (kernels :1533-1566) for
 // row l of a block of hgt rows, D pointing at row l's tridiagonal triple:
-// x1 for row l, x2 for row l + 1 when a 2x2 pivot starts at l (p2). In the
-// factor's type, as the reference computes it; a 1x1 pivot (d12 = 0) gives
-// x1 = rl / d11 exactly.
-template<typename VT>
-__device__ __forceinline__ void givens_d_solve(const double* D, int l, int hgt, VT rl, VT right,
-    bool p2, VT& x1, VT& x2)
+// x1 for row l, x2 for row l + 1 when a 2x2 pivot starts at l (p2). The
+// arithmetic runs in AT: the factor's type as the reference computes it,
+// or FP64 for an FP32 factor whose 2x2 pivots are inverted in FP64. A 1x1
+// pivot (d12 = 0) gives x1 = rl / d11 exactly either way.
+template<typename VT, typename AT>
+__device__ __forceinline__ void givens_d_solve(const double* D, int l, int hgt, VT rl_, VT right_,
+    bool p2, VT& x1_, VT& x2_)
 {
-    const VT d11 = (VT) D[1];
-    const VT d12 = (VT) D[2];
-    VT d22 = (l + 1 < hgt) ? (VT) D[4] : (VT) 0;
+    const AT rl = (AT) rl_, right = (AT) right_;
+    const AT d11 = (AT) D[1];
+    const AT d12 = (AT) D[2];
+    AT d22 = (l + 1 < hgt) ? (AT) D[4] : (AT) 0;
     if(l == 31) d22 = d11;
-    const VT r = sqrt(fma(d11, d11, d12 * d12));
-    const VT c = d11 / r, sn = d12 / r;
-    x1 = fma(c, rl, sn * right);
-    x2 = fma(-sn, rl, c * right);
-    const VT t11 = fma(c, d11, sn * d12);
-    const VT t12 = fma(c, d12, sn * d22);
-    const VT t22 = fma(-sn, d12, c * d22);
-    x2 = p2 ? (x2 / t22) : (VT) 0;
+    const AT r = sqrt(fma(d11, d11, d12 * d12));
+    const AT c = d11 / r, sn = d12 / r;
+    AT x1 = fma(c, rl, sn * right);
+    AT x2 = fma(-sn, rl, c * right);
+    const AT t11 = fma(c, d11, sn * d12);
+    const AT t12 = fma(c, d12, sn * d22);
+    const AT t22 = fma(-sn, d12, c * d22);
+    x2 = p2 ? (x2 / t22) : (AT) 0;
     x1 = fma(-t12, x2, x1) / t11;
+    x1_ = (VT) x1;
+    x2_ = (VT) x2;
+}
+
+template<typename VT>
+__device__ __forceinline__ void givens_d_solve(bool f64, const double* D, int l, int hgt, VT rl,
+    VT right, bool p2, VT& x1, VT& x2)
+{
+    if(f64) givens_d_solve<VT, double>(D, l, hgt, rl, right, p2, x1, x2);
+    else givens_d_solve<VT, VT>(D, l, hgt, rl, right, p2, x1, x2);
 }
 
 // d_ldlt_1x1_step (kernels :538-585)
@@ -306,33 +318,36 @@ __device__ void step_1x1(VT* T, double* tri, int p, int len)
     wave_sync();
 }
 
-// d_ldlt_2x2_step (kernels :589-675): Givens rotation of the 2x2 pivot,
-// in the factor's type as the reference computes it
-template<typename VT>
+// d_ldlt_2x2_step (kernels :589-675): Givens rotation of the 2x2 pivot.
+// The arithmetic runs in AT (see givens_d_solve); the multipliers are
+// stored in the factor's type and the trailing update subtracts their
+// products with the FP64-exact row factors r1, r2. A replaced near-singular
+// pivot is (VT) 1e-6 on the diagonal in both forms.
+template<typename VT, typename AT>
 __device__ void step_2x2(VT* T, double* tri, int p, int len)
 {
     const int lane = lane_id();
     const int j = lane & 31, half = lane >> 5;
     bool mod = false;
-    VT d11 = T[32 * p + p];
-    const VT d12 = T[32 * p + p + 1];
-    VT d22 = T[32 * (p + 1) + p + 1];
-    const VT det = fma(d11, d22, -(d12 * d12));
-    if(fabs((double) det) < 1e-8) { d11 = (VT) 1e-6; d22 = (VT) 1e-6; mod = true; }
-    const VT r = sqrt(fma(d11, d11, d12 * d12));
-    const VT c = d11 / r, s = d12 / r;
-    const VT t11 = fma(c, d11, s * d12);
-    const VT t12 = fma(c, d12, s * d22);
-    const VT t22 = fma(-s, d12, c * d22);
-    const VT a1 = (j < p) ? (VT) 0 : T[32 * p + j];
-    const VT a2 = (j < p) ? (VT) 0 : ((j == p) ? d12 : T[32 * (p + 1) + j]);
-    VT x1 = fma(c, a1, s * a2);
-    VT x2 = fma(-s, a1, c * a2);
+    AT d11 = (AT) T[32 * p + p];
+    const AT d12 = (AT) T[32 * p + p + 1];
+    AT d22 = (AT) T[32 * (p + 1) + p + 1];
+    const AT det = fma(d11, d22, -(d12 * d12));
+    if(fabs((double) det) < 1e-8) { d11 = (AT) (VT) 1e-6; d22 = (AT) (VT) 1e-6; mod = true; }
+    const AT r = sqrt(fma(d11, d11, d12 * d12));
+    const AT c = d11 / r, s = d12 / r;
+    const AT t11 = fma(c, d11, s * d12);
+    const AT t12 = fma(c, d12, s * d22);
+    const AT t22 = fma(-s, d12, c * d22);
+    const AT a1 = (j < p) ? (AT) 0 : (AT) T[32 * p + j];
+    const AT a2 = (j < p) ? (AT) 0 : ((j == p) ? d12 : (AT) T[32 * (p + 1) + j]);
+    AT x1 = fma(c, a1, s * a2);
+    AT x2 = fma(-s, a1, c * a2);
     x2 = x2 / t22;
     x1 = fma(-t12, x2, x1) / t11;
-    const VT u1 = x1, u2 = x2;
-    const VT r1 = fma(x1, d11, x2 * d12);
-    const VT r2 = fma(x1, d12, x2 * d22);
+    const VT u1 = (VT) x1, u2 = (VT) x2;
+    const AT r1 = fma(x1, d11, x2 * d12);
+    const AT r2 = fma(x1, d12, x2 * d22);
     wave_sync();
     if(half == 0) {
         T[32 * p + j] = (j < p) ? (VT) 0 : u1;
@@ -343,17 +358,18 @@ __device__ void step_2x2(VT* T, double* tri, int p, int len)
     const uint64_t ucols = wave_ballot(half == 0 && j > p + 1 && j < len &&
         (u1 != (VT) 0 || u2 != (VT) 0));
     uint64_t rows = ucols ? wave_ballot(half == 0 && j > p + 1 && j < len &&
-        (r1 != (VT) 0 || r2 != (VT) 0)) : 0ull;
+        (r1 != (AT) 0 || r2 != (AT) 0)) : 0ull;
     while(rows) {
         const int ia = __ffsll((long long) rows) - 1;
         rows &= rows - 1;
         const int ib = rows ? __ffsll((long long) rows) - 1 : -1;
         if(rows) rows &= rows - 1;
-        const VT r1a = wave_bcast(r1, ia), r2a = wave_bcast(r2, ia);
-        const VT r1b = wave_bcast(r1, ib < 0 ? ia : ib), r2b = wave_bcast(r2, ib < 0 ? ia : ib);
+        const AT r1a = wave_bcast(r1, ia), r2a = wave_bcast(r2, ia);
+        const AT r1b = wave_bcast(r1, ib < 0 ? ia : ib), r2b = wave_bcast(r2, ib < 0 ? ia : ib);
         const int i = half ? ib : ia;
-        const VT r1i = half ? r1b : r1a, r2i = half ? r2b : r2a;
-        if(i >= 0 && j < len) T[32 * i + j] = T[32 * i + j] - fma(r1i, u1, r2i * u2);
+        const AT r1i = half ? r1b : r1a, r2i = half ? r2b : r2a;
+        if(i >= 0 && j < len)
+            T[32 * i + j] = (VT) ((AT) T[32 * i + j] - fma(r1i, (AT) u1, r2i * (AT) u2));
     }
     wave_sync();
     if(mod && lane == 0) {
@@ -362,8 +378,8 @@ __device__ void step_2x2(VT* T, double* tri, int p, int len)
         T[32 * (p + 1) + p + 1] = 1;
     }
     if(lane < 3) {
-        tri[3 * p + lane] = (lane == 0) ? 0.0 : ((lane == 1) ? d11 : d12);
-        tri[3 * (p + 1) + lane] = (lane == 0) ? d12 : ((lane == 1) ? d22 : 0.0);
+        tri[3 * p + lane] = (lane == 0) ? 0.0 : ((lane == 1) ? (double) d11 : (double) d12);
+        tri[3 * (p + 1) + lane] = (lane == 0) ? (double) d12 : ((lane == 1) ? (double) d22 : 0.0);
     }
     wave_sync();
 }
@@ -758,7 +774,8 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
                 if(p0 != p) dense_pivot_op(T0, bperm, len, p, p0);
                 dense_pivot_op(T0, bperm, len, p + 1, p1);
             }
-            step_2x2(T0, tri, p, len);
+            if(a.piv64) step_2x2<VT, double>(T0, tri, p, len);
+            else step_2x2<VT, VT>(T0, tri, p, len);
             if(a.pivoting && lane == 0) {
                 fpiv[p] = 1;
                 fpiv[p + 1] = 0;
@@ -915,7 +932,7 @@ __device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& l
                 const VT rl = rhs[l];
                 const VT right = (l < 31) ? rhs[l + 1] : rhs[31];
                 VT x1, x2;
-                givens_d_solve(D + 3 * l, l, hgt, rl, right, p2, x1, x2);
+                givens_d_solve(a.piv64 != 0, D + 3 * l, l, hgt, rl, right, p2, x1, x2);
                 if(j < wid) {
                     if(p1) s.T[32 * l + j] = x1;
                     if(p2) s.T[32 * (l + 1) + j] = x2;
@@ -1152,7 +1169,7 @@ __device__ void offdiag_small_one(const FactorArgs<VT>& a, const LevelRange& lr,
                 const VT rl = col[32 * l];
                 const VT right = (l < 31) ? col[32 * (l + 1)] : rl;
                 VT x1, x2;
-                givens_d_solve(D + 3 * l, l, hgt, rl, right, p2, x1, x2);
+                givens_d_solve(a.piv64 != 0, D + 3 * l, l, hgt, rl, right, p2, x1, x2);
                 if(p1) col[32 * l] = x1;
                 if(p2) col[32 * (l + 1)] = x2;
             }

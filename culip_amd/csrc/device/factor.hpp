@@ -19,6 +19,9 @@ struct FactorArgs {
     // numeric pivoting); pivoting: 0 static, 1 Bunch-Kaufman, 2 rook
     int32_t* perm = nullptr;
     int32_t pivoting = 0;
+    // 1: 2x2 pivot steps and the Givens-form D solves in FP64 (an FP32
+    // factor's default); 0: in the factor's type, as BlockiLDLt<T>
+    int32_t piv64 = 0;
     const BlockMeta* bin = nullptr;       // symbolic metadata (input stores)
     BlockMeta* bout = nullptr;            // factored metadata (output stores)
     const int32_t* csr_off = nullptr;     // coarse CSR offsets (block ids)

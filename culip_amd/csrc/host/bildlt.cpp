@@ -211,6 +211,7 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.is_piv = d_fis_piv.as<int32_t>();
     a.perm = d_perm.as<int32_t>();
     a.pivoting = (int32_t) pivoting;
+    a.piv64 = (f32() && pivot_precision == CULIP_PIVOT2_F64) ? 1 : 0;
     a.bin = d_bin.as<BlockMeta>();
     a.bout = d_bout.as<BlockMeta>();
     a.csr_off = d_csr_off.as<int32_t>();
@@ -465,7 +466,7 @@ void Bildlt::trsv_u_unpermuted(double* x, hipStream_t st) const
         jacobi(x, false, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE && !use_graphs &&
+    if(solve_algo == CULIP_TRSV_SINGLE && !solve_levels &&
         chain_admitted(st, [&] {
             const int32_t par = chain_parity;
             chain_parity ^= 1;
@@ -484,7 +485,7 @@ void Bildlt::trsv_ut_unpermuted(double* x, hipStream_t st) const
         jacobi(x, true, st);
         return;
     }
-    if(solve_algo == CULIP_TRSV_SINGLE && !use_graphs &&
+    if(solve_algo == CULIP_TRSV_SINGLE && !solve_levels &&
         chain_admitted(st, [&] {
             const int32_t par = chain_parity;
             chain_parity ^= 1;
@@ -516,7 +517,7 @@ bool Bildlt::left_product(const DevCsr& K, const double* src, double* x, hipStre
     }
     // the single-launch U' solve forms each block row's K src itself; when
     // it is not admitted the caller multiplies and calls left() (same bits)
-    if(solve_algo != CULIP_TRSV_SINGLE || use_graphs) return false;
+    if(solve_algo != CULIP_TRSV_SINGLE || solve_levels) return false;
     return chain_admitted(st, [&] {
         const int32_t par = chain_parity;
         chain_parity ^= 1;

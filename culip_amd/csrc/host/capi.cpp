@@ -582,6 +582,17 @@ culip_status culip_bildlt_set_pivoting(culip_bildlt* h, culip_pivoting mode)
     });
 }
 
+culip_status culip_bildlt_set_pivot_precision(culip_bildlt* h, culip_pivot_precision p)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h, "bildlt: null argument");
+        CULIP_REQUIRE(p == CULIP_PIVOT2_F64 || p == CULIP_PIVOT2_FACTOR,
+            "bildlt: unknown pivot precision");
+        if(h->b.pivot_precision != p) h->b.release_graphs();
+        h->b.pivot_precision = p;
+    });
+}
+
 culip_status culip_bildlt_set_graphs(culip_bildlt* h, int32_t enable)
 {
     return guarded([&] {

@@ -151,6 +151,8 @@ struct Bildlt final : Precond {
     double threshold = 0.0;
     // BlockiLDLt<T, bk, rook>: numeric pivoting inside pivot blocks
     culip_pivoting pivoting = CULIP_PIVOT_STATIC;
+    // arithmetic of the 2x2 pivot steps and D solves of an FP32 factor
+    culip_pivot_precision pivot_precision = CULIP_PIVOT2_F64;
     // this factor's pivot starts and row permutation (new -> old), and a
     // work vector for applying P in the solves
     DevBuf d_fis_piv, d_perm, d_perm_tmp;
@@ -167,11 +169,13 @@ struct Bildlt final : Precond {
     int32_t plan_tot[4] = {0, 0, 0, 0};   // entries of the uo, ui, to, ti rows
     bool plan_known = false;              // plan_tot read back since the factor
     // the numeric refactor captured as a HIP graph and replayed (explicit
-    // streams only); captured again when the threshold changes. With
-    // graphs the exact solves always run as level launches: captures must
-    // not depend on the single-launch admission (its event queries would
-    // touch streams that other threads are capturing)
+    // streams only); captured again when the threshold changes
     bool use_graphs = false;
+    // exact solves as level launches only: set where solves are captured
+    // (SQMR segments of a server slot) and other threads capture too, so
+    // that no capture depends on the single-launch admission, whose event
+    // queries would touch streams other threads are capturing
+    bool solve_levels = false;
     hipGraphExec_t factor_exec = nullptr;
     double graph_threshold = 0.0;
     // single-launch triangular solves: published values, and the half of

@@ -246,6 +246,7 @@ ServeInstance* Server::instance(Slot* s, int32_t pattern, std::unique_lock<std::
         in->f->upload_pattern(s->st);
         in->f->set_solve(P.algo, P.sweeps);
         in->f->use_graphs = o_.graphs != 0;
+        in->f->solve_levels = o_.graphs != 0;
         in->op.kkt = false;
         in->op.m = in->op.n = P.N;
         in->op.A.m = in->op.A.n = P.N;

@@ -235,6 +235,18 @@ typedef enum {
     CULIP_PIVOT_ROOK = 2
 } culip_pivoting;
 culip_status culip_bildlt_set_pivoting(culip_bildlt * h, culip_pivoting mode);
+/* Arithmetic of the 2x2 pivot steps (d_ldlt_2x2_step,
+ * block_ildlt_kernels.impl.cuh:589-675) and of the Givens-form D solves of
+ * the off-diagonal blocks (:1533-1566) in an FP32 factor: F64 (the default)
+ * forms 2x2 determinants, rotations and inverses in FP64 and stores the
+ * results in FP32; FACTOR computes them in FP32 as BlockiLDLt<float> does
+ * (bit for bit). No effect on an FP64 factor or on 1x1 pivots. Set before
+ * compute / refactor. */
+typedef enum {
+    CULIP_PIVOT2_F64 = 0,
+    CULIP_PIVOT2_FACTOR = 1
+} culip_pivot_precision;
+culip_status culip_bildlt_set_pivot_precision(culip_bildlt * h, culip_pivot_precision p);
 culip_status culip_bildlt_set_solve_algorithm(culip_bildlt * h,
     culip_solve_algorithm algorithm, int32_t num_jacobi_sweeps);
 
@@ -271,8 +283,8 @@ int64_t culip_bildlt_device_bytes(const culip_bildlt * h);
  * enable != 0, a refactor on an explicit (non-NULL) stream that is not
  * being captured replays a graph of the whole launch sequence, captured on
  * the first such refactor and again after a threshold change or a new
- * symbolic phase; same results as the launches themselves. Exact solves of
- * an object with graphs run as level launches (same bits as SINGLE). */
+ * symbolic phase; same results as the launches themselves. Solves are not
+ * affected. */
 culip_status culip_bildlt_set_graphs(culip_bildlt * h, int32_t enable);
 /* Testing hook, no counterpart in the reference: records, from the host, a
  * single-launch solve whose dependency wait gave up, as the kernel would

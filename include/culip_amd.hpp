@@ -102,6 +102,13 @@ public:
             num_jacobi_sweeps));
     }
 
+    /* FP32 factor: 2x2 pivot arithmetic in FP64 (default) or in FP32 as
+     * the reference's BlockiLDLt<float> */
+    void set_pivot_precision(culip_pivot_precision p)
+    {
+        check(culip_bildlt_set_pivot_precision(h_, p));
+    }
+
     bool is_left() const { return true; }
     bool is_middle() const { return true; }
     bool is_right() const { return true; }

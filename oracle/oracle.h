@@ -33,6 +33,7 @@ typedef struct or_bildlt_f or_bildlt_f;
         double fill_factor);                                                   \
     int or_bildlt_factor_##SUF(or_bildlt_##SUF * h, double threshold);         \
     void or_bildlt_set_pivoting_##SUF(or_bildlt_##SUF * h, int mode);          \
+    void or_bildlt_set_pivot_precision_##SUF(or_bildlt_##SUF * h, int f64);    \
     void or_bildlt_solve_left_##SUF(const or_bildlt_##SUF * h,                 \
         const double * b, double * x);                                         \
     void or_bildlt_solve_middle_##SUF(const or_bildlt_##SUF * h,               \

@@ -38,6 +38,8 @@ def lib():
             f.restype, f.argtypes = I, [P, D]
             f = getattr(h, f"or_bildlt_set_pivoting_{suf}")
             f.restype, f.argtypes = None, [P, I]
+            f = getattr(h, f"or_bildlt_set_pivot_precision_{suf}")
+            f.restype, f.argtypes = None, [P, I]
             for side in ("left", "middle", "right"):
                 f = getattr(h, f"or_bildlt_solve_{side}_{suf}")
                 f.restype, f.argtypes = None, [P, pd, pd]
@@ -101,6 +103,11 @@ class OracleBildlt:
         """'static' (default), 'bk' (Bunch-Kaufman) or 'rook'."""
         m = {"static": 0, "bk": 1, "rook": 2}[mode]
         getattr(lib(), f"or_bildlt_set_pivoting_{self.t}")(self._h, m)
+
+    def set_pivot_precision(self, p):
+        """'factor' (default: 2x2 steps in the factor's type, as the
+        reference) or 'f64' (in double, the library's FP32 default)."""
+        getattr(lib(), f"or_bildlt_set_pivot_precision_{self.t}")(self._h, {"factor": 0, "f64": 1}[p])
 
     def get(self, name):
         p = C.c_void_p()

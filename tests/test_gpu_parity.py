@@ -131,19 +131,27 @@ def test_factor_fp32_barrier_conditioned_equals_oracle(hip_device):
     _factor_equal(g, o)
 
 
+@pytest.mark.parametrize("prec", ["factor", "f64"])
 @pytest.mark.parametrize("seed,fill,ff,thr,shift", [
     (1, 1, 2.0, 1e-3, 4.0), (2, 1, 2.0, 1e-3, 4.0), (3, 1, 2.0, 1e-3, 4.0),
     (2, 0, 1.0, 0.0, 20.0), (6, 0, 1.0, 0.0, 20.0), (11, 0, 1.0, 0.0, 20.0)])
-def test_factor_fp32_2x2_equals_oracle(hip_device, seed, fill, ff, thr, shift):
-    """FP32 factor (BlockiLDLt<float>) with static 2x2 pivots on random
-    indefinite matrices, with level-1 fill and dropping and with no
-    dropping: the 2x2 steps and the Givens-form D solves of the off-diagonal
-    blocks run in FP32 as the reference computes them, so the factor equals
-    the f oracle bit for bit."""
+def test_factor_fp32_2x2_equals_oracle(hip_device, seed, fill, ff, thr, shift, prec):
+    """FP32 factor with static 2x2 pivots on random indefinite matrices,
+    with level-1 fill and dropping and with no dropping, equals the f oracle
+    bit for bit in both 2x2 arithmetics: 'factor' (FP32, BlockiLDLt<float>)
+    and 'f64' (the library's FP32 default: 2x2 steps and Givens-form D
+    solves in FP64, results stored in FP32)."""
     csr, piv, bs, _ = sym_problem(n_blocks=9, bmax=32, density=0.12, seed=seed,
                                   diag_shift=shift)
     assert np.diff(np.append(piv, csr[0].size - 1)).max() == 2
-    g, o = _pair(csr, piv, bs, "f", fill_level=fill, ff=ff, thr=thr)
+    rp, ci, cv = csr
+    g = ca.BlockiLDLt(rp, ci, cv, piv, bs, "f32")
+    g.set_pivot_precision(prec)
+    g.compute(fill, ff, thr)
+    o = ol.OracleBildlt(rp, ci, cv, piv, bs, "f")
+    o.set_pivot_precision(prec)
+    o.symbolic(fill, ff)
+    o.factor(thr)
     _factor_equal(g, o)
 
 

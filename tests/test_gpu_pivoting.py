@@ -15,13 +15,15 @@ from conftest import sym_problem
 pytestmark = pytest.mark.gpu
 
 
-def _pair(csr, piv, bs, mode, t="d", fill=0, ff=1.0, thr=1e-4):
+def _pair(csr, piv, bs, mode, t="d", fill=0, ff=1.0, thr=1e-4, prec="f64"):
     rp, ci, cv = csr
     g = ca.BlockiLDLt(rp, ci, cv, piv, bs, "f32" if t == "f" else "f64")
     g.set_pivoting(mode)
+    g.set_pivot_precision(prec)
     g.compute(fill, ff, thr)
     o = ol.OracleBildlt(rp, ci, cv, piv, bs, t)
     o.set_pivoting(mode)
+    o.set_pivot_precision(prec)
     o.symbolic(fill, ff)
     o.factor(thr)
     return g, o
@@ -133,11 +135,15 @@ def test_pivoting_fp32_close_to_fp64(hip_device, mode):
         assert np.abs(a - b).max() <= 1e-3 * np.abs(b).max()
 
 
+@pytest.mark.parametrize("prec", ["factor", "f64"])
 @pytest.mark.parametrize("mode", ["bk", "rook"])
-def test_pivoting_kkt_fp32_bitwise(hip_device, mode):
-    """BlockiLDLt<float, bk, rook> on the pivoting KKT system equals the
-    FP32 oracle bit for bit (pivot choices, interchanges and values)."""
+def test_pivoting_kkt_fp32_bitwise(hip_device, mode, prec):
+    """FP32 factor with Bunch-Kaufman / rook pivoting on the pivoting KKT
+    system equals the FP32 oracle bit for bit (pivot choices, interchanges
+    and values), with the 2x2 arithmetic of BlockiLDLt<float, bk, rook>
+    ('factor') and with the library's FP64 2x2 arithmetic ('f64')."""
     k, p = _pivoting_kkt()
-    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, mode, t="f")
+    g, o = _pair((p.row_ptr, p.col, p.val), p.piv_starts, p.block_starts, mode, t="f",
+                 prec=prec)
     assert (o.get("perm") != np.arange(p.row_ptr.size - 1)).sum() > 20
     _equal(g, o)

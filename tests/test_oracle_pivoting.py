@@ -127,3 +127,119 @@ def test_static_mode_is_unchanged():
         assert np.array_equal(o1.get(nm), o2.get(nm)), nm
     assert np.array_equal(o1.get("perm"), np.arange(A.shape[0]))
     assert np.array_equal(o1.get("fis_piv"), o1.get("is_piv"))
+
+
+def _block_values(o):
+    """(metadata rows, {block: stored values}) of a factored oracle."""
+    fb = o.get("fblocks").reshape(-1, 9)
+    bs = np.asarray(o.get("block_starts"), np.int64)
+    fv = o.get("fval_store")
+    out = {}
+    for b in range(fb.shape[0]):
+        h = bs[fb[b, 5] + 1] - bs[fb[b, 5]]
+        w = bs[fb[b, 6] + 1] - bs[fb[b, 6]]
+        cnt = int(h * w) if fb[b, 1] == 1 else int(fb[b, 2])
+        out[b] = fv[fb[b, 8]:fb[b, 8] + cnt]
+    return fb, out
+
+
+@pytest.mark.parametrize("seed", [2, 3, 6, 11])
+def test_pivot_precision_f64_tracks_the_fp64_factor(seed):
+    """The oracle's FP64 arithmetic for the 2x2 steps and D solves of an
+    FP32 factor (the library's FP32 default; the reference computes them in
+    FP32). On random indefinite matrices with 32-45 static 2x2 pivots and
+    no dropping, every block of that FP32 factor is within 1e-5 of the FP64
+    factor (relative to the block's largest value) and at most twice as far
+    from it as the FP32-arithmetic factor, plus 1e-6 (both are ~1e-6: FP32
+    rounding elsewhere dominates). A dropped term or a wrong sign in the
+    FP64 form leaves errors of order one."""
+    csr, piv, bs, _ = sym_problem(n_blocks=9, bmax=32, density=0.12, seed=seed, diag_shift=20.0)
+    assert np.diff(np.append(piv, csr[0].size - 1)).max() == 2
+    ref = _oracle(csr, piv, bs, "d", "static")
+    errs = {}
+    for prec in ("factor", "f64"):
+        o = ol.OracleBildlt(*csr, piv, bs, "f")
+        o.set_pivot_precision(prec)
+        o.symbolic(0, 1.0)
+        o.factor(0.0)
+        fb, vals = _block_values(o)
+        fb_r, vals_r = _block_values(ref)
+        assert np.array_equal(fb[:, :3], fb_r[:, :3])
+        worst = 0.0
+        for b, v in vals.items():
+            scale = np.abs(vals_r[b]).max() if vals_r[b].size else 0.0
+            if scale > 0:
+                worst = max(worst, float(np.abs(v - vals_r[b]).max() / scale))
+        errs[prec] = worst
+    assert errs["f64"] < 1e-5, errs
+    assert errs["f64"] <= 2.0 * errs["factor"] + 1e-6, errs
+
+
+def _first_pivot_rows_exact(seed):
+    """{precision: whether the first 2x2 pivot's rows of block row 0 equal
+    the FP64 factor's rounded to FP32}; asserts it for 'f64'."""
+    rng = np.random.default_rng(seed)
+    n0, n1 = 6, 5
+    n = n0 + n1
+    A = rng.normal(size=(n, n)).astype(np.float32).astype(np.float64)
+    A = np.triu(A, 1)
+    A = A + A.T
+    A[np.arange(n), np.arange(n)] = rng.normal(size=n).astype(np.float32) * 0.5
+    piv = np.array([0, 2, 3, 5, 6, 7, 8, 9, 10], np.int32)   # 2x2 at rows 0-1 and 3-4
+    bs = np.array([0, n0], np.int32)
+    csr = _to_csr(A)
+    od = _oracle(csr, piv, bs, "d", "static")
+    fb, vd = _block_values(od)
+    exact = {}
+    for prec in ("f64", "factor"):
+        o = ol.OracleBildlt(*csr, piv, bs, "f")
+        o.set_pivot_precision(prec)
+        o.symbolic(0, 1.0)
+        o.factor(0.0)
+        fbf, vf = _block_values(o)
+        assert np.array_equal(fbf, fb)
+        same = True
+        for b in range(fb.shape[0]):
+            if fb[b, 5] != 0:
+                continue
+            w = n0 if fb[b, 6] == 0 else n1
+            assert fb[b, 1] == 1          # dense: row-major h x w
+            rows_f = vf[b][:2 * w]
+            rows_d = vd[b][:2 * w].astype(np.float32)
+            if prec == "f64":
+                assert np.array_equal(rows_f, rows_d.astype(np.float64)), (b, rows_f - rows_d)
+            same = same and np.array_equal(rows_f, rows_d.astype(np.float64))
+        exact[prec] = same
+    return exact
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_pivot_precision_f64_first_pivot_is_the_rounded_fp64_step(seed):
+    """Exact pin of the FP64 2x2 arithmetic. Matrix entries are FP32 values;
+    block row 0 has no Schur updates and starts with a 2x2 pivot, so the FP32
+    factor with FP64 2x2 arithmetic holds, in that pivot's two rows of U_00
+    and of the off-diagonal block U_01, the FP64 factor's values rounded to
+    FP32: the same double operations on the same inputs, rounded once. The
+    reference's FP32 arithmetic rounds every step and differs from them on
+    most draws (checked over the seeds)."""
+    assert _first_pivot_rows_exact(seed)["f64"]
+
+
+def test_pivot_precision_pin_discriminates():
+    """The exact pin above fails for the FP32-arithmetic form on most draws:
+    it is a property of the FP64 arithmetic, not of any FP32 factor."""
+    differs = sum(not _first_pivot_rows_exact(s)["factor"] for s in range(6))
+    assert differs >= 3, differs
+
+
+def test_pivot_precision_changes_nothing_in_fp64():
+    csr, piv, bs, _ = sym_problem(n_blocks=9, bmax=32, density=0.12, seed=3)
+    outs = []
+    for prec in ("factor", "f64"):
+        o = ol.OracleBildlt(*csr, piv, bs, "d")
+        o.set_pivot_precision(prec)
+        o.symbolic(1, 2.0)
+        o.factor(1e-3)
+        outs.append([o.get(nm) for nm in ("fblocks", "fix_store", "fval_store", "tri", "row_norms")])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
