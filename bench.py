@@ -631,6 +631,13 @@ def main():
         "symbolic_s_once_per_pattern": t_sym,
         "roofline": dominant,
         "roofline_secondary": secondary,
+        # no MFMA instruction runs: at level-0 fill on this generator 2.74M
+        # of the 2.77M off-diagonal blocks hold one entry and take 1.9
+        # Schur pairs each, so every Schur update is a scalar multiply-add
+        "mfma_util": 0.0,
+        "mfma_util_note": "no MFMA path: Schur updates at level-0 fill are 1-entry blocks "
+                          "(2.74M of 2.77M off-diagonal blocks, 1.9 pairs each), one scalar "
+                          "FMA per update; nothing to contract",
         "cpu_baseline": cpu,
     }
     if other is not None:

@@ -37,6 +37,13 @@ namespace {
 
 constexpr double NZ_EPS = 1e-10;
 
+// first Schur pair of an off-diagonal task: low and high 32 bits in the
+// third int4's x and w
+__device__ __forceinline__ int64_t pair_first(int4 t2)
+{
+    return (int64_t) (((uint64_t) (uint32_t) t2.w << 32) | (uint32_t) t2.x);
+}
+
 template<typename VT>
 __device__ __forceinline__ void tile_zero(VT* T)
 {
@@ -784,16 +791,19 @@ __global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
         wave_sync();
         p += order;
     }
+    uint32_t um = 0;
     if(lane < len) {
         a.rn[fr + lane] = row_ssq(T0, lane, len);
         // nonzero columns right of the diagonal, for the U_ii' solves
-        uint32_t um = 0;
         for(int c = lane + 1; c < len; ++c)
             if(T0[32 * lane + c] != (VT) 0) um |= 1u << c;
         a.piv_umask[(int64_t) q * 32 + lane] = um;
     } else if(lane < 32) {
         a.piv_umask[(int64_t) q * 32 + lane] = 0u;   // all 32 words defined
     }
+    // and whether there is any such column at all (one word per tile)
+    const bool um_some = wave_ballot(um != 0u) != 0ull;
+    if(lane == 0) a.piv_any[q] = um_some ? 1u : 0u;
     VT* out = a.if_val + (int64_t) q * 1024;
     for(int e = lane; e < len * len; e += 64) {
         const int r = e / len, c = e - r * len;
@@ -1067,7 +1077,7 @@ __device__ void offdiag_small_one(const FactorArgs<VT>& a, const LevelRange& lr,
     const bool all1x1 = ((t0.z >> 16) & 1) != 0;
     const int pq = t0.w;
     const int np = t2.y;
-    const int4* pairs = (const int4*) a.upd4 + t2.x;
+    const int4* pairs = (const int4*) a.upd4 + pair_first(t2);
     const int j = lane & 31, c0 = lane >> 5;
 
     // input block A_ij into the register tile
@@ -1315,20 +1325,12 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
     const int hgt = t0.z & 255, wid = (t0.z >> 8) & 255;
     const int pq = t0.w;
     const int np = t2.y;
-    const int4* pairs = (const int4*) a.upd4 + t2.x;
-    // the pivot block's 32 row masks of entries right of the diagonal,
-    // loaded together now and needed only after the Schur updates: a pivot
-    // block without any (nearly all at level-0 fill) leaves the forward
-    // substitution below with nothing to do
-    uint32_t um_any = 0;
-    {
-        const uint4* um4 = (const uint4*) (a.piv_umask + (int64_t) pq * 32);
-        uint4 w[8];
-#pragma unroll
-        for(int u = 0; u < 8; ++u) w[u] = um4[u];
-#pragma unroll
-        for(int u = 0; u < 8; ++u) um_any |= w[u].x | w[u].y | w[u].z | w[u].w;
-    }
+    const int4* pairs = (const int4*) a.upd4 + pair_first(t2);
+    // whether the pivot block has any entry right of its diagonal, loaded
+    // now and needed only after the Schur updates: a pivot block without
+    // (nearly all at level-0 fill) leaves the forward substitution below
+    // with nothing to do
+    const uint32_t um_any = a.piv_any[pq];
     VT* tv = &s.v[0][tid];
     int16_t* ti = &s.ix[0][tid];
     auto V = [&](int k) -> VT& { return tv[k * TINY_TPB]; };

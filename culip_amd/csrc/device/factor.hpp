@@ -57,12 +57,14 @@ struct FactorArgs {
     int64_t* chunk = nullptr;             // per scan chunk: pool map (2), sizes (2)
     // static, per level-list position of an off-diagonal block, 12 int32:
     // {block, first row, rows | cols << 8 | all-1x1 << 16, pivot position},
-    // {input format, nnz, col_ptr, val_ptr}, {first pair, pairs, flags, 0}
-    // with flags bit 0: small enough for the one-lane kernel
+    // {input format, nnz, col_ptr, val_ptr}, {first pair (low 32 bits),
+    // pairs, flags, first pair (high 32 bits)} with flags bit 0: small
+    // enough for the one-lane kernel
     const int32_t* off_task = nullptr;
     // static, per Schur pair: {block (k, i), block (k, j), first row of k, rows of k}
     const int32_t* upd4 = nullptr;
     uint32_t* piv_umask = nullptr;        // 32 per pivot position: nonzero columns > row
+    uint32_t* piv_any = nullptr;          // per pivot position: any such column
     int32_t* defer_list = nullptr;        // positions left to the general kernel
     int32_t* defer_cnt = nullptr;         // one counter per level
     int32_t* wave_list = nullptr;         // positions left to the one-wave kernel

@@ -85,7 +85,6 @@ void BildltPattern::upload_static()
     // off-diagonal task descriptors (12 int32 per level-list position) and
     // Schur pairs with the rows of their block k
     {
-        CULIP_REQUIRE(s.upd_off[K] < (int64_t) INT32_MAX, "factor: too many Schur pairs");
         std::vector<int32_t> diag_id(s.nb, -1);
         for(int32_t b = 0; b < K; ++b)
             if(s.blocks[b].block_row == s.blocks[b].block_col) diag_id[s.blocks[b].block_row] = b;
@@ -109,8 +108,12 @@ void BildltPattern::upload_static()
                 int32_t* t = &task[(size_t) 12 * p];
                 t[0] = b; t[1] = fr; t[2] = h | (w << 8) | ((all1 ? 1 : 0) << 16); t[3] = pq;
                 t[4] = m.format; t[5] = m.nnz; t[6] = m.col_ptr; t[7] = m.val_ptr;
-                t[8] = (int32_t) s.upd_off[b];
-                t[9] = (int32_t) (s.upd_off[b + 1] - s.upd_off[b]);
+                // first pair as two 32-bit halves (t[8] low, t[11] high)
+                const int64_t first = s.upd_off[b], cnt = s.upd_off[b + 1] - first;
+                CULIP_REQUIRE(cnt < (int64_t) INT32_MAX, "factor: too many Schur pairs in one block");
+                t[8] = (int32_t) (uint32_t) ((uint64_t) first & 0xffffffffu);
+                t[9] = (int32_t) cnt;
+                t[11] = (int32_t) (uint32_t) ((uint64_t) first >> 32);
                 // one-lane kernel: sparse input of at most 8 entries, a
                 // short Schur pair list, 1x1 pivots in the block row
                 t[10] = (m.format == FMT_SPARSE && m.nnz <= 8 && t[9] <= lim && all1) ? 1 : 0;
@@ -140,7 +143,8 @@ void Bildlt::upload_pattern(hipStream_t st)
     const int64_t mlb = std::max<int64_t>(1, s.max_level_blocks);
     d_keep.ensure(sizeof(int32_t) * mlb);
     d_granted.ensure(sizeof(int32_t) * mlb);
-    d_piv_umask.ensure(sizeof(uint32_t) * 32 * mlb);
+    // 32 row masks per pivot position, then one "any" word per position
+    d_piv_umask.ensure(sizeof(uint32_t) * 33 * mlb);
     d_defer_list.ensure(sizeof(int32_t) * mlb);
     d_defer_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
     d_wave_list.ensure(sizeof(int32_t) * mlb);
@@ -245,6 +249,7 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.off_task = d_off_task.as<int32_t>();
     a.upd4 = d_upd4.as<int32_t>();
     a.piv_umask = d_piv_umask.as<uint32_t>();
+    a.piv_any = a.piv_umask + 32 * std::max<int64_t>(1, sym.max_level_blocks);
     a.defer_list = d_defer_list.as<int32_t>();
     a.defer_cnt = d_defer_cnt.as<int32_t>();
     a.wave_list = d_wave_list.as<int32_t>();

@@ -3,6 +3,7 @@
 // is bit-identical to the reference's structures for the same inputs and T,
 // with the single-thread order where the reference's OpenMP order varies.
 #include "symbolic.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -40,37 +41,52 @@ int32_t ceil_in_t(T v) { return (int32_t) std::ceil(v); }
 }  // namespace
 
 // build_initial_coarse (:352-484): coarse CSR of triu(A)'s block pattern and
-// the number of fine entries per block.
+// the number of fine entries per block. Block rows are independent (their
+// fine rows are contiguous): counted, sorted and deduplicated in parallel,
+// placed by prefix sums.
 static void build_initial_coarse(Symbolic& s, const HostCsr& A)
 {
     const int32_t nb = s.nb;
-    // per coarse row: collect block columns of entries with row <= col
+    const int64_t chunk = 256;
+    // entries with row <= col per block row
     std::vector<int32_t> cnt(nb + 1, 0);
-    int32_t u = 0;
-    for(int32_t r = 0; r < s.n; ++r)
-        for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q)
-            if(r <= A.col[q]) { cnt[s.row_block[r] + 1]++; ++u; }
-    s.fine_u_nnz = u;
-    for(int32_t i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
-    std::vector<int32_t> bcol(u);
-    {
-        std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
-        for(int32_t r = 0; r < s.n; ++r)
-            for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q)
-                if(r <= A.col[q]) bcol[fillp[s.row_block[r]]++] = s.row_block[A.col[q]];
-    }
-    s.csr_off.assign(nb + 1, 0);
-    s.csr_col.clear();
-    s.a_block_nnz.clear();
-    for(int32_t i = 0; i < nb; ++i) {
-        std::sort(bcol.begin() + cnt[i], bcol.begin() + cnt[i + 1]);
-        for(int32_t q = cnt[i]; q < cnt[i + 1]; ++q) {
-            if(q > cnt[i] && bcol[q] == bcol[q - 1]) { s.a_block_nnz.back()++; continue; }
-            s.csr_col.push_back(bcol[q]);
-            s.a_block_nnz.push_back(1);
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i) {
+            int32_t c = 0;
+            for(int32_t r = s.block_starts[i]; r < s.block_starts[i + 1]; ++r)
+                for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q) c += (r <= A.col[q]);
+            cnt[i + 1] = c;
         }
-        s.csr_off[i + 1] = (int32_t) s.csr_col.size();
-    }
+    });
+    for(int32_t i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
+    s.fine_u_nnz = cnt[nb];
+    // their block columns, sorted per block row; distinct columns counted
+    std::vector<int32_t> bcol(cnt[nb]), ucnt(nb + 1, 0);
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i) {
+            int32_t at = cnt[i];
+            for(int32_t r = s.block_starts[i]; r < s.block_starts[i + 1]; ++r)
+                for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q)
+                    if(r <= A.col[q]) bcol[at++] = s.row_block[A.col[q]];
+            std::sort(bcol.begin() + cnt[i], bcol.begin() + cnt[i + 1]);
+            int32_t u = 0;
+            for(int32_t q = cnt[i]; q < cnt[i + 1]; ++q) u += (q == cnt[i] || bcol[q] != bcol[q - 1]);
+            ucnt[i + 1] = u;
+        }
+    });
+    for(int32_t i = 0; i < nb; ++i) ucnt[i + 1] += ucnt[i];
+    s.csr_off = ucnt;
+    s.csr_col.assign(ucnt[nb], 0);
+    s.a_block_nnz.assign(ucnt[nb], 0);
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i) {
+            int32_t at = ucnt[i] - 1;
+            for(int32_t q = cnt[i]; q < cnt[i + 1]; ++q) {
+                if(q == cnt[i] || bcol[q] != bcol[q - 1]) s.csr_col[++at] = bcol[q];
+                s.a_block_nnz[at]++;
+            }
+        }
+    });
     s.coarse_nnz = s.init_coarse_nnz = (int32_t) s.csr_col.size();
 }
 
@@ -83,53 +99,72 @@ static void level_fill(Symbolic& s, const HostCsr& A, int32_t max_level)
     const int32_t nb = s.nb;
     std::vector<std::vector<int32_t>> add(nb);
     const int32_t old_nnz = s.coarse_nnz;
-    std::vector<int32_t> visited(nb, -1), length(nb, 0), queue(nb);
-    for(int32_t i = 0; i < nb; ++i) {
-        if(max_level > 0) {
-            int32_t head = 0, tail = 0;
-            queue[tail++] = i;
-            visited[i] = i;
-            length[i] = 0;
-            while(head < tail) {
-                const int32_t cur = queue[head++];
-                for(int32_t r = s.block_starts[cur]; r < s.block_starts[cur + 1]; ++r)
-                    for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q) {
-                        const int32_t kc = s.row_block[A.col[q]];
-                        if(visited[kc] == i) continue;
-                        visited[kc] = i;
-                        length[kc] = 0;
-                        if(kc < i && length[cur] < max_level) {
-                            queue[tail++] = kc;
-                            length[kc] = length[cur] + 1;
-                        }
-                        if(kc > i && length[cur] > 0) add[i].push_back(kc);
-                    }
-            }
+    const int64_t chunk = 64;
+    const int workers = parallel_workers(nb, chunk);
+    // per-thread BFS state (visited stamps are block row ids, so they never
+    // need clearing)
+    std::vector<std::vector<int32_t>> visited(workers), length(workers), queue(workers);
+    if(max_level > 0)
+        for(int w = 0; w < workers; ++w) {
+            visited[w].assign(nb, -1);
+            length[w].assign(nb, 0);
+            queue[w].assign(nb, 0);
         }
-        // diagonal check on the row's first entry only; an empty trailing
-        // row (first entry index == nnz) counts as missing
-        const int32_t f = s.csr_off[i];
-        const bool has_diag = (f < old_nnz) && s.csr_col[f] == i;
-        if(!has_diag) add[i].push_back(i);
-    }
+    parallel_chunks(nb, chunk, [&](int w, int64_t lo, int64_t hi) {
+        for(int32_t i = (int32_t) lo; i < (int32_t) hi; ++i) {
+            if(max_level > 0) {
+                std::vector<int32_t>& vis = visited[w];
+                std::vector<int32_t>& len = length[w];
+                std::vector<int32_t>& que = queue[w];
+                int32_t head = 0, tail = 0;
+                que[tail++] = i;
+                vis[i] = i;
+                len[i] = 0;
+                while(head < tail) {
+                    const int32_t cur = que[head++];
+                    for(int32_t r = s.block_starts[cur]; r < s.block_starts[cur + 1]; ++r)
+                        for(int32_t q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q) {
+                            const int32_t kc = s.row_block[A.col[q]];
+                            if(vis[kc] == i) continue;
+                            vis[kc] = i;
+                            len[kc] = 0;
+                            if(kc < i && len[cur] < max_level) {
+                                que[tail++] = kc;
+                                len[kc] = len[cur] + 1;
+                            }
+                            if(kc > i && len[cur] > 0) add[i].push_back(kc);
+                        }
+                }
+            }
+            // diagonal check on the row's first entry only; an empty
+            // trailing row (first entry index == nnz) counts as missing
+            const int32_t f = s.csr_off[i];
+            const bool has_diag = (f < old_nnz) && s.csr_col[f] == i;
+            if(!has_diag) add[i].push_back(i);
+        }
+    });
     int32_t new_nnz = 0;
     for(auto& a : add) new_nnz += (int32_t) a.size();
     s.fill_in_nnz = new_nnz;
     if(new_nnz == 0) return;
 
-    std::vector<int32_t> off(nb + 1, 0), col, bn;
-    col.reserve(old_nnz + new_nnz);
-    bn.reserve(old_nnz + new_nnz);
-    std::vector<std::pair<int32_t, int32_t>> row;
-    for(int32_t i = 0; i < nb; ++i) {
-        row.clear();
-        for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q)
-            row.emplace_back(s.csr_col[q], s.a_block_nnz[q]);
-        for(int32_t c : add[i]) row.emplace_back(c, 0);
-        std::sort(row.begin(), row.end());
-        for(auto& e : row) { col.push_back(e.first); bn.push_back(e.second); }
-        off[i + 1] = (int32_t) col.size();
-    }
+    // merged rows, sorted by column, placed by a prefix sum
+    std::vector<int32_t> off(nb + 1, 0);
+    for(int32_t i = 0; i < nb; ++i)
+        off[i + 1] = off[i] + (s.csr_off[i + 1] - s.csr_off[i]) + (int32_t) add[i].size();
+    std::vector<int32_t> col(off[nb]), bn(off[nb]);
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+        std::vector<std::pair<int32_t, int32_t>> row;
+        for(int64_t i = lo; i < hi; ++i) {
+            row.clear();
+            for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q)
+                row.emplace_back(s.csr_col[q], s.a_block_nnz[q]);
+            for(int32_t c : add[i]) row.emplace_back(c, 0);
+            std::sort(row.begin(), row.end());
+            int32_t at = off[i];
+            for(auto& e : row) { col[at] = e.first; bn[at] = e.second; ++at; }
+        }
+    });
     s.csr_off.swap(off);
     s.csr_col.swap(col);
     s.a_block_nnz.swap(bn);
@@ -201,12 +236,14 @@ static void find_level_sets(Symbolic& s)
         "level sets: not every block row was scheduled (missing diagonal block?)");
 
     s.blocks.assign(K, BlockMeta{});
-    for(int32_t i = 0; i < nb; ++i)
-        for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q) {
-            s.blocks[q].id = q;
-            s.blocks[q].block_row = i;
-            s.blocks[q].block_col = s.csr_col[q];
-        }
+    parallel_chunks(nb, 256, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i)
+            for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q) {
+                s.blocks[q].id = q;
+                s.blocks[q].block_row = (int32_t) i;
+                s.blocks[q].block_col = s.csr_col[q];
+            }
+    });
 
     // level block list (:916-967): by (row level, pivot first), stable
     s.lvlb_idx = stable_bucket(K, 2 * level, [&](int32_t b) {
@@ -241,8 +278,13 @@ static void build_blocks(Symbolic& s, const HostCsr& A, T fill_factor)
 {
     const int32_t nb = s.nb, K = s.coarse_nnz;
     std::vector<int64_t> ixs(K + 1, 0), vls(K + 1, 0);
-    s.blks_sparse = s.blks_dense = 0;
-    for(int32_t i = 0; i < nb; ++i) {
+    const int64_t chunk = 256;
+    std::vector<int32_t> nsparse(nb, 0);
+    // block rows whose average block size is undefined (the lowest is
+    // reported, whatever the schedule)
+    std::vector<uint8_t> no_entries(nb, 0);
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+    for(int32_t i = (int32_t) lo; i < (int32_t) hi; ++i) {
         int32_t sum_nnz = 0, sum_blocks = 0;
         bool need_avg = false;
         for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q) {
@@ -252,9 +294,10 @@ static void build_blocks(Symbolic& s, const HostCsr& A, T fill_factor)
         }
         int32_t avg_nnz = 0;
         if(need_avg) {
-            if(sum_blocks == 0)
-                fail(CULIP_ERR_INVALID, "build_blocks: block row " + std::to_string(i) +
-                    " has no input entries (average block size undefined)");
+            if(sum_blocks == 0) {
+                no_entries[i] = 1;
+                continue;
+            }
             avg_nnz = ceil_in_t<T>((T) sum_nnz / (T) sum_blocks);
         }
         const int32_t rows = s.block_starts[i + 1] - s.block_starts[i];
@@ -274,15 +317,22 @@ static void build_blocks(Symbolic& s, const HostCsr& A, T fill_factor)
                 m.format = FMT_SPARSE;
                 ixs[li] = m.max_nnz;
                 vls[li] = m.max_nnz;
-                s.blks_sparse++;
+                nsparse[i]++;
             } else {
                 m.format = FMT_DENSE;
                 ixs[li] = 0;
                 vls[li] = (int64_t) rows * cols;
-                s.blks_dense++;
             }
         }
     }
+    });
+    for(int32_t i = 0; i < nb; ++i)
+        if(no_entries[i])
+            fail(CULIP_ERR_INVALID, "build_blocks: block row " + std::to_string(i) +
+                " has no input entries (average block size undefined)");
+    s.blks_sparse = 0;
+    for(int32_t i = 0; i < nb; ++i) s.blks_sparse += nsparse[i];
+    s.blks_dense = K - s.blks_sparse;
     int64_t a = 0, v = 0;
     for(int32_t p = 0; p <= K; ++p) {
         const int64_t ca = ixs[p], cv = vls[p];
@@ -292,17 +342,21 @@ static void build_blocks(Symbolic& s, const HostCsr& A, T fill_factor)
     s.val_space = vls[K];
     CULIP_REQUIRE(s.val_space < (int64_t) INT32_MAX,
         "build_blocks: value storage exceeds int32 offsets (mat_int_t)");
-    for(int32_t b = 0; b < K; ++b) {
-        s.blocks[b].col_ptr = (int32_t) ixs[s.block_in_lvl[b]];
-        s.blocks[b].val_ptr = (int32_t) vls[s.block_in_lvl[b]];
-    }
+    parallel_chunks(K, 1 << 16, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t b = lo; b < hi; ++b) {
+            s.blocks[b].col_ptr = (int32_t) ixs[s.block_in_lvl[b]];
+            s.blocks[b].val_ptr = (int32_t) vls[s.block_in_lvl[b]];
+        }
+    });
 
     // gather map (:1214-1266): sparse entries in CSR scan order of the
-    // block's fine rows, dense entries row-major
+    // block's fine rows, dense entries row-major; a block row's fine rows
+    // fill only its own blocks, so block rows run in parallel
     s.ix_store.assign(s.ix_space, -1);
     s.value_slot.assign(A.nnz(), -1);
     std::vector<int32_t> rec(K, 0);
-    for(int32_t r = 0; r < s.n; ++r) {
+    parallel_chunks(nb, chunk, [&](int, int64_t lo, int64_t hi) {
+    for(int32_t r = s.block_starts[lo]; r < s.block_starts[hi]; ++r) {
         const int32_t bi = s.row_block[r];
         const int32_t lr = r - s.block_starts[bi];
         int32_t e = s.csr_off[bi];
@@ -323,6 +377,7 @@ static void build_blocks(Symbolic& s, const HostCsr& A, T fill_factor)
             rec[e]++;
         }
     }
+    });
 }
 
 // solve_analysis_block (:2050-2150)
@@ -340,36 +395,60 @@ static void solve_analysis(Symbolic& s)
 }
 
 // Schur-update pairs for each off-diagonal block (merge walk, kernels
-// :1692-1727), computed once per pattern.
+// :1692-1727), computed once per pattern: for block (i, j), j != i, the
+// blocks (k, i) and (k, j) for every k < i present in both columns, in
+// ascending k. Block rows are independent: a first parallel pass counts the
+// pairs of every block, a prefix sum places them, a second pass writes them.
 static void update_pairs(Symbolic& s)
 {
-    const int32_t K = s.coarse_nnz;
-    s.upd_off.assign(K + 1, 0);
-    std::vector<int32_t> mark(s.nb, -1);
-    std::vector<int32_t> ki, kj;
-    for(int32_t i = 0; i < s.nb; ++i) {
-        // rows k < i present in column i (excluding the diagonal)
-        for(int32_t p = s.csc_off[i]; p < s.csc_off[i + 1] - 1; ++p)
-            mark[s.blocks[s.csc_idx[p]].block_row] = s.csc_idx[p];
+    const int32_t K = s.coarse_nnz, nb = s.nb;
+    // block row of every column entry, in column order
+    std::vector<int32_t> crow(K);
+    parallel_chunks(K, 1 << 16, [&](int, int64_t lo, int64_t hi) {
+        for(int64_t p = lo; p < hi; ++p) crow[p] = s.blocks[s.csc_idx[p]].block_row;
+    });
+    const int64_t chunk = 64;
+    const int workers = parallel_workers(nb, chunk);
+    std::vector<std::vector<int32_t>> marks(workers, std::vector<int32_t>(nb, -1));
+    // emit(q, block (k, i), block (k, j)) for the pairs of block row i
+    auto row_pairs = [&](int32_t i, std::vector<int32_t>& mark, auto&& emit) {
+        for(int32_t p = s.csc_off[i]; p < s.csc_off[i + 1] - 1; ++p) mark[crow[p]] = s.csc_idx[p];
         for(int32_t q = s.csr_off[i]; q < s.csr_off[i + 1]; ++q) {
             const int32_t j = s.csr_col[q];
-            if(j != i)
-                for(int32_t p = s.csc_off[j]; p < s.csc_off[j + 1] - 1; ++p) {
-                    const int32_t bkj = s.csc_idx[p];
-                    const int32_t k = s.blocks[bkj].block_row;
-                    if(k >= i) break;
-                    if(mark[k] >= 0 && s.blocks[mark[k]].block_row == k) {
-                        ki.push_back(mark[k]);
-                        kj.push_back(bkj);
-                    }
-                }
-            s.upd_off[q + 1] = (int64_t) ki.size();
+            if(j == i) continue;
+            for(int32_t p = s.csc_off[j]; p < s.csc_off[j + 1] - 1; ++p) {
+                const int32_t k = crow[p];
+                if(k >= i) break;
+                if(mark[k] >= 0) emit(q, mark[k], s.csc_idx[p]);
+            }
         }
-        for(int32_t p = s.csc_off[i]; p < s.csc_off[i + 1] - 1; ++p)
-            mark[s.blocks[s.csc_idx[p]].block_row] = -1;
-    }
-    s.upd_ki.swap(ki);
-    s.upd_kj.swap(kj);
+        for(int32_t p = s.csc_off[i]; p < s.csc_off[i + 1] - 1; ++p) mark[crow[p]] = -1;
+    };
+    s.upd_off.assign(K + 1, 0);
+    parallel_chunks(nb, chunk, [&](int w, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i)
+            row_pairs((int32_t) i, marks[w], [&](int32_t q, int32_t, int32_t) { s.upd_off[q + 1]++; });
+    });
+    for(int32_t q = 0; q < K; ++q) s.upd_off[q + 1] += s.upd_off[q];
+    // 24 bytes per pair on the device (pair lists and the task records):
+    // 2^33 pairs is 206 GB, most of the GPU
+    if(s.upd_off[K] > ((int64_t) 1 << 33))
+        fail(CULIP_ERR_INVALID, "symbolic: " + std::to_string(s.upd_off[K]) +
+            " Schur pairs exceed the 2^33 supported (fill level too high for this pattern)");
+    s.upd_ki.assign(s.upd_off[K], 0);
+    s.upd_kj.assign(s.upd_off[K], 0);
+    parallel_chunks(nb, chunk, [&](int w, int64_t lo, int64_t hi) {
+        for(int64_t i = lo; i < hi; ++i) {
+            int64_t at = -1;
+            int32_t cur = -1;
+            row_pairs((int32_t) i, marks[w], [&](int32_t q, int32_t bki, int32_t bkj) {
+                if(q != cur) { cur = q; at = s.upd_off[q]; }
+                s.upd_ki[at] = bki;
+                s.upd_kj[at] = bkj;
+                ++at;
+            });
+        }
+    });
 }
 
 void symbolic_analyze(Symbolic& s, const HostCsr& A, const std::vector<int32_t>& piv_starts,

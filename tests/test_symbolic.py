@@ -41,7 +41,16 @@ CASES = [
 ] + [(e, (lambda e=e: edge_problem(e)[:3]), 0, 1.0) for e in EDGE_CASES]
 
 
-@pytest.mark.parametrize("name,make,fl,ff", CASES, ids=[c[0] for c in CASES])
+# thousands of block rows: the host threads split every per-row pass (the
+# oracle's own brute-force checks stay on the small cases)
+LARGE_CASES = [
+    ("kkt40k-colour", lambda: _kkt_case(40000, 15000, 16, ca.ORDER_BLOCK_COLOR), 0, 1.0),
+    ("kkt12k-fill1", lambda: _kkt_case(12000, 4800, 16, ca.ORDER_NATURAL), 1, 1.5),
+]
+
+
+@pytest.mark.parametrize("name,make,fl,ff", CASES + LARGE_CASES,
+                         ids=[c[0] for c in CASES + LARGE_CASES])
 @pytest.mark.parametrize("t", ["d", "f"])
 def test_product_symbolic_bit_exact(name, make, fl, ff, t):
     csr, piv, bs = make()
