@@ -400,8 +400,10 @@ __device__ VT row_ssq(const VT* T, int r, int w)
 }
 
 // dependency blocks staged per round of the pivot update; the first tile
-// holds the pivot tile for the factorization afterwards
-constexpr int PIV_ROUND = 4;
+// holds the pivot tile for the factorization afterwards. Two keep the pivot
+// blocks' LDS (17.8 KB in FP64) below that of the one-lane off-diagonal
+// form they share launches with, so eight workgroups fit a CU.
+constexpr int PIV_ROUND = 2;
 
 template<typename VT>
 struct alignas(16) PivLds {
@@ -658,13 +660,13 @@ __device__ void dense_pivot_op(VT* T, int32_t* perm, int len, int pa, int pb)
     wave_sync();
 }
 
-// k_factor_pivot_blocks (kernels :1146-1283)
+// k_factor_pivot_blocks (kernels :1146-1283) for the pivot block at level
+// position q, one wave
 template<typename VT>
-__global__ __launch_bounds__(64) void k_pivots(FactorArgs<VT> a, LevelRange lr)
+__device__ void pivots_one(const FactorArgs<VT>& a, const LevelRange& lr, const int q,
+    PivLds<VT>& s)
 {
-    __shared__ PivLds<VT> s;
     const int lane = lane_id();
-    const int q = blockIdx.x;
     const int b = a.lvlb_idx[lr.lo + q];
     const BlockMeta m = a.bin[b];
     const int i = m.block_row;
@@ -1304,13 +1306,17 @@ __device__ __forceinline__ void tiny_raw(const int32_t* ix_out, const VT* val_ou
     }
 }
 
+// One-lane form, first phase: the Schur updates of the off-diagonal block at
+// level position q (thread tid of its workgroup), which read only blocks of
+// earlier levels and so run beside the level's pivot blocks. The tile list
+// is parked in the position's compact store (if_idx / if_val, count in
+// if_cnt); if_cnt[q] = -1 marks a position left to the other kernels.
 template<typename VT>
-__global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, LevelRange lr)
+__device__ void tiny_pairs_one(const FactorArgs<VT>& a, const LevelRange& lr, const int q,
+    const int tid, TinyLds<VT>& s)
 {
-    __shared__ TinyLds<VT> s;
-    const int tid = threadIdx.x;
-    const int q = lr.npiv + (int) blockIdx.x * TINY_TPB + tid;
     if(q >= lr.size) return;
+    a.if_cnt[q] = -1;
     const int64_t p = (int64_t) lr.lo + q;
     const int4 t0 = ((const int4*) a.off_task)[3 * p];
     const int4 t1 = ((const int4*) a.off_task)[3 * p + 1];
@@ -1321,16 +1327,9 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         return;
     }
     if(!(t2.z & 1)) { to_wave_form(a, lr, q); return; }
-    const int fr = t0.y;
     const int hgt = t0.z & 255, wid = (t0.z >> 8) & 255;
-    const int pq = t0.w;
     const int np = t2.y;
     const int4* pairs = (const int4*) a.upd4 + pair_first(t2);
-    // whether the pivot block has any entry right of its diagonal, loaded
-    // now and needed only after the Schur updates: a pivot block without
-    // (nearly all at level-0 fill) leaves the forward substitution below
-    // with nothing to do
-    const uint32_t um_any = a.piv_any[pq];
     VT* tv = &s.v[0][tid];
     int16_t* ti = &s.ix[0][tid];
     auto V = [&](int k) -> VT& { return tv[k * TINY_TPB]; };
@@ -1505,6 +1504,53 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
             }
         }
     }
+    // the tile list for the second phase
+    VT* lv = a.if_val + (int64_t) q * 1024;
+    uint16_t* li = a.if_idx + (int64_t) q * 1024;
+    for(int k = 0; k < n; ++k) {
+        li[k] = (uint16_t) I(k);
+        lv[k] = V(k);
+    }
+    a.if_cnt[q] = n;
+}
+
+// One-lane form, second phase (after the level's pivot blocks): the tile
+// list of the first phase through U_ii' and D, then the row partial sums and
+// the compact entries.
+template<typename VT>
+__global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny_scale(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ TinyLds<VT> s;
+    const int tid = threadIdx.x;
+    const int q = lr.npiv + (int) blockIdx.x * TINY_TPB + tid;
+    if(q >= lr.size) return;
+    int n = a.if_cnt[q];
+    if(n < 0) return;
+    const int64_t p = (int64_t) lr.lo + q;
+    const int4 t0 = ((const int4*) a.off_task)[3 * p];
+    const int fr = t0.y;
+    const int hgt = t0.z & 255;
+    const int pq = t0.w;
+    // a pivot block without entries right of its diagonal (nearly all at
+    // level-0 fill) leaves the forward substitution with nothing to do
+    const uint32_t um_any = a.piv_any[pq];
+    VT* tv = &s.v[0][tid];
+    int16_t* ti = &s.ix[0][tid];
+    auto V = [&](int k) -> VT& { return tv[k * TINY_TPB]; };
+    auto I = [&](int k) -> int16_t& { return ti[k * TINY_TPB]; };
+    auto find = [&](int idx) {
+        for(int k = 0; k < n; ++k)
+            if(I(k) == idx) return k;
+        return -1;
+    };
+    {
+        const VT* lv = a.if_val + (int64_t) q * 1024;
+        const uint16_t* li = a.if_idx + (int64_t) q * 1024;
+        for(int k = 0; k < n; ++k) {
+            I(k) = (int16_t) li[k];
+            V(k) = lv[k];
+        }
+    }
     // U_ii' forward substitution over the rows in ascending order
     const VT* piv = a.if_val + (int64_t) pq * 1024;
     for(int ii = 0; ii < hgt && um_any != 0; ++ii) {
@@ -1557,10 +1603,13 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         I(m + 1) = ik;
         V(m + 1) = vk;
     }
-    // row partial sums of squares (columns ascending) and the compact list
+    // row partial sums of squares (columns ascending) of the rows with
+    // entries; the others stay at the zero the first phase stored (a row
+    // without entries sums to +0)
     VT* rp = a.rn_part + (int64_t) q * 32;
     int k = 0;
-    for(int r = 0; r < hgt; ++r) {
+    while(k < n) {
+        const int r = I(k) >> 5;
         VT ssq = 0;
         while(k < n && (I(k) >> 5) == r) {
             ssq = fma(V(k), V(k), ssq);
@@ -1568,6 +1617,7 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
         }
         rp[r] = ssq;
     }
+    // and the compact list
     VT* oval = a.if_val + (int64_t) q * 1024;
     uint16_t* oix = a.if_idx + (int64_t) q * 1024;
     int cnt = 0;
@@ -1581,6 +1631,34 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny(FactorArgs<VT> a, Lev
     }
     a.if_cnt[q] = cnt;
 }
+// The level's pivot blocks (workgroups 0 .. npiv - 1) and the first phase
+// of the one-lane off-diagonal form (the rest, 64 positions each) in one
+// launch: the two read disjoint data and never wait for each other.
+template<typename VT>
+union PivPairsLds {
+    PivLds<VT> piv;
+    TinyLds<VT> tiny;
+};
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_pivots_pairs(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ PivPairsLds<VT> u;
+    const int w = blockIdx.x;
+    if(w < lr.npiv) {
+        pivots_one<VT>(a, lr, w, u.piv);
+        return;
+    }
+    // the workgroup's positions' row partial sums start at zero, stored
+    // coalesced here (the second phase writes only the rows with entries;
+    // the other off-diagonal kernels write every row of theirs)
+    const int q0 = lr.npiv + (w - lr.npiv) * TINY_TPB;
+    const int q1 = min(q0 + TINY_TPB, lr.size);
+    VT* rp = a.rn_part + (int64_t) q0 * 32;
+    for(int e = (int) threadIdx.x; e < (q1 - q0) * 32; e += TINY_TPB) rp[e] = (VT) 0;
+    tiny_pairs_one<VT>(a, lr, q0 + (int) threadIdx.x, (int) threadIdx.x, u.tiny);
+}
+
 
 template<typename VT>
 __device__ __forceinline__ VT drop_norm(VT rn) { return (VT) sqrtf((float) rn); }
@@ -2101,11 +2179,12 @@ template<typename VT>
 void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
-    if(r.npiv > 0) k_pivots<VT><<<r.npiv, 64, 0, st>>>(a, r);
-    if(r.size > r.npiv) {
-        const int noff = r.size - r.npiv;
+    const int noff = r.size - r.npiv;
+    const int tiny_blocks = noff > 0 ? (noff + TINY_TPB - 1) / TINY_TPB : 0;
+    if(r.npiv + tiny_blocks > 0) k_pivots_pairs<VT><<<r.npiv + tiny_blocks, 64, 0, st>>>(a, r);
+    if(noff > 0) {
         const int grid = noff < 2048 ? noff : 2048;
-        k_offdiag_tiny<VT><<<(noff + TINY_TPB - 1) / TINY_TPB, TINY_TPB, 0, st>>>(a, r);
+        k_offdiag_tiny_scale<VT><<<tiny_blocks, TINY_TPB, 0, st>>>(a, r);
         k_offdiag<VT><<<grid, 64, 0, st>>>(a, r);
         k_offdiag_general<VT><<<grid, 64, 0, st>>>(a, r);
     }
