@@ -437,11 +437,12 @@ def main():
         sys.exit(2)
     if a.slots <= 0:
         a.slots = 16 if a.workload == "batch" else 8
-    if a.workload in ("serve", "batch"):
-        # one hardware queue per slot stream, so concurrent solves are not
-        # serialised behind each other (HIP maps streams onto this many
-        # queues; read when the runtime starts)
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, a.slots))))
+    # one hardware queue per slot stream, so concurrent solves are not
+    # serialised behind each other (HIP maps streams onto this many queues;
+    # read when the runtime starts); the default line also runs the batch
+    # on 16 slots
+    nq = a.slots if a.workload in ("serve", "batch") else 16
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, nq))))
     import torch
     import culip_amd as ca
 
@@ -598,6 +599,21 @@ def main():
         log("precision variant: barrier-conditioned n=2,000, FP32 factor vs the oracle")
         prec_var = precision_variant(order, a.seed, a.fill_factor, a.threshold)
 
+    # the batching configuration (512 systems, FP32 factor, autotuned), in
+    # the same line so that every run of the default bench measures it
+    batch = None
+    if not a.no_variants:
+        log(f"batch: {a.systems} systems")
+        ab = argparse.Namespace(**vars(a))
+        ab.workload, ab.slots, ab.prec, ab.graphs = "batch", 16, "f32", 0
+        bo = bench_batch(ab)
+        batch = {k: bo[k] for k in ("value", "unit", "converged_fraction", "max_true_rel_residual",
+                                    "sqmr_iterations_mean", "p50_ms", "p99_ms",
+                                    "plain_f64_per_system_solves_per_s",
+                                    "speedup_vs_plain_f64_per_system",
+                                    "end_to_end_solves_per_s", "setup")}
+        batch["config"] = bo["config"]
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline: oracle on the host")
@@ -650,6 +666,8 @@ def main():
         out["jacobi_solves"] = jac
     if prec_var is not None:
         out["precision_variant"] = prec_var
+    if batch is not None:
+        out["batch"] = batch
     print(json.dumps(out))
 
 
