@@ -2,22 +2,28 @@
 //
 // Reference: libs/la/preconditioner/block_ildlt.impl.cuh:1519-1671 (schedule)
 // and block_ildlt_kernels.impl.cuh:1146-2074 (kernels). Per level:
-//   k_pivots    one wave per pivot block: A_ii -= sum_k U_ki' D_k U_ki over
-//               the block column, static 1x1/2x2 LDL' of the tile, row sums
+//   k_pivots_pairs
+//               one wave per pivot block: A_ii -= sum_k U_ki' D_k U_ki over
+//               the block column, 1x1/2x2 LDL' of the tile (static or
+//               numeric pivots), row sums; in the same launch, one lane per
+//               small off-diagonal block: its Schur updates
+//   k_offdiag_tiny_scale
+//               those blocks through U_ii' and D, row partial sums, compact
+//               entries
 //   k_offdiag   one wave per off-diagonal block: A_ij -= sum_k U_ki' D_k U_kj
 //               over the block's symbolic Schur pairs, U_ij = D_i \ (U_ii' \
 //               A_ij), per-row partial sums of squares; the result leaves as
 //               a dense tile or as a compact (index, value) list of the
 //               entries with |v| > 1e-10
 //   k_rownorm   one wave per block row: row sums of squares in a fixed order
-//   k_level_maps, k_level_grant, k_level_write
-//               chunked scans in level-list order: entries surviving the
+//   k_level_scan
+//               chunked scan in level-list order: entries surviving the
 //               threshold, leftover-space pool, granted extra space, output
 //               offsets and the factored metadata
 //   k_store     one wave per block: compacted final storage
 //
 // The dependency blocks of a tile are walked with their metadata fetched
-// lane-parallel up front. In k_pivots a block whose nonzero entries share
+// lane-parallel up front. In the pivot blocks a block whose nonzero entries share
 // one column (one entry, on sparse KKT patterns) has its whole update -- one
 // diagonal element -- formed by its own lane, so the serial walk over the
 // block column costs a few scalar instructions per block; other blocks are
@@ -1696,8 +1702,8 @@ __global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
 // in level-list order (kernels :1948-2008 with the atomics serialised in
 // that order). The pool after a sparse block is max(pool + max_nnz - kept,
 // 0); such maps p -> max(p + a, b) compose associatively, so the pool
-// sequence is a scan: chunk maps (k_level_maps), pools and grants
-// (k_level_grant), output offsets (k_level_write).
+// sequence is a scan (k_level_scan): each chunk's composed map, then the
+// pools and grants, then the output offsets from the chunks' size sums.
 struct PoolMap {
     int64_t a, b;
     __device__ static PoolMap shfl_up(const PoolMap& m, int o)
@@ -1749,17 +1755,58 @@ __device__ V block_prefix(int count, Load load, Op op, const V& id, V* sh)
     return tot;
 }
 
-template<typename VT>
-__global__ __launch_bounds__(SC_TPB) void k_level_maps(FactorArgs<VT> a, LevelRange lr)
+// One launch per level, in chunks of SCAN_CHUNK positions. Every chunk runs
+// the threshold counts, publishes its pool map (release store of its flag),
+// waits for the maps of the chunks before it (acquire loads; chunks are
+// dispatched in order, so a waiting chunk's predecessors are resident and
+// never wait for it), runs the pools and grants, publishes its sizes, waits
+// for the sizes before it and writes the factored metadata.
+constexpr int SCAN_SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ int64_t load_published(const int64_t* p)
 {
-    __shared__ PoolMap sh[SC_NW];
+    return __hip_atomic_load(const_cast<int64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the first wave polls the flags of chunks [0, c); then the workgroup goes
+// on together. A wait that gives up records 2 in *err and goes on.
+__device__ void wait_chunk_flags(const int32_t* flags, int c, int32_t* err)
+{
+    if(threadIdx.x < WAVE) {
+        int spins = 0;
+        for(;;) {
+            bool set = true;
+            for(int i = (int) threadIdx.x; i < c; i += WAVE)
+                set = set && __hip_atomic_load(const_cast<int32_t*>(flags + i), __ATOMIC_ACQUIRE,
+                    __HIP_MEMORY_SCOPE_AGENT) != 0;
+            if(wave_ballot(!set) == 0ull) break;
+            __builtin_amdgcn_s_sleep(1);
+            if(++spins > SCAN_SPIN_LIMIT) {
+                if(threadIdx.x == 0 && err)
+                    __hip_atomic_store(err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void k_level_scan(FactorArgs<VT> a, LevelRange lr)
+{
+    __shared__ PoolMap shm[SC_NW];
+    __shared__ Sizes shs[SC_NW];
     const int c = blockIdx.x;
-    // entries of a sparse position that survive |v| >= tau * sqrtf(norm)
+    int32_t* fmap = a.scan_flags + (int64_t) lr.level * 2 * a.scan_maxch;
+    int32_t* fsize = fmap + a.scan_maxch;
+    const int32_t* cin = a.carry + 4 * (lr.level & 1);
+    int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
+    // -- entries of a sparse position that survive |v| >= tau * sqrtf(norm)
     // (kernels :1919-1946); the thread's positions are read in three rounds
-    // of independent loads
-    // (capacity / first row / entry count, then each block's first entry,
-    // then that entry's row norm) instead of one dependent chain per
-    // position; further entries (rare at level-0 fill) are read in order
+    // of independent loads (capacity / first row / entry count, then each
+    // block's first entry, then that entry's row norm) instead of one
+    // dependent chain per position; further entries (rare at level-0 fill)
+    // are read in order
     int capk[SC_IPT], frk[SC_IPT], cntk[SC_IPT];
 #pragma unroll
     for(int k = 0; k < SC_IPT; ++k) {
@@ -1791,10 +1838,12 @@ __global__ __launch_bounds__(SC_TPB) void k_level_maps(FactorArgs<VT> a, LevelRa
         n0[k] = (VT) 0;
         if(capk[k] >= 0 && cntk[k] > 0) n0[k] = a.rn[frk[k] + (ix0[k] >> 5)];
     }
+    int dcnt[SC_IPT];
     PoolMap m{0, 0};
 #pragma unroll
     for(int k = 0; k < SC_IPT; ++k) {
         const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
+        dcnt[k] = 0;
         if(capk[k] >= 0) {
             int keep = 0;
             if(cntk[k] > 0) {
@@ -1807,124 +1856,93 @@ __global__ __launch_bounds__(SC_TPB) void k_level_maps(FactorArgs<VT> a, LevelRa
                 }
             }
             a.drop_cnt[q] = keep;
+            dcnt[k] = keep;
             m = MapOp()(m, PoolMap{(int64_t) capk[k] - keep, 0});
         }
     }
-    PoolMap tot;
-    (void) block_exclusive_scan<PoolMap, MapOp, SC_NW>(m, MapOp(), PoolMap{0, 0}, sh, &tot);
+    PoolMap mtot;
+    const PoolMap mex = block_exclusive_scan<PoolMap, MapOp, SC_NW>(m, MapOp(), PoolMap{0, 0}, shm,
+        &mtot);
     if(threadIdx.x == 0) {
-        a.chunk[4 * c + 0] = tot.a;
-        a.chunk[4 * c + 1] = tot.b;
+        a.chunk[4 * c + 0] = mtot.a;
+        a.chunk[4 * c + 1] = mtot.b;
+        __hip_atomic_store(fmap + c, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-
-template<typename VT>
-__global__ __launch_bounds__(SC_TPB) void k_level_grant(FactorArgs<VT> a, LevelRange lr)
-{
-    __shared__ PoolMap shm[SC_NW];
-    __shared__ Sizes shs[SC_NW];
-    const int c = blockIdx.x;
-    const int32_t* cin = a.carry + 4 * (lr.level & 1);
-    int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
+    wait_chunk_flags(fmap, c, a.err);
+    // -- pools, grants and kept counts
     const int64_t* ch = a.chunk;
     const PoolMap before = block_prefix<PoolMap>(c,
-        [ch](int i) { return PoolMap{ch[4 * i + 0], ch[4 * i + 1]}; }, MapOp(), PoolMap{0, 0}, shm);
+        [ch](int i) { return PoolMap{load_published(ch + 4 * i), load_published(ch + 4 * i + 1)}; },
+        MapOp(), PoolMap{0, 0}, shm);
     const int64_t pool_c = map_apply(before, cin[0]);
-    int cap[SC_IPT], cnt[SC_IPT];
-    PoolMap m{0, 0};
-#pragma unroll
-    for(int k = 0; k < SC_IPT; ++k) {
-        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
-        cap[k] = (q < lr.size) ? a.lq_cap[lr.lo + q] : CAP_NONE;
-        cnt[k] = (cap[k] >= 0) ? a.drop_cnt[q] : 0;
-        if(cap[k] >= 0) m = MapOp()(m, PoolMap{(int64_t) cap[k] - cnt[k], 0});
-    }
-    PoolMap tot;
-    const PoolMap ex = block_exclusive_scan<PoolMap, MapOp, SC_NW>(m, MapOp(), PoolMap{0, 0}, shm, &tot);
-    int64_t pl = map_apply(ex, pool_c);
+    int64_t pl = map_apply(mex, pool_c);
+    int kp[SC_IPT], gr[SC_IPT];
     Sizes sz{0, 0};
 #pragma unroll
     for(int k = 0; k < SC_IPT; ++k) {
         const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
-        if(cap[k] >= 0) {
+        kp[k] = 0;
+        gr[k] = 0;
+        if(capk[k] >= 0) {
             int64_t g = 0;
-            if(cnt[k] > cap[k]) {
-                const int64_t e = (int64_t) cnt[k] - cap[k];
+            if(dcnt[k] > capk[k]) {
+                const int64_t e = (int64_t) dcnt[k] - capk[k];
                 g = e < pl ? e : pl;
             }
-            const int64_t room = cap[k] + g;
-            const int32_t kp = (int32_t) (cnt[k] < room ? cnt[k] : room);
-            const int64_t np = pl + cap[k] - cnt[k];
+            const int64_t room = capk[k] + g;
+            kp[k] = (int32_t) (dcnt[k] < room ? dcnt[k] : room);
+            gr[k] = (int32_t) g;
+            const int64_t np = pl + capk[k] - dcnt[k];
             pl = np > 0 ? np : 0;
-            a.keep[q] = kp;
-            a.granted[q] = (int32_t) g;
-            sz.x += kp;
-            sz.v += kp;
-        } else if(cap[k] != CAP_NONE) {
-            sz.v += -1 - cap[k];
+            a.keep[q] = kp[k];
+            a.granted[q] = gr[k];
+            sz.x += kp[k];
+            sz.v += kp[k];
+        } else if(capk[k] != CAP_NONE) {
+            sz.v += -1 - capk[k];
         }
     }
     Sizes stot;
-    (void) block_exclusive_scan<Sizes, SumOp, SC_NW>(sz, SumOp(), Sizes{0, 0}, shs, &stot);
+    const Sizes sex = block_exclusive_scan<Sizes, SumOp, SC_NW>(sz, SumOp(), Sizes{0, 0}, shs, &stot);
     if(threadIdx.x == 0) {
         a.chunk[4 * c + 2] = stot.x;
         a.chunk[4 * c + 3] = stot.v;
-        if(c == (int) gridDim.x - 1) cout[0] = (int32_t) map_apply(tot, pool_c);
+        __hip_atomic_store(fsize + c, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-}
-
-template<typename VT>
-__global__ __launch_bounds__(SC_TPB) void k_level_write(FactorArgs<VT> a, LevelRange lr)
-{
-    __shared__ Sizes shs[SC_NW];
-    const int c = blockIdx.x;
-    const int32_t* cin = a.carry + 4 * (lr.level & 1);
-    int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
-    const int64_t* ch = a.chunk;
-    const Sizes before = block_prefix<Sizes>(c,
-        [ch](int i) { return Sizes{ch[4 * i + 2], ch[4 * i + 3]}; }, SumOp(), Sizes{0, 0}, shs);
-    int cap[SC_IPT], kp[SC_IPT];
-    Sizes sz{0, 0};
+    wait_chunk_flags(fsize, c, a.err);
+    // -- output offsets and factored metadata
+    const Sizes sbefore = block_prefix<Sizes>(c,
+        [ch](int i) { return Sizes{load_published(ch + 4 * i + 2), load_published(ch + 4 * i + 3)}; },
+        SumOp(), Sizes{0, 0}, shs);
+    int64_t px = cin[1] + sbefore.x + sex.x, pv = cin[2] + sbefore.v + sex.v;
 #pragma unroll
     for(int k = 0; k < SC_IPT; ++k) {
-        const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
-        cap[k] = (q < lr.size) ? a.lq_cap[lr.lo + q] : CAP_NONE;
-        kp[k] = (cap[k] >= 0) ? a.keep[q] : 0;
-        if(cap[k] >= 0) { sz.x += kp[k]; sz.v += kp[k]; }
-        else if(cap[k] != CAP_NONE) sz.v += -1 - cap[k];
-    }
-    Sizes stot;
-    const Sizes ex = block_exclusive_scan<Sizes, SumOp, SC_NW>(sz, SumOp(), Sizes{0, 0}, shs, &stot);
-    int64_t px = cin[1] + before.x + ex.x, pv = cin[2] + before.v + ex.v;
-#pragma unroll
-    for(int k = 0; k < SC_IPT; ++k) {
-        if(cap[k] == CAP_NONE) continue;
+        if(capk[k] == CAP_NONE) continue;
         const int q = c * SCAN_CHUNK + (int) threadIdx.x * SC_IPT + k;
         const int b = a.lvlb_idx[lr.lo + q];
         BlockMeta o = a.bin[b];
         int64_t nv;
-        if(cap[k] >= 0) {
-            const int g = a.granted[q];
-            const int cn = a.drop_cnt[q];
+        if(capk[k] >= 0) {
             o.nnz = kp[k];
-            o.max_nnz = cap[k] + g;
-            if(cn > cap[k]) o.got_extra = g;
-            a.overflow[q] = (uint8_t) (cn > cap[k] + g);
+            o.max_nnz = capk[k] + gr[k];
+            if(dcnt[k] > capk[k]) o.got_extra = gr[k];
+            a.overflow[q] = (uint8_t) (dcnt[k] > capk[k] + gr[k]);
             nv = kp[k];
         } else {
             o.nnz = 0;
             a.overflow[q] = 0;
-            nv = -1 - cap[k];
+            nv = -1 - capk[k];
         }
         o.col_ptr = (int32_t) px;
         o.val_ptr = (int32_t) pv;
-        px += (cap[k] >= 0) ? kp[k] : 0;
+        px += (capk[k] >= 0) ? kp[k] : 0;
         pv += nv;
         a.bout[o.id] = o;
     }
     if(threadIdx.x == 0 && c == (int) gridDim.x - 1) {
-        cout[1] = (int32_t) (cin[1] + before.x + stot.x);
-        cout[2] = (int32_t) (cin[2] + before.v + stot.v);
+        cout[0] = (int32_t) map_apply(mtot, pool_c);
+        cout[1] = (int32_t) (cin[1] + sbefore.x + stot.x);
+        cout[2] = (int32_t) (cin[2] + sbefore.v + stot.v);
     }
 }
 
@@ -2167,6 +2185,10 @@ void factor_begin(const FactorArgs<VT>& a, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
     k_zero_carry<VT><<<1, 64, 0, st>>>(a.carry);
+    if(a.num_levels > 0 && a.scan_maxch > 0) {
+        const int64_t nf = (int64_t) 2 * a.scan_maxch * a.num_levels;
+        k_zero<int32_t><<<(unsigned) ((nf + 255) / 256), 256, 0, st>>>(a.scan_flags, nf);
+    }
     if(a.num_levels > 0) {
         const unsigned g = (unsigned) ((a.num_levels + 255) / 256);
         k_zero<int32_t><<<g, 256, 0, st>>>(a.defer_cnt, a.num_levels);
@@ -2191,9 +2213,7 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
     if(a.pivoting && r.npiv > 0) k_pivot_cols<VT><<<r.npiv, 64, 0, st>>>(a, r);
     if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
     const unsigned nch = (unsigned) ((r.size + SCAN_CHUNK - 1) / SCAN_CHUNK);
-    k_level_maps<VT><<<nch, SC_TPB, 0, st>>>(a, r);
-    k_level_grant<VT><<<nch, SC_TPB, 0, st>>>(a, r);
-    k_level_write<VT><<<nch, SC_TPB, 0, st>>>(a, r);
+    k_level_scan<VT><<<nch, SC_TPB, 0, st>>>(a, r);
     k_store_tiny<VT><<<(r.size + 255) / 256, 256, 0, st>>>(a, r);
     k_store<VT><<<r.size < 2048 ? r.size : 2048, 64, 0, st>>>(a, r);
 }

@@ -55,6 +55,12 @@ struct FactorArgs {
     const int32_t* lq_cap = nullptr;
     const int32_t* lq_fr = nullptr;
     int64_t* chunk = nullptr;             // per scan chunk: pool map (2), sizes (2)
+    // per level and scan chunk two flags (pool map, sizes published);
+    // scan_maxch chunks per level; zeroed at the start of every factor
+    int32_t* scan_flags = nullptr;
+    int32_t scan_maxch = 0;
+    // pinned host flag: 2 when a scan chunk's bounded wait gave up
+    int32_t* err = nullptr;
     // static, per level-list position of an off-diagonal block, 12 int32:
     // {block, first row, rows | cols << 8 | all-1x1 << 16, pivot position},
     // {input format, nnz, col_ptr, val_ptr}, {first pair (low 32 bits),

@@ -151,7 +151,12 @@ void Bildlt::upload_pattern(hipStream_t st)
     d_wave_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
     d_store_list.ensure(sizeof(int32_t) * mlb);
     d_store_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
-    d_chunk.ensure(sizeof(int64_t) * 4 * (size_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK));
+    // per scan chunk 4 int64, then 2 int32 flags per chunk and level
+    {
+        const size_t maxch = (size_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK);
+        d_chunk.ensure(sizeof(int64_t) * 4 * maxch +
+            sizeof(int32_t) * 2 * maxch * (size_t) std::max(1, s.num_levels));
+    }
     d_vals.ensure(sizeof(double) * std::max<int64_t>(1, A.nnz()));
     d_val_in.ensure(vt() * std::max<int64_t>(1, s.val_space));
     d_bout.ensure(sizeof(BlockMeta) * std::max(1, K));
@@ -246,6 +251,12 @@ FactorArgs<VT> Bildlt::factor_args() const
     a.lq_cap = d_lq_cap.as<int32_t>();
     a.lq_fr = d_lq_fr.as<int32_t>();
     a.chunk = d_chunk.as<int64_t>();
+    {
+        const int64_t mlb = std::max<int64_t>(1, sym.max_level_blocks);
+        a.scan_maxch = (int32_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK);
+        a.scan_flags = (int32_t*) (a.chunk + 4 * (int64_t) a.scan_maxch);
+    }
+    a.err = h_trsv_err.p;
     a.off_task = d_off_task.as<int32_t>();
     a.upd4 = d_upd4.as<int32_t>();
     a.piv_umask = d_piv_umask.as<uint32_t>();
@@ -541,7 +552,10 @@ void Bildlt::check_solve_error(hipStream_t st) const
 
 void Bildlt::take_async_error() const
 {
-    if(h_trsv_err.take())
+    const int32_t e = h_trsv_err.take();
+    if(e == 2)
+        fail(CULIP_ERR_HIP, "factor: a chunk wait of a level's dropping scan timed out");
+    if(e)
         fail(CULIP_ERR_HIP, "triangular solve: a dependency wait of a single-launch solve timed "
                             "out (its x holds NaN)");
 }
