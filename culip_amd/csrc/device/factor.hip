@@ -1530,10 +1530,12 @@ __global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny_scale(FactorArgs<VT> 
     const int tid = threadIdx.x;
     const int q = lr.npiv + (int) blockIdx.x * TINY_TPB + tid;
     if(q >= lr.size) return;
-    int n = a.if_cnt[q];
-    if(n < 0) return;
+    // the parked count and the task record together (neither waits for
+    // the other)
     const int64_t p = (int64_t) lr.lo + q;
     const int4 t0 = ((const int4*) a.off_task)[3 * p];
+    int n = a.if_cnt[q];
+    if(n < 0) return;
     const int fr = t0.y;
     const int hgt = t0.z & 255;
     const int pq = t0.w;
