@@ -1757,12 +1757,15 @@ __device__ V block_prefix(int count, Load load, Op op, const V& id, V* sh)
     return tot;
 }
 
-// One launch per level, in chunks of SCAN_CHUNK positions. Every chunk runs
-// the threshold counts, publishes its pool map (release store of its flag),
-// waits for the maps of the chunks before it (acquire loads; chunks are
-// dispatched in order, so a waiting chunk's predecessors are resident and
-// never wait for it), runs the pools and grants, publishes its sizes, waits
-// for the sizes before it and writes the factored metadata.
+// One launch per level, in chunks of SCAN_CHUNK positions. A workgroup
+// takes its chunk from a ticket counter, so every chunk it later waits for
+// belongs to a workgroup that is already running, whatever the dispatch
+// order. Every chunk runs the threshold counts, publishes its pool map
+// (release store of its flag), waits for the maps of the chunks before it
+// (acquire loads), runs the pools and grants, publishes its sizes, waits
+// for the sizes before it and writes the factored metadata. The waits are
+// bounded only as a guard against a hang; given the tickets they end as
+// soon as the earlier chunks have published.
 constexpr int SCAN_SPIN_LIMIT = 1 << 22;
 
 __device__ __forceinline__ int64_t load_published(const int64_t* p)
@@ -1798,9 +1801,12 @@ __global__ __launch_bounds__(SC_TPB) void k_level_scan(FactorArgs<VT> a, LevelRa
 {
     __shared__ PoolMap shm[SC_NW];
     __shared__ Sizes shs[SC_NW];
-    const int c = blockIdx.x;
-    int32_t* fmap = a.scan_flags + (int64_t) lr.level * 2 * a.scan_maxch;
+    __shared__ int ticket;
+    int32_t* fmap = a.scan_flags + (int64_t) lr.level * (2 * a.scan_maxch + 1);
     int32_t* fsize = fmap + a.scan_maxch;
+    if(threadIdx.x == 0) ticket = atomicAdd(fsize + a.scan_maxch, 1);
+    __syncthreads();
+    const int c = ticket;
     const int32_t* cin = a.carry + 4 * (lr.level & 1);
     int32_t* cout = a.carry + 4 * ((lr.level + 1) & 1);
     // -- entries of a sparse position that survive |v| >= tau * sqrtf(norm)
@@ -2188,7 +2194,7 @@ void factor_begin(const FactorArgs<VT>& a, void* stream)
     hipStream_t st = (hipStream_t) stream;
     k_zero_carry<VT><<<1, 64, 0, st>>>(a.carry);
     if(a.num_levels > 0 && a.scan_maxch > 0) {
-        const int64_t nf = (int64_t) 2 * a.scan_maxch * a.num_levels;
+        const int64_t nf = (int64_t) (2 * a.scan_maxch + 1) * a.num_levels;
         k_zero<int32_t><<<(unsigned) ((nf + 255) / 256), 256, 0, st>>>(a.scan_flags, nf);
     }
     if(a.num_levels > 0) {

@@ -55,8 +55,9 @@ struct FactorArgs {
     const int32_t* lq_cap = nullptr;
     const int32_t* lq_fr = nullptr;
     int64_t* chunk = nullptr;             // per scan chunk: pool map (2), sizes (2)
-    // per level and scan chunk two flags (pool map, sizes published);
-    // scan_maxch chunks per level; zeroed at the start of every factor
+    // per level: two flags per scan chunk (pool map, sizes published) and a
+    // chunk ticket counter (2 scan_maxch + 1 words); zeroed at the start of
+    // every factor
     int32_t* scan_flags = nullptr;
     int32_t scan_maxch = 0;
     // pinned host flag: 2 when a scan chunk's bounded wait gave up

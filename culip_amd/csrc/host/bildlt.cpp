@@ -151,11 +151,12 @@ void Bildlt::upload_pattern(hipStream_t st)
     d_wave_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
     d_store_list.ensure(sizeof(int32_t) * mlb);
     d_store_cnt.ensure(sizeof(int32_t) * std::max(1, s.num_levels));
-    // per scan chunk 4 int64, then 2 int32 flags per chunk and level
+    // per scan chunk 4 int64, then per level 2 int32 flags per chunk and
+    // a chunk ticket counter
     {
         const size_t maxch = (size_t) ((mlb + SCAN_CHUNK - 1) / SCAN_CHUNK);
         d_chunk.ensure(sizeof(int64_t) * 4 * maxch +
-            sizeof(int32_t) * 2 * maxch * (size_t) std::max(1, s.num_levels));
+            sizeof(int32_t) * (2 * maxch + 1) * (size_t) std::max(1, s.num_levels));
     }
     d_vals.ensure(sizeof(double) * std::max<int64_t>(1, A.nnz()));
     d_val_in.ensure(vt() * std::max<int64_t>(1, s.val_space));
@@ -401,6 +402,10 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
         else factor_impl<double>(st);
     }
     HIP_CHECK(hipStreamSynchronize(st));
+    // a level's dropping scan whose bounded wait gave up leaves the object
+    // unfactored
+    factored = false;
+    take_async_error();
     factored = true;
     plan_known = false;
     host_int.erase("fblocks");
