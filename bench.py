@@ -359,26 +359,30 @@ def bench_serve(a):
 
 def bench_batch(a):
     """512 independent systems (n ~ U[1k, 40k], m = 0.4 n, seeds 0..511)
-    solved as one batch on `slots` concurrent streams, with the block size
-    ({8, 16, 32}) and the triangular-solve schedule picked per size bucket
-    by the autotuner (culip_amd.batch); beside it the plain FP64 path
-    (exact single-launch solves), one system at a time, on the same
-    systems. The value is the rate of a batch pass with the patterns
-    resident (refactor + solve of every system); the end-to-end rate also
-    counts generating the systems and registering the patterns (host
-    symbolic analysis on 16 threads, upload) and the first pass, which
-    creates the numeric instances."""
-    import concurrent.futures as cf
+    solved as one batch (culip_amd.Batch): every system refactored
+    concurrently on 16 streams, then one SQMR recurrence whose steps are
+    single launches over all systems still iterating, with block-Jacobi
+    triangular solves; the block size per size bucket ({8, 16, 32}) and the
+    sweep count ({2, 3}) picked by the autotuner (culip_amd.batch). Beside it
+    the plain FP64 path (exact single-launch solves), one system at a time,
+    on the same systems. The value is the rate of a batch pass with the
+    systems registered: refactor + solve of every system, values and
+    right-hand sides from host memory and solutions back to it. Every
+    system of a pass returns when the pass ends, so its latency is the pass
+    time. The end-to-end rate also counts generating the systems,
+    registering them (host symbolic analysis on 16 threads, upload) and the
+    first pass, which allocates the pooled work space."""
     import culip_amd.batch as cb
+    import concurrent.futures as cf
     rng = np.random.default_rng(a.seed)
     sizes = [int(v) for v in rng.integers(1000, 40001, size=a.systems)]
     t0 = time.perf_counter()
-    tuned, tune_ms = cb.autotune(sizes, lambda i, B: cb.kkt_system(sizes[i], i, B),
-                                 prec=a.prec, slots=a.slots, log=log)
+    tuned, sweeps, tune_ms = cb.autotune_batch(
+        sizes, lambda i, B: cb.kkt_system(sizes[i], i, B), prec=a.prec, log=log)
     t_tune = time.perf_counter() - t0
 
     def make(i):
-        k, p = cb.kkt_system(sizes[i], i, tuned[cb.bucket_of(sizes[i])][0])
+        k, p = cb.kkt_system(sizes[i], i, tuned[cb.bucket_of(sizes[i])])
         return p, k.rhs[p.perm].copy()
 
     t0 = time.perf_counter()
@@ -386,8 +390,7 @@ def bench_batch(a):
         systems = list(ex.map(make, range(a.systems)))
     t_gen = time.perf_counter() - t0
     log(f"{len(systems)} systems generated in {t_gen:.1f}s")
-    res, secs, t_reg = cb.solve_batch(systems, lambda i: tuned[cb.bucket_of(sizes[i])][1:],
-                                      a.slots, a.prec, log=log, graphs=bool(a.graphs))
+    res, secs, t_reg, phases = cb.solve_batch_engine(systems, a.prec, sweeps, log=log)
     el = secs[-1]
     # the plain path: FP64 factor, exact single-launch solves (the reference
     # driver's SOLVE_BLOCK), one system at a time
@@ -395,7 +398,6 @@ def bench_batch(a):
     el_base = secs_b[-1]
     trr = max(r[1]["true_rel_residual"] for r in res)
     conv = [r[1]["true_rel_residual"] <= 1e-8 for r in res]
-    lat = np.array([r[2]["latency"] for r in res]) * 1e3
     return {
         "metric": "KKT solves/sec to rel. residual 1e-8 (FP32-factor vs FP64); p50/p99 solve latency",
         "value": len(res) / el, "unit": "solves/s", "n_gpus": 1, "steps": len(res),
@@ -403,30 +405,32 @@ def bench_batch(a):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.prec == "f64" else "f32-factor/f64-sqmr", "data": "synthetic",
         "config": {"workload": "batch", "systems": a.systems, "n_range": [1000, 40000],
-                   "m": "0.4n", "seeds": f"0..{a.systems - 1}", "slots": a.slots,
+                   "m": "0.4n", "seeds": f"0..{a.systems - 1}",
+                   "engine": "one batch: concurrent refactors on 16 streams, one launch per "
+                             "SQMR step over all systems still iterating",
                    "max_block_by_bucket": {f"{cb.BUCKETS[b][0]}-{cb.BUCKETS[b][1] - 1}":
-                                           tuned[b][0] for b in tuned},
-                   "solve_schedule_by_bucket": {
-                       f"{cb.BUCKETS[b][0]}-{cb.BUCKETS[b][1] - 1}":
-                           tuned[b][1] if tuned[b][1] != "jacobi" else f"{tuned[b][1]}{tuned[b][2]}"
-                       for b in tuned},
-                   "tol": 1e-8, "graphs": bool(a.graphs), "parallelism": "none"},
-        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                                           tuned[b] for b in tuned},
+                   "solve_schedule": f"jacobi{sweeps}",
+                   "tol": 1e-8, "parallelism": "none"},
+        "p50_ms": el * 1e3, "p99_ms": el * 1e3,
+        "latency_note": "every system of a pass returns when the pass ends",
         "max_true_rel_residual": trr,
         "converged_fraction": float(np.mean(conv)),
         "converged_solves_per_s": float(np.sum(conv)) / el,
         "sqmr_iterations_mean": float(np.mean([r[1]["iterations"] for r in res])),
+        "sqmr_iterations_max": int(max(r[1]["iterations"] for r in res)),
         "plain_f64_per_system_solves_per_s": len(base) / el_base,
         "speedup_vs_plain_f64_per_system": (len(res) / el) / (len(base) / el_base),
         "plain_max_true_rel_residual": max(r[1]["true_rel_residual"] for r in base),
+        "phases_s": {"refactors": phases["factor_s"], "sqmr": phases["sqmr_s"]},
+        "device_bytes": phases["device_bytes"],
         "setup": {"autotune_s": t_tune, "generate_s": t_gen,
                   "register_s_symbolic_and_upload": t_reg,
-                  "first_pass_s_with_instance_creation": secs[0]},
+                  "first_pass_s_with_allocation": secs[0]},
         "end_to_end_solves_per_s": len(res) / (t_gen + t_reg + secs[0]),
         "autotune_ms": tune_ms,
         "roofline": None, "cpu_baseline": None,
     }
-
 
 def main():
     a = parse()
@@ -611,7 +615,7 @@ def main():
                                     "sqmr_iterations_mean", "p50_ms", "p99_ms",
                                     "plain_f64_per_system_solves_per_s",
                                     "speedup_vs_plain_f64_per_system",
-                                    "end_to_end_solves_per_s", "setup")}
+                                    "end_to_end_solves_per_s", "phases_s", "setup")}
         batch["config"] = bo["config"]
 
     cpu = None

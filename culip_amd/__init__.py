@@ -22,7 +22,7 @@ from ._lib import (CulipError, PREC_F32, PREC_F64, SOLVE_LEFT, SOLVE_MIDDLE, SOL
                    ORDER_NATURAL, ORDER_BLOCK_COLOR, TRSV_LEVELS, TRSV_SINGLE, TRSV_JACOBI,
                    check, lib)
 
-__all__ = ["CulipError", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
+__all__ = ["CulipError", "Batch", "KKT", "Prep", "BlockiLDLt", "CSRMatrixSPMV", "AugmentedMatrixSPMV",
            "SQMR", "ImportLDUPreconditioner", "NormalMatrixSPMV", "Server", "read_matrix_csr", "read_dense_vector", "csr_struct", "PREC_F32",
            "PREC_F64", "ORDER_NATURAL", "ORDER_BLOCK_COLOR", "permute_scale",
            "weighted_matching", "structural_rank", "btf_rank_bound"]
@@ -556,6 +556,74 @@ class Server(_Handle):
     def close(self):
         if getattr(self, "_h", None):
             lib().culip_server_destroy(self._h)
+            self._h = None
+
+    def _release(self):
+        self.close()
+
+
+class Batch(_Handle):
+    """Independent systems refactored and solved together (culip_batch):
+    one launch per SQMR step over all systems still iterating, block-Jacobi
+    triangular solves with `jacobi_sweeps` sweeps. add(K, piv_starts,
+    block_starts) -> index (symbolic analysis and upload; thread-safe,
+    before the first solve); solve(vals, rhs) -> (xs, results) for lists of
+    host arrays, one per system. Each system's x and result equal its solve
+    alone with the same options (BlockiLDLt with Jacobi sweeps + SQMR)."""
+
+    def __init__(self, prec="f32", fill_level=0, fill_factor=1.0, threshold=1e-4, tol=1e-8,
+                 max_it=1000, true_residual_period=20, max_restarts=10, jacobi_sweeps=2,
+                 factor_streams=16):
+        pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
+        o = _lib.BatchOpts(pr, fill_level, fill_factor, threshold,
+                           _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts),
+                           jacobi_sweeps, factor_streams)
+        h = C.c_void_p()
+        check(lib().culip_batch_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._track()
+        self._n = {}
+
+    def add(self, row_ptr, col, val, piv_starts, block_starts):
+        s, keep = csr_struct(row_ptr, col, val)
+        ps, bs = _i32(piv_starts), _i32(block_starts)
+        i = C.c_int32()
+        check(lib().culip_batch_add(self._h, C.byref(s), len(ps), _ptr(ps, C.c_int32), len(bs),
+                                    _ptr(bs, C.c_int32), C.byref(i)))
+        self._n[i.value] = len(row_ptr) - 1
+        return i.value
+
+    def solve(self, vals, rhs, stream=None):
+        S = len(self._n)
+        if len(vals) != S or len(rhs) != S:
+            raise ValueError(f"batch: {S} systems, got {len(vals)} value and {len(rhs)} "
+                             "right-hand-side arrays")
+        v = [_f64(a) for a in vals]
+        b = [_f64(a) for a in rhs]
+        xs = [np.empty(self._n[i]) for i in range(S)]
+        P = C.POINTER(C.c_double)
+        pv = (P * S)(*[_ptr(a, C.c_double) for a in v])
+        pb = (P * S)(*[_ptr(a, C.c_double) for a in b])
+        px = (P * S)(*[_ptr(a, C.c_double) for a in xs])
+        res = (_lib.SqmrResult * S)()
+        check(lib().culip_batch_solve(self._h, pv, pb, px, res, _stream(stream)))
+        out = [{"residual": r.residual, "iterations": r.iterations,
+                "converged": bool(r.converged), "restarts": r.restarts,
+                "true_rel_residual": r.true_rel_residual} for r in res]
+        return xs, out
+
+    def info(self):
+        """systems, device bytes, and the last solve's seconds: refactors
+        and SQMR"""
+        c, b = C.c_int32(), C.c_int64()
+        t = np.zeros(2)
+        check(lib().culip_batch_info(self._h, C.byref(c), C.byref(b), _ptr(t, C.c_double)))
+        return {"systems": c.value, "device_bytes": b.value, "factor_s": float(t[0]),
+                "sqmr_s": float(t[1])}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().culip_batch_destroy(self._h)
             self._h = None
 
     def _release(self):

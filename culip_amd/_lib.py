@@ -59,6 +59,12 @@ class ServerOpts(C.Structure):
                 ("equilibrate_sweeps", C.c_int32), ("graphs", C.c_int32)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("prec", C.c_int), ("fill_level", C.c_int32), ("fill_factor", C.c_double),
+                ("threshold", C.c_double), ("sqmr", SqmrOpts), ("jacobi_sweeps", C.c_int32),
+                ("factor_streams", C.c_int32)]
+
+
 _lib = None
 
 # (name, restype, argtypes)
@@ -156,6 +162,16 @@ _SIGS = [
                                     C.POINTER(C.c_int32)]),
     ("culip_server_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("culip_server_destroy", None, [C.c_void_p]),
+    ("culip_batch_create", C.c_int, [C.POINTER(BatchOpts), C.POINTER(C.c_void_p)]),
+    ("culip_batch_add", C.c_int, [C.c_void_p, C.POINTER(Csr), C.c_int32, C.POINTER(C.c_int32),
+                                  C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("culip_batch_solve", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)),
+                                    C.POINTER(C.POINTER(C.c_double)),
+                                    C.POINTER(C.POINTER(C.c_double)), C.POINTER(SqmrResult),
+                                    C.c_void_p]),
+    ("culip_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_double)]),
+    ("culip_batch_destroy", None, [C.c_void_p]),
     ("culip_spmv_create_kkt_csr", C.c_int, [C.POINTER(Csr), C.POINTER(Csr), C.c_double,
                                             C.c_int32, C.POINTER(C.c_int32),
                                             C.POINTER(C.c_void_p)]),

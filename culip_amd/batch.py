@@ -5,15 +5,20 @@ The autotuner sorts systems into size buckets and, per bucket, times every
 candidate (maximum block size, triangular-solve schedule) on the bucket's
 first system: numeric refactor plus SQMR to the FP64 true-residual
 tolerance, best of a few repetitions after a warm-up. A candidate that
-misses the tolerance is never picked. The batch then runs on a Server with
-each system's pattern registered under its bucket's block size and
-schedule, so that the systems' solves overlap on the device.
+misses the tolerance is never picked.
+
+The batch runs on culip_amd.Batch: every system registered under its
+bucket's block size, all refactored concurrently and solved by one SQMR
+recurrence whose steps are single launches over all systems still
+iterating, with block-Jacobi triangular solves (one sweep count for the
+batch). solve_batch runs the same systems on a Server instead (one request
+per system, `slots` streams), the path with exact triangular solves.
 """
 import time
 
 import numpy as np
 
-from . import (BlockiLDLt, CSRMatrixSPMV, KKT, ORDER_BLOCK_COLOR, Prep, SQMR, Server)
+from . import (Batch, BlockiLDLt, CSRMatrixSPMV, KKT, ORDER_BLOCK_COLOR, Prep, SQMR, Server)
 
 BUCKETS = ((1000, 10000), (10000, 20000), (20000, 30000), (30000, 40001))
 BLOCK_SIZES = (8, 16, 32)
@@ -96,6 +101,81 @@ def autotune(sizes, make_system, prec="f32", slots=16, block_sizes=BLOCK_SIZES, 
         if log:
             log(f"autotune bucket {lo}-{hi - 1}: block size {best[1]}, {best[2]} {best[3]}")
     return tuned, timings
+
+
+def autotune_batch(sizes, make_system, prec="f32", block_sizes=BLOCK_SIZES, sweep_options=(2, 3),
+                   reps=3, tol=1e-8, buckets=BUCKETS, log=None):
+    """For the batched engine: a block size per size bucket and one
+    block-Jacobi sweep count for the whole batch. Every (block size, sweeps)
+    pair is timed on each bucket's first system as in autotune; the sweep
+    count is the one whose per-bucket best times sum lowest, and each
+    bucket takes its fastest block size at that count.
+    Returns (block size by bucket index, sweeps, all timings in ms)."""
+    import torch
+    dev = torch.device("cuda:0")
+    per, timings, present = {}, {}, []
+    for bi, (lo, hi) in enumerate(buckets):
+        idx = [i for i, n in enumerate(sizes) if lo <= n < hi]
+        if not idx:
+            continue
+        present.append(bi)
+        for B in block_sizes:
+            k, p = make_system(idx[0], B)
+            rhs = k.rhs[p.perm]
+            for sw in sweep_options:
+                ms = _time_candidate(p, rhs, prec, B, "jacobi", sw, reps, tol, dev)
+                per[(bi, B, sw)] = ms
+                timings[f"{lo}-{hi - 1}:B{B}:jacobi{sw}"] = ms
+    if not present:
+        raise ValueError("autotune_batch: no system in any bucket")
+    total = {sw: sum(min(per[(bi, B, sw)] for B in block_sizes) for bi in present)
+             for sw in sweep_options}
+    sweeps = min(sweep_options, key=lambda sw: total[sw])
+    if total[sweeps] == float("inf"):
+        raise RuntimeError("autotune_batch: no sweep count meets the tolerance in every bucket")
+    tuned = {bi: min(block_sizes, key=lambda B: per[(bi, B, sweeps)]) for bi in present}
+    if log:
+        for bi in present:
+            lo, hi = buckets[bi]
+            log(f"autotune bucket {lo}-{hi - 1}: block size {tuned[bi]}, jacobi {sweeps}")
+    return tuned, sweeps, timings
+
+
+def solve_batch_engine(systems, prec, sweeps, tol=1e-8, max_it=1000, passes=2, log=None,
+                       factor_streams=16):
+    """Solves every system (prep, rhs) on one Batch. Registration (symbolic
+    analysis and upload, 16 threads) is timed apart; each pass refactors
+    and solves all systems; the first pass also allocates the batch's
+    pooled work space. Returns (results of the last pass as (x, result)
+    per system, seconds per pass, registration seconds, Batch.info() after
+    the last pass)."""
+    import concurrent.futures as cf
+    bt = Batch(prec=prec, tol=tol, max_it=max_it, jacobi_sweeps=sweeps,
+               factor_streams=factor_streams)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(16) as ex:
+        idx = list(ex.map(lambda s: bt.add(s[0].row_ptr, s[0].col, s[0].val, s[0].piv_starts,
+                                           s[0].block_starts), systems))
+    t_reg = time.perf_counter() - t0
+    S = len(systems)
+    vals, rhs = [None] * S, [None] * S
+    for j, i in enumerate(idx):
+        vals[i] = systems[j][0].val
+        rhs[i] = systems[j][1]
+    secs = []
+    xs = res = None
+    for rep in range(passes):
+        t0 = time.perf_counter()
+        xs, res = bt.solve(vals, rhs)
+        secs.append(time.perf_counter() - t0)
+        if log:
+            inf = bt.info()
+            log(f"batch pass {rep} (one batch, {prec}, jacobi {sweeps}): "
+                f"{S / secs[-1]:.1f} solves/s (refactors {inf['factor_s'] * 1e3:.0f} ms, "
+                f"SQMR {inf['sqmr_s'] * 1e3:.0f} ms)")
+    phases = bt.info()
+    bt.close()
+    return [(xs[idx[j]], res[idx[j]]) for j in range(S)], secs, t_reg, phases
 
 
 def solve_batch(systems, schedule_of, slots, prec, tol=1e-8, max_it=1000, passes=2, log=None,

@@ -15,6 +15,7 @@
 // recurrences need (FP32 sums make it vary by ~1e-7 from apply to apply and
 // cost the recurrence most of its progress below that level).
 #include "solve.hpp"
+#include "batch.hpp"
 #include "wave.hpp"
 
 #include <hip/hip_runtime.h>
@@ -778,6 +779,26 @@ __global__ __launch_bounds__(64) void k_jacobi_first_product(const int4* task,
         i_lr, b, nullptr, xout, y, res);
 }
 
+// Batched block-Jacobi sweeps: item {s, r} is block row r of system s, the
+// workgroup r of k_jacobi_sweep on system s; tr selects the layout (0 the U
+// solve, 1 the U' solve). b may be the output vector itself: a block row
+// reads its own rows of b before it writes its own rows of the output, and
+// no other block row reads them.
+template<typename VT>
+__global__ __launch_bounds__(64) void bk_jacobi_sweep(const int2* items, const BatchSys* sys,
+    int32_t tr, int32_t b, int32_t in, int32_t out)
+{
+    __shared__ double y[33];
+    __shared__ double res[33];
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int4* task = S.task[tr];
+    const double* xin = in == BV_NONE ? nullptr : batch_vec(S, in);
+    block_row_solve<VT>(task[2 * it.y], task[2 * it.y + 1], xin != nullptr, S.o_col[tr],
+        (const VT*) S.o_val[tr], S.o_lr[tr], S.i_col[tr], (const VT*) S.i_val[tr], S.i_lr[tr],
+        batch_vec(S, b), xin, batch_vec(S, out), y, res);
+}
+
 __global__ void k_fill_bits(double* p, int64_t n, unsigned long long bits)
 {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -1032,5 +1053,30 @@ template void trsv_upper<double>(const SolveTopo&, const SolvePlan<double>&, dou
 template void trsv_upper<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
 template void trsv_upper_t<double>(const SolveTopo&, const SolvePlan<double>&, double*, void*);
 template void trsv_upper_t<float>(const SolveTopo&, const SolvePlan<float>&, double*, void*);
+
+
+// the batched form of Bildlt::jacobi (copy, then jacobi_sweeps) and of
+// left_product (K src, then the sweeps): the same sweeps and values. The
+// product is the batched CSR product, whose rows equal those the fused
+// first sweep forms; the sweeps read b from the target itself instead of a
+// copy.
+template<typename VT>
+void batch_jacobi(const BatchLists& L, const BatchSys* sys, bool transpose, int32_t sweeps,
+    int32_t target, int32_t product_src, hipStream_t st)
+{
+    if(L.blk.count <= 0 || sweeps <= 0) return;
+    if(product_src != BV_NONE) batch_csr_spmv(L, sys, product_src, target, st);
+    const int32_t tr = transpose ? 1 : 0;
+    int32_t in = BV_NONE;
+    for(int32_t s = 0; s < sweeps; ++s) {
+        const int32_t out = (s == sweeps - 1) ? target : ((s & 1) ? BV_JB2 : BV_JB1);
+        bk_jacobi_sweep<VT><<<(unsigned) L.blk.count, 64, 0, st>>>(L.blk.items, sys, tr, target,
+            in, out);
+        in = out;
+    }
+}
+
+template void batch_jacobi<double>(const BatchLists&, const BatchSys*, bool, int32_t, int32_t, int32_t, hipStream_t);
+template void batch_jacobi<float>(const BatchLists&, const BatchSys*, bool, int32_t, int32_t, int32_t, hipStream_t);
 
 }  // namespace culip

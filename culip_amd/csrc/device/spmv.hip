@@ -4,6 +4,7 @@
 // y = alpha * A x + beta * y. Eight lanes per row, coalesced entry loads,
 // fixed-order reduction (xor tree within the eight lanes).
 #include "spmv.hpp"
+#include "batch.hpp"
 #include "wave.hpp"
 
 #include <hip/hip_runtime.h>
@@ -15,6 +16,36 @@ namespace {
 constexpr int LPR = 8;   // lanes per row
 constexpr int SPMV_U = 3; // entries per lane and round of a CSR row
 
+// the sum of one CSR row over the eight lanes sub = 0..7 of a group (every
+// lane returns it)
+__device__ __forceinline__ double csr_row_sum(const int32_t* __restrict__ rp,
+    const int32_t* __restrict__ ci, const double* __restrict__ cv, const double* __restrict__ x,
+    int64_t row, int sub)
+{
+    double acc = 0.0;
+    const int32_t s = rp[row], e = rp[row + 1];
+    // the lane's entries q0, q0 + LPR, q0 + 2 LPR are loaded together
+    // (clamped in range), then summed in that order
+    for(int32_t q0 = s + sub; q0 < e; q0 += LPR * SPMV_U) {
+        int32_t c[SPMV_U];
+        double v[SPMV_U], xv[SPMV_U];
+#pragma unroll
+        for(int u = 0; u < SPMV_U; ++u) {
+            const int32_t q = min(q0 + u * LPR, e - 1);
+            c[u] = ci[q];
+            v[u] = cv[q];
+        }
+#pragma unroll
+        for(int u = 0; u < SPMV_U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for(int u = 0; u < SPMV_U; ++u)
+            if(q0 + u * LPR < e) acc = fma(v[u], xv[u], acc);
+    }
+#pragma unroll
+    for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+    return acc;
+}
+
 __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __restrict__ rp,
     const int32_t* __restrict__ ci, const double* __restrict__ cv, const double* __restrict__ x,
     double* __restrict__ y, double alpha, double beta)
@@ -24,32 +55,26 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __re
     const int64_t row0 = tid / LPR;
     const int64_t stride = (int64_t) gridDim.x * blockDim.x / LPR;
     for(int64_t row = row0; row < m; row += stride) {
-        double acc = 0.0;
-        const int32_t s = rp[row], e = rp[row + 1];
-        // the lane's entries q0, q0 + LPR, q0 + 2 LPR are loaded together
-        // (clamped in range), then summed in that order
-        for(int32_t q0 = s + sub; q0 < e; q0 += LPR * SPMV_U) {
-            int32_t c[SPMV_U];
-            double v[SPMV_U], xv[SPMV_U];
-#pragma unroll
-            for(int u = 0; u < SPMV_U; ++u) {
-                const int32_t q = min(q0 + u * LPR, e - 1);
-                c[u] = ci[q];
-                v[u] = cv[q];
-            }
-#pragma unroll
-            for(int u = 0; u < SPMV_U; ++u) xv[u] = x[c[u]];
-#pragma unroll
-            for(int u = 0; u < SPMV_U; ++u)
-                if(q0 + u * LPR < e) acc = fma(v[u], xv[u], acc);
-        }
-#pragma unroll
-        for(int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        const double acc = csr_row_sum(rp, ci, cv, x, row, sub);
         if(sub == 0) {
             const double r = alpha * acc;
             y[row] = (beta == 0.0) ? r : fma(beta, y[row], r);
         }
     }
+}
+
+// y = K x of every listed system: item {s, c} is rows 32 c .. 32 c + 31 of
+// system s, each summed as k_csr_spmv sums it (alpha 1, beta 0)
+__global__ __launch_bounds__(256) void bk_csr_spmv(const int2* items, const BatchSys* sys,
+    int32_t xid, int32_t yid)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int sub = threadIdx.x & (LPR - 1);
+    const int64_t row = (int64_t) it.y * (256 / LPR) + threadIdx.x / LPR;
+    if(row >= S.n) return;
+    const double acc = csr_row_sum(S.k_rp, S.k_col, S.k_val, batch_vec(S, xid), row, sub);
+    if(sub == 0) batch_vec(S, yid)[row] = 1.0 * acc;
 }
 
 // KKT: b1 = H x1 + A' x2 (H = diag(d), or a CSR matrix hp/hi/hv) ;
@@ -164,6 +189,14 @@ void kkt_spmv(const DevKkt& K, const double* x, double* y, double alpha, double 
     if(K.m > 0)
         k_kkt_dual<<<(unsigned) (((int64_t) K.m * LPR + 255) / 256), 256, 0, st>>>(K.n, K.m,
             K.dix, K.A.row_ptr, K.A.col, K.A.val, x, y, alpha, beta);
+}
+
+
+void batch_csr_spmv(const BatchLists& L, const BatchSys* sys, int32_t x, int32_t y,
+    hipStream_t st)
+{
+    if(L.spmv.count > 0)
+        bk_csr_spmv<<<(unsigned) L.spmv.count, 256, 0, st>>>(L.spmv.items, sys, x, y);
 }
 
 }  // namespace culip

@@ -17,6 +17,7 @@
 // The host checks the stopping flags with the reference's 4-iteration
 // throttle (:309-331).
 #include "sqmr.hpp"
+#include "batch.hpp"
 #include "wave.hpp"
 #include "../host/common.hpp"
 
@@ -37,6 +38,7 @@ enum {
     S_AROT = 11, S_C = 12, S_S = 15, S_TAU = 18, S_RHO = 19, S_GAMMA = 20, S_NORMPB = 21,
     S_NORMX = 22, S_ETA = 26, S_ITER = 27, NSCAL = 32
 };
+static_assert(S_GAMMA == SQMR_S_GAMMA && S_NORMPB == SQMR_S_NORMPB, "scalar slots");
 
 #define HIPCHK(x)                                                                       \
     do {                                                                                \
@@ -130,9 +132,9 @@ __device__ void block_sum2(double& a, double& b, double* sh)
     __syncthreads();
 }
 
-__device__ __forceinline__ double& part_at(double* part, int slot)
+__device__ __forceinline__ double& part_at(double* part, int slot, int q, int G)
 {
-    return part[slot * (int) gridDim.x + (int) blockIdx.x];
+    return part[slot * G + q];
 }
 
 struct DP { double g1, g2; bool w1, w2; };
@@ -248,13 +250,13 @@ __device__ __forceinline__ DP dsolve_loaded(const DRow& d, double r1, double r2)
     return o;
 }
 
-__global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, const double* v,
+__device__ __forceinline__ void alpha_body(int q_, int G_, int32_t n, const double* Av, const double* v,
     const double* z, const double* tri, const int32_t* ip, double* part)
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
-    const int64_t S = (int64_t) gridDim.x * TPB;
-    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+    const int64_t S = (int64_t) G_ * TPB;
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
         double av[VR], av1[VR], vv[VR], vv1[VR], zz[VR];
         DRow d[VR];
 #pragma unroll
@@ -281,21 +283,21 @@ __global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, cons
         }
     }
     block_sum2(num, den, sh);
-    if(threadIdx.x == 0) { part_at(part, 0) = num; part_at(part, 1) = den; }
+    if(threadIdx.x == 0) { part_at(part, 0, q_, G_) = num; part_at(part, 1, q_, G_) = den; }
 }
 
-__global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* Av,
+__device__ __forceinline__ void vnext_body(int q_, int G_, int32_t n, int k, const double* Av,
     const double* vk, const double* vkm1, double* tv, const double* scal, double* part)
 {
     __shared__ double sh[16];
-    const int G = gridDim.x;
+    const int G = G_;
     double r[2];
     reduce_parts<2>(part, G, {0, 1}, r, sh);
     const double alpha = r[0] / r[1];
     const double beta = scal[S_BETA + k % 3];
     double ss = 0.0;
     const int64_t S = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
         double a[VR], b[VR], c[VR];
 #pragma unroll
         for(int r = 0; r < VR; ++r) {
@@ -314,20 +316,20 @@ __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* A
         }
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part_at(part, 2) = ss;
+    if(threadIdx.x == 0) part_at(part, 2, q_, G_) = ss;
 }
 
-__global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, double* v1,
+__device__ __forceinline__ void vscale_body(int q_, int G_, int32_t n, const double* tv, double* v1,
     double* z, double* w1, double* ev, const double* tri, const int32_t* ip, const double* scal,
     const double* part)
 {
     __shared__ double sh[16];
-    const int G = gridDim.x;
+    const int G = G_;
     const double delta = sqrt(reduce_part(part, G, 2, sh));
     const double t = 1.0 / delta;
     const double we = scal[S_ETA] * delta;
     const int64_t S = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
         double a[VR], a1[VR];
         DRow d[VR];
 #pragma unroll
@@ -353,13 +355,13 @@ __global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, dou
     }
 }
 
-__global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const double* Av,
+__device__ __forceinline__ void beta_body(int q_, int G_, int32_t n, const double* wk, const double* Av,
     const double* vk, double* part)
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
-    const int64_t S = (int64_t) gridDim.x * TPB;
-    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
+    const int64_t S = (int64_t) G_ * TPB;
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
         double a[VR], b[VR], c[VR];
 #pragma unroll
         for(int r = 0; r < VR; ++r) {
@@ -376,7 +378,7 @@ __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const
         }
     }
     block_sum2(num, den, sh);
-    if(threadIdx.x == 0) { part_at(part, 3) = num; part_at(part, 4) = den; }
+    if(threadIdx.x == 0) { part_at(part, 3, q_, G_) = num; part_at(part, 4, q_, G_) = den; }
 }
 
 template<int K0>
@@ -392,13 +394,13 @@ __device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
 }
 
 template<int K0>
-__device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk, double* pk,
+__device__ __forceinline__ void pupdate(int q_, int G_, int32_t n, double* w1, const double* vk, double* pk,
     const double* pkm1, const double* pkm2, double* x, const double* scal, double* part,
     const int32_t* flags)
 {
     __shared__ double sh[64];
     double S[NSCAL];
-    const int G = gridDim.x;
+    const int G = G_;
     double r[5];
     reduce_parts<5>(part, G, {0, 1, 2, 3, 4}, r, sh);
     iteration_scalars<K0>(S, scal, r);
@@ -410,7 +412,7 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
     const bool frozen = flags[0] != 0;
     double ss = 0.0;
     const int64_t SG = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) blockIdx.x * TPB + threadIdx.x; g0 < n; g0 += SG * VR) {
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += SG * VR) {
         double ww[VR], a[VR], b[VR], c[VR], xx[VR];
 #pragma unroll
         for(int r = 0; r < VR; ++r) {
@@ -437,17 +439,17 @@ __device__ __forceinline__ void pupdate(int32_t n, double* w1, const double* vk,
         }
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part_at(part, 5) = ss;
+    if(threadIdx.x == 0) part_at(part, 5, q_, G_) = ss;
 }
 
-__global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
+__device__ __forceinline__ void pupdate_body(int q_, int G_, int32_t n, int k, double* w1, const double* vk,
     double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
     double* part, const int32_t* flags)
 {
     switch(k % 3) {
-    case 0: pupdate<0>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
-    case 1: pupdate<1>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
-    default: pupdate<2>(n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    case 0: pupdate<0>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    case 1: pupdate<1>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    default: pupdate<2>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
     }
 }
 
@@ -490,23 +492,23 @@ __global__ __launch_bounds__(TPB) void k_scalars(int k, int G, double* scal, con
 }
 
 // start: tv = Pl \ b (done by the caller); v0 = tv / ||tv||, z = D \ v0
-__global__ __launch_bounds__(TPB) void k_sumsq(int32_t n, const double* a, double* part, int slot)
+__device__ __forceinline__ void sumsq_body(int q_, int G_, int32_t n, const double* a, double* part, int slot)
 {
     __shared__ double sh[16];
     double ss = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) ss = fma(a[g], a[g], ss);
+    for(int32_t g = q_ * TPB + threadIdx.x; g < n; g += G_ * TPB) ss = fma(a[g], a[g], ss);
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part_at(part, slot) = ss;
+    if(threadIdx.x == 0) part_at(part, slot, q_, G_) = ss;
 }
 
-__global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, double* v0, double* z,
+__device__ __forceinline__ void init_v_body(int q_, int G_, int32_t n, const double* tv, double* v0, double* z,
     const double* tri, const int32_t* ip, double* part)
 {
     __shared__ double sh[16];
-    const int G = gridDim.x;
+    const int G = G_;
     const double t = 1.0 / sqrt(reduce_part(part, G, 6, sh));
     double ss = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+    for(int32_t g = q_ * TPB + threadIdx.x; g < n; g += G * TPB) {
         const double r1 = tv[g] * t;
         const double r2 = (g < n - 1) ? tv[g + 1] * t : 0.0;
         v0[g] = r1;
@@ -515,21 +517,21 @@ __global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, dou
         if(d.w2) { z[g + 1] = d.g2; ss = fma(d.g2, d.g2, ss); }
     }
     ss = block_sum(ss, sh);
-    if(threadIdx.x == 0) part_at(part, 7) = ss;
+    if(threadIdx.x == 0) part_at(part, 7, q_, G_) = ss;
 }
 
-__global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, double* w0, double* ev,
+__device__ __forceinline__ void init_w_body(int q_, int G_, int32_t n, const double* z, double* w0, double* ev,
     double* scal, const double* part, int32_t* flags)
 {
     __shared__ double sh[16];
-    const int G = gridDim.x;
+    const int G = G_;
     const double norm_pb = sqrt(reduce_part(part, G, 6, sh));
     const double eta = 1.0 / sqrt(reduce_part(part, G, 7, sh));
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += G * TPB) {
+    for(int32_t g = q_ * TPB + threadIdx.x; g < n; g += G * TPB) {
         w0[g] = z[g] * eta;
         ev[g] = z[g];
     }
-    if(blockIdx.x == 0 && threadIdx.x == 0) {
+    if(q_ == 0 && threadIdx.x == 0) {
         for(int q = 0; q < NSCAL; ++q) scal[q] = 0.0;
         scal[S_C + 0] = -1.0;
         scal[23] = 1.0;
@@ -540,22 +542,22 @@ __global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, doub
     }
 }
 
-__global__ void k_fill(int32_t n, double* a, double v)
+__device__ __forceinline__ void fill_body(int q_, int G_, int32_t n, double* a, double v)
 {
-    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = q_ * blockDim.x + threadIdx.x;
     if(g < n) a[g] = v;
 }
 
-__global__ void k_copy(int32_t n, const double* a, double* b)
+__device__ __forceinline__ void copy_body(int q_, int G_, int32_t n, const double* a, double* b)
 {
-    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = q_ * blockDim.x + threadIdx.x;
     if(g < n) b[g] = a[g];
 }
 
 // b <- D \ a (out of place)
-__global__ void k_dsolve(int32_t n, const double* a, double* b, const double* tri, const int32_t* ip)
+__device__ __forceinline__ void dsolve_body(int q_, int G_, int32_t n, const double* a, double* b, const double* tri, const int32_t* ip)
 {
-    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = q_ * blockDim.x + threadIdx.x;
     if(g >= n) return;
     const DP d = dsolve(g, n, tri, ip, a[g], (g < n - 1) ? a[g + 1] : 0.0);
     if(d.w1) b[g] = d.g1;
@@ -563,26 +565,26 @@ __global__ void k_dsolve(int32_t n, const double* a, double* b, const double* tr
 }
 
 // r = b - r (r holds A x); x_acc += x
-__global__ void k_residual(int32_t n, const double* b, double* r)
+__device__ __forceinline__ void residual_body(int q_, int G_, int32_t n, const double* b, double* r)
 {
-    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = q_ * blockDim.x + threadIdx.x;
     if(g < n) r[g] = b[g] - r[g];
 }
 
-__global__ void k_axpy1(int32_t n, const double* a, double* b)
+__device__ __forceinline__ void axpy1_body(int q_, int G_, int32_t n, const double* a, double* b)
 {
-    const int32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t g = q_ * blockDim.x + threadIdx.x;
     if(g < n) b[g] += a[g];
 }
 
 // sums of squares of a and b, each element divided by s[g] when s is set
 // (the norms of the unscaled system when a and b belong to D A D)
-__global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, const double* b,
+__device__ __forceinline__ void norm2_pair_body(int q_, int G_, int32_t n, const double* a, const double* b,
     const double* s, double* part)
 {
     __shared__ double sh[16];
     double sa = 0.0, sb = 0.0;
-    for(int32_t g = blockIdx.x * TPB + threadIdx.x; g < n; g += gridDim.x * TPB) {
+    for(int32_t g = q_ * TPB + threadIdx.x; g < n; g += G_ * TPB) {
         double av = a[g], bv = b[g];
         if(s) {
             av /= s[g];
@@ -593,15 +595,234 @@ __global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, 
     }
     sa = block_sum(sa, sh);
     sb = block_sum(sb, sh);
-    if(threadIdx.x == 0) { part_at(part, 0) = sa; part_at(part, 1) = sb; }
+    if(threadIdx.x == 0) { part_at(part, 0, q_, G_) = sa; part_at(part, 1, q_, G_) = sb; }
 }
 
-__global__ __launch_bounds__(TPB) void k_norm_final(int G, const double* part, double* out)
+// the single-system kernels: the bodies at their own grid position
+__global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, const double* v,
+    const double* z, const double* tri, const int32_t* ip, double* part)
+{
+    alpha_body(blockIdx.x, gridDim.x, n, Av, v, z, tri, ip, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* Av,
+    const double* vk, const double* vkm1, double* tv, const double* scal, double* part)
+{
+    vnext_body(blockIdx.x, gridDim.x, n, k, Av, vk, vkm1, tv, scal, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, double* v1,
+    double* z, double* w1, double* ev, const double* tri, const int32_t* ip, const double* scal,
+    const double* part)
+{
+    vscale_body(blockIdx.x, gridDim.x, n, tv, v1, z, w1, ev, tri, ip, scal, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const double* Av,
+    const double* vk, double* part)
+{
+    beta_body(blockIdx.x, gridDim.x, n, wk, Av, vk, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
+    double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
+    double* part, const int32_t* flags)
+{
+    pupdate_body(blockIdx.x, gridDim.x, n, k, w1, vk, pk, pkm1, pkm2, x, scal, part, flags);
+}
+
+__global__ __launch_bounds__(TPB) void k_sumsq(int32_t n, const double* a, double* part, int slot)
+{
+    sumsq_body(blockIdx.x, gridDim.x, n, a, part, slot);
+}
+
+__global__ __launch_bounds__(TPB) void k_init_v(int32_t n, const double* tv, double* v0, double* z,
+    const double* tri, const int32_t* ip, double* part)
+{
+    init_v_body(blockIdx.x, gridDim.x, n, tv, v0, z, tri, ip, part);
+}
+
+__global__ __launch_bounds__(TPB) void k_init_w(int32_t n, const double* z, double* w0, double* ev,
+    double* scal, const double* part, int32_t* flags)
+{
+    init_w_body(blockIdx.x, gridDim.x, n, z, w0, ev, scal, part, flags);
+}
+
+__global__ void k_fill(int32_t n, double* a, double v)
+{
+    fill_body(blockIdx.x, gridDim.x, n, a, v);
+}
+
+__global__ void k_copy(int32_t n, const double* a, double* b)
+{
+    copy_body(blockIdx.x, gridDim.x, n, a, b);
+}
+
+__global__ void k_dsolve(int32_t n, const double* a, double* b, const double* tri, const int32_t* ip)
+{
+    dsolve_body(blockIdx.x, gridDim.x, n, a, b, tri, ip);
+}
+
+__global__ void k_residual(int32_t n, const double* b, double* r)
+{
+    residual_body(blockIdx.x, gridDim.x, n, b, r);
+}
+
+__global__ void k_axpy1(int32_t n, const double* a, double* b)
+{
+    axpy1_body(blockIdx.x, gridDim.x, n, a, b);
+}
+
+__global__ __launch_bounds__(TPB) void k_norm2_pair(int32_t n, const double* a, const double* b,
+    const double* s, double* part)
+{
+    norm2_pair_body(blockIdx.x, gridDim.x, n, a, b, s, part);
+}
+
+__device__ __forceinline__ void norm_final_body(int G, const double* part, double* out)
 {
     __shared__ double sh[16];
     const double a = reduce_part(part, G, 0, sh);
     const double b = reduce_part(part, G, 1, sh);
     if(threadIdx.x == 0) { out[0] = sqrt(a); out[1] = sqrt(b); }
+}
+
+__global__ __launch_bounds__(TPB) void k_norm_final(int G, const double* part, double* out)
+{
+    norm_final_body(G, part, out);
+}
+
+// ---------------------------------------------------------------------------
+// Batched steps: one launch over the items of a list; each item is the
+// single-system kernel's workgroup `it.y` of system `it.x` (the iteration's
+// cyclic vectors chosen from the system's own iteration counter).
+
+__global__ void k_batch_expand(const int4* ent, int2* items)
+{
+    const int4 e = ent[blockIdx.x];
+    for(int j = threadIdx.x; j < e.y; j += blockDim.x) items[e.z + j] = make_int2(e.x, j);
+}
+
+__global__ __launch_bounds__(TPB) void bk_alpha(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k = *S.iter;
+    alpha_body(it.y, S.G, S.n, S.av[k & 1], S.v[k % 3], S.z, S.tri, S.ip, S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_vnext(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k = *S.iter;
+    vnext_body(it.y, S.G, S.n, k, S.av[k & 1], S.v[k % 3], S.v[(k + 2) % 3], S.tv, S.scal,
+        S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_vscale(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k1 = (*S.iter + 1) % 3;
+    vscale_body(it.y, S.G, S.n, S.tv, S.v[k1], S.z, S.w[k1], S.ev, S.tri, S.ip, S.scal, S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_beta(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k = *S.iter;
+    beta_body(it.y, S.G, S.n, S.w[k % 3], S.av[(k & 1) ^ 1], S.v[k % 3], S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_pupdate(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k = *S.iter;
+    const int k0 = k % 3, k1 = (k + 1) % 3, km1 = (k + 2) % 3;
+    pupdate_body(it.y, S.G, S.n, k, S.w[k1], S.v[k0], S.p[k0], S.p[km1], S.p[k1], S.xt, S.scal,
+        S.part, S.flags);
+}
+
+// the scalar recurrence of every listed system, then its iteration counter
+// moves on (every thread has read it before the first barrier in scalars)
+__global__ __launch_bounds__(TPB) void bk_scalars(const int2* items, const BatchSys* sys,
+    double tol, double brel)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    const int k = *S.iter;
+    switch(k % 3) {
+    case 0: scalars<0>(k, S.G, S.scal, S.part, S.flags, tol, brel); break;
+    case 1: scalars<1>(k, S.G, S.scal, S.part, S.flags, tol, brel); break;
+    default: scalars<2>(k, S.G, S.scal, S.part, S.flags, tol, brel); break;
+    }
+    if(threadIdx.x == 0) *S.iter = k + 1;
+}
+
+// the vectors a run starts from zero (sqmr.impl.cuh:126-135)
+__global__ void bk_init_fill(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    for(double* a : {S.v[0], S.v[1], S.v[2], S.w[0], S.w[1], S.w[2], S.p[0], S.p[1], S.p[2], S.xt})
+        fill_body(it.y, 0, S.n, a, 0.0);
+}
+
+__global__ __launch_bounds__(TPB) void bk_sumsq(const int2* items, const BatchSys* sys, int32_t a)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    sumsq_body(it.y, S.G, S.n, batch_vec(S, a), S.part, 6);
+}
+
+__global__ __launch_bounds__(TPB) void bk_init_v(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    init_v_body(it.y, S.G, S.n, S.tv, S.v[0], S.z, S.tri, S.ip, S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_init_w(const int2* items, const BatchSys* sys)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    init_w_body(it.y, S.G, S.n, S.z, S.w[0], S.ev, S.scal, S.part, S.flags);
+    if(it.y == 0 && threadIdx.x == 0) *S.iter = 0;
+}
+
+__global__ __launch_bounds__(TPB) void bk_norm2_pair(const int2* items, const BatchSys* sys,
+    int32_t a, int32_t b)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    norm2_pair_body(it.y, S.G, S.n, batch_vec(S, a), batch_vec(S, b), nullptr, S.part);
+}
+
+__global__ __launch_bounds__(TPB) void bk_norm_final(const int2* items, const BatchSys* sys)
+{
+    const BatchSys& S = sys[items[blockIdx.x].x];
+    norm_final_body(S.G, S.part, S.scal + 40);
+}
+
+// elementwise steps over 256-row chunks; op 0 copy b = a, 1 b = D \ a,
+// 2 b = a - b, 3 b += a, 4 a = 0
+__global__ void bk_elementwise(const int2* items, const BatchSys* sys, int32_t op, int32_t a,
+    int32_t b)
+{
+    const int2 it = items[blockIdx.x];
+    const BatchSys& S = sys[it.x];
+    double* va = batch_vec(S, a);
+    double* vb = batch_vec(S, b);
+    switch(op) {
+    case 0: copy_body(it.y, 0, S.n, va, vb); break;
+    case 1: dsolve_body(it.y, 0, S.n, va, vb, S.tri, S.ip); break;
+    case 2: residual_body(it.y, 0, S.n, va, vb); break;
+    case 3: axpy1_body(it.y, 0, S.n, va, vb); break;
+    default: fill_body(it.y, 0, S.n, va, 0.0); break;
+    }
 }
 
 inline unsigned blocks_for(int32_t n) { return (unsigned) ((n + 255) / 256); }
@@ -703,7 +924,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     // long plateaus of the true residual are normal on barrier-conditioned
     // systems (a few hundred iterations); restarting inside one throws the
     // Krylov space away
-    constexpr int STALL_PERIODS = 25;
+    constexpr int STALL_PERIODS = SQMR_STALL_PERIODS;
     double best_nr = -1.0, gam_at_best = 0.0;
     int stall = 0;
     // the kernels of iteration `i`: they depend on i only through i mod 3
@@ -722,7 +943,7 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
         k_pupdate<<<G, TPB, 0, st>>>(n, i, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
             w.scal, w.part, w.flags);
         k_scalars<<<1, TPB, 0, st>>>(i, G, w.scal, w.part, w.flags, opt.tol,
-            period > 0 ? 1e-14 : -1.0);
+            period > 0 ? SQMR_BREAKDOWN_REL : -1.0);
         cur = nxt;
         HIPCHK(hipGetLastError());
     };
@@ -900,6 +1121,66 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
     res.restarts = restarts;
     res.true_rel_residual = rr;
     if(opt.true_residual_period > 0 || opt.max_restarts > 0) res.converged = rr <= opt.tol;
+}
+
+
+void batch_expand(const int4* ent, int32_t count, int2* items, hipStream_t st)
+{
+    if(count > 0) k_batch_expand<<<count, 256, 0, st>>>(ent, items);
+}
+
+void batch_step(BatchStep s, const BatchLists& L, const BatchSys* sys, int32_t a, int32_t b,
+    double tol, double brel, hipStream_t st)
+{
+    auto grid = [](const BatchList& l) { return (unsigned) l.count; };
+    switch(s) {
+    case BS_ALPHA:
+        if(L.vec.count) bk_alpha<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_VNEXT:
+        if(L.vec.count) bk_vnext<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_VSCALE:
+        if(L.vec.count) bk_vscale<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_BETA:
+        if(L.vec.count) bk_beta<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_PUPDATE:
+        if(L.vec.count) bk_pupdate<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_SCALARS:
+        if(L.sys.count) bk_scalars<<<grid(L.sys), TPB, 0, st>>>(L.sys.items, sys, tol, brel);
+        break;
+    case BS_INIT_FILL:
+        if(L.row.count) bk_init_fill<<<grid(L.row), 256, 0, st>>>(L.row.items, sys);
+        break;
+    case BS_SUMSQ:
+        if(L.vec.count) bk_sumsq<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys, a);
+        break;
+    case BS_INIT_V:
+        if(L.vec.count) bk_init_v<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_INIT_W:
+        if(L.vec.count) bk_init_w<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        break;
+    case BS_NORM2_PAIR:
+        if(L.vec.count) bk_norm2_pair<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys, a, b);
+        break;
+    case BS_NORM_FINAL:
+        if(L.sys.count) bk_norm_final<<<grid(L.sys), TPB, 0, st>>>(L.sys.items, sys);
+        break;
+    case BS_COPY:
+    case BS_DSOLVE:
+    case BS_RESIDUAL:
+    case BS_AXPY1:
+    case BS_ZERO: {
+        const int op = s == BS_COPY ? 0 : s == BS_DSOLVE ? 1 : s == BS_RESIDUAL ? 2 : s == BS_AXPY1 ? 3 : 4;
+        if(L.row.count) bk_elementwise<<<grid(L.row), 256, 0, st>>>(L.row.items, sys, op, a, b);
+        break;
+    }
+    }
+    HIPCHK(hipGetLastError());
 }
 
 }  // namespace culip

@@ -46,6 +46,15 @@ struct SqmrWork {
     double* host_scal = nullptr;    // pinned, 32 doubles
 };
 
+// scalar slots of SqmrWork::scal the host reads: the recurrence's gamma and
+// ||Pl \ b||; [40] and [41] hold the norms of a residual check
+constexpr int SQMR_S_GAMMA = 20, SQMR_S_NORMPB = 21;
+// the true-residual gate's restart rules: a run ends when the best true
+// residual has not halved for this many checks; Lanczos breakdown is
+// |delta| <= SQMR_BREAKDOWN_REL * ||A||est
+constexpr int SQMR_STALL_PERIODS = 25;
+constexpr double SQMR_BREAKDOWN_REL = 1e-14;
+
 int32_t sqmr_grid(int32_t n);
 size_t sqmr_work_bytes(int32_t n);
 void sqmr_work_bind(SqmrWork& w, int32_t n, char* dev_base, int32_t* host_flags,

@@ -293,6 +293,9 @@ SolvePlan<VT> Bildlt::plan() const
     return p;
 }
 
+template SolvePlan<float> Bildlt::plan<float>() const;
+template SolvePlan<double> Bildlt::plan<double>() const;
+
 SolveTopo Bildlt::topo() const
 {
     SolveTopo t;

@@ -11,8 +11,10 @@
 #include "kkt.hpp"
 #include "matching.hpp"
 #include "serve.hpp"
+#include "batch_solver.hpp"
 
 struct culip_server { std::unique_ptr<culip::Server> s; };
+struct culip_batch { std::unique_ptr<culip::BatchSolver> b; };
 struct culip_kkt { culip::Kkt* k; };
 struct culip_prep { culip::Prep* p; };
 struct culip_host_matrix { culip::HostCsr a; };
@@ -965,5 +967,61 @@ culip_status culip_server_stats(const culip_server* s, int64_t* out)
 }
 
 void culip_server_destroy(culip_server* s) { delete s; }
+
+culip_status culip_batch_create(const culip_batch_opts* o, culip_batch** out)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(o && out, "batch: null argument");
+        device_present();
+        BatchSolverOpts bo;
+        bo.prec = o->prec;
+        bo.fill_level = o->fill_level;
+        bo.fill_factor = o->fill_factor;
+        bo.threshold = o->threshold;
+        bo.sqmr = o->sqmr;
+        bo.jacobi_sweeps = o->jacobi_sweeps;
+        bo.factor_streams = o->factor_streams;
+        auto* h = new culip_batch;
+        try {
+            h->b = std::make_unique<BatchSolver>(bo);
+        } catch(...) { delete h; throw; }
+        *out = h;
+    });
+}
+
+culip_status culip_batch_add(culip_batch* b, const culip_csr* K, int32_t num_pivs,
+    const int32_t* piv_starts, int32_t num_blocks, const int32_t* block_starts, int32_t* index)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(b && K && index, "batch: null argument");
+        CULIP_REQUIRE(num_pivs > 0 && piv_starts, "batch: pivot starts required");
+        CULIP_REQUIRE(num_blocks > 0 && block_starts, "batch: block starts required");
+        HostCsr h = to_host(K);
+        *index = b->b->add(h, std::vector<int32_t>(piv_starts, piv_starts + num_pivs),
+            std::vector<int32_t>(block_starts, block_starts + num_blocks));
+    });
+}
+
+culip_status culip_batch_solve(culip_batch* b, const double* const* vals,
+    const double* const* rhs, double* const* x, culip_sqmr_result* res, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(b, "batch: null handle");
+        b->b->solve(vals, rhs, x, res, (hipStream_t) stream);
+    });
+}
+
+culip_status culip_batch_info(const culip_batch* b, int32_t* count, int64_t* device_bytes,
+    double* seconds)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(b, "batch: null handle");
+        if(count) *count = b->b->count();
+        if(device_bytes) *device_bytes = b->b->device_bytes();
+        if(seconds) b->b->last_seconds(seconds);
+    });
+}
+
+void culip_batch_destroy(culip_batch* b) { delete b; }
 
 }  // extern "C"
