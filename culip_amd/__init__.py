@@ -564,20 +564,22 @@ class Server(_Handle):
 
 class Batch(_Handle):
     """Independent systems refactored and solved together (culip_batch):
-    one launch per SQMR step over all systems still iterating, block-Jacobi
-    triangular solves with `jacobi_sweeps` sweeps. add(K, piv_starts,
+    each step of the refactor (a level of all systems) and of the SQMR
+    recurrence is one launch over all systems taking part, with block-Jacobi
+    triangular solves of `jacobi_sweeps` sweeps. add(K, piv_starts,
     block_starts) -> index (symbolic analysis and upload; thread-safe,
     before the first solve); solve(vals, rhs) -> (xs, results) for lists of
-    host arrays, one per system. Each system's x and result equal its solve
-    alone with the same options (BlockiLDLt with Jacobi sweeps + SQMR)."""
+    arrays, one per system: numpy arrays (host), or float64 CUDA tensors
+    (device; xs are then CUDA tensors). Each system's x and result equal its
+    solve alone with the same options (BlockiLDLt with Jacobi sweeps +
+    SQMR)."""
 
     def __init__(self, prec="f32", fill_level=0, fill_factor=1.0, threshold=1e-4, tol=1e-8,
-                 max_it=1000, true_residual_period=20, max_restarts=10, jacobi_sweeps=2,
-                 factor_streams=16):
+                 max_it=1000, true_residual_period=20, max_restarts=10, jacobi_sweeps=2):
         pr = PREC_F32 if prec in ("f32", PREC_F32) else PREC_F64
         o = _lib.BatchOpts(pr, fill_level, fill_factor, threshold,
                            _lib.SqmrOpts(tol, max_it, true_residual_period, max_restarts),
-                           jacobi_sweeps, factor_streams)
+                           jacobi_sweeps)
         h = C.c_void_p()
         check(lib().culip_batch_create(C.byref(o), C.byref(h)))
         self._h = h
@@ -598,15 +600,30 @@ class Batch(_Handle):
         if len(vals) != S or len(rhs) != S:
             raise ValueError(f"batch: {S} systems, got {len(vals)} value and {len(rhs)} "
                              "right-hand-side arrays")
-        v = [_f64(a) for a in vals]
-        b = [_f64(a) for a in rhs]
-        xs = [np.empty(self._n[i]) for i in range(S)]
         P = C.POINTER(C.c_double)
-        pv = (P * S)(*[_ptr(a, C.c_double) for a in v])
-        pb = (P * S)(*[_ptr(a, C.c_double) for a in b])
-        px = (P * S)(*[_ptr(a, C.c_double) for a in xs])
+        device = S > 0 and hasattr(vals[0], "is_cuda")
+        if device:
+            import torch
+            for a in list(vals) + list(rhs):
+                if not (a.is_cuda and a.dtype == torch.float64 and a.is_contiguous()):
+                    raise ValueError("batch: device arrays must be contiguous float64 CUDA "
+                                     "tensors")
+            xs = [torch.empty(self._n[i], dtype=torch.float64, device=vals[0].device)
+                  for i in range(S)]
+            addr = lambda a: C.cast(C.c_void_p(a.data_ptr()), P)
+            keep = (vals, rhs)
+        else:
+            v = [_f64(a) for a in vals]
+            b = [_f64(a) for a in rhs]
+            xs = [np.empty(self._n[i]) for i in range(S)]
+            addr = lambda a: _ptr(a, C.c_double)
+            keep = (v, b)
+        pv = (P * S)(*[addr(a) for a in keep[0]])
+        pb = (P * S)(*[addr(a) for a in keep[1]])
+        px = (P * S)(*[addr(a) for a in xs])
         res = (_lib.SqmrResult * S)()
-        check(lib().culip_batch_solve(self._h, pv, pb, px, res, _stream(stream)))
+        check(lib().culip_batch_solve(self._h, pv, pb, px, res, 1 if device else 0,
+                                      _stream(stream)))
         out = [{"residual": r.residual, "iterations": r.iterations,
                 "converged": bool(r.converged), "restarts": r.restarts,
                 "true_rel_residual": r.true_rel_residual} for r in res]

@@ -61,8 +61,7 @@ class ServerOpts(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("prec", C.c_int), ("fill_level", C.c_int32), ("fill_factor", C.c_double),
-                ("threshold", C.c_double), ("sqmr", SqmrOpts), ("jacobi_sweeps", C.c_int32),
-                ("factor_streams", C.c_int32)]
+                ("threshold", C.c_double), ("sqmr", SqmrOpts), ("jacobi_sweeps", C.c_int32)]
 
 
 _lib = None
@@ -168,7 +167,7 @@ _SIGS = [
     ("culip_batch_solve", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(C.c_double)),
                                     C.POINTER(C.POINTER(C.c_double)),
                                     C.POINTER(C.POINTER(C.c_double)), C.POINTER(SqmrResult),
-                                    C.c_void_p]),
+                                    C.c_int32, C.c_void_p]),
     ("culip_batch_info", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_double)]),
     ("culip_batch_destroy", None, [C.c_void_p]),

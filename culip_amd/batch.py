@@ -142,16 +142,18 @@ def autotune_batch(sizes, make_system, prec="f32", block_sizes=BLOCK_SIZES, swee
 
 
 def solve_batch_engine(systems, prec, sweeps, tol=1e-8, max_it=1000, passes=2, log=None,
-                       factor_streams=16):
+                       device_arrays=True):
     """Solves every system (prep, rhs) on one Batch. Registration (symbolic
     analysis and upload, 16 threads) is timed apart; each pass refactors
     and solves all systems; the first pass also allocates the batch's
-    pooled work space. Returns (results of the last pass as (x, result)
+    pooled work space and builds its launch lists. device_arrays: values,
+    right-hand sides and solutions stay in device memory (copied there
+    before the passes), else every pass takes them from host memory and
+    returns x to it. Returns (results of the last pass as (x, result)
     per system, seconds per pass, registration seconds, Batch.info() after
     the last pass)."""
     import concurrent.futures as cf
-    bt = Batch(prec=prec, tol=tol, max_it=max_it, jacobi_sweeps=sweeps,
-               factor_streams=factor_streams)
+    bt = Batch(prec=prec, tol=tol, max_it=max_it, jacobi_sweeps=sweeps)
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(16) as ex:
         idx = list(ex.map(lambda s: bt.add(s[0].row_ptr, s[0].col, s[0].val, s[0].piv_starts,
@@ -162,6 +164,13 @@ def solve_batch_engine(systems, prec, sweeps, tol=1e-8, max_it=1000, passes=2, l
     for j, i in enumerate(idx):
         vals[i] = systems[j][0].val
         rhs[i] = systems[j][1]
+    if device_arrays:
+        import torch
+        vals = [torch.tensor(np.ascontiguousarray(v), dtype=torch.float64, device="cuda:0")
+                for v in vals]
+        rhs = [torch.tensor(np.ascontiguousarray(r), dtype=torch.float64, device="cuda:0")
+               for r in rhs]
+        torch.cuda.synchronize()
     secs = []
     xs = res = None
     for rep in range(passes):
@@ -175,6 +184,8 @@ def solve_batch_engine(systems, prec, sweeps, tol=1e-8, max_it=1000, passes=2, l
                 f"SQMR {inf['sqmr_s'] * 1e3:.0f} ms)")
     phases = bt.info()
     bt.close()
+    if device_arrays:
+        xs = [x.cpu().numpy() for x in xs]
     return [(xs[idx[j]], res[idx[j]]) for j in range(S)], secs, t_reg, phases
 
 

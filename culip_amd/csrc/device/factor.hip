@@ -831,11 +831,12 @@ __device__ void offdiag_general_one(const FactorArgs<VT>& a, const LevelRange& l
     OffLds<VT>& s);
 
 template<typename VT>
-__global__ __launch_bounds__(64) void k_offdiag_general(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void offdiag_general_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ OffLds<VT> s;
     const int ndef = a.defer_cnt[lr.level];
-    for(int t = blockIdx.x; t < ndef; t += gridDim.x) {
+    for(int t = bid_; t < ndef; t += nblk_) {
         offdiag_general_one(a, lr, a.defer_list[t], s);
         __syncthreads();
     }
@@ -1225,11 +1226,12 @@ __device__ void offdiag_small_one(const FactorArgs<VT>& a, const LevelRange& lr,
 
 // the positions k_offdiag_tiny leaves to the wave form, grid-stride
 template<typename VT>
-__global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void offdiag_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ OffLdsSmall<VT> s;
     const int nw = a.wave_cnt[lr.level];
-    for(int t = blockIdx.x; t < nw; t += gridDim.x) {
+    for(int t = bid_; t < nw; t += nblk_) {
         offdiag_small_one(a, lr, a.wave_list[t], s);
         wave_sync();
     }
@@ -1249,7 +1251,6 @@ __global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange 
 // zeros in the wave form are skipped).
 constexpr int TINY_E = 8;
 constexpr int TINY_T = 32;
-constexpr int TINY_TPB = 64;
 
 template<typename VT>
 struct TinyLds {
@@ -1524,11 +1525,12 @@ __device__ void tiny_pairs_one(const FactorArgs<VT>& a, const LevelRange& lr, co
 // list of the first phase through U_ii' and D, then the row partial sums and
 // the compact entries.
 template<typename VT>
-__global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny_scale(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void offdiag_tiny_scale_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ TinyLds<VT> s;
     const int tid = threadIdx.x;
-    const int q = lr.npiv + (int) blockIdx.x * TINY_TPB + tid;
+    const int q = lr.npiv + bid_ * TINY_TPB + tid;
     if(q >= lr.size) return;
     // the parked count and the task record together (neither waits for
     // the other)
@@ -1649,10 +1651,11 @@ union PivPairsLds {
 };
 
 template<typename VT>
-__global__ __launch_bounds__(64) void k_pivots_pairs(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void pivots_pairs_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ PivPairsLds<VT> u;
-    const int w = blockIdx.x;
+    const int w = bid_;
     if(w < lr.npiv) {
         pivots_one<VT>(a, lr, w, u.piv);
         return;
@@ -1674,10 +1677,11 @@ __device__ __forceinline__ VT drop_norm(VT rn) { return (VT) sqrtf((float) rn); 
 // row sums of squares: pivot block first, then the off-diagonal blocks of
 // the row by block id (contiguous in the level list at row_first_block_offset)
 template<typename VT>
-__global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void rownorm_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     const int lane = lane_id();
-    const int b = a.lvlb_idx[lr.lo + blockIdx.x];
+    const int b = a.lvlb_idx[lr.lo + bid_];
     const int i = a.bin[b].block_row;
     const int fr = a.bs[i];
     const int h = a.bs[i + 1] - fr;
@@ -1797,7 +1801,8 @@ __device__ void wait_chunk_flags(const int32_t* flags, int c, int32_t* err)
 }
 
 template<typename VT>
-__global__ __launch_bounds__(SC_TPB) void k_level_scan(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void level_scan_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ PoolMap shm[SC_NW];
     __shared__ Sizes shs[SC_NW];
@@ -1947,7 +1952,7 @@ __global__ __launch_bounds__(SC_TPB) void k_level_scan(FactorArgs<VT> a, LevelRa
         pv += nv;
         a.bout[o.id] = o;
     }
-    if(threadIdx.x == 0 && c == (int) gridDim.x - 1) {
+    if(threadIdx.x == 0 && c == nblk_ - 1) {
         cout[0] = (int32_t) map_apply(mtot, pool_c);
         cout[1] = (int32_t) (cin[1] + sbefore.x + stot.x);
         cout[2] = (int32_t) (cin[2] + sbefore.v + stot.v);
@@ -2027,13 +2032,14 @@ __device__ void store_one(const FactorArgs<VT>& a, const LevelRange& lr, int q, 
 
 // the positions k_store_tiny leaves to the wave form, grid-stride
 template<typename VT>
-__global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void store_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
     __shared__ VT nrm[32];
     __shared__ VT key[1024];
     __shared__ uint16_t kix[1024];
     const int ns = a.store_cnt[lr.level];
-    for(int t = blockIdx.x; t < ns; t += gridDim.x) {
+    for(int t = bid_; t < ns; t += nblk_) {
         store_one(a, lr, a.store_list[t], nrm, key, kix);
         wave_sync();
     }
@@ -2044,9 +2050,10 @@ __global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
 constexpr int STORE_TINY = 16;
 
 template<typename VT>
-__global__ __launch_bounds__(256) void k_store_tiny(FactorArgs<VT> a, LevelRange lr)
+__device__ __forceinline__ void store_tiny_body(const FactorArgs<VT>& a, const LevelRange& lr,
+    int bid_, int nblk_)
 {
-    const int q = (int) blockIdx.x * 256 + (int) threadIdx.x;
+    const int q = bid_ * 256 + (int) threadIdx.x;
     if(q >= lr.size) return;
     const int cap = a.lq_cap[lr.lo + q];
     const int cnt = a.if_cnt[q];
@@ -2107,6 +2114,55 @@ __global__ __launch_bounds__(256) void k_store_tiny(FactorArgs<VT> a, LevelRange
 // its pivot block (earlier levels, final storage) take its pivoting order in
 // their columns: sparse entries get new column indices in place, dense rows
 // are gathered column by column.
+// the single-system kernels: the bodies at their own grid position
+template<typename VT>
+__global__ __launch_bounds__(64) void k_offdiag_general(FactorArgs<VT> a, LevelRange lr)
+{
+    offdiag_general_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64, 3) void k_offdiag(FactorArgs<VT> a, LevelRange lr)
+{
+    offdiag_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(TINY_TPB) void k_offdiag_tiny_scale(FactorArgs<VT> a, LevelRange lr)
+{
+    offdiag_tiny_scale_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_pivots_pairs(FactorArgs<VT> a, LevelRange lr)
+{
+    pivots_pairs_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_rownorm(FactorArgs<VT> a, LevelRange lr)
+{
+    rownorm_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void k_level_scan(FactorArgs<VT> a, LevelRange lr)
+{
+    level_scan_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void k_store(FactorArgs<VT> a, LevelRange lr)
+{
+    store_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(256) void k_store_tiny(FactorArgs<VT> a, LevelRange lr)
+{
+    store_tiny_body<VT>(a, lr, blockIdx.x, gridDim.x);
+}
+
 template<typename VT>
 __global__ __launch_bounds__(64) void k_pivot_cols(FactorArgs<VT> a, LevelRange lr)
 {
@@ -2160,6 +2216,125 @@ __global__ void k_zero(VT* a, int64_t n)
     if(q < n) a[q] = (VT) 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched refactor: one launch per step covers every listed system; item
+// {s, q} is workgroup q of the single-system launch on system s (the same
+// grid position and grid size, so the same results).
+
+// factor_reset_pivots, the zeroing of gather_values and factor_begin
+template<typename VT>
+__global__ void bk_factor_begin(const FactorSys<VT>* fs, const int2* items)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const FactorArgs<VT>& a = F.a;
+    const int64_t g = (int64_t) it.y * 256 + threadIdx.x;
+    if(g < a.n) {
+        a.is_piv[g] = F.static_is_piv[g];
+        a.perm[g] = (int32_t) g;
+    }
+    if(g < F.val_space) F.val_in[g] = (VT) 0;
+    const int64_t nf = (a.num_levels > 0 && a.scan_maxch > 0)
+        ? (int64_t) (2 * a.scan_maxch + 1) * a.num_levels : 0;
+    if(g < nf) a.scan_flags[g] = 0;
+    if(g < a.num_levels) {
+        a.defer_cnt[g] = 0;
+        a.wave_cnt[g] = 0;
+        a.store_cnt[g] = 0;
+    }
+    if(it.y == 0 && threadIdx.x < 8) a.carry[threadIdx.x] = 0;
+}
+
+// the gather of gather_values
+template<typename VT>
+__global__ void bk_gather(const FactorSys<VT>* fs, const int2* items)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const int64_t q = (int64_t) it.y * 256 + threadIdx.x;
+    if(q < F.nnz && F.value_slot[q] >= 0) F.val_in[F.value_slot[q]] = (VT) F.vals[q];
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void bk_pivots_pairs(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    pivots_pairs_body<VT>(F.a, lr, it.y, fl_grid_pivots_pairs(lr));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(TINY_TPB) void bk_offdiag_tiny_scale(const FactorSys<VT>* fs,
+    const int2* items, int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    offdiag_tiny_scale_body<VT>(F.a, lr, it.y, fl_tiny_blocks(lr));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64, 3) void bk_offdiag(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    offdiag_body<VT>(F.a, lr, it.y, fl_grid_tasks(lr.size - lr.npiv, BATCH_TASK_GRID));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void bk_offdiag_general(const FactorSys<VT>* fs,
+    const int2* items, int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    offdiag_general_body<VT>(F.a, lr, it.y, fl_grid_tasks(lr.size - lr.npiv, BATCH_TASK_GRID));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void bk_rownorm(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    rownorm_body<VT>(F.a, lr, it.y, lr.npiv);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(SC_TPB) void bk_level_scan(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    level_scan_body<VT>(F.a, lr, it.y, fl_grid_level_scan(lr));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(256) void bk_store_tiny(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    store_tiny_body<VT>(F.a, lr, it.y, fl_grid_store_tiny(lr));
+}
+
+template<typename VT>
+__global__ __launch_bounds__(64) void bk_store(const FactorSys<VT>* fs, const int2* items,
+    int32_t level)
+{
+    const int2 it = items[blockIdx.x];
+    const FactorSys<VT>& F = fs[it.x];
+    const LevelRange lr = F.lr[level];
+    store_body<VT>(F.a, lr, it.y, fl_grid_tasks(lr.size, BATCH_TASK_GRID));
+}
+
 }  // namespace
 
 __global__ void k_reset_pivots(int32_t n, const int32_t* sp, int32_t* is_piv, int32_t* perm)
@@ -2210,20 +2385,20 @@ void factor_level(const FactorArgs<VT>& a, const LevelRange& r, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
     const int noff = r.size - r.npiv;
-    const int tiny_blocks = noff > 0 ? (noff + TINY_TPB - 1) / TINY_TPB : 0;
-    if(r.npiv + tiny_blocks > 0) k_pivots_pairs<VT><<<r.npiv + tiny_blocks, 64, 0, st>>>(a, r);
+    const int tiny_blocks = fl_tiny_blocks(r);
+    if(fl_grid_pivots_pairs(r) > 0)
+        k_pivots_pairs<VT><<<fl_grid_pivots_pairs(r), 64, 0, st>>>(a, r);
     if(noff > 0) {
-        const int grid = noff < 2048 ? noff : 2048;
+        const int grid = fl_grid_tasks(noff, 2048);
         k_offdiag_tiny_scale<VT><<<tiny_blocks, TINY_TPB, 0, st>>>(a, r);
         k_offdiag<VT><<<grid, 64, 0, st>>>(a, r);
         k_offdiag_general<VT><<<grid, 64, 0, st>>>(a, r);
     }
     if(a.pivoting && r.npiv > 0) k_pivot_cols<VT><<<r.npiv, 64, 0, st>>>(a, r);
     if(r.npiv > 0) k_rownorm<VT><<<r.npiv, 64, 0, st>>>(a, r);
-    const unsigned nch = (unsigned) ((r.size + SCAN_CHUNK - 1) / SCAN_CHUNK);
-    k_level_scan<VT><<<nch, SC_TPB, 0, st>>>(a, r);
-    k_store_tiny<VT><<<(r.size + 255) / 256, 256, 0, st>>>(a, r);
-    k_store<VT><<<r.size < 2048 ? r.size : 2048, 64, 0, st>>>(a, r);
+    k_level_scan<VT><<<fl_grid_level_scan(r), SC_TPB, 0, st>>>(a, r);
+    k_store_tiny<VT><<<fl_grid_store_tiny(r), 256, 0, st>>>(a, r);
+    k_store<VT><<<fl_grid_tasks(r.size, 2048), 64, 0, st>>>(a, r);
 }
 
 template void gather_values<double>(const double*, const int64_t*, int64_t, double*, int64_t, void*);
@@ -2232,5 +2407,48 @@ template void factor_begin<double>(const FactorArgs<double>&, void*);
 template void factor_begin<float>(const FactorArgs<float>&, void*);
 template void factor_level<double>(const FactorArgs<double>&, const LevelRange&, void*);
 template void factor_level<float>(const FactorArgs<float>&, const LevelRange&, void*);
+
+
+int64_t factor_batch_chunks(int32_t n, int64_t nnz, int64_t val_space, int32_t num_levels,
+    int32_t scan_maxch)
+{
+    int64_t m = std::max<int64_t>(std::max<int64_t>(n, nnz), std::max<int64_t>(val_space, 1));
+    if(num_levels > 0 && scan_maxch > 0)
+        m = std::max<int64_t>(m, (int64_t) (2 * scan_maxch + 1) * num_levels);
+    m = std::max<int64_t>(m, num_levels);
+    return (m + 255) / 256;
+}
+
+template<typename VT>
+void factor_batch_begin(const FactorSys<VT>* fs, FactorItems chunks, hipStream_t st)
+{
+    if(chunks.count <= 0) return;
+    bk_factor_begin<VT><<<(unsigned) chunks.count, 256, 0, st>>>(fs, chunks.items);
+    bk_gather<VT><<<(unsigned) chunks.count, 256, 0, st>>>(fs, chunks.items);
+}
+
+// the launches of factor_level, in its order, each over the level's items
+// of every system that has the level
+template<typename VT>
+void factor_batch_level(const FactorSys<VT>* fs, const FactorBatchLevel& L, int32_t level,
+    hipStream_t st)
+{
+    auto grid = [](const FactorItems& f) { return (unsigned) f.count; };
+    if(L.pp.count) bk_pivots_pairs<VT><<<grid(L.pp), 64, 0, st>>>(fs, L.pp.items, level);
+    if(L.ts.count) bk_offdiag_tiny_scale<VT><<<grid(L.ts), TINY_TPB, 0, st>>>(fs, L.ts.items, level);
+    if(L.off.count) {
+        bk_offdiag<VT><<<grid(L.off), 64, 0, st>>>(fs, L.off.items, level);
+        bk_offdiag_general<VT><<<grid(L.off), 64, 0, st>>>(fs, L.off.items, level);
+    }
+    if(L.rn.count) bk_rownorm<VT><<<grid(L.rn), 64, 0, st>>>(fs, L.rn.items, level);
+    if(L.sc.count) bk_level_scan<VT><<<grid(L.sc), SC_TPB, 0, st>>>(fs, L.sc.items, level);
+    if(L.stt.count) bk_store_tiny<VT><<<grid(L.stt), 256, 0, st>>>(fs, L.stt.items, level);
+    if(L.st.count) bk_store<VT><<<grid(L.st), 64, 0, st>>>(fs, L.st.items, level);
+}
+
+template void factor_batch_begin<double>(const FactorSys<double>*, FactorItems, hipStream_t);
+template void factor_batch_begin<float>(const FactorSys<float>*, FactorItems, hipStream_t);
+template void factor_batch_level<double>(const FactorSys<double>*, const FactorBatchLevel&, int32_t, hipStream_t);
+template void factor_batch_level<float>(const FactorSys<float>*, const FactorBatchLevel&, int32_t, hipStream_t);
 
 }  // namespace culip

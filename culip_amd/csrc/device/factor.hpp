@@ -3,6 +3,8 @@
 
 #include <cstdint>
 
+#include <hip/hip_runtime.h>
+
 #include "../host/symbolic.hpp"
 
 namespace culip {
@@ -96,7 +98,77 @@ struct LevelRange {
     int32_t level;     // level index
 };
 
+// threads of a k_offdiag_tiny_scale workgroup (one per one-lane block)
+constexpr int TINY_TPB = 64;
+// per-system grid of the loops over device-counted tasks (k_offdiag,
+// k_offdiag_general, k_store) in a batched refactor: each task is one
+// workgroup's work, so the grid changes no result
+constexpr int BATCH_TASK_GRID = 64;
+
+// Workgroups of the launches of one level (factor_level); the batched
+// refactor launches the same workgroups for every system of a batch.
+__host__ __device__ inline int32_t fl_tiny_blocks(const LevelRange& r)
+{
+    const int32_t noff = r.size - r.npiv;
+    return noff > 0 ? (noff + TINY_TPB - 1) / TINY_TPB : 0;
+}
+__host__ __device__ inline int32_t fl_grid_pivots_pairs(const LevelRange& r)
+{
+    return r.npiv + fl_tiny_blocks(r);
+}
+__host__ __device__ inline int32_t fl_grid_tasks(int32_t count, int32_t cap)
+{
+    return count > 0 ? (count < cap ? count : cap) : 0;
+}
+__host__ __device__ inline int32_t fl_grid_level_scan(const LevelRange& r)
+{
+    return (r.size + SCAN_CHUNK - 1) / SCAN_CHUNK;
+}
+__host__ __device__ inline int32_t fl_grid_store_tiny(const LevelRange& r)
+{
+    return (r.size + 255) / 256;
+}
+
+// One system of a batched refactor (device table entry).
+template<typename VT>
+struct FactorSys {
+    FactorArgs<VT> a;
+    const LevelRange* lr = nullptr;       // per level (device)
+    const double* vals = nullptr;         // K's values
+    const int64_t* value_slot = nullptr;  // their slots in val_in (or -1)
+    int64_t nnz = 0, val_space = 0;
+    VT* val_in = nullptr;
+    const int32_t* static_is_piv = nullptr;
+};
+
+// work items {system, workgroup of that system} of one batched launch
+struct FactorItems {
+    const int2* items = nullptr;
+    int32_t count = 0;
+};
+// the launches of one level of a batched refactor
+struct FactorBatchLevel {
+    FactorItems pp;    // k_pivots_pairs
+    FactorItems ts;    // k_offdiag_tiny_scale
+    FactorItems off;   // k_offdiag and k_offdiag_general (BATCH_TASK_GRID per system)
+    FactorItems rn;    // k_rownorm
+    FactorItems sc;    // k_level_scan
+    FactorItems stt;   // k_store_tiny
+    FactorItems st;    // k_store (BATCH_TASK_GRID per system)
+};
+
 // Host entry points (factor.hip). All launches go to `stream`.
+// Batched refactor: the start of every listed system's factor (pivot
+// starts, zeroed and gathered input values, zeroed level counters), over
+// chunk items {system, 256-entry chunk}; then one level of all systems.
+template<typename VT>
+void factor_batch_begin(const FactorSys<VT>* fs, FactorItems chunks, hipStream_t st);
+template<typename VT>
+void factor_batch_level(const FactorSys<VT>* fs, const FactorBatchLevel& L, int32_t level,
+    hipStream_t st);
+// chunks of 256 entries factor_batch_begin needs for a system
+int64_t factor_batch_chunks(int32_t n, int64_t nnz, int64_t val_space, int32_t num_levels,
+    int32_t scan_maxch);
 template<typename VT>
 void factor_begin(const FactorArgs<VT>& a, void* stream);
 template<typename VT>

@@ -128,11 +128,12 @@ __device__ __forceinline__ void plan_rank(const PlanStage<VT>& sg, int nblk, int
 }
 
 template<typename VT, bool FILL>
-__global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+__device__ __forceinline__ void plan_rows_body(const SolveTopo& t, const VT* val_out,
+    const SolvePlan<VT>& p, int bid_)
 {
     __shared__ double X[1024];
     const int lane = lane_id();
-    const int i = blockIdx.x;
+    const int i = bid_;
     const int fr = t.bs[i];
     const int h = t.bs[i + 1] - fr;
     // pivot block inverse rows (ui) and columns (ti)
@@ -225,11 +226,18 @@ __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out
 }
 
 template<typename VT, bool FILL>
-__global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+__global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+{
+    plan_rows_body<VT, FILL>(t, val_out, p, blockIdx.x);
+}
+
+template<typename VT, bool FILL>
+__device__ __forceinline__ void plan_cols_body(const SolveTopo& t, const VT* val_out,
+    const SolvePlan<VT>& p, int bid_)
 {
     __shared__ PlanStage<VT> sg;
     const int lane = lane_id();
-    const int j = blockIdx.x;
+    const int j = bid_;
     const int fc = t.bs[j];
     const int w = t.bs[j + 1] - fc;
     int cnt = 0;
@@ -297,6 +305,12 @@ __global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out
     if(!FILL && lane < w) p.to_ptr[fc + lane + 1] = cnt;
 }
 
+template<typename VT, bool FILL>
+__global__ __launch_bounds__(64) void k_plan_cols(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
+{
+    plan_cols_body<VT, FILL>(t, val_out, p, blockIdx.x);
+}
+
 // inclusive scan of a[1..n] in place (a[0] = 0), one workgroup, in chunks
 // prefix sums of a[1..n] in place, in chunks of PS_TPB * PS_IPT entries:
 // chunk totals first, then each chunk adds the totals before it
@@ -312,25 +326,30 @@ struct ISumOp {
     __device__ ISum operator()(const ISum& p, const ISum& q) const { return {p.s + q.s}; }
 };
 
-__global__ __launch_bounds__(PS_TPB) void k_chunk_sums(const int32_t* a, int32_t n, int32_t* tmp)
+__device__ __forceinline__ void chunk_sums_body(const int32_t* a, int32_t n, int32_t* tmp, int bid_)
 {
     __shared__ ISum sh[PS_TPB / WAVE];
     int32_t v = 0;
 #pragma unroll
     for(int k = 0; k < PS_IPT; ++k) {
-        const int64_t i = 1 + (int64_t) blockIdx.x * PS_CHUNK + (int64_t) threadIdx.x * PS_IPT + k;
+        const int64_t i = 1 + (int64_t) bid_ * PS_CHUNK + (int64_t) threadIdx.x * PS_IPT + k;
         if(i <= n) v += a[i];
     }
     ISum tot;
     (void) block_exclusive_scan<ISum, ISumOp, PS_TPB / WAVE>(ISum{v}, ISumOp(), ISum{0}, sh, &tot);
-    if(threadIdx.x == 0) tmp[blockIdx.x] = tot.s;
+    if(threadIdx.x == 0) tmp[bid_] = tot.s;
 }
 
-__global__ __launch_bounds__(PS_TPB) void k_chunk_scan(int32_t* a, int32_t n, const int32_t* tmp,
-    int32_t* total)
+__global__ __launch_bounds__(PS_TPB) void k_chunk_sums(const int32_t* a, int32_t n, int32_t* tmp)
+{
+    chunk_sums_body(a, n, tmp, blockIdx.x);
+}
+
+__device__ __forceinline__ void chunk_scan_body(int32_t* a, int32_t n, const int32_t* tmp,
+    int32_t* total, int bid_, int nblk_)
 {
     __shared__ ISum sh[PS_TPB / WAVE];
-    const int c = blockIdx.x;
+    const int c = bid_;
     // totals of the chunks before this one
     int32_t pre = 0;
     for(int q = threadIdx.x; q < c; q += PS_TPB) pre += tmp[q];
@@ -354,21 +373,33 @@ __global__ __launch_bounds__(PS_TPB) void k_chunk_scan(int32_t* a, int32_t n, co
     }
     if(threadIdx.x == 0) {
         if(c == 0) a[0] = 0;
-        if(c == (int) gridDim.x - 1 && total) *total = ptot.s + tot.s;
+        if(c == nblk_ - 1 && total) *total = ptot.s + tot.s;
     }
+}
+
+__global__ __launch_bounds__(PS_TPB) void k_chunk_scan(int32_t* a, int32_t n, const int32_t* tmp,
+    int32_t* total)
+{
+    chunk_scan_body(a, n, tmp, total, blockIdx.x, gridDim.x);
 }
 
 // task descriptors of the block rows in level order, two int4 each:
 // {fr, h, off-diagonal begin, end}, {inverse begin, end, 0, 0}
-__global__ void k_plan_tasks(int32_t nb, const int32_t* lvl_rows, const int32_t* bs,
-    const int32_t* o_ptr, const int32_t* i_ptr, int4* task)
+__device__ __forceinline__ void plan_tasks_body(int32_t nb, const int32_t* lvl_rows,
+    const int32_t* bs, const int32_t* o_ptr, const int32_t* i_ptr, int4* task, int bid_)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bid_ * blockDim.x + threadIdx.x;
     if(t >= nb) return;
     const int i = lvl_rows[t];
     const int fr = bs[i], h = bs[i + 1] - fr;
     task[2 * t] = make_int4(fr, h, o_ptr[fr], o_ptr[fr + h]);
     task[2 * t + 1] = make_int4(i_ptr[fr], i_ptr[fr + h], 0, 0);
+}
+
+__global__ void k_plan_tasks(int32_t nb, const int32_t* lvl_rows, const int32_t* bs,
+    const int32_t* o_ptr, const int32_t* i_ptr, int4* task)
+{
+    plan_tasks_body(nb, lvl_rows, bs, o_ptr, i_ptr, task, blockIdx.x);
 }
 
 // res = U_ii^-1 y for one block row (y, res: LDS, res zeroed; all lanes
@@ -799,6 +830,60 @@ __global__ __launch_bounds__(64) void bk_jacobi_sweep(const int2* items, const B
         batch_vec(S, b), xin, batch_vec(S, out), y, res);
 }
 
+// Batched layout build: item {s, q} is workgroup q of the single-system
+// launch on system s.
+template<typename VT, bool FILL>
+__global__ __launch_bounds__(64) void bk_plan_rows(const PlanSys<VT>* ps, const int2* items)
+{
+    const int2 it = items[blockIdx.x];
+    const PlanSys<VT>& P = ps[it.x];
+    plan_rows_body<VT, FILL>(P.t, P.val_out, P.p, it.y);
+}
+
+template<typename VT, bool FILL>
+__global__ __launch_bounds__(64) void bk_plan_cols(const PlanSys<VT>* ps, const int2* items)
+{
+    const int2 it = items[blockIdx.x];
+    const PlanSys<VT>& P = ps[it.x];
+    plan_cols_body<VT, FILL>(P.t, P.val_out, P.p, it.y);
+}
+
+template<typename VT>
+__device__ __forceinline__ int32_t* plan_ptr(const SolvePlan<VT>& p, int q)
+{
+    return q == 0 ? p.uo_ptr : q == 1 ? p.ui_ptr : q == 2 ? p.to_ptr : p.ti_ptr;
+}
+
+template<typename VT>
+__global__ __launch_bounds__(PS_TPB) void bk_chunk_sums(const PlanSys<VT>* ps, const int2* items,
+    int32_t q)
+{
+    const int2 it = items[blockIdx.x];
+    const PlanSys<VT>& P = ps[it.x];
+    chunk_sums_body(plan_ptr(P.p, q), P.t.n, P.scan_tmp, it.y);
+}
+
+template<typename VT>
+__global__ __launch_bounds__(PS_TPB) void bk_chunk_scan(const PlanSys<VT>* ps, const int2* items,
+    int32_t q)
+{
+    const int2 it = items[blockIdx.x];
+    const PlanSys<VT>& P = ps[it.x];
+    chunk_scan_body(plan_ptr(P.p, q), P.t.n, P.scan_tmp, P.totals + q, it.y,
+        (P.t.n + PS_CHUNK - 1) / PS_CHUNK);
+}
+
+template<typename VT>
+__global__ void bk_plan_tasks(const PlanSys<VT>* ps, const int2* items, int32_t transpose)
+{
+    const int2 it = items[blockIdx.x];
+    const PlanSys<VT>& P = ps[it.x];
+    if(transpose)
+        plan_tasks_body(P.t.nb, P.t.lvl_rows, P.t.bs, P.p.to_ptr, P.p.ti_ptr, P.p.task_t, it.y);
+    else
+        plan_tasks_body(P.t.nb, P.t.lvl_rows, P.t.bs, P.p.uo_ptr, P.p.ui_ptr, P.p.task_u, it.y);
+}
+
 __global__ void k_fill_bits(double* p, int64_t n, unsigned long long bits)
 {
     const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -1078,5 +1163,29 @@ void batch_jacobi(const BatchLists& L, const BatchSys* sys, bool transpose, int3
 
 template void batch_jacobi<double>(const BatchLists&, const BatchSys*, bool, int32_t, int32_t, int32_t, hipStream_t);
 template void batch_jacobi<float>(const BatchLists&, const BatchSys*, bool, int32_t, int32_t, int32_t, hipStream_t);
+
+
+int32_t scan_chunks(int32_t n) { return (n + PS_CHUNK - 1) / PS_CHUNK; }
+
+template<typename VT>
+void plan_batch(const PlanSys<VT>* ps, FactorItems blk, FactorItems scan, FactorItems task,
+    hipStream_t st)
+{
+    if(blk.count <= 0) return;
+    const unsigned nb = (unsigned) blk.count;
+    bk_plan_rows<VT, false><<<nb, 64, 0, st>>>(ps, blk.items);
+    bk_plan_cols<VT, false><<<nb, 64, 0, st>>>(ps, blk.items);
+    for(int32_t q = 0; q < 4; ++q) {
+        bk_chunk_sums<VT><<<(unsigned) scan.count, PS_TPB, 0, st>>>(ps, scan.items, q);
+        bk_chunk_scan<VT><<<(unsigned) scan.count, PS_TPB, 0, st>>>(ps, scan.items, q);
+    }
+    bk_plan_rows<VT, true><<<nb, 64, 0, st>>>(ps, blk.items);
+    bk_plan_cols<VT, true><<<nb, 64, 0, st>>>(ps, blk.items);
+    bk_plan_tasks<VT><<<(unsigned) task.count, 256, 0, st>>>(ps, task.items, 0);
+    bk_plan_tasks<VT><<<(unsigned) task.count, 256, 0, st>>>(ps, task.items, 1);
+}
+
+template void plan_batch<double>(const PlanSys<double>*, FactorItems, FactorItems, FactorItems, hipStream_t);
+template void plan_batch<float>(const PlanSys<float>*, FactorItems, FactorItems, FactorItems, hipStream_t);
 
 }  // namespace culip

@@ -58,6 +58,25 @@ void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT* dinv, Solv
 size_t scan_tmp_ints(int32_t n);
 void scan_counts(int32_t* ptr, int32_t n, int32_t* total_dev, int32_t* tmp, void* stream);
 
+// chunks of the prefix sums of scan_counts over n entries
+int32_t scan_chunks(int32_t n);
+
+// One system of a batched layout build (device table entry).
+template<typename VT>
+struct PlanSys {
+    SolveTopo t;
+    SolvePlan<VT> p;
+    const VT* val_out = nullptr;
+    int32_t* totals = nullptr;    // 4 entry totals (uo, ui, to, ti)
+    int32_t* scan_tmp = nullptr;  // scan_tmp_ints(n) ints
+};
+// solve_plan_count, the four scan_counts and solve_plan_fill of every
+// listed system: blk = {system, block row} items, scan = {system, chunk <
+// scan_chunks(n)}, task = {system, chunk of 256 block rows}
+template<typename VT>
+void plan_batch(const PlanSys<VT>* ps, FactorItems blk, FactorItems scan, FactorItems task,
+    hipStream_t st);
+
 // x <- U \ x (in place); x <- U' \ x; x <- D \ x.
 template<typename VT>
 void trsv_upper(const SolveTopo& t, const SolvePlan<VT>& p, double* x, void* stream);

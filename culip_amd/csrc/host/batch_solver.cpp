@@ -13,13 +13,13 @@
 #include "batch_solver.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <exception>
 #include <numeric>
-#include <thread>
 
 namespace culip {
 
@@ -47,13 +47,11 @@ BatchSolver::BatchSolver(const BatchSolverOpts& o) : o_(o)
         "batch: the batched SQMR needs a true-residual period > 0");
     CULIP_REQUIRE(o.sqmr.max_it >= 0 && o.sqmr.max_restarts >= 0 && o.sqmr.tol >= 0,
         "batch: bad SQMR options");
-    CULIP_REQUIRE(o.factor_streams >= 1 && o.factor_streams <= 64, "batch: 1..64 factor streams");
     HIP_CHECK(hipGetDevice(&device_));
 }
 
 BatchSolver::~BatchSolver()
 {
-    for(hipStream_t s : streams_) (void) hipStreamDestroy(s);
     if(h_scal_) (void) hipHostFree(h_scal_);
     if(h_flags_) (void) hipHostFree(h_flags_);
     if(h_sys_pinned_) (void) hipHostFree(h_sys_pinned_);
@@ -116,6 +114,9 @@ int64_t BatchSolver::device_bytes() const
     std::lock_guard<std::mutex> g(mu_);
     int64_t b = (int64_t) (d_work_.bytes + d_scal_.bytes + d_flags_.bytes + d_iter_.bytes +
         d_sys_.bytes);
+    for(const DevBuf* d : {&d_fsys_, &d_psys_, &d_lr_, &d_chunks_, &d_blk_, &d_scan_, &d_task_})
+        b += (int64_t) d->bytes;
+    for(const DevBuf& d : d_fit_) b += (int64_t) d.bytes;
     for(const ListSlot& l : slot_) {
         b += (int64_t) l.ent.bytes;
         for(const DevBuf& d : l.items) b += (int64_t) d.bytes;
@@ -232,12 +233,8 @@ void BatchSolver::finalize(hipStream_t st)
             hipHostMallocDefault));
         HIP_CHECK(hipEventCreateWithFlags(&l.ev, hipEventDisableTiming));
     }
-    const int nst = std::min<int>(o_.factor_streams, S);
-    for(int t = 0; t < nst; ++t) {
-        hipStream_t q = nullptr;
-        HIP_CHECK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
-        streams_.push_back(q);
-    }
+    if(o_.prec == CULIP_PREC_F32) build_factor<float>();
+    else build_factor<double>();
     st_.assign(S, State{});
     upload_sys(st);
 }
@@ -354,41 +351,120 @@ void BatchSolver::residual_norms(const BatchLists& L, int32_t bvec, hipStream_t 
     batch_step(BS_NORM_FINAL, L, d, 0, 0, 0, 0, st);
 }
 
-void BatchSolver::factor_all(const double* const* vals)
+// the tables and work items of the batched refactor (static for the batch:
+// they follow from the symbolic analyses)
+template<typename VT>
+void BatchSolver::build_factor()
 {
     const int32_t S = (int32_t) sys_.size();
-    std::atomic<int32_t> next{0};
-    std::exception_ptr err;
-    std::mutex em;
-    auto run = [&](int t) {
-        const hipError_t e = hipSetDevice(device_);
-        if(e != hipSuccess) {
-            std::lock_guard<std::mutex> g(em);
-            if(!err) err = std::make_exception_ptr(Error(CULIP_ERR_HIP, "batch: hipSetDevice"));
-            return;
+    std::vector<LevelRange> lrs;
+    std::vector<size_t> lr_off(S);
+    int32_t max_levels = 0;
+    for(int32_t s = 0; s < S; ++s) {
+        const Symbolic& y = sys_[s]->f->sym;
+        lr_off[s] = lrs.size();
+        for(int32_t l = 0; l < y.num_levels; ++l) {
+            LevelRange r;
+            r.lo = y.lvlb_off[l];
+            r.size = y.lvlb_off[l + 1] - r.lo;
+            r.npiv = y.lvl_off[l + 1] - y.lvl_off[l];
+            r.row_lo = y.lvl_off[l];
+            r.level = l;
+            lrs.push_back(r);
         }
-        for(;;) {
-            const int32_t s = next.fetch_add(1);
-            if(s >= S) return;
-            try {
-                sys_[s]->f->factor(vals[s], false, o_.threshold, streams_[t]);
-            } catch(...) {
-                std::lock_guard<std::mutex> g(em);
-                if(!err) err = std::current_exception();
-                next.store(S);
-                return;
-            }
-        }
+        max_levels = std::max(max_levels, y.num_levels);
+    }
+    upload(d_lr_, lrs);
+    std::vector<FactorSys<VT>> F(S);
+    std::vector<PlanSys<VT>> P(S);
+    std::vector<int2> chunks, blk, scan, task;
+    for(int32_t s = 0; s < S; ++s) {
+        Bildlt& f = *sys_[s]->f;
+        f.threshold = o_.threshold;
+        FactorSys<VT>& q = F[s];
+        q.a = f.factor_args<VT>();
+        q.lr = d_lr_.as<LevelRange>() + lr_off[s];
+        q.vals = f.d_vals.as<double>();
+        q.value_slot = f.d_value_slot.as<int64_t>();
+        q.nnz = f.A.nnz();
+        q.val_space = f.sym.val_space;
+        q.val_in = f.d_val_in.as<VT>();
+        q.static_is_piv = f.d_is_piv.as<int32_t>();
+        PlanSys<VT>& p = P[s];
+        p.t = f.topo();
+        p.p = f.plan<VT>();
+        p.val_out = q.a.val_out;
+        p.totals = f.d_totals.as<int32_t>();
+        p.scan_tmp = f.d_scan_tmp.as<int32_t>();
+        const int64_t nch = factor_batch_chunks(f.n(), q.nnz, q.val_space, f.sym.num_levels,
+            q.a.scan_maxch);
+        for(int64_t c = 0; c < nch; ++c) chunks.push_back(make_int2(s, (int) c));
+        for(int32_t r = 0; r < p.t.nb; ++r) blk.push_back(make_int2(s, r));
+        for(int32_t c = 0; c < scan_chunks(f.n()); ++c) scan.push_back(make_int2(s, c));
+        for(int32_t c = 0; c < (p.t.nb + 255) / 256; ++c) task.push_back(make_int2(s, c));
+    }
+    d_fsys_.ensure(sizeof(FactorSys<VT>) * (size_t) S);
+    HIP_CHECK(hipMemcpy(d_fsys_.p, F.data(), sizeof(FactorSys<VT>) * (size_t) S,
+        hipMemcpyHostToDevice));
+    d_psys_.ensure(sizeof(PlanSys<VT>) * (size_t) S);
+    HIP_CHECK(hipMemcpy(d_psys_.p, P.data(), sizeof(PlanSys<VT>) * (size_t) S,
+        hipMemcpyHostToDevice));
+    auto put = [](DevBuf& d, const std::vector<int2>& v) {
+        CULIP_REQUIRE(v.size() < (size_t) INT32_MAX, "batch: too many work items for one launch");
+        upload(d, v);
+        return FactorItems{d.as<int2>(), (int32_t) v.size()};
     };
-    std::vector<std::thread> th;
-    for(size_t t = 1; t < streams_.size(); ++t) th.emplace_back(run, (int) t);
-    run(0);
-    for(auto& t : th) t.join();
-    if(err) std::rethrow_exception(err);
+    chunks_ = put(d_chunks_, chunks);
+    blk_ = put(d_blk_, blk);
+    scan_ = put(d_scan_, scan);
+    task_ = put(d_task_, task);
+    // per level and launch: the workgroups of every system that has the level
+    std::vector<int2> it[7];
+    std::vector<std::array<size_t, 7>> off(max_levels + 1);
+    for(int32_t l = 0; l < max_levels; ++l) {
+        for(int k = 0; k < 7; ++k) off[l][k] = it[k].size();
+        for(int32_t s = 0; s < S; ++s) {
+            if(l >= sys_[s]->f->sym.num_levels) continue;
+            const LevelRange& r = lrs[lr_off[s] + l];
+            const int32_t noff = r.size - r.npiv;
+            const int32_t cnt[7] = {fl_grid_pivots_pairs(r), fl_tiny_blocks(r),
+                fl_grid_tasks(noff, BATCH_TASK_GRID), r.npiv, fl_grid_level_scan(r),
+                fl_grid_store_tiny(r), fl_grid_tasks(r.size, BATCH_TASK_GRID)};
+            for(int k = 0; k < 7; ++k)
+                for(int32_t q = 0; q < cnt[k]; ++q) it[k].push_back(make_int2(s, q));
+        }
+    }
+    for(int k = 0; k < 7; ++k) off[max_levels][k] = it[k].size();
+    FactorItems all[7];
+    for(int k = 0; k < 7; ++k) all[k] = put(d_fit_[k], it[k]);
+    flevels_.assign(max_levels, FactorBatchLevel{});
+    for(int32_t l = 0; l < max_levels; ++l) {
+        FactorItems* dst[7] = {&flevels_[l].pp, &flevels_[l].ts, &flevels_[l].off,
+            &flevels_[l].rn, &flevels_[l].sc, &flevels_[l].stt, &flevels_[l].st};
+        for(int k = 0; k < 7; ++k) {
+            dst[k]->items = all[k].items + off[l][k];
+            dst[k]->count = (int32_t) (off[l + 1][k] - off[l][k]);
+        }
+    }
+}
+
+// factor_impl of every system: one launch per step over all systems (the
+// values are already in each factor's value buffer)
+template<typename VT>
+void BatchSolver::factor_batched(hipStream_t st)
+{
+    const FactorSys<VT>* fs = d_fsys_.as<FactorSys<VT>>();
+    factor_batch_begin<VT>(fs, chunks_, st);
+    for(int32_t l = 0; l < (int32_t) flevels_.size(); ++l)
+        factor_batch_level<VT>(fs, flevels_[l], l, st);
+    plan_batch<VT>(d_psys_.as<PlanSys<VT>>(), blk_, scan_, task_, st);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st));
+    for(auto& e : sys_) e->f->after_factor();
 }
 
 void BatchSolver::solve(const double* const* vals, const double* const* rhs, double* const* x,
-    culip_sqmr_result* res, hipStream_t st)
+    culip_sqmr_result* res, bool device_arrays, hipStream_t st)
 {
     CULIP_REQUIRE(vals && rhs && x, "batch: null argument");
     HIP_CHECK(hipSetDevice(device_));
@@ -399,12 +475,17 @@ void BatchSolver::solve(const double* const* vals, const double* const* rhs, dou
     const int32_t period = O.true_residual_period;
     const BatchSys* d = d_sys_.as<BatchSys>();
 
+    const hipMemcpyKind in_kind = device_arrays ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const auto t0 = std::chrono::steady_clock::now();
-    factor_all(vals);
+    for(int32_t s = 0; s < S; ++s)
+        HIP_CHECK(hipMemcpyAsync(sys_[s]->f->d_vals.p, vals[s], sizeof(double) * (size_t) sys_[s]->nnz,
+            in_kind, st));
+    if(o_.prec == CULIP_PREC_F32) factor_batched<float>(st);
+    else factor_batched<double>(st);
     const auto t1 = std::chrono::steady_clock::now();
     for(int32_t s = 0; s < S; ++s) {
         HIP_CHECK(hipMemcpyAsync(const_cast<double*>(h_sys_[s].b), rhs[s],
-            sizeof(double) * (size_t) sys_[s]->n, hipMemcpyHostToDevice, st));
+            sizeof(double) * (size_t) sys_[s]->n, in_kind, st));
         h_sys_[s].brun_rhs = 0;
         st_[s] = State{};
     }
@@ -590,7 +671,7 @@ void BatchSolver::solve(const double* const* vals, const double* const* rhs, dou
 
     for(int32_t s = 0; s < S; ++s) {
         HIP_CHECK(hipMemcpyAsync(x[s], h_sys_[s].x, sizeof(double) * (size_t) sys_[s]->n,
-            hipMemcpyDeviceToHost, st));
+            device_arrays ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         if(res) {
             State& q = st_[s];
             culip_sqmr_result r = q.res;

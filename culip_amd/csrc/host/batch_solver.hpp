@@ -1,7 +1,7 @@
 // A batch of independent KKT systems of uneven size, factored and solved
-// together: the refactors run concurrently on a pool of streams, and every
-// step of the SQMR recurrence is one launch over all systems still
-// iterating (device/batch.hpp). Each system follows exactly the control flow
+// together: every step of the refactor (a level of all systems) and of the
+// SQMR recurrence is one launch over all systems that take part in it
+// (device/batch.hpp, factor_batch_level). Each system follows exactly the control flow
 // of the one-system SQMR with the FP64 true-residual gate and refinement
 // restarts (sqmr_solve), and ends with the bits it ends with when it is
 // solved alone with the same options and block-Jacobi schedule.
@@ -23,7 +23,6 @@ struct BatchSolverOpts {
     double threshold = 1e-4;
     culip_sqmr_opts sqmr{1e-8, 1000, 20, 10};
     int32_t jacobi_sweeps = 2;   // block-Jacobi sweeps per triangular solve
-    int32_t factor_streams = 16; // concurrent refactors
 };
 
 class BatchSolver {
@@ -41,9 +40,10 @@ public:
     int32_t system_n(int32_t s) const;
     int64_t system_nnz(int32_t s) const;
     // refactors system s with vals[s] and solves K_s x_s = rhs_s for every
-    // system (host arrays); res may be null
+    // system (host arrays, or device arrays when device_arrays); res may be
+    // null
     void solve(const double* const* vals, const double* const* rhs, double* const* x,
-        culip_sqmr_result* res, hipStream_t st);
+        culip_sqmr_result* res, bool device_arrays, hipStream_t st);
     int64_t device_bytes() const;
     // wall seconds of the last solve: refactors, SQMR
     void last_seconds(double* out) const { out[0] = sec_[0]; out[1] = sec_[1]; }
@@ -87,7 +87,8 @@ private:
     };
 
     void finalize(hipStream_t st);
-    void factor_all(const double* const* vals);
+    template<typename VT> void build_factor();
+    template<typename VT> void factor_batched(hipStream_t st);
     const BatchLists& lists(ListSlot& slot, const std::vector<int32_t>& ids, hipStream_t st);
     void start_runs(const BatchLists& L, hipStream_t st);
     void iteration(const BatchLists& L, hipStream_t st);
@@ -111,7 +112,10 @@ private:
     double* h_scal_ = nullptr;      // pinned mirrors of the scalar and flag pools
     int32_t* h_flags_ = nullptr;
     BatchSys* h_sys_pinned_ = nullptr;
-    std::vector<hipStream_t> streams_;
+    // batched refactor: system tables, level ranges, work items
+    DevBuf d_fsys_, d_psys_, d_lr_, d_fit_[7], d_chunks_, d_blk_, d_scan_, d_task_;
+    std::vector<FactorBatchLevel> flevels_;
+    FactorItems chunks_, blk_, scan_, task_;
     ListSlot slot_[4];
     std::vector<State> st_;
     double sec_[2] = {0.0, 0.0};

@@ -294,6 +294,8 @@ SolvePlan<VT> Bildlt::plan() const
 }
 
 template SolvePlan<float> Bildlt::plan<float>() const;
+template FactorArgs<float> Bildlt::factor_args<float>() const;
+template FactorArgs<double> Bildlt::factor_args<double>() const;
 template SolvePlan<double> Bildlt::plan<double>() const;
 
 SolveTopo Bildlt::topo() const
@@ -405,6 +407,11 @@ void Bildlt::factor(const double* vals, bool on_device, double thr, hipStream_t 
         else factor_impl<double>(st);
     }
     HIP_CHECK(hipStreamSynchronize(st));
+    after_factor();
+}
+
+void Bildlt::after_factor()
+{
     // a level's dropping scan whose bounded wait gave up leaves the object
     // unfactored
     factored = false;

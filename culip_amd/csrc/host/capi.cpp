@@ -980,7 +980,6 @@ culip_status culip_batch_create(const culip_batch_opts* o, culip_batch** out)
         bo.threshold = o->threshold;
         bo.sqmr = o->sqmr;
         bo.jacobi_sweeps = o->jacobi_sweeps;
-        bo.factor_streams = o->factor_streams;
         auto* h = new culip_batch;
         try {
             h->b = std::make_unique<BatchSolver>(bo);
@@ -1003,11 +1002,12 @@ culip_status culip_batch_add(culip_batch* b, const culip_csr* K, int32_t num_piv
 }
 
 culip_status culip_batch_solve(culip_batch* b, const double* const* vals,
-    const double* const* rhs, double* const* x, culip_sqmr_result* res, void* stream)
+    const double* const* rhs, double* const* x, culip_sqmr_result* res, int32_t device_arrays,
+    void* stream)
 {
     return guarded([&] {
         CULIP_REQUIRE(b, "batch: null handle");
-        b->b->solve(vals, rhs, x, res, (hipStream_t) stream);
+        b->b->solve(vals, rhs, x, res, device_arrays != 0, (hipStream_t) stream);
     });
 }
 

@@ -222,6 +222,9 @@ struct Bildlt final : Precond {
     // with_layout is set and left out otherwise
     int64_t numeric_bytes_estimate(bool with_layout) const;
     void factor(const double* vals, bool vals_on_device, double thr, hipStream_t st);
+    // the end of a factor whose launches have completed (also after a
+    // batched refactor): errors the device recorded, then the object is factored
+    void after_factor();
     void solve(culip_side side, bool transpose, const double* b, double* x, hipStream_t st) const;
     // the same without allocating: exact solves as level launches,
     // block-Jacobi work vectors in the static scratch array (n must not

@@ -486,10 +486,10 @@ void culip_server_destroy(culip_server * s);
 
 /* ------------------------------------------------------------------------ */
 /* Batches of independent systems (no counterpart in the reference): every
- * system of the batch is refactored (concurrently, on `factor_streams`
- * streams) and solved by SQMR with the FP64 true-residual gate and
- * block-Jacobi triangular solves, where each step of the recurrence is one
- * launch over all systems still iterating. Each system's x, residuals and
+ * system of the batch is refactored and solved by SQMR with the FP64
+ * true-residual gate and block-Jacobi triangular solves; each step of the
+ * refactor (a level of all systems) and of the recurrence is one launch over
+ * all systems that take part in it. Each system's x, residuals and
  * iteration counts are bit-identical to its solve alone (culip_bildlt with
  * CULIP_TRSV_JACOBI and the same sweeps, culip_sqmr with the same options).
  * Not thread-safe: one solve at a time per batch.                        */
@@ -501,7 +501,6 @@ typedef struct {
     double threshold;           /* drop tolerance of every refactor */
     culip_sqmr_opts sqmr;       /* true_residual_period must be > 0 */
     int32_t jacobi_sweeps;      /* block-Jacobi sweeps per triangular solve, >= 1 */
-    int32_t factor_streams;     /* concurrent refactors, 1..64 */
 } culip_batch_opts;
 
 typedef struct culip_batch culip_batch;
@@ -514,10 +513,12 @@ culip_status culip_batch_add(culip_batch * b, const culip_csr * K, int32_t num_p
     const int32_t * piv_starts, int32_t num_blocks, const int32_t * block_starts,
     int32_t * index);
 /* Refactors system i with vals[i] (nnz values in its entry order) and
- * solves it for rhs[i] into x[i] (all host arrays) for every system;
- * res: one result per system (may be NULL). Synchronises stream. */
+ * solves it for rhs[i] into x[i] for every system; the arrays are host
+ * memory, or device memory when device_arrays != 0. res: one result per
+ * system (may be NULL). Synchronises stream. */
 culip_status culip_batch_solve(culip_batch * b, const double * const * vals,
-    const double * const * rhs, double * const * x, culip_sqmr_result * res, void * stream);
+    const double * const * rhs, double * const * x, culip_sqmr_result * res,
+    int32_t device_arrays, void * stream);
 /* systems registered, device bytes held, and seconds[2] of the last solve:
  * the refactors, then the SQMR (any may be NULL) */
 culip_status culip_batch_info(const culip_batch * b, int32_t * count, int64_t * device_bytes,

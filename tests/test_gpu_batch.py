@@ -1,7 +1,8 @@
 """The batched engine (culip_amd.Batch) on the GPU: every system's solution,
 residuals and iteration counts are bit for bit those of the same system
-solved alone by BlockiLDLt (block-Jacobi solves, same sweeps) + SQMR with
-the same options, whatever else is in the batch. Iteration limits that are
+factored and solved alone by BlockiLDLt (block-Jacobi solves, same sweeps)
++ SQMR with the same options, whatever else is in the batch (the batched
+refactor runs each level of all systems as one launch per step). Iteration limits that are
 not a multiple of the true-residual period end runs between checks, and an
 unattainable tolerance drives refinement restarts and undone corrections."""
 import numpy as np
@@ -45,7 +46,7 @@ def _check_equal(batch_out, alone_out, label):
 def _run(dev, systems, prec, sweeps, tol=1e-8, max_it=1000, period=20, restarts=10,
          vals=None):
     bt = ca.Batch(prec=prec, tol=tol, max_it=max_it, true_residual_period=period,
-                  max_restarts=restarts, jacobi_sweeps=sweeps, factor_streams=4)
+                  max_restarts=restarts, jacobi_sweeps=sweeps)
     for _, p, _ in systems:
         bt.add(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
     vals = vals or [p.val for _, p, _ in systems]
@@ -87,7 +88,7 @@ def test_batch_new_values(hip_device):
     # two solves of one batch with different values: the second equals the
     # systems solved alone with those values
     systems = _systems([1700, 2800, 2200], seed0=700)
-    bt = ca.Batch(prec="f32", jacobi_sweeps=2, factor_streams=2)
+    bt = ca.Batch(prec="f32", jacobi_sweeps=2)
     for _, p, _ in systems:
         bt.add(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
     rhs = [r for _, _, r in systems]
@@ -117,3 +118,24 @@ def test_batch_rejects_bad_use(hip_device):
     with pytest.raises(ca.CulipError):
         bt.add(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
     bt.close()
+
+
+def test_batch_device_arrays(hip_device):
+    # values, right-hand sides and solutions in device memory: the same
+    # results as with host arrays
+    import torch
+    systems = _systems([1300, 2400, 3100], seed0=900)
+    bt = ca.Batch(prec="f64", jacobi_sweeps=3)
+    for _, p, _ in systems:
+        bt.add(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+    vals = [p.val for _, p, _ in systems]
+    rhs = [r for _, _, r in systems]
+    xh, rh = bt.solve(vals, rhs)
+    dv = [torch.tensor(v, device=hip_device) for v in vals]
+    db = [torch.tensor(r, device=hip_device) for r in rhs]
+    xd, rd = bt.solve(dv, db)
+    bt.close()
+    for i in range(len(systems)):
+        assert xd[i].is_cuda
+        assert np.array_equal(xd[i].cpu().numpy(), xh[i])
+        assert rd[i] == rh[i]
