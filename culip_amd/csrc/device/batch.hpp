@@ -76,6 +76,11 @@ enum BatchVec : int32_t {
     BV_NONE = -1
 };
 
+// rows of K per item of the batched CSR product, block rows per item of the
+// batched block-Jacobi sweep
+constexpr int BATCH_SPMV_ROWS = 128;
+constexpr int JACOBI_BATCH_ROWS = 8;
+
 // Work items of one kind for a set of systems: items[i] = {system, local}.
 struct BatchList {
     const int2* items = nullptr;
@@ -86,8 +91,8 @@ struct BatchList {
 struct BatchLists {
     BatchList vec;    // sqmr_grid(n) items of 256 threads (vector kernels)
     BatchList row;    // ceil(n / 256) items of 256 threads (elementwise)
-    BatchList blk;    // one item of 64 threads per block row (block-Jacobi)
-    BatchList spmv;   // ceil(n / 32) items of 256 threads (CSR product)
+    BatchList blk;    // ceil(nb / JACOBI_BATCH_ROWS) items of 64 threads (block-Jacobi)
+    BatchList spmv;   // ceil(n / BATCH_SPMV_ROWS) items of 256 threads (CSR product)
     BatchList sys;    // one item per system (scalars)
 };
 

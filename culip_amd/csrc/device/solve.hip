@@ -416,7 +416,7 @@ __device__ __forceinline__ double apply_inverse(int fr, int h, int ib, int ie, i
 {
     const int lane = lane_id();
     if(ie - ib == h && wave_ballot(lane < h && (il0 != lane || ic0 != fr + lane)) == 0)
-        return (lane < h) ? 0.0 + (double) iv0 * y[lane] : 0.0;
+        return (lane < h) ? fma((double) iv0, y[lane], 0.0) : 0.0;
     const bool in0 = ib + lane < ie;
     int icol = in0 ? ic0 : fr;
     VT ival = in0 ? iv0 : (VT) 0;
@@ -427,8 +427,7 @@ __device__ __forceinline__ double apply_inverse(int fr, int h, int ib, int ie, i
             isg = 64;
             if(e < ie) { icol = i_col[e]; ival = i_val[e]; isg = i_lr[e]; }
         }
-        double p = (isg < 64) ? (double) ival * y[icol - fr] : 0.0;
-        p = seg_scan_runs(p, isg);
+        double p = seg_scan_runs_fma((isg < 64) ? (double) ival : 0.0, y[icol - fr], isg);
         const int nsg = next_lane(isg, -1);
         if(isg < 64 && (lane == 63 || nsg != isg)) res[isg] += p;
         wave_sync();
@@ -500,7 +499,7 @@ __device__ __forceinline__ void block_row_solve(int4 t0, int4 t1, bool offdiag,
 #pragma unroll
         for(int k = 0; k < TRSV_PF; ++k) {
             if(e0 + 64 * k >= oe) break;
-            double p = seg_scan_runs((double) v[k] * xv[k], sg[k]);
+            double p = seg_scan_runs_fma((double) v[k], xv[k], sg[k]);
             const int nsg = next_lane(sg[k], -1);
             if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
             wave_sync();
@@ -767,7 +766,7 @@ __global__ __launch_bounds__(64) void k_trsv_chain(const int4* task, int32_t nb,
 #pragma unroll
             for(int k = 0; k < TRSV_PF; ++k) {
                 if(e0 + 64 * k >= oe) break;
-                double p = seg_scan_runs((double) v[k] * (sg[k] < 64 ? xv[k] : 0.0), sg[k]);
+                double p = seg_scan_runs_fma((double) v[k], sg[k] < 64 ? xv[k] : 0.0, sg[k]);
                 const int nsg = next_lane(sg[k], -1);
                 if(sg[k] < 64 && (lane == 63 || nsg != sg[k])) y[sg[k]] -= p;
                 wave_sync();
@@ -810,24 +809,143 @@ __global__ __launch_bounds__(64) void k_jacobi_first_product(const int4* task,
         i_lr, b, nullptr, xout, y, res);
 }
 
-// Batched block-Jacobi sweeps: item {s, r} is block row r of system s, the
-// workgroup r of k_jacobi_sweep on system s; tr selects the layout (0 the U
-// solve, 1 the U' solve). b may be the output vector itself: a block row
-// reads its own rows of b before it writes its own rows of the output, and
-// no other block row reads them.
+// Batched block-Jacobi sweeps. Item {s, c} is block rows JACOBI_ROWS c ..
+// JACOBI_ROWS c + JACOBI_ROWS - 1 of system s, one after the other in one
+// wave; each block row is computed exactly as block_row_solve computes it
+// (the workgroup of k_jacobi_sweep for that block row): the same masked
+// 64-entry chunks, segmented sums in chunk order and U_ii^-1 rows. What a
+// block row reads before its first wait (its first chunk, the x values of
+// that chunk, its b and its first 64 entries of U_ii^-1) is loaded while the
+// block row before it computes; the task records of all the item's block
+// rows come in with one load. tr selects the layout (0 the U solve, 1 the
+// U' solve). b may be the output vector itself: a block row reads its own
+// rows of b before it writes its own rows of the output, and no other block
+// row reads them.
+constexpr int JACOBI_ROWS = JACOBI_BATCH_ROWS;
+
+template<typename VT>
+struct JacobiHead {
+    int c, sg;
+    VT v;
+    double xv;
+    double bo;
+    int ic, il;
+    VT iv;
+};
+
 template<typename VT>
 __global__ __launch_bounds__(64) void bk_jacobi_sweep(const int2* items, const BatchSys* sys,
-    int32_t tr, int32_t b, int32_t in, int32_t out)
+    int32_t tr, int32_t b_id, int32_t in, int32_t out)
 {
     __shared__ double y[33];
     __shared__ double res[33];
+    const int lane = lane_id();
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int4* task = S.task[tr];
-    const double* xin = in == BV_NONE ? nullptr : batch_vec(S, in);
-    block_row_solve<VT>(task[2 * it.y], task[2 * it.y + 1], xin != nullptr, S.o_col[tr],
-        (const VT*) S.o_val[tr], S.o_lr[tr], S.i_col[tr], (const VT*) S.i_val[tr], S.i_lr[tr],
-        batch_vec(S, b), xin, batch_vec(S, out), y, res);
+    const int32_t* o_col = S.o_col[tr];
+    const VT* o_val = (const VT*) S.o_val[tr];
+    const uint8_t* o_lr = S.o_lr[tr];
+    const int32_t* i_col = S.i_col[tr];
+    const VT* i_val = (const VT*) S.i_val[tr];
+    const uint8_t* i_lr = S.i_lr[tr];
+    const double* xsrc = in == BV_NONE ? nullptr : batch_vec(S, in);
+    const double* b = batch_vec(S, b_id);
+    double* xout = batch_vec(S, out);
+    const bool offdiag = xsrc != nullptr;
+    const int r0 = it.y * JACOBI_ROWS;
+    const int cnt = min(JACOBI_ROWS, S.nb - r0);
+    // lane 2 j holds block row r0 + j's first task record, lane 2 j + 1 its
+    // second
+    const int4 tk = task[2 * r0 + min(lane, 2 * cnt - 1)];
+    auto rec = [&](int j, int4& t0, int4& t1) {
+        t0.x = __shfl(tk.x, 2 * j);
+        t0.y = __shfl(tk.y, 2 * j);
+        t0.z = __shfl(tk.z, 2 * j);
+        t0.w = __shfl(tk.w, 2 * j);
+        t1.x = __shfl(tk.x, 2 * j + 1);
+        t1.y = __shfl(tk.y, 2 * j + 1);
+    };
+    // the loads of block row j that do not depend on other rows' results
+    auto head = [&](int j, JacobiHead<VT>& H) {
+        int4 t0, t1;
+        rec(j, t0, t1);
+        const int fr = t0.x, h = t0.y, ob = t0.z, oe = offdiag ? t0.w : t0.z;
+        const int ib = t1.x, ie = t1.y;
+        H.bo = b[fr + min(lane, h - 1)];
+        const int ei = ib + min(lane, max(ie - ib - 1, 0));
+        H.ic = i_col[ei];
+        H.iv = i_val[ei];
+        H.il = i_lr[ei];
+        H.c = 0;
+        H.v = (VT) 0;
+        H.sg = 64;
+        if(ob < oe) {
+            const int e = min(ob + lane, oe - 1);
+            H.c = o_col[e];
+            H.v = o_val[e];
+            H.sg = o_lr[e];
+        }
+    };
+    auto gather = [&](int j, JacobiHead<VT>& H) {
+        int4 t0, t1;
+        rec(j, t0, t1);
+        const int ob = t0.z, oe = offdiag ? t0.w : t0.z;
+        H.xv = 0.0;
+        if(ob < oe) {
+            const double xr = xsrc[H.c];
+            const bool in0 = ob + lane < oe;
+            H.c = in0 ? H.c : 0;
+            H.v = in0 ? H.v : (VT) 0;
+            H.sg = in0 ? H.sg : 64;
+            H.xv = in0 ? xr : 0.0;
+        }
+    };
+    auto apply_chunk = [&](int c, VT v, int sg, double xv) {
+        (void) c;
+        const double p = seg_scan_runs_fma((double) v, xv, sg);
+        const int nsg = next_lane(sg, -1);
+        if(sg < 64 && (lane == 63 || nsg != sg)) y[sg] -= p;
+        wave_sync();
+    };
+    auto compute = [&](int j, const JacobiHead<VT>& H) {
+        int4 t0, t1;
+        rec(j, t0, t1);
+        const int fr = t0.x, h = t0.y, ob = t0.z, oe = offdiag ? t0.w : t0.z;
+        const int ib = t1.x, ie = t1.y;
+        const double bo = (lane < h) ? H.bo : 0.0;
+        if(lane < h) { y[lane] = bo; res[lane] = 0; }
+        wave_sync();
+        if(ob < oe) {
+            apply_chunk(H.c, H.v, H.sg, H.xv);
+            for(int e0 = ob + 64; e0 < oe; e0 += 64) {
+                const int e = min(e0 + lane, oe - 1);
+                const int cr = o_col[e];
+                const VT vr = o_val[e];
+                const int sr = o_lr[e];
+                const double xr = xsrc[cr];
+                const bool in0 = e0 + lane < oe;
+                apply_chunk(in0 ? cr : 0, in0 ? vr : (VT) 0, in0 ? sr : 64, in0 ? xr : 0.0);
+            }
+        }
+        wave_sync();
+        const double r = apply_inverse<VT>(fr, h, ib, ie, H.ic, H.iv, H.il, i_col, i_val, i_lr,
+            y, res);
+        if(lane < h) xout[fr + lane] = r;
+        wave_sync();
+    };
+    JacobiHead<VT> A, B;
+    head(0, A);
+    gather(0, A);
+    for(int j = 0; j < cnt; j += 2) {
+        if(j + 1 < cnt) head(j + 1, B);
+        compute(j, A);
+        if(j + 1 >= cnt) break;
+        gather(j + 1, B);
+        if(j + 2 < cnt) head(j + 2, A);
+        compute(j + 1, B);
+        if(j + 2 < cnt) gather(j + 2, A);
+    }
 }
 
 // Batched layout build: item {s, q} is workgroup q of the single-system

@@ -63,18 +63,68 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int32_t m, const int32_t* __re
     }
 }
 
-// y = K x of every listed system: item {s, c} is rows 32 c .. 32 c + 31 of
-// system s, each summed as k_csr_spmv sums it (alpha 1, beta 0)
+// y = K x of every listed system: item {s, c} is rows BATCH_SPMV_ROWS c ..
+// of system s. Each group of eight lanes sums BATCH_SPMV_ROWS / 32 rows,
+// rows 32 k + group, with the loads of all its rows issued together; every
+// row is summed exactly as csr_row_sum sums it (the lane's entries in
+// order, then the xor tree over the eight lanes), alpha 1, beta 0.
 __global__ __launch_bounds__(256) void bk_csr_spmv(const int2* items, const BatchSys* sys,
     int32_t xid, int32_t yid)
 {
+    constexpr int RB = BATCH_SPMV_ROWS / 32;
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int sub = threadIdx.x & (LPR - 1);
-    const int64_t row = (int64_t) it.y * (256 / LPR) + threadIdx.x / LPR;
-    if(row >= S.n) return;
-    const double acc = csr_row_sum(S.k_rp, S.k_col, S.k_val, batch_vec(S, xid), row, sub);
-    if(sub == 0) batch_vec(S, yid)[row] = 1.0 * acc;
+    const int grp = threadIdx.x / LPR;
+    const int32_t* __restrict__ rp = S.k_rp;
+    const int32_t* __restrict__ ci = S.k_col;
+    const double* __restrict__ cv = S.k_val;
+    const double* __restrict__ x = batch_vec(S, xid);
+    double* __restrict__ y = batch_vec(S, yid);
+    const int32_t n = S.n;
+    int32_t row[RB], s[RB], e[RB];
+    double acc[RB];
+#pragma unroll
+    for(int j = 0; j < RB; ++j) {
+        row[j] = it.y * BATCH_SPMV_ROWS + 32 * j + grp;
+        const int32_t r = min(row[j], n - 1);
+        s[j] = rp[r];
+        e[j] = rp[r + 1];
+        if(row[j] >= n) e[j] = s[j];
+        acc[j] = 0.0;
+    }
+    for(int32_t off = sub;; off += LPR * SPMV_U) {
+        int32_t c[RB][SPMV_U];
+        double v[RB][SPMV_U], xv[RB][SPMV_U];
+#pragma unroll
+        for(int j = 0; j < RB; ++j)
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u) {
+                const int32_t q = max(min(s[j] + off + u * LPR, e[j] - 1), 0);
+                c[j][u] = ci[q];
+                v[j][u] = cv[q];
+            }
+#pragma unroll
+        for(int j = 0; j < RB; ++j)
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u) xv[j][u] = x[c[j][u]];
+        bool more = false;
+#pragma unroll
+        for(int j = 0; j < RB; ++j) {
+#pragma unroll
+            for(int u = 0; u < SPMV_U; ++u)
+                if(s[j] + off + u * LPR < e[j]) acc[j] = fma(v[j][u], xv[j][u], acc[j]);
+            more = more || s[j] + off + LPR * SPMV_U < e[j];
+        }
+        if(wave_ballot(more) == 0) break;
+    }
+#pragma unroll
+    for(int j = 0; j < RB; ++j) {
+        double a = acc[j];
+#pragma unroll
+        for(int o = LPR / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, LPR);
+        if(sub == 0 && row[j] < n) y[row[j]] = 1.0 * a;
+    }
 }
 
 // KKT: b1 = H x1 + A' x2 (H = diag(d), or a CSR matrix hp/hi/hv) ;

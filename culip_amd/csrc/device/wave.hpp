@@ -92,6 +92,25 @@ __device__ __forceinline__ double seg_scan_runs(double v, int sg)
     return v;
 }
 
+// seg_scan_runs(a * b, sg) with the first step's sum written out as
+// fma(a, b, u), u the neighbour's rounded product: each lane's own product
+// enters its first partial sum unrounded. Every kernel that sums products
+// through this rounds them identically, independently of what the compiler
+// would contract.
+__device__ __forceinline__ double seg_scan_runs_fma(double a, double b, int sg)
+{
+    const double v0 = a * b;
+    const double u = dpp_move<0x111, 0xf>(0.0, v0);
+    const int su = dpp_move<0x111, 0xf>(-1, sg);
+    double v = (su == sg) ? fma(a, b, u) : v0;
+    seg_scan_step<0x112, 0xf>(v, sg);
+    seg_scan_step<0x114, 0xf>(v, sg);
+    seg_scan_step<0x118, 0xf>(v, sg);
+    seg_scan_step<0x142, 0xa>(v, sg);
+    seg_scan_step<0x143, 0xc>(v, sg);
+    return v;
+}
+
 // v of lane i + 1 (fill on lane 63), all lanes active
 __device__ __forceinline__ int next_lane(int v, int fill) { return dpp_move<0x130, 0xf>(fill, v); }
 

@@ -220,8 +220,8 @@ void BatchSolver::finalize(hipStream_t st)
         else set_plan(f.plan<double>());
         tot[0] += B.G;
         tot[1] += cdiv(B.n, 256);
-        tot[2] += B.nb;
-        tot[3] += cdiv(B.n, 32);
+        tot[2] += cdiv(B.nb, JACOBI_BATCH_ROWS);
+        tot[3] += cdiv(B.n, BATCH_SPMV_ROWS);
         tot[4] += 1;
     }
     for(int k = 0; k < 5; ++k)
@@ -270,7 +270,8 @@ const BatchLists& BatchSolver::lists(ListSlot& slot, const std::vector<int32_t>&
     int32_t total[5] = {0, 0, 0, 0, 0};
     for(int32_t k = 0; k < m; ++k) {
         const BatchSys& B = h_sys_[ids[k]];
-        const int32_t cnt[5] = {B.G, cdiv(B.n, 256), B.nb, cdiv(B.n, 32), 1};
+        const int32_t cnt[5] = {B.G, cdiv(B.n, 256), cdiv(B.nb, JACOBI_BATCH_ROWS),
+            cdiv(B.n, BATCH_SPMV_ROWS), 1};
         for(int q = 0; q < 5; ++q) {
             slot.h_ent[q * m + k] = make_int4(ids[k], cnt[q], total[q], 0);
             total[q] += cnt[q];
