@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--threshold", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batch workload: skip the plain FP64 per-system comparison "
+                         "(profiling runs)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the FP32-factor and block-Jacobi sub-runs")
     ap.add_argument("--solve-algorithm", default="single",
@@ -394,8 +397,11 @@ def bench_batch(a):
     el = secs[-1]
     # the plain path: FP64 factor, exact single-launch solves (the reference
     # driver's SOLVE_BLOCK), one system at a time
-    base, secs_b, _ = cb.solve_batch(systems, lambda i: None, 1, "f64", log=log)
-    el_base = secs_b[-1]
+    if a.no_plain:
+        base, el_base = res, el
+    else:
+        base, secs_b, _ = cb.solve_batch(systems, lambda i: None, 1, "f64", log=log)
+        el_base = secs_b[-1]
     trr = max(r[1]["true_rel_residual"] for r in res)
     conv = [r[1]["true_rel_residual"] <= 1e-8 for r in res]
     return {
@@ -406,8 +412,8 @@ def bench_batch(a):
         "dtype": "f64" if a.prec == "f64" else "f32-factor/f64-sqmr", "data": "synthetic",
         "config": {"workload": "batch", "systems": a.systems, "n_range": [1000, 40000],
                    "m": "0.4n", "seeds": f"0..{a.systems - 1}",
-                   "engine": "one batch: concurrent refactors on 16 streams, one launch per "
-                             "SQMR step over all systems still iterating",
+                   "engine": "one batch: one launch per refactor step and per SQMR step over "
+                             "all systems taking part",
                    "max_block_by_bucket": {f"{cb.BUCKETS[b][0]}-{cb.BUCKETS[b][1] - 1}":
                                            tuned[b] for b in tuned},
                    "solve_schedule": f"jacobi{sweeps}",
@@ -419,9 +425,11 @@ def bench_batch(a):
         "converged_solves_per_s": float(np.sum(conv)) / el,
         "sqmr_iterations_mean": float(np.mean([r[1]["iterations"] for r in res])),
         "sqmr_iterations_max": int(max(r[1]["iterations"] for r in res)),
-        "plain_f64_per_system_solves_per_s": len(base) / el_base,
-        "speedup_vs_plain_f64_per_system": (len(res) / el) / (len(base) / el_base),
-        "plain_max_true_rel_residual": max(r[1]["true_rel_residual"] for r in base),
+        "plain_f64_per_system_solves_per_s": None if a.no_plain else len(base) / el_base,
+        "speedup_vs_plain_f64_per_system":
+            None if a.no_plain else (len(res) / el) / (len(base) / el_base),
+        "plain_max_true_rel_residual":
+            None if a.no_plain else max(r[1]["true_rel_residual"] for r in base),
         "phases_s": {"refactors": phases["factor_s"], "sqmr": phases["sqmr_s"]},
         "device_bytes": phases["device_bytes"],
         "setup": {"autotune_s": t_tune, "generate_s": t_gen,
