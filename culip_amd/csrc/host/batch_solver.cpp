@@ -52,6 +52,8 @@ BatchSolver::BatchSolver(const BatchSolverOpts& o) : o_(o)
 
 BatchSolver::~BatchSolver()
 {
+    if(seg_exec_) (void) hipGraphExecDestroy(seg_exec_);
+    if(sq_) (void) hipStreamDestroy(sq_);
     if(h_scal_) (void) hipHostFree(h_scal_);
     if(h_flags_) (void) hipHostFree(h_flags_);
     if(h_sys_pinned_) (void) hipHostFree(h_sys_pinned_);
@@ -235,6 +237,7 @@ void BatchSolver::finalize(hipStream_t st)
     }
     if(o_.prec == CULIP_PREC_F32) build_factor<float>();
     else build_factor<double>();
+    HIP_CHECK(hipStreamCreateWithFlags(&sq_, hipStreamNonBlocking));
     st_.assign(S, State{});
     upload_sys(st);
 }
@@ -464,12 +467,40 @@ void BatchSolver::factor_batched(hipStream_t st)
     for(auto& e : sys_) e->f->after_factor();
 }
 
+void BatchSolver::capture_segment(const BatchLists& L, int32_t period,
+    const std::vector<int32_t>& ids, hipStream_t st)
+{
+    if(seg_exec_) (void) hipGraphExecDestroy(seg_exec_);
+    seg_exec_ = nullptr;
+    seg_ids_.clear();
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        for(int32_t j = 0; j < period; ++j) iteration(L, st);
+    } catch(...) {
+        hipGraph_t bad = nullptr;
+        (void) hipStreamEndCapture(st, &bad);
+        if(bad) (void) hipGraphDestroy(bad);
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamEndCapture(st, &g));
+    const hipError_t e = hipGraphInstantiate(&seg_exec_, g, nullptr, nullptr, 0);
+    (void) hipGraphDestroy(g);
+    hip_check(e, "batch: segment graph instantiation");
+    seg_ids_ = ids;
+}
+
 void BatchSolver::solve(const double* const* vals, const double* const* rhs, double* const* x,
-    culip_sqmr_result* res, bool device_arrays, hipStream_t st)
+    culip_sqmr_result* res, bool device_arrays, hipStream_t user)
 {
     CULIP_REQUIRE(vals && rhs && x, "batch: null argument");
     HIP_CHECK(hipSetDevice(device_));
-    finalize(st);
+    finalize(user);
+    // the caller's stream is done with everything before the call; the
+    // batch runs on its own stream and ends synchronised
+    HIP_CHECK(hipStreamSynchronize(user));
+    hipStream_t st = sq_;
+    seg_prev_.clear();
     const int32_t S = (int32_t) sys_.size();
     for(int32_t s = 0; s < S; ++s) CULIP_REQUIRE(vals[s] && rhs[s] && x[s], "batch: null array");
     const culip_sqmr_opts& O = o_.sqmr;
@@ -531,7 +562,25 @@ void BatchSolver::solve(const double* const* vals, const double* const* rhs, dou
         std::fill(is_end.begin(), is_end.end(), 0);
         if(!act.empty()) {
             const BatchLists& L = lists(slot_[1], act, st);
-            iteration(L, st);
+            // a whole segment (every running system at a check boundary with
+            // `period` iterations left) replays the captured graph when the
+            // same systems ran the previous segment too
+            bool aligned = period > 1;
+            for(int32_t s : act)
+                if(st_[s].it % period != 0 || st_[s].it + period > st_[s].run_max) {
+                    aligned = false;
+                    break;
+                }
+            int32_t steps = 1;
+            if(aligned && act == seg_prev_) {
+                if(!seg_exec_ || seg_ids_ != act) capture_segment(L, period, act, st);
+                HIP_CHECK(hipGraphLaunch(seg_exec_, st));
+                steps = period;
+            } else {
+                iteration(L, st);
+            }
+            if(aligned) seg_prev_ = act;
+            for(int32_t s : act) st_[s].it += steps - 1;
             chk.clear();
             for(int32_t s : act)
                 if((st_[s].it + 1) % period == 0) {

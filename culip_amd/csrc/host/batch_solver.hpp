@@ -100,6 +100,9 @@ private:
     void residual_norms(const BatchLists& L, int32_t bvec, hipStream_t st);
     void read_back(hipStream_t st);
     void upload_sys(hipStream_t st);
+    // `period` iterations of the listed systems as a captured graph
+    void capture_segment(const BatchLists& L, int32_t period, const std::vector<int32_t>& ids,
+        hipStream_t st);
 
     BatchSolverOpts o_;
     int device_ = 0;
@@ -118,6 +121,12 @@ private:
     FactorItems chunks_, blk_, scan_, task_;
     ListSlot slot_[4];
     std::vector<State> st_;
+    // the batch's own stream (graphs cannot be captured on the null stream)
+    hipStream_t sq_ = nullptr;
+    // a segment of `period` iterations captured for the set of running
+    // systems seg_ids_ (replayed while that set runs whole segments)
+    hipGraphExec_t seg_exec_ = nullptr;
+    std::vector<int32_t> seg_ids_, seg_prev_;
     double sec_[2] = {0.0, 0.0};
 };
 
