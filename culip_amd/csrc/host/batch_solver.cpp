@@ -128,15 +128,23 @@ int64_t BatchSolver::device_bytes() const
     return b;
 }
 
-// pooled SQMR work, scalar and flag pools, the device system table, item
-// lists and factor streams, once all systems are known
+// pooled SQMR work, scalar and flag pools, the device system tables, work
+// lists and the batch's stream, once all systems are known (the first
+// solve); a set-up that fails leaves the batch unusable
 void BatchSolver::finalize(hipStream_t st)
 {
     {
         std::lock_guard<std::mutex> g(mu_);
-        if(finalized_) return;
+        if(ready_) return;
+        if(finalized_) fail(CULIP_ERR_STATE, "batch: its set-up failed earlier");
         finalized_ = true;
     }
+    setup(st);
+    ready_ = true;
+}
+
+void BatchSolver::setup(hipStream_t st)
+{
     const int32_t S = (int32_t) sys_.size();
     CULIP_REQUIRE(S > 0, "batch: no systems");
     std::vector<size_t> off(S);

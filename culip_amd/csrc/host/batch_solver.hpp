@@ -87,6 +87,7 @@ private:
     };
 
     void finalize(hipStream_t st);
+    void setup(hipStream_t st);
     template<typename VT> void build_factor();
     template<typename VT> void factor_batched(hipStream_t st);
     const BatchLists& lists(ListSlot& slot, const std::vector<int32_t>& ids, hipStream_t st);
@@ -108,7 +109,8 @@ private:
     int device_ = 0;
     mutable std::mutex mu_;
     std::vector<std::unique_ptr<Entry>> sys_;
-    bool finalized_ = false;
+    bool finalized_ = false;   // no more systems (set-up started)
+    bool ready_ = false;       // set-up done
     DevBuf d_work_, d_scal_, d_flags_, d_iter_, d_sys_;
     std::vector<SqmrWork> work_;
     std::vector<BatchSys> h_sys_;
