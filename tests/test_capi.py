@@ -155,6 +155,27 @@ def test_server_needs_a_device():
     assert e.value.status == 8
 
 
+def test_batch_needs_a_device_and_checks_options():
+    """The batch refuses to start without a HIP device (NO_DEVICE), and its
+    C entry points reject null arguments with INVALID before any device
+    work."""
+    import ctypes as C
+    import torch
+    from culip_amd import _lib
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(ca.CulipError) as e:
+        ca.Batch(prec="f32", jacobi_sweeps=2)
+    assert e.value.status == 8
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.culip_batch_create(None, C.byref(h)) == 1
+    assert L.culip_batch_solve(None, None, None, None, None, 0, None) == 1
+    assert L.culip_batch_add(None, None, 0, None, 0, None, None) == 1
+    assert L.culip_batch_info(None, None, None, None) == 1
+    L.culip_batch_destroy(None)
+
+
 def test_imported_factor_checks_inputs_before_the_device():
     """ImportLDUPreconditioner: malformed inputs are rejected on the host
     with INVALID; well-formed ones need a device (NO_DEVICE here)."""
