@@ -258,6 +258,14 @@ class BlockiLDLt(_Handle):
     def solve_right(self, b, x, transpose=False, stream=None):
         self._solve(SOLVE_RIGHT, b, x, transpose, stream)
 
+    def left_product(self, A, src, x, stream=None):
+        """x = U' \\ (P' (A src)) for a CSRMatrixSPMV A, folded into one
+        launch where the solve schedule allows; returns whether it was."""
+        f = C.c_int32()
+        check(lib().culip_bildlt_solve_left_product(self._h, A._h, _dptr(src), _dptr(x),
+                                                    C.byref(f), _stream(stream)))
+        return bool(f.value)
+
     def get_int(self, name):
         p = C.POINTER(C.c_int32)()
         n = C.c_int64()

@@ -115,6 +115,9 @@ _SIGS = [
     ("culip_bildlt_set_graphs", C.c_int, [C.c_void_p, C.c_int32]),
     ("culip_bildlt_set_pivoting", C.c_int, [C.c_void_p, C.c_int]),
     ("culip_bildlt_set_pivot_precision", C.c_int, [C.c_void_p, C.c_int]),
+    ("culip_bildlt_solve_left_product", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.POINTER(C.c_int32),
+                                                  C.c_void_p]),
     ("culip_bildlt_solve_noalloc", C.c_int, [C.c_void_p, C.c_int, C.c_int32, C.c_void_p,
                                              C.c_void_p, C.c_void_p]),
     ("culip_bildlt_destroy", None, [C.c_void_p]),

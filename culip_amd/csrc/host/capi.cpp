@@ -443,6 +443,28 @@ culip_status culip_bildlt_solve(const culip_bildlt* h, culip_side side, int32_t 
     });
 }
 
+culip_status culip_bildlt_solve_left_product(const culip_bildlt* h, const culip_spmv* K,
+    const double* src, double* x, int32_t* fused, void* stream)
+{
+    return guarded([&] {
+        CULIP_REQUIRE(h && K && src && x, "bildlt: null argument");
+        CULIP_REQUIRE(src != x, "bildlt: the product's source and result must differ");
+        const Spmv& op = K->s;
+        CULIP_REQUIRE(!op.kkt && !op.normal && op.m == h->b.n() && op.n == h->b.n(),
+            "bildlt: the product needs a CSR operator of the factor's size");
+        if(!h->b.factored) fail(CULIP_ERR_STATE, "solve: preconditioner has not been computed");
+        h->b.take_async_error();
+        hipStream_t st = (hipStream_t) stream;
+        const bool f = h->b.left_product(op.A, src, x, st);
+        if(!f) {
+            op.multiply(src, x, 1.0, 0.0, false, st);
+            h->b.left(x, st);
+        }
+        HIP_CHECK(hipGetLastError());
+        if(fused) *fused = f ? 1 : 0;
+    });
+}
+
 culip_status culip_bildlt_solve_noalloc(const culip_bildlt* h, culip_side side,
     int32_t transpose, const double* b, double* x, void* stream)
 {

@@ -337,6 +337,18 @@ int32_t culip_spmv_m(const culip_spmv * op);
 int32_t culip_spmv_n(const culip_spmv * op);
 void culip_spmv_destroy(culip_spmv * op);
 
+/* x = U' \ (P' (K src)): the left solve of an operator product, as SQMR
+ * applies it every iteration. Where the factor's solve forms its rows of
+ * K src itself (single-launch and block-Jacobi schedules, static pivots) it
+ * does so in the same launch; otherwise K src is formed first and solved
+ * (the same bits either way). K: a CSR operator (culip_spmv_create_csr) of
+ * the factor's size; src and x: device doubles, distinct. fused (may be
+ * NULL) receives 1 when the product was folded into the solve. No
+ * counterpart in the reference (its SQMR calls the SpMV and then the left
+ * solve). */
+culip_status culip_bildlt_solve_left_product(const culip_bildlt * h, const culip_spmv * K,
+    const double * src, double * x, int32_t * fused, void * stream);
+
 /* ------------------------------------------------------------------------ */
 /* SQMR (libs/la/iterative_solvers/sqmr.cuh:21-72).                         */
 

@@ -488,6 +488,32 @@ def test_folded_product_equals_separate(hip_device, algo, sw, prec):
     assert np.array_equal(xf, xs)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_left_product_entry_equals_product_then_solve(hip_device, prec):
+    """culip_bildlt_solve_left_product: U' \\ (K src) folded into the solve
+    (single launch, block-Jacobi) or formed first (level launches) gives the
+    bits of the SpMV followed by solve_left, and says which it did."""
+    import torch
+    k, p = _kkt(3000, 1200, 16, ca.ORDER_BLOCK_COLOR)
+    g = ca.BlockiLDLt(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts, prec)
+    g.compute(0, 1.0, 1e-4)
+    A = ca.CSRMatrixSPMV(p.row_ptr, p.col, p.val)
+    src = torch.tensor(np.random.default_rng(9).normal(size=p.row_ptr.size - 1),
+                       device=hip_device)
+    for algo, sw, folded in (("single", 0, True), ("levels", 0, False), ("jacobi", 2, True)):
+        g.set_solve_algorithm(algo, sw)
+        x1 = torch.empty_like(src)
+        assert g.left_product(A, src, x1) == folded, algo
+        y = torch.empty_like(src)
+        A.multiply(src, y)
+        x2 = torch.empty_like(src)
+        g.solve_left(y, x2)
+        torch.cuda.synchronize()
+        assert torch.equal(x1, x2), algo
+    with pytest.raises(ca.CulipError):
+        g.left_product(A, src, src)
+
+
 def test_in_place_solves_equal_out_of_place(hip_device):
     """solve_left / solve_right with x = b (in place, as SQMR applies them)
     give the same bits as with separate arrays, for every schedule."""
