@@ -4,7 +4,7 @@ solve-schedule autotuner, and the batched solve over a request server.
 The autotuner sorts systems into size buckets and, per bucket, times every
 candidate (maximum block size, triangular-solve schedule) on the bucket's
 first system: numeric refactor plus SQMR to the FP64 true-residual
-tolerance, best of a few repetitions after a warm-up. A candidate that
+tolerance, the median of a few repetitions after a warm-up. A candidate that
 misses the tolerance is never picked.
 
 The batch runs on culip_amd.Batch: every system registered under its
@@ -59,16 +59,16 @@ def _time_candidate(p, rhs, prec, B, algo, sweeps, reps, tol, dev):
     g.refactor(dv, 1e-4)
     q.solve(b, x)
     torch.cuda.synchronize()
-    best = float("inf")
+    times = []
     for _ in range(reps):
         t0 = time.perf_counter()
         g.refactor(dv, 1e-4)
         q.solve(b, x)
         torch.cuda.synchronize()
-        best = min(best, time.perf_counter() - t0)
+        times.append(time.perf_counter() - t0)
     if q.result["true_rel_residual"] > tol:
         return float("inf")
-    return best * 1e3
+    return float(np.median(times)) * 1e3
 
 
 def autotune(sizes, make_system, prec="f32", slots=16, block_sizes=BLOCK_SIZES, reps=3,
@@ -104,7 +104,7 @@ def autotune(sizes, make_system, prec="f32", slots=16, block_sizes=BLOCK_SIZES, 
 
 
 def autotune_batch(sizes, make_system, prec="f32", block_sizes=BLOCK_SIZES, sweep_options=(2, 3),
-                   reps=3, tol=1e-8, buckets=BUCKETS, log=None):
+                   reps=5, tol=1e-8, buckets=BUCKETS, log=None):
     """For the batched engine: a block size per size bucket and one
     block-Jacobi sweep count for the whole batch. Every (block size, sweeps)
     pair is timed on each bucket's first system as in autotune; the sweep

@@ -79,7 +79,7 @@ enum BatchVec : int32_t {
 // rows of K per item of the batched CSR product, block rows per item of the
 // batched block-Jacobi sweep
 constexpr int BATCH_SPMV_ROWS = 128;
-constexpr int JACOBI_BATCH_ROWS = 8;
+constexpr int JACOBI_BATCH_ROWS = 4;
 
 // Work items of one kind for a set of systems: items[i] = {system, local}.
 struct BatchList {
