@@ -68,6 +68,9 @@ def parse():
     ap.add_argument("--threshold", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-requests", type=int, default=64,
+                    help="serve workload: requests of the stream served by the plain FP64 "
+                         "per-system path for the speedup (0: skip)")
     ap.add_argument("--no-plain", action="store_true",
                     help="batch workload: skip the plain FP64 per-system comparison "
                          "(profiling runs)")
@@ -332,6 +335,33 @@ def bench_serve(a):
     srv.close()
     conv = np.array([r[1]["true_rel_residual"] <= 1e-8 for r in res])
     span = last - first
+    # the plain FP64 per-system path on the first requests of the stream: one
+    # request at a time (one slot), FP64 factor, exact single-launch solves,
+    # the same equilibration, back to back; converged solves only
+    plain = None
+    if a.plain_requests > 0:
+        sel = list(range(min(a.plain_requests, a.requests)))
+        srv1 = ca.Server(slots=1, prec="f64", max_it=a.max_it or 3000,
+                         solve_algorithm="single", equilibrate_sweeps=a.equilibrate_sweeps)
+        used = sorted({int(picks[j]) for j in sel})
+        reg = {}
+        for i in used:
+            k, p = _pattern(ca, int(sizes[i]), 1000 + i, a.max_block, order)
+            reg[i] = srv1.add_pattern(p.row_ptr, p.col, p.val, p.piv_starts, p.block_starts)
+        warm1 = srv1.submit(reg[used[0]], pats[used[0]][1][0], pats[used[0]][2])
+        srv1.wait(warm1)
+        t0 = time.perf_counter()
+        ids1 = [srv1.submit(reg[int(picks[j])], pats[picks[j]][1][sets[j]], pats[picks[j]][2])
+                for j in sel]
+        out1 = [srv1.wait(r) for r in ids1]
+        t1 = time.perf_counter() - t0
+        srv1.close()
+        c1 = sum(r[1]["true_rel_residual"] <= 1e-8 for r in out1)
+        plain = {"requests": len(sel), "converged": int(c1), "solves_per_s": c1 / t1,
+                 "sqmr_iterations_mean": float(np.mean([r[1]["iterations"] for r in out1])),
+                 "max_true_rel_residual": max(r[1]["true_rel_residual"] for r in out1)}
+        log(f"plain FP64 per-system path: {plain['solves_per_s']:.1f} solves/s over "
+            f"{len(sel)} requests")
     return {
         "metric": "KKT solves/sec to rel. residual 1e-8 (FP32-factor vs FP64); p50/p99 solve latency",
         # only requests that reached the tolerance count as solves
@@ -355,6 +385,10 @@ def bench_serve(a):
         "p50_ms_converged": float(np.percentile(lat[conv], 50)) if conv.any() else None,
         "p99_ms_converged": float(np.percentile(lat[conv], 99)) if conv.any() else None,
         "warmup_solves_per_s": thr, "setup_s": t_setup,
+        "plain_f64_per_system": plain,
+        "speedup_vs_plain_f64_per_system":
+            None if not plain or plain["solves_per_s"] <= 0
+            else float(conv.sum()) / span / plain["solves_per_s"],
         "device_bytes_peak": st["peak"], "device_budget": st["budget"],
         "instances": st["instances"],
         "roofline": None, "cpu_baseline": None,
