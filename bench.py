@@ -430,6 +430,17 @@ def bench_batch(a):
     log(f"{len(systems)} systems generated in {t_gen:.1f}s")
     res, secs, t_reg, phases = cb.solve_batch_engine(systems, a.prec, sweeps, log=log)
     el = secs[-1]
+    # the same batch with the FP64 factor: batching alone, without the FP32
+    # factor's halved factor bytes
+    f64 = None
+    if a.prec != "f64" and not a.no_plain:
+        r64, s64, _, ph64 = cb.solve_batch_engine(systems, "f64", sweeps, log=log)
+        f64 = {"value": len(r64) / s64[-1], "phases_s": {"refactors": ph64["factor_s"],
+                                                         "sqmr": ph64["sqmr_s"]},
+               "sqmr_iterations_mean": float(np.mean([r[1]["iterations"] for r in r64])),
+               "sqmr_iterations_max": int(max(r[1]["iterations"] for r in r64)),
+               "converged_fraction": float(np.mean([r[1]["true_rel_residual"] <= 1e-8
+                                                    for r in r64]))}
     # the plain path: FP64 factor, exact single-launch solves (the reference
     # driver's SOLVE_BLOCK), one system at a time
     if a.no_plain:
@@ -466,6 +477,7 @@ def bench_batch(a):
         "plain_max_true_rel_residual":
             None if a.no_plain else max(r[1]["true_rel_residual"] for r in base),
         "phases_s": {"refactors": phases["factor_s"], "sqmr": phases["sqmr_s"]},
+        "f64_factor_batch": f64,
         "device_bytes": phases["device_bytes"],
         "setup": {"autotune_s": t_tune, "generate_s": t_gen,
                   "register_s_symbolic_and_upload": t_reg,
@@ -686,7 +698,8 @@ def main():
                                     "sqmr_iterations_mean", "p50_ms", "p99_ms",
                                     "plain_f64_per_system_solves_per_s",
                                     "speedup_vs_plain_f64_per_system",
-                                    "end_to_end_solves_per_s", "phases_s", "setup")}
+                                    "end_to_end_solves_per_s", "phases_s", "f64_factor_batch",
+                                    "setup")}
         batch["config"] = bo["config"]
 
     cpu = None
