@@ -94,6 +94,10 @@ struct BatchLists {
     BatchList blk;    // ceil(nb / JACOBI_BATCH_ROWS) items of 64 threads (block-Jacobi)
     BatchList spmv;   // ceil(n / BATCH_SPMV_ROWS) items of 256 threads (CSR product)
     BatchList sys;    // one item per system (scalars)
+    // every listed system's vector grid covers its rows in one round
+    // (n <= G * 256): the vector kernels then hold one element per thread
+    // instead of a round of four (same arithmetic, fewer registers)
+    bool one_round = false;
 };
 
 // the vector `id` of system s

@@ -250,17 +250,18 @@ __device__ __forceinline__ DP dsolve_loaded(const DRow& d, double r1, double r2)
     return o;
 }
 
+template<int R = VR>
 __device__ __forceinline__ void alpha_body(int q_, int G_, int32_t n, const double* Av, const double* v,
     const double* z, const double* tri, const int32_t* ip, double* part)
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
     const int64_t S = (int64_t) G_ * TPB;
-    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
-        double av[VR], av1[VR], vv[VR], vv1[VR], zz[VR];
-        DRow d[VR];
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * R) {
+        double av[R], av1[R], vv[R], vv1[R], zz[R];
+        DRow d[R];
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = min(g0 + r * S, (int64_t) n - 1);
             const int64_t g1 = min(g + 1, (int64_t) n - 1);
             av[r] = Av[g];
@@ -271,7 +272,7 @@ __device__ __forceinline__ void alpha_body(int q_, int G_, int32_t n, const doub
             if(tri) d[r] = dload(g, n, tri, ip);
         }
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = g0 + r * S;
             if(g >= n) break;
             DP o;
@@ -286,61 +287,75 @@ __device__ __forceinline__ void alpha_body(int q_, int G_, int32_t n, const doub
     if(threadIdx.x == 0) { part_at(part, 0, q_, G_) = num; part_at(part, 1, q_, G_) = den; }
 }
 
+// The kernels that start with a reduction of the previous kernel's partial
+// sums issue the loads of their first round before it, so its memory round
+// trip overlaps the reduction's.
+template<int R = VR>
 __device__ __forceinline__ void vnext_body(int q_, int G_, int32_t n, int k, const double* Av,
     const double* vk, const double* vkm1, double* tv, const double* scal, double* part)
 {
     __shared__ double sh[16];
     const int G = G_;
-    double r[2];
-    reduce_parts<2>(part, G, {0, 1}, r, sh);
-    const double alpha = r[0] / r[1];
-    const double beta = scal[S_BETA + k % 3];
-    double ss = 0.0;
     const int64_t S = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
-        double a[VR], b[VR], c[VR];
+    double a[R], b[R], c[R];
+    auto load = [&](int64_t g0) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = min(g0 + r * S, (int64_t) n - 1);
             a[r] = Av[g];
             b[r] = vk[g];
             c[r] = vkm1[g];
         }
+    };
+    int64_t g0 = (int64_t) q_ * TPB + threadIdx.x;
+    if(g0 < n) load(g0);
+    double rp[2];
+    reduce_parts<2>(part, G, {0, 1}, rp, sh);
+    const double alpha = rp[0] / rp[1];
+    const double beta = scal[S_BETA + k % 3];
+    double ss = 0.0;
+    for(; g0 < n; g0 += S * R) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = g0 + r * S;
             if(g >= n) break;
             const double t = fma(-beta, c[r], fma(-alpha, b[r], a[r]));
             tv[g] = t;
             ss = fma(t, t, ss);
         }
+        if(g0 + S * R < n) load(g0 + S * R);
     }
     ss = block_sum(ss, sh);
     if(threadIdx.x == 0) part_at(part, 2, q_, G_) = ss;
 }
 
+template<int R = VR>
 __device__ __forceinline__ void vscale_body(int q_, int G_, int32_t n, const double* tv, double* v1,
     double* z, double* w1, double* ev, const double* tri, const int32_t* ip, const double* scal,
     const double* part)
 {
     __shared__ double sh[16];
     const int G = G_;
-    const double delta = sqrt(reduce_part(part, G, 2, sh));
-    const double t = 1.0 / delta;
-    const double we = scal[S_ETA] * delta;
     const int64_t S = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
-        double a[VR], a1[VR];
-        DRow d[VR];
+    double a[R], a1[R];
+    DRow d[R];
+    auto load = [&](int64_t g0) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = min(g0 + r * S, (int64_t) n - 1);
             a[r] = tv[g];
             a1[r] = tv[min(g + 1, (int64_t) n - 1)];
             if(tri) d[r] = dload(g, n, tri, ip);
         }
+    };
+    int64_t g0 = (int64_t) q_ * TPB + threadIdx.x;
+    if(g0 < n) load(g0);
+    const double delta = sqrt(reduce_part(part, G, 2, sh));
+    const double t = 1.0 / delta;
+    const double we = scal[S_ETA] * delta;
+    for(; g0 < n; g0 += S * R) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = g0 + r * S;
             if(g >= n) break;
             const double r1 = a[r] * t;
@@ -352,26 +367,28 @@ __device__ __forceinline__ void vscale_body(int q_, int G_, int32_t n, const dou
             if(o.w1) { z[g] = o.g1; w1[g] = o.g1 * we; ev[g] = o.g1; }
             if(o.w2) { z[g + 1] = o.g2; w1[g + 1] = o.g2 * we; ev[g + 1] = o.g2; }
         }
+        if(g0 + S * R < n) load(g0 + S * R);
     }
 }
 
+template<int R = VR>
 __device__ __forceinline__ void beta_body(int q_, int G_, int32_t n, const double* wk, const double* Av,
     const double* vk, double* part)
 {
     __shared__ double sh[16];
     double num = 0.0, den = 0.0;
     const int64_t S = (int64_t) G_ * TPB;
-    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * VR) {
-        double a[VR], b[VR], c[VR];
+    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += S * R) {
+        double a[R], b[R], c[R];
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = min(g0 + r * S, (int64_t) n - 1);
             a[r] = wk[g];
             b[r] = Av[g];
             c[r] = vk[g];
         }
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             if(g0 + r * S >= n) break;
             num = fma(a[r], b[r], num);
             den = fma(a[r], c[r], den);
@@ -393,29 +410,18 @@ __device__ __forceinline__ void iteration_scalars(double* S, const double* scal,
     scalar_step<K0>(S, alpha, delta, beta);
 }
 
-template<int K0>
+template<int K0, int R>
 __device__ __forceinline__ void pupdate(int q_, int G_, int32_t n, double* w1, const double* vk, double* pk,
     const double* pkm1, const double* pkm2, double* x, const double* scal, double* part,
     const int32_t* flags)
 {
     __shared__ double sh[64];
-    double S[NSCAL];
     const int G = G_;
-    double r[5];
-    reduce_parts<5>(part, G, {0, 1, 2, 3, 4}, r, sh);
-    iteration_scalars<K0>(S, scal, r);
-    const double ib = 1.0 / S[S_BETA + (K0 + 1) % 3];
-    const double nbrr = -S[S_BRR], neps = -S[S_EPS], ir = 1.0 / S[S_RHO], tau = S[S_TAU];
-    // after a Lanczos breakdown (flagged by the previous iteration) the
-    // recurrence divides by a vanishing delta: x keeps the value it had when
-    // the breakdown was detected
-    const bool frozen = flags[0] != 0;
-    double ss = 0.0;
     const int64_t SG = (int64_t) G * TPB;
-    for(int64_t g0 = (int64_t) q_ * TPB + threadIdx.x; g0 < n; g0 += SG * VR) {
-        double ww[VR], a[VR], b[VR], c[VR], xx[VR];
+    double ww[R], a[R], b[R], c[R], xx[R];
+    auto load = [&](int64_t g0) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = min(g0 + r * SG, (int64_t) n - 1);
             ww[r] = w1[g];
             a[r] = vk[g];
@@ -423,8 +429,33 @@ __device__ __forceinline__ void pupdate(int q_, int G_, int32_t n, double* w1, c
             c[r] = pkm2[g];
             xx[r] = x[g];
         }
+    };
+    int64_t g0 = (int64_t) q_ * TPB + threadIdx.x;
+    if(g0 < n) load(g0);
+    double rp[5];
+    reduce_parts<5>(part, G, {0, 1, 2, 3, 4}, rp, sh);
+    // one thread runs the scalar step and shares the five factors the
+    // update needs (the step's 32 scalars held by every thread would cost
+    // the registers of two more waves per SIMD)
+    if(threadIdx.x == 0) {
+        double S[NSCAL];
+        iteration_scalars<K0>(S, scal, rp);
+        sh[32] = 1.0 / S[S_BETA + (K0 + 1) % 3];
+        sh[33] = -S[S_BRR];
+        sh[34] = -S[S_EPS];
+        sh[35] = 1.0 / S[S_RHO];
+        sh[36] = S[S_TAU];
+    }
+    __syncthreads();
+    const double ib = sh[32], nbrr = sh[33], neps = sh[34], ir = sh[35], tau = sh[36];
+    // after a Lanczos breakdown (flagged by the previous iteration) the
+    // recurrence divides by a vanishing delta: x keeps the value it had when
+    // the breakdown was detected
+    const bool frozen = flags[0] != 0;
+    double ss = 0.0;
+    for(; g0 < n; g0 += SG * R) {
 #pragma unroll
-        for(int r = 0; r < VR; ++r) {
+        for(int r = 0; r < R; ++r) {
             const int64_t g = g0 + r * SG;
             if(g >= n) break;
             w1[g] = ww[r] * ib;
@@ -437,19 +468,21 @@ __device__ __forceinline__ void pupdate(int q_, int G_, int32_t n, double* w1, c
             if(!frozen) x[g] = xv;
             ss = fma(xv, xv, ss);
         }
+        if(g0 + SG * R < n) load(g0 + SG * R);
     }
     ss = block_sum(ss, sh);
     if(threadIdx.x == 0) part_at(part, 5, q_, G_) = ss;
 }
 
+template<int R = VR>
 __device__ __forceinline__ void pupdate_body(int q_, int G_, int32_t n, int k, double* w1, const double* vk,
     double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
     double* part, const int32_t* flags)
 {
     switch(k % 3) {
-    case 0: pupdate<0>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
-    case 1: pupdate<1>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
-    default: pupdate<2>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    case 0: pupdate<0, R>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    case 1: pupdate<1, R>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
+    default: pupdate<2, R>(q_, G_, n, w1, vk, pk, pkm1, pkm2, x, scal, part, flags); break;
     }
 }
 
@@ -703,46 +736,51 @@ __global__ void k_batch_expand(const int4* ent, int2* items)
     for(int j = threadIdx.x; j < e.y; j += blockDim.x) items[e.z + j] = make_int2(e.x, j);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void bk_alpha(const int2* items, const BatchSys* sys)
 {
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int k = *S.iter;
-    alpha_body(it.y, S.G, S.n, S.av[k & 1], S.v[k % 3], S.z, S.tri, S.ip, S.part);
+    alpha_body<R>(it.y, S.G, S.n, S.av[k & 1], S.v[k % 3], S.z, S.tri, S.ip, S.part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void bk_vnext(const int2* items, const BatchSys* sys)
 {
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int k = *S.iter;
-    vnext_body(it.y, S.G, S.n, k, S.av[k & 1], S.v[k % 3], S.v[(k + 2) % 3], S.tv, S.scal,
+    vnext_body<R>(it.y, S.G, S.n, k, S.av[k & 1], S.v[k % 3], S.v[(k + 2) % 3], S.tv, S.scal,
         S.part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void bk_vscale(const int2* items, const BatchSys* sys)
 {
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int k1 = (*S.iter + 1) % 3;
-    vscale_body(it.y, S.G, S.n, S.tv, S.v[k1], S.z, S.w[k1], S.ev, S.tri, S.ip, S.scal, S.part);
+    vscale_body<R>(it.y, S.G, S.n, S.tv, S.v[k1], S.z, S.w[k1], S.ev, S.tri, S.ip, S.scal, S.part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void bk_beta(const int2* items, const BatchSys* sys)
 {
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int k = *S.iter;
-    beta_body(it.y, S.G, S.n, S.w[k % 3], S.av[(k & 1) ^ 1], S.v[k % 3], S.part);
+    beta_body<R>(it.y, S.G, S.n, S.w[k % 3], S.av[(k & 1) ^ 1], S.v[k % 3], S.part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void bk_pupdate(const int2* items, const BatchSys* sys)
 {
     const int2 it = items[blockIdx.x];
     const BatchSys& S = sys[it.x];
     const int k = *S.iter;
     const int k0 = k % 3, k1 = (k + 1) % 3, km1 = (k + 2) % 3;
-    pupdate_body(it.y, S.G, S.n, k, S.w[k1], S.v[k0], S.p[k0], S.p[km1], S.p[k1], S.xt, S.scal,
+    pupdate_body<R>(it.y, S.G, S.n, k, S.w[k1], S.v[k0], S.p[k0], S.p[km1], S.p[k1], S.xt, S.scal,
         S.part, S.flags);
 }
 
@@ -1135,19 +1173,34 @@ void batch_step(BatchStep s, const BatchLists& L, const BatchSys* sys, int32_t a
     auto grid = [](const BatchList& l) { return (unsigned) l.count; };
     switch(s) {
     case BS_ALPHA:
-        if(L.vec.count) bk_alpha<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        if(L.vec.count) {
+            if(L.one_round) bk_alpha<1><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+            else bk_alpha<VR><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        }
         break;
     case BS_VNEXT:
-        if(L.vec.count) bk_vnext<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        if(L.vec.count) {
+            if(L.one_round) bk_vnext<1><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+            else bk_vnext<VR><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        }
         break;
     case BS_VSCALE:
-        if(L.vec.count) bk_vscale<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        if(L.vec.count) {
+            if(L.one_round) bk_vscale<1><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+            else bk_vscale<VR><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        }
         break;
     case BS_BETA:
-        if(L.vec.count) bk_beta<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        if(L.vec.count) {
+            if(L.one_round) bk_beta<1><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+            else bk_beta<VR><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        }
         break;
     case BS_PUPDATE:
-        if(L.vec.count) bk_pupdate<<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        if(L.vec.count) {
+            if(L.one_round) bk_pupdate<1><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+            else bk_pupdate<VR><<<grid(L.vec), TPB, 0, st>>>(L.vec.items, sys);
+        }
         break;
     case BS_SCALARS:
         if(L.sys.count) bk_scalars<<<grid(L.sys), TPB, 0, st>>>(L.sys.items, sys, tol, brel);

@@ -297,6 +297,11 @@ const BatchLists& BatchSolver::lists(ListSlot& slot, const std::vector<int32_t>&
             batch_expand(slot.ent.as<int4>() + (size_t) q * m, m, slot.items[q].as<int2>(), st);
     }
     BatchList* dst[5] = {&slot.L.vec, &slot.L.row, &slot.L.blk, &slot.L.spmv, &slot.L.sys};
+    slot.L.one_round = true;
+    for(int32_t k = 0; k < m; ++k) {
+        const BatchSys& B = h_sys_[ids[k]];
+        if((int64_t) B.n > (int64_t) B.G * 256) slot.L.one_round = false;
+    }
     for(int q = 0; q < 5; ++q) {
         dst[q]->items = slot.items[q].as<int2>();
         dst[q]->count = total[q];
