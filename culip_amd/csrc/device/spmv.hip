@@ -15,6 +15,9 @@ namespace {
 
 constexpr int LPR = 8;   // lanes per row
 constexpr int SPMV_U = 3; // entries per lane and round of a CSR row
+// the same for the batched product, whose lane groups hold four rows: two
+// entries per row and round keep it at six waves per SIMD (three: four)
+constexpr int BATCH_SPMV_U = 2;
 
 // the sum of one CSR row over the eight lanes sub = 0..7 of a group (every
 // lane returns it)
@@ -93,13 +96,13 @@ __global__ __launch_bounds__(256) void bk_csr_spmv(const int2* items, const Batc
         if(row[j] >= n) e[j] = s[j];
         acc[j] = 0.0;
     }
-    for(int32_t off = sub;; off += LPR * SPMV_U) {
-        int32_t c[RB][SPMV_U];
-        double v[RB][SPMV_U], xv[RB][SPMV_U];
+    for(int32_t off = sub;; off += LPR * BATCH_SPMV_U) {
+        int32_t c[RB][BATCH_SPMV_U];
+        double v[RB][BATCH_SPMV_U], xv[RB][BATCH_SPMV_U];
 #pragma unroll
         for(int j = 0; j < RB; ++j)
 #pragma unroll
-            for(int u = 0; u < SPMV_U; ++u) {
+            for(int u = 0; u < BATCH_SPMV_U; ++u) {
                 const int32_t q = max(min(s[j] + off + u * LPR, e[j] - 1), 0);
                 c[j][u] = ci[q];
                 v[j][u] = cv[q];
@@ -107,14 +110,14 @@ __global__ __launch_bounds__(256) void bk_csr_spmv(const int2* items, const Batc
 #pragma unroll
         for(int j = 0; j < RB; ++j)
 #pragma unroll
-            for(int u = 0; u < SPMV_U; ++u) xv[j][u] = x[c[j][u]];
+            for(int u = 0; u < BATCH_SPMV_U; ++u) xv[j][u] = x[c[j][u]];
         bool more = false;
 #pragma unroll
         for(int j = 0; j < RB; ++j) {
 #pragma unroll
-            for(int u = 0; u < SPMV_U; ++u)
+            for(int u = 0; u < BATCH_SPMV_U; ++u)
                 if(s[j] + off + u * LPR < e[j]) acc[j] = fma(v[j][u], xv[j][u], acc[j]);
-            more = more || s[j] + off + LPR * SPMV_U < e[j];
+            more = more || s[j] + off + LPR * BATCH_SPMV_U < e[j];
         }
         if(wave_ballot(more) == 0) break;
     }
