@@ -631,37 +631,44 @@ __device__ __forceinline__ void norm2_pair_body(int q_, int G_, int32_t n, const
     if(threadIdx.x == 0) { part_at(part, 0, q_, G_) = sa; part_at(part, 1, q_, G_) = sb; }
 }
 
-// the single-system kernels: the bodies at their own grid position
+// the single-system kernels: the bodies at their own grid position; R = 1
+// when the grid covers the rows in one round (fewer registers, same
+// arithmetic), else VR
+template<int R>
 __global__ __launch_bounds__(TPB) void k_alpha(int32_t n, const double* Av, const double* v,
     const double* z, const double* tri, const int32_t* ip, double* part)
 {
-    alpha_body(blockIdx.x, gridDim.x, n, Av, v, z, tri, ip, part);
+    alpha_body<R>(blockIdx.x, gridDim.x, n, Av, v, z, tri, ip, part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void k_vnext(int32_t n, int k, const double* Av,
     const double* vk, const double* vkm1, double* tv, const double* scal, double* part)
 {
-    vnext_body(blockIdx.x, gridDim.x, n, k, Av, vk, vkm1, tv, scal, part);
+    vnext_body<R>(blockIdx.x, gridDim.x, n, k, Av, vk, vkm1, tv, scal, part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void k_vscale(int32_t n, const double* tv, double* v1,
     double* z, double* w1, double* ev, const double* tri, const int32_t* ip, const double* scal,
     const double* part)
 {
-    vscale_body(blockIdx.x, gridDim.x, n, tv, v1, z, w1, ev, tri, ip, scal, part);
+    vscale_body<R>(blockIdx.x, gridDim.x, n, tv, v1, z, w1, ev, tri, ip, scal, part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void k_beta(int32_t n, const double* wk, const double* Av,
     const double* vk, double* part)
 {
-    beta_body(blockIdx.x, gridDim.x, n, wk, Av, vk, part);
+    beta_body<R>(blockIdx.x, gridDim.x, n, wk, Av, vk, part);
 }
 
+template<int R>
 __global__ __launch_bounds__(TPB) void k_pupdate(int32_t n, int k, double* w1, const double* vk,
     double* pk, const double* pkm1, const double* pkm2, double* x, const double* scal,
     double* part, const int32_t* flags)
 {
-    pupdate_body(blockIdx.x, gridDim.x, n, k, w1, vk, pk, pkm1, pkm2, x, scal, part, flags);
+    pupdate_body<R>(blockIdx.x, gridDim.x, n, k, w1, vk, pk, pkm1, pkm2, x, scal, part, flags);
 }
 
 __global__ __launch_bounds__(TPB) void k_sumsq(int32_t n, const double* a, double* part, int slot)
@@ -968,18 +975,33 @@ static void sqmr_run(const SqmrOps& ops, SqmrWork& w, const double* b, double* x
     // the kernels of iteration `i`: they depend on i only through i mod 3
     // (the 3-cyclic vectors) and the parity of `cur` (the two A ev vectors),
     // so iterations i and i + 6 launch the same work
+    const bool one_round = (int64_t) n <= (int64_t) G * TPB;
     auto launch_iteration = [&](int i) {
         const int k0 = i % 3, k1 = (i + 1) % 3, km1 = (i + 2) % 3, km2 = (i + 1) % 3;
-        k_alpha<<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
-        k_vnext<<<G, TPB, 0, st>>>(n, i, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal, w.part);
-        k_vscale<<<G, TPB, 0, st>>>(n, w.tv, w.v[k1], w.z, w.w[k1], w.ev, ops.tri, ops.is_piv,
-            w.scal, w.part);
         const int nxt = cur ^ 1;
+        if(one_round) {
+            k_alpha<1><<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
+            k_vnext<1><<<G, TPB, 0, st>>>(n, i, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal, w.part);
+            k_vscale<1><<<G, TPB, 0, st>>>(n, w.tv, w.v[k1], w.z, w.w[k1], w.ev, ops.tri,
+                ops.is_piv, w.scal, w.part);
+        } else {
+            k_alpha<VR><<<G, TPB, 0, st>>>(n, w.av[cur], w.v[k0], w.z, ops.tri, ops.is_piv, w.part);
+            k_vnext<VR><<<G, TPB, 0, st>>>(n, i, w.av[cur], w.v[k0], w.v[km1], w.tv, w.scal,
+                w.part);
+            k_vscale<VR><<<G, TPB, 0, st>>>(n, w.tv, w.v[k1], w.z, w.w[k1], w.ev, ops.tri,
+                ops.is_piv, w.scal, w.part);
+        }
         if(ops.right) ops.right(w.ev, st);
         apply_left_op(w.ev, w.av[nxt]);
-        k_beta<<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
-        k_pupdate<<<G, TPB, 0, st>>>(n, i, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2], w.xt,
-            w.scal, w.part, w.flags);
+        if(one_round) {
+            k_beta<1><<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
+            k_pupdate<1><<<G, TPB, 0, st>>>(n, i, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2],
+                w.xt, w.scal, w.part, w.flags);
+        } else {
+            k_beta<VR><<<G, TPB, 0, st>>>(n, w.w[k0], w.av[nxt], w.v[k0], w.part);
+            k_pupdate<VR><<<G, TPB, 0, st>>>(n, i, w.w[k1], w.v[k0], w.p[k0], w.p[km1], w.p[km2],
+                w.xt, w.scal, w.part, w.flags);
+        }
         k_scalars<<<1, TPB, 0, st>>>(i, G, w.scal, w.part, w.flags, opt.tol,
             period > 0 ? SQMR_BREAKDOWN_REL : -1.0);
         cur = nxt;
