@@ -673,13 +673,16 @@ def main():
                 "true_rel_residual": rv["true_rel_residual"],
                 "rel_diff_vs_fp64_solution": float((xv - x).norm() / x.norm())}
 
-    # FP32 factor (FP64 SQMR, FP64 true residual); block-Jacobi solves
-    other = jac = None
+    # FP32 factor (FP64 SQMR, FP64 true residual); block-Jacobi solves, with
+    # the FP64 and the FP32 factor (the sweeps read the factor's entries in
+    # its storage precision, so that is where the FP32 factor can pay)
+    other = jac = jac32 = None
     if a.prec == "f64" and not a.no_variants:
         other = variant("f32", a.solve_algorithm, a.jacobi_sweeps)
         if a.solve_algorithm in ("single", "levels"):
             log("variant: jacobi solves, 2 sweeps")
             jac = variant("f64", "jacobi", 2)
+            jac32 = variant("f32", "jacobi", 2)
 
     prec_var = None
     if not a.no_variants and not a.no_cpu_baseline:
@@ -753,6 +756,8 @@ def main():
         out["f32_factor"] = other
     if jac is not None:
         out["jacobi_solves"] = jac
+    if jac32 is not None:
+        out["f32_factor_jacobi_solves"] = jac32
     if prec_var is not None:
         out["precision_variant"] = prec_var
     if batch is not None:
