@@ -81,6 +81,17 @@ enum BatchVec : int32_t {
 constexpr int BATCH_SPMV_ROWS = 128;
 constexpr int JACOBI_BATCH_ROWS = 4;
 
+// XCD-aware lists: a 1D grid's workgroups go to the XCDs round-robin
+// (workgroup w to XCD w % BATCH_XCDS). The lists of the product and of the
+// sweeps give each system one XCD: item t of a group sits at position
+// BATCH_XCDS t + group, so every XCD runs the items of its own systems and
+// keeps their gathered x in its own L2. The groups are balanced by item
+// count; positions past a group's end hold {-1, 0}, which the kernels skip.
+// Below BATCH_XCD_MIN_SYSTEMS running systems the lists stay plain (one
+// system on one XCD would leave seven idle).
+constexpr int BATCH_XCDS = 8;
+constexpr int BATCH_XCD_MIN_SYSTEMS = 64;
+
 // Work items of one kind for a set of systems: items[i] = {system, local}.
 struct BatchList {
     const int2* items = nullptr;
@@ -122,8 +133,12 @@ __device__ inline double* batch_vec(const BatchSys& s, int32_t id)
     }
 }
 
-// ent[i] = {system, count, offset, -}: items[offset + j] = {system, j}
-void batch_expand(const int4* ent, int32_t count, int2* items, hipStream_t st);
+// ent[i] = {system, count, offset, group}: items[offset + j] = {system, j}
+// (plain), or with interleaved_total > 0 items[BATCH_XCDS (offset + j) +
+// group] = {system, j} after every one of the interleaved_total items is
+// set to {-1, 0}
+void batch_expand(const int4* ent, int32_t count, int2* items, int32_t interleaved_total,
+    hipStream_t st);
 
 // the steps of the recurrence (sqmr.hip)
 enum BatchStep : int32_t {

@@ -841,6 +841,7 @@ __global__ __launch_bounds__(64) void bk_jacobi_sweep(const int2* items, const B
     __shared__ double res[33];
     const int lane = lane_id();
     const int2 it = items[blockIdx.x];
+    if(it.x < 0) return;   // a hole of an XCD-aware list
     const BatchSys& S = sys[it.x];
     const int4* task = S.task[tr];
     const int32_t* o_col = S.o_col[tr];

@@ -76,6 +76,7 @@ __global__ __launch_bounds__(256) void bk_csr_spmv(const int2* items, const Batc
 {
     constexpr int RB = BATCH_SPMV_ROWS / 32;
     const int2 it = items[blockIdx.x];
+    if(it.x < 0) return;   // a hole of an XCD-aware list (the whole workgroup)
     const BatchSys& S = sys[it.x];
     const int sub = threadIdx.x & (LPR - 1);
     const int grp = threadIdx.x / LPR;

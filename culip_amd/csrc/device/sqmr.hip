@@ -737,10 +737,19 @@ __global__ __launch_bounds__(TPB) void k_norm_final(int G, const double* part, d
 // single-system kernel's workgroup `it.y` of system `it.x` (the iteration's
 // cyclic vectors chosen from the system's own iteration counter).
 
-__global__ void k_batch_expand(const int4* ent, int2* items)
+__global__ void k_batch_expand(const int4* ent, int2* items, int32_t interleaved)
 {
     const int4 e = ent[blockIdx.x];
-    for(int j = threadIdx.x; j < e.y; j += blockDim.x) items[e.z + j] = make_int2(e.x, j);
+    for(int j = threadIdx.x; j < e.y; j += blockDim.x) {
+        const int64_t pos = interleaved ? (int64_t) BATCH_XCDS * (e.z + j) + e.w : e.z + j;
+        items[pos] = make_int2(e.x, j);
+    }
+}
+
+__global__ void k_batch_holes(int2* items, int32_t total)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < total) items[i] = make_int2(-1, 0);
 }
 
 template<int R>
@@ -1184,9 +1193,13 @@ void sqmr_solve(const SqmrOps& ops, SqmrWork& w, const double* b, double* x,
 }
 
 
-void batch_expand(const int4* ent, int32_t count, int2* items, hipStream_t st)
+void batch_expand(const int4* ent, int32_t count, int2* items, int32_t interleaved_total,
+    hipStream_t st)
 {
-    if(count > 0) k_batch_expand<<<count, 256, 0, st>>>(ent, items);
+    if(interleaved_total > 0)
+        k_batch_holes<<<(unsigned) ((interleaved_total + 255) / 256), 256, 0, st>>>(items,
+            interleaved_total);
+    if(count > 0) k_batch_expand<<<count, 256, 0, st>>>(ent, items, interleaved_total > 0);
 }
 
 void batch_step(BatchStep s, const BatchLists& L, const BatchSys* sys, int32_t a, int32_t b,

@@ -170,6 +170,7 @@ void BatchSolver::setup(hipStream_t st)
     work_.assign(S, SqmrWork{});
     h_sys_.assign(S, BatchSys{});
     int64_t tot[5] = {0, 0, 0, 0, 0};
+    int64_t maxc[5] = {0, 0, 0, 0, 0};
     for(int32_t s = 0; s < S; ++s) {
         Entry& e = *sys_[s];
         Bildlt& f = *e.f;
@@ -228,12 +229,17 @@ void BatchSolver::setup(hipStream_t st)
         };
         if(f.f32()) set_plan(f.plan<float>());
         else set_plan(f.plan<double>());
-        tot[0] += B.G;
-        tot[1] += cdiv(B.n, 256);
-        tot[2] += cdiv(B.nb, JACOBI_BATCH_ROWS);
-        tot[3] += cdiv(B.n, BATCH_SPMV_ROWS);
-        tot[4] += 1;
+        const int64_t c[5] = {B.G, cdiv(B.n, 256), cdiv(B.nb, JACOBI_BATCH_ROWS),
+            cdiv(B.n, BATCH_SPMV_ROWS), 1};
+        for(int k = 0; k < 5; ++k) {
+            tot[k] += c[k];
+            maxc[k] = std::max(maxc[k], c[k]);
+        }
     }
+    // an XCD-aware list is at most BATCH_XCDS times its largest group, and
+    // a group (filled largest system first into the lightest group) exceeds
+    // the mean by at most one system's items
+    for(int k = 0; k < 5; ++k) tot[k] += (int64_t) BATCH_XCDS * maxc[k];
     for(int k = 0; k < 5; ++k)
         CULIP_REQUIRE(tot[k] < (int64_t) INT32_MAX, "batch: too many work items for one launch");
     for(ListSlot& l : slot_) {
@@ -288,13 +294,40 @@ const BatchLists& BatchSolver::lists(ListSlot& slot, const std::vector<int32_t>&
             total[q] += cnt[q];
         }
     }
+    // the product's and the sweeps' lists XCD-aware (device/batch.hpp)
+    int32_t inter[5] = {0, 0, 0, 0, 0};
+    if(m >= BATCH_XCD_MIN_SYSTEMS) {
+        std::vector<int32_t> order(m);
+        for(int q : {2, 3}) {
+            int4* e = slot.h_ent + (size_t) q * m;
+            for(int32_t k = 0; k < m; ++k) order[k] = k;
+            std::stable_sort(order.begin(), order.end(),
+                [&](int32_t a, int32_t b) { return e[a].y > e[b].y; });
+            int64_t load[BATCH_XCDS] = {};
+            for(int32_t k : order) {
+                int g = 0;
+                for(int x = 1; x < BATCH_XCDS; ++x)
+                    if(load[x] < load[g]) g = x;
+                e[k].z = (int32_t) load[g];
+                e[k].w = g;
+                load[g] += e[k].y;
+            }
+            const int64_t mx = *std::max_element(load, load + BATCH_XCDS);
+            const int64_t t = (int64_t) BATCH_XCDS * mx;
+            CULIP_REQUIRE(t <= (int64_t) (slot.items[q].bytes / sizeof(int2)),
+                "batch: XCD-aware list exceeds its buffer");
+            inter[q] = (int32_t) t;
+            total[q] = (int32_t) t;
+        }
+    }
     if(m > 0) {
         HIP_CHECK(hipMemcpyAsync(slot.ent.p, slot.h_ent, sizeof(int4) * 5 * (size_t) m,
             hipMemcpyHostToDevice, st));
         HIP_CHECK(hipEventRecord(slot.ev, st));
         slot.ev_used = true;
         for(int q = 0; q < 5; ++q)
-            batch_expand(slot.ent.as<int4>() + (size_t) q * m, m, slot.items[q].as<int2>(), st);
+            batch_expand(slot.ent.as<int4>() + (size_t) q * m, m, slot.items[q].as<int2>(),
+                inter[q], st);
     }
     BatchList* dst[5] = {&slot.L.vec, &slot.L.row, &slot.L.blk, &slot.L.spmv, &slot.L.sys};
     slot.L.one_round = true;

@@ -139,3 +139,11 @@ def test_batch_device_arrays(hip_device):
         assert xd[i].is_cuda
         assert np.array_equal(xd[i].cpu().numpy(), xh[i])
         assert rd[i] == rh[i]
+
+
+def test_batch_xcd_lists_equal_alone(hip_device):
+    # 70 running systems: the product's and the sweeps' lists are XCD-aware
+    # (each system's items on one XCD, holes between groups), same bits
+    systems = _systems([600 + 37 * i for i in range(70)], seed0=1100)
+    res = _run(hip_device, systems, "f32", 2)
+    assert all(r["converged"] for r in res), res
