@@ -24,9 +24,10 @@ namespace culip {
 
 namespace {
 
-// columns of U_ii^-1 by back substitution, in double; X[r][c] (32 x 32 LDS)
+// columns of U_ii^-1 by back substitution, in double; X[r][c] at X[hm r + c]
+// (hm x hm LDS, hm >= h: the largest block height of the launch)
 template<typename VT>
-__device__ void diag_inverse(const VT* U, int h, double* X)
+__device__ void diag_inverse(const VT* U, int h, double* X, int hm)
 {
     const int c = lane_id();
     // a tile with no entry right of its diagonal (most pivot blocks of a
@@ -36,26 +37,26 @@ __device__ void diag_inverse(const VT* U, int h, double* X)
     if(c < h)
         for(int k = c + 1; k < h; ++k) offd = offd || (U[c * h + k] != (VT) 0);
     if(wave_ballot(offd) == 0) {
-        for(int e = c; e < 1024; e += 64) {
-            const int r = e >> 5, q = e & 31;
+        for(int e = c; e < hm * hm; e += 64) {
+            const int r = e / hm, q = e - r * hm;
             X[e] = (r == q && r < h) ? 1.0 / (double) U[r * h + r] : 0.0;
         }
         wave_sync();
         return;
     }
-    for(int r = 31; r >= 0; --r) {
-        if(c < 32) {
+    for(int r = hm - 1; r >= 0; --r) {
+        if(c < hm) {
             double v = 0.0;
             if(c < h && r < h && r <= c) {
                 if(r == c) {
                     v = 1.0 / (double) U[r * h + r];
                 } else {
                     double s = 0.0;
-                    for(int k = r + 1; k <= c; ++k) s = fma((double) U[r * h + k], X[32 * k + c], s);
+                    for(int k = r + 1; k <= c; ++k) s = fma((double) U[r * h + k], X[hm * k + c], s);
                     v = -s / (double) U[r * h + r];
                 }
             }
-            X[32 * r + c] = v;
+            X[hm * r + c] = v;
         }
     }
     wave_sync();
@@ -127,29 +128,30 @@ __device__ __forceinline__ void plan_rank(const PlanStage<VT>& sg, int nblk, int
     }
 }
 
+// X: hm x hm doubles of LDS (the launch's dynamic LDS), hm >= every block
+// height of the launch
 template<typename VT, bool FILL>
 __device__ __forceinline__ void plan_rows_body(const SolveTopo& t, const VT* val_out,
-    const SolvePlan<VT>& p, int bid_)
+    const SolvePlan<VT>& p, int bid_, double* X, int hm)
 {
-    __shared__ double X[1024];
     const int lane = lane_id();
     const int i = bid_;
     const int fr = t.bs[i];
     const int h = t.bs[i + 1] - fr;
     // pivot block inverse rows (ui) and columns (ti)
     const BlockMeta dm = t.bout[t.csr_off[i]];
-    diag_inverse(val_out + dm.val_ptr, h, X);
+    diag_inverse(val_out + dm.val_ptr, h, X, hm);
     if(lane < h) {
         int cr = 0, cc = 0;
         int32_t wr = 0, wc = 0;
         if(FILL) { wr = p.ui_ptr[fr + lane]; wc = p.ti_ptr[fr + lane]; }
         for(int c = 0; c < h; ++c) {
-            const double v = X[32 * lane + c];  // row lane of U^-1
+            const double v = X[hm * lane + c];  // row lane of U^-1
             if(v != 0.0) {
                 if(FILL) { p.ui_col[wr + cr] = fr + c; p.ui_val[wr + cr] = (VT) v; p.ui_lr[wr + cr] = (uint8_t) lane; }
                 ++cr;
             }
-            const double u = X[32 * c + lane];  // column lane of U^-1
+            const double u = X[hm * c + lane];  // column lane of U^-1
             if(u != 0.0) {
                 if(FILL) { p.ti_col[wc + cc] = fr + c; p.ti_val[wc + cc] = (VT) u; p.ti_lr[wc + cc] = (uint8_t) lane; }
                 ++cc;
@@ -228,7 +230,8 @@ __device__ __forceinline__ void plan_rows_body(const SolveTopo& t, const VT* val
 template<typename VT, bool FILL>
 __global__ __launch_bounds__(64) void k_plan_rows(SolveTopo t, const VT* val_out, SolvePlan<VT> p)
 {
-    plan_rows_body<VT, FILL>(t, val_out, p, blockIdx.x);
+    extern __shared__ double plan_x[];
+    plan_rows_body<VT, FILL>(t, val_out, p, blockIdx.x, plan_x, 32);
 }
 
 template<typename VT, bool FILL>
@@ -952,11 +955,13 @@ __global__ __launch_bounds__(64) void bk_jacobi_sweep(const int2* items, const B
 // Batched layout build: item {s, q} is workgroup q of the single-system
 // launch on system s.
 template<typename VT, bool FILL>
-__global__ __launch_bounds__(64) void bk_plan_rows(const PlanSys<VT>* ps, const int2* items)
+__global__ __launch_bounds__(64) void bk_plan_rows(const PlanSys<VT>* ps, const int2* items,
+    int32_t hm)
 {
+    extern __shared__ double plan_x[];
     const int2 it = items[blockIdx.x];
     const PlanSys<VT>& P = ps[it.x];
-    plan_rows_body<VT, FILL>(P.t, P.val_out, P.p, it.y);
+    plan_rows_body<VT, FILL>(P.t, P.val_out, P.p, it.y, plan_x, hm);
 }
 
 template<typename VT, bool FILL>
@@ -1039,7 +1044,7 @@ template<typename VT>
 void solve_plan_count(const SolveTopo& t, const VT* val_out, VT*, SolvePlan<VT>& p, void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
-    k_plan_rows<VT, false><<<t.nb, 64, 0, st>>>(t, val_out, p);
+    k_plan_rows<VT, false><<<t.nb, 64, sizeof(double) * 1024, st>>>(t, val_out, p);
     k_plan_cols<VT, false><<<t.nb, 64, 0, st>>>(t, val_out, p);
 }
 
@@ -1048,7 +1053,7 @@ void solve_plan_fill(const SolveTopo& t, const VT* val_out, const VT*, SolvePlan
     void* stream)
 {
     hipStream_t st = (hipStream_t) stream;
-    k_plan_rows<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
+    k_plan_rows<VT, true><<<t.nb, 64, sizeof(double) * 1024, st>>>(t, val_out, p);
     k_plan_cols<VT, true><<<t.nb, 64, 0, st>>>(t, val_out, p);
     const unsigned g = (unsigned) ((t.nb + 255) / 256);
     k_plan_tasks<<<g, 256, 0, st>>>(t.nb, t.lvl_rows, t.bs, p.uo_ptr, p.ui_ptr, p.task_u);
@@ -1288,23 +1293,24 @@ int32_t scan_chunks(int32_t n) { return (n + PS_CHUNK - 1) / PS_CHUNK; }
 
 template<typename VT>
 void plan_batch(const PlanSys<VT>* ps, FactorItems blk, FactorItems scan, FactorItems task,
-    hipStream_t st)
+    int32_t hm, hipStream_t st)
 {
     if(blk.count <= 0) return;
     const unsigned nb = (unsigned) blk.count;
-    bk_plan_rows<VT, false><<<nb, 64, 0, st>>>(ps, blk.items);
+    const size_t lds = sizeof(double) * (size_t) hm * (size_t) hm;
+    bk_plan_rows<VT, false><<<nb, 64, lds, st>>>(ps, blk.items, hm);
     bk_plan_cols<VT, false><<<nb, 64, 0, st>>>(ps, blk.items);
     for(int32_t q = 0; q < 4; ++q) {
         bk_chunk_sums<VT><<<(unsigned) scan.count, PS_TPB, 0, st>>>(ps, scan.items, q);
         bk_chunk_scan<VT><<<(unsigned) scan.count, PS_TPB, 0, st>>>(ps, scan.items, q);
     }
-    bk_plan_rows<VT, true><<<nb, 64, 0, st>>>(ps, blk.items);
+    bk_plan_rows<VT, true><<<nb, 64, lds, st>>>(ps, blk.items, hm);
     bk_plan_cols<VT, true><<<nb, 64, 0, st>>>(ps, blk.items);
     bk_plan_tasks<VT><<<(unsigned) task.count, 256, 0, st>>>(ps, task.items, 0);
     bk_plan_tasks<VT><<<(unsigned) task.count, 256, 0, st>>>(ps, task.items, 1);
 }
 
-template void plan_batch<double>(const PlanSys<double>*, FactorItems, FactorItems, FactorItems, hipStream_t);
-template void plan_batch<float>(const PlanSys<float>*, FactorItems, FactorItems, FactorItems, hipStream_t);
+template void plan_batch<double>(const PlanSys<double>*, FactorItems, FactorItems, FactorItems, int32_t, hipStream_t);
+template void plan_batch<float>(const PlanSys<float>*, FactorItems, FactorItems, FactorItems, int32_t, hipStream_t);
 
 }  // namespace culip

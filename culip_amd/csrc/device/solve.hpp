@@ -72,10 +72,12 @@ struct PlanSys {
 };
 // solve_plan_count, the four scan_counts and solve_plan_fill of every
 // listed system: blk = {system, block row} items, scan = {system, chunk <
-// scan_chunks(n)}, task = {system, chunk of 256 block rows}
+// scan_chunks(n)}, task = {system, chunk of 256 block rows}; hm: the largest
+// block height of the listed systems (1..32; the LDS of the pivot-block
+// inverses is hm x hm per block row)
 template<typename VT>
 void plan_batch(const PlanSys<VT>* ps, FactorItems blk, FactorItems scan, FactorItems task,
-    hipStream_t st);
+    int32_t hm, hipStream_t st);
 
 // x <- U \ x (in place); x <- U' \ x; x <- D \ x.
 template<typename VT>

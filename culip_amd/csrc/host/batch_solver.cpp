@@ -412,6 +412,8 @@ void BatchSolver::build_factor()
     int32_t max_levels = 0;
     for(int32_t s = 0; s < S; ++s) {
         const Symbolic& y = sys_[s]->f->sym;
+        for(int32_t b = 0; b < y.nb; ++b)
+            max_h_ = std::max(max_h_, y.block_starts[b + 1] - y.block_starts[b]);
         lr_off[s] = lrs.size();
         for(int32_t l = 0; l < y.num_levels; ++l) {
             LevelRange r;
@@ -424,6 +426,8 @@ void BatchSolver::build_factor()
         }
         max_levels = std::max(max_levels, y.num_levels);
     }
+    // the solve layout's LDS tile of a pivot-block inverse is max_h_ squared
+    CULIP_REQUIRE(max_h_ >= 1 && max_h_ <= 32, "batch: block height outside 1..32");
     upload(d_lr_, lrs);
     std::vector<FactorSys<VT>> F(S);
     std::vector<PlanSys<VT>> P(S);
@@ -507,7 +511,7 @@ void BatchSolver::factor_batched(hipStream_t st)
     factor_batch_begin<VT>(fs, chunks_, st);
     for(int32_t l = 0; l < (int32_t) flevels_.size(); ++l)
         factor_batch_level<VT>(fs, flevels_[l], l, st);
-    plan_batch<VT>(d_psys_.as<PlanSys<VT>>(), blk_, scan_, task_, st);
+    plan_batch<VT>(d_psys_.as<PlanSys<VT>>(), blk_, scan_, task_, max_h_, st);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st));
     for(auto& e : sys_) e->f->after_factor();

@@ -120,6 +120,7 @@ private:
     // batched refactor: system tables, level ranges, work items
     DevBuf d_fsys_, d_psys_, d_lr_, d_fit_[7], d_chunks_, d_blk_, d_scan_, d_task_;
     std::vector<FactorBatchLevel> flevels_;
+    int32_t max_h_ = 1;   // largest block height of the batch
     FactorItems chunks_, blk_, scan_, task_;
     ListSlot slot_[4];
     std::vector<State> st_;
